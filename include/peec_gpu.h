@@ -1,0 +1,103 @@
+/* Device-resident PEEC frequency-point engine for one MI355X (gfx950).
+ *
+ * No counterpart in the reference: its drivers (main.c:662-900 assembly,
+ * main.c:1249-1568 harmonic solve) call the drop-in functions one by one
+ * through host memory. This engine keeps the mesh's static Lp/P, the MNA
+ * matrix and its factors in HBM and runs every stage of a frequency point on
+ * the GPU:
+ *
+ *   retarded Lp(w), P(w) (optional) -> MNA fill -> equilibration ->
+ *   LU (FP64, or FP32 MFMA factor + FP64 iterative refinement) ->
+ *   port right-hand sides -> solves -> component-wise backward error
+ *   (harmonic.c:932-969).
+ *
+ * Ports follow the reference's harmonic excitation (harmonic.h:118-126,
+ * harmonic.c:503-549): port k injects +1 A at node port_nodes[2k] and draws
+ * it at port_nodes[2k+1]; Z_in[k] = V_a - V_b of its own solution.
+ * Calls on one engine are not thread-safe; distinct engines are. */
+#ifndef PEEC_GPU_H
+#define PEEC_GPU_H
+
+#include <stddef.h>
+
+#include "complex_matrix.h"
+#include "dual_mesh.h"
+#include "inductance.h"
+#include "mesh.h"
+#include "potential.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+    PEEC_SOLVE_FP64 = 0,  /* FP64 LU: the drop-in / correctness path */
+    PEEC_SOLVE_MIXED = 1  /* FP32 MFMA factor + FP64 iterative refinement */
+} PeecSolveMode;
+
+typedef struct
+{
+    InductanceOptions inductance;
+    PotentialOptions potential;
+    int retarded;               /* 0 quasi-static (reference model), 1 exp(-jkR)/R */
+    int smooth_order;           /* GL order of the retarded remainder rule, default 2 */
+    int near_smooth_order;      /* same, for entries of nearby regions, default 4 */
+    int block_size;             /* LU panel width, 0 = automatic */
+    int max_refinement_steps;   /* mixed mode cap, default 30 */
+    double refinement_target;   /* mixed mode stops at this backward error, default 1e-14 */
+    double backward_error_gate; /* a point is accepted at or below this, default 1e-10 */
+} PeecEngineOptions;
+
+typedef struct
+{
+    double frequency;
+    double backward_error;      /* worst over the point's right-hand sides */
+    int refinement_steps;       /* mixed mode */
+    int used_fp64_fallback;     /* mixed mode did not reach the gate and refactored in FP64 */
+    double ms_assembly;         /* retarded Lp(w)/P(w) */
+    double ms_fill;             /* MNA fill + equilibration */
+    double ms_factor;
+    double ms_solve;            /* right-hand sides, solves and refinement */
+    double ms_check;            /* backward error */
+} PeecPointReport;
+
+typedef struct PeecEngine PeecEngine;
+
+void peec_engine_options_default(PeecEngineOptions *options);
+
+/* Builds the device copies of the mesh regions and assembles the static
+ * Lp and P (FP64). 0 on success, -1 on bad input or GPU failure. */
+int peec_engine_create(const Mesh *mesh, const DualMesh *dual,
+                       const PeecEngineOptions *options, PeecEngine **engine);
+
+/* Solves one frequency point for n_ports port excitations on one factor.
+ * z_in: n_ports values; edge_current / node_voltage: optional, rhs-major
+ * (n_ports x Ne, n_ports x Nv). Returns 0, -1 (bad input / GPU failure),
+ * -2 (singular) or -3 (backward error above the gate). */
+int peec_engine_solve(PeecEngine *engine, double frequency, int mode,
+                      const int *port_nodes, size_t n_ports, Complex *z_in,
+                      Complex *edge_current, Complex *node_voltage,
+                      PeecPointReport *report);
+
+/* Copies the static matrices (Ne x Ne, Nv x Nv, row-major) to the host. */
+int peec_engine_static_matrices(const PeecEngine *engine, double *L, double *P);
+
+/* Copies the retarded matrices at wave number k to the host. */
+int peec_engine_retarded_matrices(PeecEngine *engine, double wave_number,
+                                  Complex *L, Complex *P);
+
+size_t peec_engine_edges(const PeecEngine *engine);
+size_t peec_engine_nodes(const PeecEngine *engine);
+
+/* FLOP and evaluation counts of the static assembly (roofline accounting). */
+long long peec_engine_far_evaluations(const PeecEngine *engine);
+long long peec_engine_smooth_evaluations(const PeecEngine *engine);
+
+void peec_engine_destroy(PeecEngine *engine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

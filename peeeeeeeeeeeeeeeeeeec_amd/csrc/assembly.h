@@ -1,0 +1,91 @@
+// Device-resident assembly of the static Lp/P matrices (analytic 1/R panel
+// integrals) and of their retarded complex counterparts.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/dual_mesh.h"
+#include "../../include/inductance.h"
+#include "../../include/mesh.h"
+#include "../../include/potential.h"
+#include "common.h"
+
+namespace peec {
+
+struct QuadratureOrders {
+    int far_order = 4, near_order = 10, touching_order = 12, self_order = 8;
+    double near_factor = 0.5;
+};
+
+enum class RegionKind { Node = 0, Edge = 1 };
+
+// Work done by one assembly pass, for roofline accounting.
+struct AssemblyWork {
+    long long far_evaluations = 0;      // 1/R point evaluations in the tile kernel
+    long long near_entries = 0;         // entries recomputed by the near-field kernel
+    long long smooth_evaluations = 0;   // exp(-jkR) evaluations (retarded pass)
+};
+
+// One set of dual regions (all node regions for P, all edge regions for
+// Lp) staged in HBM for the tile kernels.
+class RegionSet {
+  public:
+    RegionSet(const Mesh *mesh, const DualMesh *dual, RegionKind kind, const QuadratureOrders &q,
+              int smooth_order, double direction_tolerance, hipStream_t stream,
+              bool count_work = false, int near_smooth_order = 4);
+
+    int count() const { return R_; }
+    RegionKind kind() const { return kind_; }
+
+    // static matrix (row-major R x R, original numbering) into d_out
+    void assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work);
+    // complex retarded matrix: d_out = d_static + scale * smooth remainder
+    void assemble_retarded(const double *d_static, double wave_number, double2 *d_out,
+                           hipStream_t s, AssemblyWork *work);
+    // single entries through the near-field kernel (drop-in *_compute_pair)
+    void assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,
+                        int *d_error);
+
+  private:
+    void build_points(const std::vector<double> &patch_vertices, const std::vector<int> &ptr,
+                      int order, std::vector<double> &pi, std::vector<double> &pj, int &slots,
+                      std::vector<int> &npts);
+    void build_near_list(const Mesh *mesh);
+
+    RegionKind kind_;
+    QuadratureOrders q_;
+    int smooth_order_;
+    int near_smooth_order_;
+    double dir_tol_;
+    int R_ = 0;
+    int maxp_ = 0;
+    int far_slots_ = 0, smooth_slots_ = 0;
+    std::vector<int> h_ptr_;
+    std::vector<double> h_pv_, h_aux_, h_dir_;
+    std::vector<int> h_perm_;           // kernel order -> original index
+    std::vector<int> h_near_;           // candidate near entries (i <= j), original ids
+    std::vector<int> h_tiles_;          // upper-triangle tile list (bi, bj)
+    DeviceBuffer<int> d_ptr_, d_perm_, d_near_, d_tiles_, d_far_n_, d_smooth_n_;
+    DeviceBuffer<double> d_pv_, d_aux_, d_dir_, d_aux_k_, d_dir_k_;
+    DeviceBuffer<int> d_smooth_err_;
+    DeviceBuffer<double4> d_far_i_, d_far_j_, d_smooth_i_, d_smooth_j_;
+    long long far_evals_ = 0, smooth_evals_ = 0;
+
+  public:
+    long long far_evaluations() const { return far_evals_; }
+    long long smooth_evaluations() const { return smooth_evals_; }
+    const std::vector<int> &near_list() const { return h_near_; }
+};
+
+// Raw integral over all piece pairs of two piece lists (12 doubles each).
+double region_pair_integral(const std::vector<double> &pa, const std::vector<double> &pb, bool same,
+                            const QuadratureOrders &q);
+
+// K(a, b) of two free pieces (NaN on failure), evaluated on the GPU.
+double patch_pair_integral(const QuadPatch *a, const QuadPatch *b, const QuadratureOrders &q);
+
+// Upload the Gauss-Legendre table to __constant__ memory (once per process).
+void ensure_device_tables();
+
+}  // namespace peec

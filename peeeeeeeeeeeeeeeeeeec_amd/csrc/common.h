@@ -1,0 +1,129 @@
+// Shared host/device helpers for the gfx950 PEEC library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/complex_matrix.h"
+
+namespace peec {
+
+// Errors are reported the reference's way: an "ERROR: ..." line on stderr
+// and a non-zero status to the caller.
+#define PEEC_ERR(...)                                   \
+    do {                                                \
+        std::fprintf(stderr, "ERROR: " __VA_ARGS__);    \
+        std::fprintf(stderr, "\n");                     \
+    } while (0)
+
+// Evaluate a HIP call; on failure print it and return `ret` from the caller.
+#define HIP_OR_RETURN(call, ret)                                              \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            std::fprintf(stderr, "ERROR: HIP %s failed at %s:%d: %s\n", #call, \
+                         __FILE__, __LINE__, hipGetErrorString(e_));          \
+            return ret;                                                       \
+        }                                                                     \
+    } while (0)
+
+// Throwing variant used inside C++ helpers; the C ABI catches and converts.
+struct HipError {
+    std::string what;
+};
+
+#define HIP_CHECK(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            char buf_[512];                                                    \
+            std::snprintf(buf_, sizeof buf_, "HIP %s failed at %s:%d: %s", #call, \
+                          __FILE__, __LINE__, hipGetErrorString(e_));         \
+            throw ::peec::HipError{buf_};                                      \
+        }                                                                      \
+    } while (0)
+
+// Owning device buffer (hipMalloc / hipFree).
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) { resize(n); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : ptr(o.ptr), count(o.count) {
+        o.ptr = nullptr;
+        o.count = 0;
+    }
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            count = o.count;
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+    void resize(size_t n) {
+        if (n == count && ptr) return;
+        release();
+        if (n) HIP_CHECK(hipMalloc(&ptr, n * sizeof(T)));
+        count = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    size_t bytes() const { return count * sizeof(T); }
+    void upload(const T *host, size_t n, hipStream_t s) {
+        resize(n);
+        if (n) HIP_CHECK(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void download(T *host, size_t n, hipStream_t s) const {
+        if (n) HIP_CHECK(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// One stream per call keeps concurrent drop-in calls on distinct outputs
+// independent (the reference functions are reentrant).
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    Stream(const Stream &) = delete;
+    Stream &operator=(const Stream &) = delete;
+    operator hipStream_t() const { return s; }
+    void sync() const { HIP_CHECK(hipStreamSynchronize(s)); }
+};
+
+// Fails loudly when no GPU is visible: there is no CPU fallback.
+bool require_device();
+
+// Gauss-Legendre nodes/weights for the supported orders, computed once on
+// the host from the Legendre three-term recurrence in binary128.
+bool gauss_legendre(int order, double *x, double *w);
+inline bool gl_order_supported(int o) {
+    return o == 2 || o == 3 || o == 4 || o == 5 || o == 6 || o == 8 || o == 10 || o == 12 ||
+           o == 16;
+}
+
+// Device constant table: nodes at [order][i], weights likewise.
+struct GLTable {
+    double x[17][16];
+    double w[17][16];
+};
+const GLTable &gl_table();
+
+}  // namespace peec

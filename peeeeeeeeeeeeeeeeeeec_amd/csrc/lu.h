@@ -1,0 +1,59 @@
+// Blocked right-looking LU with partial pivoting on the GPU (row-major,
+// in place), templated on the storage element:
+//   double2  complex FP64   (drop-in complex LU, FP64 oracle path)
+//   float2   complex FP32   (mixed-precision factor; panel arithmetic and
+//                            pivot magnitudes in FP64, trailing update on
+//                            FP32 MFMA)
+//   double   real FP64      (drop-in real LU)
+#pragma once
+
+#include "common.h"
+
+namespace peec {
+
+struct LUStatus {
+    int code = 0;        // 0 ok, -2 singular/invalid pivot, -3 internal timeout
+    long long column = -1;
+};
+
+// Scratch owned by one factorization context (one stream).
+struct LUContext {
+    DeviceBuffer<unsigned> barrier;   // grid barrier counter
+    DeviceBuffer<int> status;         // [0] code, [1] column
+    DeviceBuffer<double> slots;       // pivot-row exchange for the panel kernel
+    DeviceBuffer<double> cand;        // per-workgroup pivot candidates
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    explicit LUContext(hipStream_t s);
+};
+
+// Per-row / per-column max equilibration (complex_lu.c:275-320) of a
+// row-major complex FP64 matrix in place: A <- diag(1/r) A diag(1/c).
+// Returns 0, or -2 when a row or column is zero or not finite.
+// `max_scaled` receives max |A_eq|.
+int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d_col,
+                        double *max_scaled, LUContext &ctx);
+
+// Factor in place; d_ipiv[k] = row swapped with k at step k (complex_lu.c:330-510).
+template <class S>
+LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LUContext &ctx,
+                   int block_size = 0);
+
+// Solve with the factors for `nrhs` right-hand sides stored rhs-major
+// (X[r*n + i]); applies the row scale, pivots, L, U and column scale like
+// complex_lu_solve (complex_lu.c:724-778). Scales may be null (real LU).
+// The factor may be FP32 (float2): arithmetic is FP64 throughout.
+template <class S, class V>
+void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *d_row,
+              const double *d_col, V *dX, int nrhs, LUContext &ctx);
+
+// Convert an FP64 complex matrix to FP32 (for the mixed factor).
+void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s);
+
+// y = A x for complex FP64 (rhs-major vectors), and the reference's
+// component-wise backward error terms per row:
+//   out[i] = (|r_i|, sum_j |A_ij||x_j| + |b_i|, |b_i|) with r = A x - b.
+void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, const double2 *db,
+                      double2 *dr, double *dterms, hipStream_t s);
+
+}  // namespace peec

@@ -1,0 +1,393 @@
+"""Python host mirror of the drop-in C ABI (libpeec_mi355x.so).
+
+Function names and argument meaning follow the reference's C interface
+(inductance.h, potential.h, quadrature.h, incidence.h, lu.h, complex_lu.h,
+harmonic.h) so the tests read like the reference's own; results come back as
+numpy arrays. There is no fallback: if the HIP library is missing the import
+raises, and every call that needs a GPU fails with the library's error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ctypes_api as api
+
+_dp = C.POINTER(C.c_double)
+_cp = C.POINTER(api.Complex)
+_ip = C.POINTER(C.c_int)
+_M = C.POINTER(api.Mesh)
+_D = C.POINTER(api.DualMesh)
+_Q = C.POINTER(api.QuadPatch)
+
+
+class PeecEngineOptions(C.Structure):
+    _fields_ = [("inductance", api.InductanceOptions), ("potential", api.PotentialOptions),
+                ("retarded", C.c_int), ("smooth_order", C.c_int), ("near_smooth_order", C.c_int),
+                ("block_size", C.c_int),
+                ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
+                ("backward_error_gate", C.c_double)]
+
+
+class PeecPointReport(C.Structure):
+    _fields_ = [("frequency", C.c_double), ("backward_error", C.c_double),
+                ("refinement_steps", C.c_int), ("used_fp64_fallback", C.c_int),
+                ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
+                ("ms_solve", C.c_double), ("ms_check", C.c_double)]
+
+
+class RetardationOptions(C.Structure):
+    _fields_ = [("inductance", api.InductanceOptions), ("potential", api.PotentialOptions),
+                ("smooth_order", C.c_int), ("near_smooth_order", C.c_int)]
+
+
+# every symbol the public headers under include/ declare (checked by the CPU
+# test suite against the built library)
+EXPORTED = [
+    # mesh.h / dual_mesh.h
+    "mesh_init", "mesh_free", "mesh_load_gmsh", "mesh_build_topology", "mesh_print_info",
+    "dual_mesh_init", "dual_mesh_free", "dual_mesh_build", "dual_mesh_print_info",
+    # quadrature.h
+    "gauss_legendre_rule", "triangle_area", "quad_patch_area", "quad_patch_size",
+    "quad_patch_min_vertex_distance", "quad_patch_classify_pair", "quad_patch_kernel_1_over_r",
+    # inductance.h / potential.h
+    "inductance_options_default", "inductance_init", "inductance_free", "inductance_compute_pair",
+    "inductance_compute_matrix", "inductance_print_info",
+    "potential_options_default", "potential_init", "potential_free", "potential_compute_pair",
+    "potential_compute_matrix", "potential_print_info",
+    # retardation.h
+    "retardation_options_default", "retardation_compute_matrices",
+    # incidence.h / resistance.h
+    "incidence_init", "incidence_free", "incidence_build", "incidence_build_PAT",
+    "resistance_options_default", "resistance_init", "resistance_free", "resistance_compute_matrix",
+    "resistance_get",
+    # matrix.h / complex_matrix.h
+    "matrix_init", "matrix_alloc", "matrix_free", "complex_matrix_init", "complex_matrix_alloc",
+    "complex_matrix_free", "complex_matrix_vector_multiply",
+    # lu.h / complex_lu.h
+    "lu_options_default", "lu_init", "lu_free", "lu_factorize", "lu_solve", "lu_solve_multiple",
+    "lu_determinant", "lu_print_info",
+    "complex_lu_options_default", "complex_lu_init", "complex_lu_free", "complex_lu_factorize",
+    "complex_lu_solve", "complex_lu_solve_multiple", "complex_lu_print_info",
+    # harmonic.h
+    "harmonic_system_init", "harmonic_system_free", "harmonic_solution_init",
+    "harmonic_solution_free", "harmonic_build_system", "harmonic_factorize", "harmonic_build_rhs",
+    "harmonic_solve", "harmonic_compute_residual", "harmonic_print_info",
+    # peec_gpu.h
+    "peec_engine_options_default", "peec_engine_create", "peec_engine_solve",
+    "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
+    "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
+    "peec_engine_destroy",
+]
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(api.LIB_PATH):
+        raise ImportError(
+            f"{api.LIB_PATH} is missing: build it with `python -m peeeeeeeeeeeeeeeeeeec_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(api.LIB_PATH)
+    _sig(lib, "mesh_init", None, _M)
+    _sig(lib, "mesh_free", None, _M)
+    _sig(lib, "mesh_load_gmsh", C.c_int, C.c_char_p, _M)
+    _sig(lib, "mesh_build_topology", C.c_int, _M)
+    _sig(lib, "dual_mesh_init", None, _D)
+    _sig(lib, "dual_mesh_free", None, _D)
+    _sig(lib, "dual_mesh_build", C.c_int, _M, _D)
+    _sig(lib, "gauss_legendre_rule", C.c_int, C.c_int, _dp, _dp)
+    _sig(lib, "quad_patch_classify_pair", C.c_int, _Q, _Q, C.c_double)
+    _sig(lib, "quad_patch_kernel_1_over_r", C.c_double, _Q, _Q, C.c_int, C.c_int, C.c_int,
+         C.c_int, C.c_double)
+    _sig(lib, "quad_patch_area", C.c_double, _Q)
+    _sig(lib, "quad_patch_size", C.c_double, _Q)
+    _sig(lib, "inductance_options_default", None, C.POINTER(api.InductanceOptions), C.c_int)
+    _sig(lib, "potential_options_default", None, C.POINTER(api.PotentialOptions), C.c_int)
+    _sig(lib, "inductance_compute_matrix", C.c_int, _M, _D, C.POINTER(api.InductanceOptions),
+         C.POINTER(api.InductanceMatrix))
+    _sig(lib, "potential_compute_matrix", C.c_int, _M, _D, C.POINTER(api.PotentialOptions),
+         C.POINTER(api.PotentialMatrix))
+    _sig(lib, "inductance_compute_pair", C.c_int, _M, _D, C.c_size_t, C.c_size_t,
+         C.POINTER(api.InductanceOptions), _dp)
+    _sig(lib, "potential_compute_pair", C.c_int, _M, _D, C.c_size_t, C.c_size_t,
+         C.POINTER(api.PotentialOptions), _dp)
+    _sig(lib, "inductance_free", None, C.POINTER(api.InductanceMatrix))
+    _sig(lib, "potential_free", None, C.POINTER(api.PotentialMatrix))
+    _sig(lib, "retardation_options_default", None, C.POINTER(RetardationOptions), C.c_int)
+    _sig(lib, "retardation_compute_matrices", C.c_int, _M, _D, C.c_double,
+         C.POINTER(RetardationOptions), C.POINTER(api.ComplexMatrix), C.POINTER(api.ComplexMatrix))
+    _sig(lib, "complex_matrix_free", None, C.POINTER(api.ComplexMatrix))
+    _sig(lib, "matrix_free", None, C.POINTER(api.Matrix))
+    _sig(lib, "incidence_build", C.c_int, _M, C.POINTER(api.IncidenceMatrix))
+    _sig(lib, "incidence_free", None, C.POINTER(api.IncidenceMatrix))
+    _sig(lib, "incidence_build_PAT", C.c_int, _M, C.POINTER(api.PotentialMatrix),
+         C.POINTER(api.Matrix), C.c_int)
+    _sig(lib, "resistance_options_default", None, C.POINTER(api.ResistanceOptions), C.c_int)
+    _sig(lib, "resistance_compute_matrix", C.c_int, _M, _D, C.POINTER(api.ResistanceOptions),
+         C.POINTER(api.ResistanceMatrix))
+    _sig(lib, "resistance_free", None, C.POINTER(api.ResistanceMatrix))
+    _sig(lib, "complex_matrix_vector_multiply", C.c_int, C.POINTER(api.ComplexMatrix), _cp, _cp, C.c_int)
+    _sig(lib, "lu_options_default", None, C.POINTER(api.LUOptions), C.c_int)
+    _sig(lib, "lu_init", None, C.POINTER(api.LUFactorization))
+    _sig(lib, "lu_free", None, C.POINTER(api.LUFactorization))
+    _sig(lib, "lu_factorize", C.c_int, _dp, C.c_size_t, C.POINTER(api.LUOptions),
+         C.POINTER(api.LUFactorization))
+    _sig(lib, "lu_solve", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp)
+    _sig(lib, "lu_solve_multiple", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp, C.c_size_t, C.c_int)
+    _sig(lib, "lu_determinant", C.c_double, C.POINTER(api.LUFactorization))
+    _sig(lib, "complex_lu_options_default", None, C.POINTER(api.ComplexLUOptions), C.c_int)
+    _sig(lib, "complex_lu_init", None, C.POINTER(api.ComplexLUFactorization))
+    _sig(lib, "complex_lu_free", None, C.POINTER(api.ComplexLUFactorization))
+    _sig(lib, "complex_lu_factorize", C.c_int, _cp, C.c_size_t, C.POINTER(api.ComplexLUOptions),
+         C.POINTER(api.ComplexLUFactorization))
+    _sig(lib, "complex_lu_solve", C.c_int, C.POINTER(api.ComplexLUFactorization), _cp, _cp)
+    _sig(lib, "complex_lu_solve_multiple", C.c_int, C.POINTER(api.ComplexLUFactorization), _cp, _cp,
+         C.c_size_t, C.c_int)
+    _sig(lib, "harmonic_system_init", None, C.POINTER(api.HarmonicSystem))
+    _sig(lib, "harmonic_system_free", None, C.POINTER(api.HarmonicSystem))
+    _sig(lib, "harmonic_solution_init", None, C.POINTER(api.HarmonicSolution))
+    _sig(lib, "harmonic_solution_free", None, C.POINTER(api.HarmonicSolution))
+    _sig(lib, "harmonic_build_system", C.c_int, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.ResistanceMatrix), C.POINTER(api.IncidenceMatrix), C.POINTER(api.Matrix),
+         C.c_double, C.c_int, C.POINTER(api.HarmonicSystem))
+    _sig(lib, "harmonic_factorize", C.c_int, C.POINTER(api.HarmonicSystem), C.c_int)
+    _sig(lib, "harmonic_build_rhs", C.c_int, C.POINTER(api.PotentialMatrix),
+         C.POINTER(api.HarmonicExcitation), _cp, C.c_int)
+    _sig(lib, "harmonic_solve", C.c_int, C.POINTER(api.HarmonicSystem), C.POINTER(api.PotentialMatrix),
+         C.POINTER(api.HarmonicExcitation), C.POINTER(api.HarmonicSolution), C.c_int)
+    _sig(lib, "harmonic_compute_residual", C.c_int, C.POINTER(api.HarmonicSystem),
+         C.POINTER(api.PotentialMatrix), C.POINTER(api.HarmonicExcitation),
+         C.POINTER(api.HarmonicSolution), C.c_int, C.POINTER(api.HarmonicResidual))
+    _sig(lib, "peec_engine_options_default", None, C.POINTER(PeecEngineOptions))
+    _sig(lib, "peec_engine_create", C.c_int, _M, _D, C.POINTER(PeecEngineOptions), C.POINTER(C.c_void_p))
+    _sig(lib, "peec_engine_solve", C.c_int, C.c_void_p, C.c_double, C.c_int, _ip, C.c_size_t, _cp, _cp,
+         _cp, C.POINTER(PeecPointReport))
+    _sig(lib, "peec_engine_static_matrices", C.c_int, C.c_void_p, _dp, _dp)
+    _sig(lib, "peec_engine_retarded_matrices", C.c_int, C.c_void_p, C.c_double, _cp, _cp)
+    _sig(lib, "peec_engine_edges", C.c_size_t, C.c_void_p)
+    _sig(lib, "peec_engine_nodes", C.c_size_t, C.c_void_p)
+    _sig(lib, "peec_engine_far_evaluations", C.c_longlong, C.c_void_p)
+    _sig(lib, "peec_engine_smooth_evaluations", C.c_longlong, C.c_void_p)
+    _sig(lib, "peec_engine_destroy", None, C.c_void_p)
+    _lib = lib
+    return lib
+
+
+# ---------------------------------------------------------------------------
+# thin numpy-level wrappers
+
+
+class Model:
+    """A Mesh + DualMesh built by the product library from xyz/quads or .msh."""
+
+    def __init__(self, xyz=None, quads=None, msh=None):
+        lib = load()
+        if msh is not None:
+            self.mesh = api.Mesh()
+            if lib.mesh_load_gmsh(msh.encode(), C.byref(self.mesh)) != 0:
+                raise RuntimeError(f"cannot load {msh}")
+        else:
+            self.mesh = api.make_mesh(xyz, quads)
+        if lib.mesh_build_topology(C.byref(self.mesh)) != 0:
+            raise RuntimeError("mesh_build_topology failed")
+        self.dual = api.DualMesh()
+        if lib.dual_mesh_build(C.byref(self.mesh), C.byref(self.dual)) != 0:
+            raise RuntimeError("dual_mesh_build failed")
+        self.ne = self.mesh.n_edges
+        self.nv = self.mesh.n_nodes
+
+    def close(self):
+        lib = load()
+        lib.dual_mesh_free(C.byref(self.dual))
+        lib.mesh_free(C.byref(self.mesh))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def edges(self):
+        return api.as_array(self.mesh.edges, 2 * self.ne, np.int32).reshape(-1, 2)
+
+    def inductance(self, options=None):
+        lib = load()
+        o = options or api.InductanceOptions()
+        if options is None:
+            lib.inductance_options_default(C.byref(o), 1)
+        L = api.InductanceMatrix()
+        st = lib.inductance_compute_matrix(C.byref(self.mesh), C.byref(self.dual), C.byref(o), C.byref(L))
+        if st != 0:
+            raise RuntimeError(f"inductance_compute_matrix failed ({st})")
+        out = api.as_array(L.data, L.n * L.n).reshape(L.n, L.n)
+        lib.inductance_free(C.byref(L))
+        return out
+
+    def potential(self, options=None):
+        lib = load()
+        o = options or api.PotentialOptions()
+        if options is None:
+            lib.potential_options_default(C.byref(o), 1)
+        P = api.PotentialMatrix()
+        st = lib.potential_compute_matrix(C.byref(self.mesh), C.byref(self.dual), C.byref(o), C.byref(P))
+        if st != 0:
+            raise RuntimeError(f"potential_compute_matrix failed ({st})")
+        out = api.as_array(P.data, P.n * P.n).reshape(P.n, P.n)
+        lib.potential_free(C.byref(P))
+        return out
+
+    def retarded(self, wave_number, smooth_order=2, near_smooth_order=4):
+        lib = load()
+        o = RetardationOptions()
+        lib.retardation_options_default(C.byref(o), 1)
+        o.smooth_order = smooth_order
+        o.near_smooth_order = near_smooth_order
+        L = api.ComplexMatrix()
+        P = api.ComplexMatrix()
+        st = lib.retardation_compute_matrices(C.byref(self.mesh), C.byref(self.dual), wave_number,
+                                              C.byref(o), C.byref(L), C.byref(P))
+        if st != 0:
+            raise RuntimeError(f"retardation_compute_matrices failed ({st})")
+        Lc = api.complex_array(L.data, L.rows * L.cols).reshape(L.rows, L.cols)
+        Pc = api.complex_array(P.data, P.rows * P.cols).reshape(P.rows, P.cols)
+        lib.complex_matrix_free(C.byref(L))
+        lib.complex_matrix_free(C.byref(P))
+        return Lc, Pc
+
+
+def complex_lu_factor_solve(Z, B):
+    """Factor Z with complex_lu_factorize and solve the columns of B (n x k)."""
+    lib = load()
+    Z = np.ascontiguousarray(Z, dtype=np.complex128)
+    n = Z.shape[0]
+    o = api.ComplexLUOptions()
+    lib.complex_lu_options_default(C.byref(o), 1)
+    lu = api.ComplexLUFactorization()
+    st = lib.complex_lu_factorize(Z.ctypes.data_as(_cp), n, C.byref(o), C.byref(lu))
+    if st != 0:
+        lib.complex_lu_free(C.byref(lu))
+        return st, None, None
+    Bm = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).reshape(-1, n))
+    X = np.zeros_like(Bm)
+    st = lib.complex_lu_solve_multiple(C.byref(lu), Bm.ctypes.data_as(_cp), X.ctypes.data_as(_cp),
+                                       Bm.shape[0], 1)
+    info = {
+        "pivots": api.as_array(lu.pivots, n, np.uint64).astype(np.int64),
+        "row_scale": api.as_array(lu.row_scale, n),
+        "column_scale": api.as_array(lu.column_scale, n),
+        "matrix_scale": lu.matrix_scale,
+        "swap_count": lu.swap_count,
+    }
+    lib.complex_lu_free(C.byref(lu))
+    return st, X, info
+
+
+def real_lu_factor_solve(A, B):
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    o = api.LUOptions()
+    lib.lu_options_default(C.byref(o), 1)
+    lu = api.LUFactorization()
+    st = lib.lu_factorize(A.ctypes.data_as(_dp), n, C.byref(o), C.byref(lu))
+    if st != 0:
+        lib.lu_free(C.byref(lu))
+        return st, None, None
+    Bm = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(-1, n))
+    X = np.zeros_like(Bm)
+    st = lib.lu_solve_multiple(C.byref(lu), Bm.ctypes.data_as(_dp), X.ctypes.data_as(_dp), Bm.shape[0], 1)
+    det = lib.lu_determinant(C.byref(lu))
+    lib.lu_free(C.byref(lu))
+    return st, X, det
+
+
+class Engine:
+    """Device-resident frequency-point engine (include/peec_gpu.h)."""
+
+    FP64 = 0
+    MIXED = 1
+
+    def __init__(self, model: Model, retarded=False, smooth_order=2, near_smooth_order=4, block_size=0,
+                 **opts):
+        lib = load()
+        self.model = model
+        o = PeecEngineOptions()
+        lib.peec_engine_options_default(C.byref(o))
+        o.retarded = int(bool(retarded))
+        o.smooth_order = smooth_order
+        o.near_smooth_order = near_smooth_order
+        o.block_size = block_size
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.options = o
+        h = C.c_void_p()
+        st = lib.peec_engine_create(C.byref(model.mesh), C.byref(model.dual), C.byref(o), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"peec_engine_create failed ({st})")
+        self.handle = h
+        self.ne = model.ne
+        self.nv = model.nv
+
+    def close(self):
+        if self.handle:
+            load().peec_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def static_matrices(self):
+        L = np.zeros(self.ne * self.ne)
+        P = np.zeros(self.nv * self.nv)
+        if load().peec_engine_static_matrices(self.handle, L.ctypes.data_as(_dp), P.ctypes.data_as(_dp)) != 0:
+            raise RuntimeError("peec_engine_static_matrices failed")
+        return L.reshape(self.ne, self.ne), P.reshape(self.nv, self.nv)
+
+    def retarded_matrices(self, k):
+        L = np.zeros(self.ne * self.ne, dtype=np.complex128)
+        P = np.zeros(self.nv * self.nv, dtype=np.complex128)
+        if load().peec_engine_retarded_matrices(self.handle, k, L.ctypes.data_as(_cp), P.ctypes.data_as(_cp)) != 0:
+            raise RuntimeError("peec_engine_retarded_matrices failed")
+        return L.reshape(self.ne, self.ne), P.reshape(self.nv, self.nv)
+
+    def solve(self, frequency, ports, mode=0, want_solution=True):
+        """ports: list of (a, b) node pairs. Returns (status, z_in, I, V, report)."""
+        ports = np.ascontiguousarray(np.asarray(ports, dtype=np.int32).reshape(-1, 2))
+        k = ports.shape[0]
+        z = np.zeros(k, dtype=np.complex128)
+        I = np.zeros(k * self.ne, dtype=np.complex128) if want_solution else None
+        V = np.zeros(k * self.nv, dtype=np.complex128) if want_solution else None
+        rep = PeecPointReport()
+        st = load().peec_engine_solve(
+            self.handle, float(frequency), int(mode), ports.ctypes.data_as(_ip), k,
+            z.ctypes.data_as(_cp), I.ctypes.data_as(_cp) if I is not None else None,
+            V.ctypes.data_as(_cp) if V is not None else None, C.byref(rep))
+        if I is not None:
+            I = I.reshape(k, self.ne)
+            V = V.reshape(k, self.nv)
+        return st, z, I, V, rep
+
+    @property
+    def far_evaluations(self):
+        return load().peec_engine_far_evaluations(self.handle)
+
+    @property
+    def smooth_evaluations(self):
+        return load().peec_engine_smooth_evaluations(self.handle)

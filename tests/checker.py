@@ -1,0 +1,318 @@
+"""Loaders for the CPU checker libraries used by the tests.
+
+* oracle/liboracle.so      — this repository's C restatement (always built)
+* oracle/_ref/libpeecref.so — the reference's own hot-path sources compiled
+  by oracle/Makefile (present when the reference was available at build
+  time; it travels with the tree as a built library).
+
+Both take the same Mesh/DualMesh structs as the product library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from peeeeeeeeeeeeeeeeeeec_amd import ctypes_api as api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libpeecref.so")
+PLATE_MSH = os.path.join(ROOT, "tests", "golden", "plate.msh")
+
+_dp = C.POINTER(C.c_double)
+_cp = C.POINTER(api.Complex)
+_sp = C.POINTER(C.c_size_t)
+_M = C.POINTER(api.Mesh)
+_D = C.POINTER(api.DualMesh)
+_Q = C.POINTER(api.QuadPatch)
+
+
+class OracleCounters(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in (
+        "far_evaluations", "potential_calls", "near_triangle_pairs",
+        "far_patch_pairs", "near_patch_pairs")]
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = C.CDLL(ORACLE_PATH)
+        _sig(lib, "oracle_gauss_legendre", C.c_int, C.c_int, _dp, _dp)
+        _sig(lib, "oracle_load_gmsh", C.c_int, C.c_char_p, _M)
+        _sig(lib, "oracle_build_topology", C.c_int, _M)
+        _sig(lib, "oracle_build_dual", C.c_int, _M, _D)
+        _sig(lib, "oracle_free_mesh", None, _M)
+        _sig(lib, "oracle_free_dual", None, _D)
+        _sig(lib, "oracle_classify", C.c_int, _Q, _Q, C.c_double)
+        _sig(lib, "oracle_patch_kernel", C.c_double, _Q, _Q, C.c_int, C.c_int, C.c_int,
+             C.c_int, C.c_double)
+        _sig(lib, "oracle_inductance_matrix", C.c_int, _M, _D,
+             C.POINTER(api.InductanceOptions), _dp, C.POINTER(OracleCounters))
+        _sig(lib, "oracle_potential_matrix", C.c_int, _M, _D,
+             C.POINTER(api.PotentialOptions), _dp, C.POINTER(OracleCounters))
+        _sig(lib, "oracle_pat", C.c_int, _M, _dp, _dp)
+        _sig(lib, "oracle_mna", C.c_int, _M, _dp, _dp, _dp, _cp, _cp, C.c_double, _cp)
+        _sig(lib, "oracle_complex_lu", C.c_int, _cp, C.c_size_t, C.c_double, _cp, _sp,
+             _dp, _dp, _dp)
+        _sig(lib, "oracle_complex_solve", C.c_int, _cp, _sp, _dp, _dp, C.c_size_t, _cp, _cp)
+        _sig(lib, "oracle_real_lu", C.c_int, _dp, C.c_size_t, C.c_double, _dp, _sp, _dp)
+        _sig(lib, "oracle_real_solve", C.c_int, _dp, _sp, C.c_size_t, _dp, _dp)
+        _sig(lib, "oracle_backward_error", C.c_int, _cp, _cp, _cp, C.c_size_t, C.c_size_t, _dp)
+        _sig(lib, "oracle_rhs", C.c_int, C.c_size_t, C.c_size_t, _dp, _cp, _cp, _cp, _cp)
+        _sig(lib, "oracle_retarded_matrices", C.c_int, _M, _D,
+             C.POINTER(api.InductanceOptions), C.POINTER(api.PotentialOptions), C.c_double,
+             C.c_int, C.c_int, _dp, _dp, _cp, _cp)
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The compiled reference (its own symbol names)."""
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF_PATH)
+        _sig(lib, "mesh_load_gmsh", C.c_int, C.c_char_p, _M)
+        _sig(lib, "mesh_build_topology", C.c_int, _M)
+        _sig(lib, "mesh_free", None, _M)
+        _sig(lib, "dual_mesh_build", C.c_int, _M, _D)
+        _sig(lib, "dual_mesh_free", None, _D)
+        _sig(lib, "gauss_legendre_rule", C.c_int, C.c_int, _dp, _dp)
+        _sig(lib, "quad_patch_kernel_1_over_r", C.c_double, _Q, _Q, C.c_int, C.c_int,
+             C.c_int, C.c_int, C.c_double)
+        _sig(lib, "quad_patch_classify_pair", C.c_int, _Q, _Q, C.c_double)
+        _sig(lib, "inductance_options_default", None, C.POINTER(api.InductanceOptions), C.c_int)
+        _sig(lib, "potential_options_default", None, C.POINTER(api.PotentialOptions), C.c_int)
+        _sig(lib, "inductance_compute_matrix", C.c_int, _M, _D,
+             C.POINTER(api.InductanceOptions), C.POINTER(api.InductanceMatrix))
+        _sig(lib, "potential_compute_matrix", C.c_int, _M, _D,
+             C.POINTER(api.PotentialOptions), C.POINTER(api.PotentialMatrix))
+        _sig(lib, "inductance_free", None, C.POINTER(api.InductanceMatrix))
+        _sig(lib, "potential_free", None, C.POINTER(api.PotentialMatrix))
+        _sig(lib, "incidence_build", C.c_int, _M, C.POINTER(api.IncidenceMatrix))
+        _sig(lib, "incidence_free", None, C.POINTER(api.IncidenceMatrix))
+        _sig(lib, "incidence_build_PAT", C.c_int, _M, C.POINTER(api.PotentialMatrix),
+             C.POINTER(api.Matrix), C.c_int)
+        _sig(lib, "matrix_free", None, C.POINTER(api.Matrix))
+        _sig(lib, "resistance_options_default", None, C.POINTER(api.ResistanceOptions), C.c_int)
+        _sig(lib, "resistance_compute_matrix", C.c_int, _M, _D,
+             C.POINTER(api.ResistanceOptions), C.POINTER(api.ResistanceMatrix))
+        _sig(lib, "resistance_free", None, C.POINTER(api.ResistanceMatrix))
+        _sig(lib, "harmonic_system_init", None, C.POINTER(api.HarmonicSystem))
+        _sig(lib, "harmonic_system_free", None, C.POINTER(api.HarmonicSystem))
+        _sig(lib, "harmonic_solution_init", None, C.POINTER(api.HarmonicSolution))
+        _sig(lib, "harmonic_solution_free", None, C.POINTER(api.HarmonicSolution))
+        _sig(lib, "harmonic_build_system", C.c_int, C.POINTER(api.InductanceMatrix),
+             C.POINTER(api.ResistanceMatrix), C.POINTER(api.IncidenceMatrix),
+             C.POINTER(api.Matrix), C.c_double, C.c_int, C.POINTER(api.HarmonicSystem))
+        _sig(lib, "harmonic_factorize", C.c_int, C.POINTER(api.HarmonicSystem), C.c_int)
+        _sig(lib, "harmonic_solve", C.c_int, C.POINTER(api.HarmonicSystem),
+             C.POINTER(api.PotentialMatrix), C.POINTER(api.HarmonicExcitation),
+             C.POINTER(api.HarmonicSolution), C.c_int)
+        _sig(lib, "harmonic_compute_residual", C.c_int, C.POINTER(api.HarmonicSystem),
+             C.POINTER(api.PotentialMatrix), C.POINTER(api.HarmonicExcitation),
+             C.POINTER(api.HarmonicSolution), C.c_int, C.POINTER(api.HarmonicResidual))
+        _sig(lib, "complex_lu_options_default", None, C.POINTER(api.ComplexLUOptions), C.c_int)
+        _sig(lib, "complex_lu_init", None, C.POINTER(api.ComplexLUFactorization))
+        _sig(lib, "complex_lu_free", None, C.POINTER(api.ComplexLUFactorization))
+        _sig(lib, "complex_lu_factorize", C.c_int, _cp, C.c_size_t,
+             C.POINTER(api.ComplexLUOptions), C.POINTER(api.ComplexLUFactorization))
+        _sig(lib, "complex_lu_solve", C.c_int, C.POINTER(api.ComplexLUFactorization), _cp, _cp)
+        _sig(lib, "lu_options_default", None, C.POINTER(api.LUOptions), C.c_int)
+        _sig(lib, "lu_init", None, C.POINTER(api.LUFactorization))
+        _sig(lib, "lu_free", None, C.POINTER(api.LUFactorization))
+        _sig(lib, "lu_factorize", C.c_int, _dp, C.c_size_t, C.POINTER(api.LUOptions),
+             C.POINTER(api.LUFactorization))
+        _sig(lib, "lu_solve", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp)
+        _ref = lib
+    return _ref
+
+
+# ---------------------------------------------------------------------------
+# numpy-level conveniences over the oracle
+
+
+def default_inductance_options(threads=1):
+    o = api.InductanceOptions()
+    o.far_order, o.near_order, o.touching_order, o.self_order = 4, 10, 12, 8
+    o.near_factor, o.parallel_threads, o.direction_tolerance = 0.5, threads, 1e-10
+    return o
+
+
+def default_potential_options(threads=1):
+    o = api.PotentialOptions()
+    o.far_order, o.near_order, o.touching_order, o.self_order = 4, 10, 12, 8
+    o.near_factor, o.parallel_threads = 0.5, threads
+    return o
+
+
+class OracleModel:
+    """Mesh + dual mesh built by the oracle from xyz/quads (or a .msh)."""
+
+    def __init__(self, xyz=None, quads=None, msh=None):
+        lib = oracle()
+        if msh is not None:
+            self.mesh = api.Mesh()
+            if lib.oracle_load_gmsh(msh.encode(), C.byref(self.mesh)) != 0:
+                raise RuntimeError("oracle gmsh load failed")
+        else:
+            self.mesh = api.make_mesh(xyz, quads)
+        if lib.oracle_build_topology(C.byref(self.mesh)) != 0:
+            raise RuntimeError("oracle topology failed")
+        self.dual = api.DualMesh()
+        if lib.oracle_build_dual(C.byref(self.mesh), C.byref(self.dual)) != 0:
+            raise RuntimeError("oracle dual failed")
+        self.ne = self.mesh.n_edges
+        self.nv = self.mesh.n_nodes
+
+    def close(self):
+        lib = oracle()
+        lib.oracle_free_dual(C.byref(self.dual))
+        lib.oracle_free_mesh(C.byref(self.mesh))
+
+    def inductance(self, counters=None, opt=None):
+        L = np.zeros(self.ne * self.ne)
+        o = opt or default_inductance_options()
+        st = oracle().oracle_inductance_matrix(
+            C.byref(self.mesh), C.byref(self.dual), C.byref(o),
+            L.ctypes.data_as(_dp), C.byref(counters) if counters is not None else None)
+        if st != 0:
+            raise RuntimeError("oracle inductance failed")
+        return L.reshape(self.ne, self.ne)
+
+    def potential(self, counters=None, opt=None):
+        P = np.zeros(self.nv * self.nv)
+        o = opt or default_potential_options()
+        st = oracle().oracle_potential_matrix(
+            C.byref(self.mesh), C.byref(self.dual), C.byref(o),
+            P.ctypes.data_as(_dp), C.byref(counters) if counters is not None else None)
+        if st != 0:
+            raise RuntimeError("oracle potential failed")
+        return P.reshape(self.nv, self.nv)
+
+    def edges(self):
+        return api.as_array(self.mesh.edges, 2 * self.ne, np.int32).reshape(-1, 2)
+
+    def pat(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        PAT = np.zeros(self.nv * self.ne)
+        oracle().oracle_pat(C.byref(self.mesh), P.ctypes.data_as(_dp), PAT.ctypes.data_as(_dp))
+        return PAT.reshape(self.nv, self.ne)
+
+    def mna(self, L, PAT, frequency, Lc=None, Pc=None):
+        n = self.ne + self.nv
+        Z = np.zeros(n * n, dtype=np.complex128)
+        L = np.ascontiguousarray(L, dtype=np.float64) if L is not None else None
+        PAT = np.ascontiguousarray(PAT, dtype=np.float64) if PAT is not None else None
+        Lc = np.ascontiguousarray(Lc, dtype=np.complex128) if Lc is not None else None
+        Pc = np.ascontiguousarray(Pc, dtype=np.complex128) if Pc is not None else None
+        st = oracle().oracle_mna(
+            C.byref(self.mesh),
+            L.ctypes.data_as(_dp) if L is not None else None, None,
+            PAT.ctypes.data_as(_dp) if PAT is not None else None,
+            Lc.ctypes.data_as(_cp) if Lc is not None else None,
+            Pc.ctypes.data_as(_cp) if Pc is not None else None,
+            frequency, Z.ctypes.data_as(_cp))
+        if st != 0:
+            raise RuntimeError("oracle mna failed")
+        return Z.reshape(n, n)
+
+    def retarded(self, k, order, Ls, Ps, near_order=4):
+        Lc = np.zeros(self.ne * self.ne, dtype=np.complex128)
+        Pc = np.zeros(self.nv * self.nv, dtype=np.complex128)
+        Ls = np.ascontiguousarray(Ls); Ps = np.ascontiguousarray(Ps)
+        st = oracle().oracle_retarded_matrices(
+            C.byref(self.mesh), C.byref(self.dual), C.byref(default_inductance_options()),
+            C.byref(default_potential_options()), k, order, near_order, Ls.ctypes.data_as(_dp),
+            Ps.ctypes.data_as(_dp), Lc.ctypes.data_as(_cp), Pc.ctypes.data_as(_cp))
+        if st != 0:
+            raise RuntimeError("oracle retardation failed")
+        return Lc.reshape(self.ne, self.ne), Pc.reshape(self.nv, self.nv)
+
+
+def port_current(nv, a, b):
+    Is = np.zeros(nv, dtype=np.complex128)
+    Is[a] += 1.0
+    Is[b] -= 1.0
+    return Is
+
+
+def oracle_rhs(ne, nv, P=None, Pc=None, U=None, Is=None):
+    rhs = np.zeros(ne + nv, dtype=np.complex128)
+    Pr = np.ascontiguousarray(P, dtype=np.float64) if P is not None else None
+    Pcc = np.ascontiguousarray(Pc, dtype=np.complex128) if Pc is not None else None
+    Uc = np.ascontiguousarray(U, dtype=np.complex128) if U is not None else None
+    Ic = np.ascontiguousarray(Is, dtype=np.complex128) if Is is not None else None
+    oracle().oracle_rhs(ne, nv, Pr.ctypes.data_as(_dp) if Pr is not None else None,
+                        Pcc.ctypes.data_as(_cp) if Pcc is not None else None,
+                        Uc.ctypes.data_as(_cp) if Uc is not None else None,
+                        Ic.ctypes.data_as(_cp) if Ic is not None else None,
+                        rhs.ctypes.data_as(_cp))
+    return rhs
+
+
+def oracle_complex_lu(Z, tol=1e-14):
+    n = Z.shape[0]
+    Z = np.ascontiguousarray(Z, dtype=np.complex128)
+    LU = np.zeros_like(Z)
+    piv = np.zeros(n, dtype=np.uintp)
+    rs = np.zeros(n); cs = np.zeros(n); ms = C.c_double()
+    st = oracle().oracle_complex_lu(Z.ctypes.data_as(_cp), n, tol, LU.ctypes.data_as(_cp),
+                                    piv.ctypes.data_as(_sp), rs.ctypes.data_as(_dp),
+                                    cs.ctypes.data_as(_dp), C.byref(ms))
+    return st, (LU, piv, rs, cs, ms.value)
+
+
+def oracle_complex_solve(fac, b):
+    LU, piv, rs, cs, _ = fac
+    n = LU.shape[0]
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    x = np.zeros(n, dtype=np.complex128)
+    st = oracle().oracle_complex_solve(LU.ctypes.data_as(_cp), piv.ctypes.data_as(_sp),
+                                       rs.ctypes.data_as(_dp), cs.ctypes.data_as(_dp), n,
+                                       b.ctypes.data_as(_cp), x.ctypes.data_as(_cp))
+    if st != 0:
+        raise RuntimeError("oracle solve failed")
+    return x
+
+
+def oracle_backward_error(Z, x, b, ne):
+    n = Z.shape[0]
+    out = np.zeros(7)
+    Z = np.ascontiguousarray(Z, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    oracle().oracle_backward_error(Z.ctypes.data_as(_cp), x.ctypes.data_as(_cp),
+                                   b.ctypes.data_as(_cp), n, ne, out.ctypes.data_as(_dp))
+    return out
+
+
+def oracle_port_solve(model: OracleModel, L, P, frequency, a, b):
+    """Quasi-static port solve through the oracle: returns (Z_in, I, V, eta)."""
+    PAT = model.pat(P)
+    Z = model.mna(L, PAT, frequency)
+    Is = port_current(model.nv, a, b)
+    rhs = oracle_rhs(model.ne, model.nv, P=P, Is=Is)
+    st, fac = oracle_complex_lu(Z)
+    if st != 0:
+        raise RuntimeError(f"oracle LU failed {st}")
+    x = oracle_complex_solve(fac, rhs)
+    eta = oracle_backward_error(Z, x, rhs, model.ne)
+    I, V = x[:model.ne], x[model.ne:]
+    return V[a] - V[b], I, V, eta

@@ -1,0 +1,110 @@
+"""The C-ABI library: loads, exports every symbol include/*.h declares, and
+its host-side geometry producers match the oracle (no GPU needed)."""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from peeeeeeeeeeeeeeeeeeec_amd import ctypes_api as api
+from peeeeeeeeeeeeeeeeeeec_amd import lib as plib
+from peeeeeeeeeeeeeeeeeeec_amd import meshgen
+from tests import checker as ck
+from tests.conftest import gpu_available
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    """Function names of every prototype (a line starting with a return type)."""
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^(?!static|typedef|#)[A-Za-z_][A-Za-z0-9_ \t\*]*?\b([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = plib.load()
+    declared = declared_functions()
+    assert len(declared) > 60
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(plib.EXPORTED) == declared
+
+
+def test_struct_sizes_match_reference_layout():
+    """sizeof() of the reference's structs as gcc lays them out on x86-64
+    from the reference headers (Mesh, QuadPatch, DualMesh, InductanceOptions,
+    PotentialOptions, ComplexLUFactorization, HarmonicSystem,
+    LUFactorization); the ctypes mirrors and include/*.h must agree."""
+    expected = {"Mesh": 120, "QuadPatch": 104, "DualMesh": 64, "InductanceOptions": 40,
+                "PotentialOptions": 32, "ComplexLUFactorization": 64, "HarmonicSystem": 136,
+                "LUFactorization": 48}
+    for name, size in expected.items():
+        assert C.sizeof(getattr(api, name)) == size, name
+
+
+def test_headers_compile_to_reference_layout(tmp_path):
+    src = tmp_path / "sz.c"
+    src.write_text(
+        '#include <stdio.h>\n#include "harmonic.h"\n#include "lu.h"\n#include "dual_mesh.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu", sizeof(Mesh), sizeof(QuadPatch),'
+        'sizeof(DualMesh), sizeof(InductanceOptions), sizeof(PotentialOptions),'
+        'sizeof(ComplexLUFactorization), sizeof(HarmonicSystem), sizeof(LUFactorization));return 0;}')
+    exe = tmp_path / "sz"
+    import subprocess
+    subprocess.run(["gcc", "-std=c17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["120", "104", "64", "40", "32", "64", "136", "48"]
+
+
+@pytest.mark.parametrize("which", ["plate_msh", "box"])
+def test_host_geometry_matches_oracle(which):
+    if which == "plate_msh":
+        m = plib.Model(msh=ck.PLATE_MSH)
+        o = ck.OracleModel(msh=ck.PLATE_MSH)
+    else:
+        xyz, quads = meshgen.box(1.0, 0.6, 0.4, 5, 3, 2)
+        m = plib.Model(xyz, quads)
+        o = ck.OracleModel(xyz, quads)
+    try:
+        a, b = api.mesh_arrays(m.mesh), api.mesh_arrays(o.mesh)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+        da, db = api.dual_arrays(m.dual), api.dual_arrays(o.dual)
+        assert np.array_equal(da["node_region_areas"], db["node_region_areas"])
+        assert np.array_equal(da["edge_region_areas"], db["edge_region_areas"])
+        for x, y in zip(da["node_patches"] + da["edge_patches"], db["node_patches"] + db["edge_patches"]):
+            assert np.array_equal(x, y)
+    finally:
+        m.close()
+        o.close()
+
+
+def test_host_gauss_legendre_matches_oracle():
+    lib = plib.load()
+    for order in (2, 3, 4, 5, 6, 8, 10, 12, 16):
+        x1, w1, x2, w2 = ((C.c_double * 16)() for _ in range(4))
+        assert lib.gauss_legendre_rule(order, x1, w1) == 0
+        ck.oracle().oracle_gauss_legendre(order, x2, w2)
+        assert list(x1[:order]) == list(x2[:order]) and list(w1[:order]) == list(w2[:order])
+    assert lib.gauss_legendre_rule(7, (C.c_double * 16)(), (C.c_double * 16)()) == -1
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU behaviour")
+def test_gpu_entry_points_fail_loudly_without_gpu():
+    m = plib.Model(*meshgen.plate(2, 2))
+    try:
+        with pytest.raises(RuntimeError):
+            m.potential()
+        st, _, _ = plib.complex_lu_factor_solve(np.eye(3, dtype=complex), np.ones(3))
+        assert st != 0
+    finally:
+        m.close()

@@ -1,0 +1,119 @@
+"""GPU assembly parity (FP64) against the golden vectors of the reference
+and the oracle: static Lp/P to 1e-12 relative per entry, retarded matrices
+to 1e-11 of the largest entry."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from peeeeeeeeeeeeeeeeeeec_amd import ctypes_api as api
+from peeeeeeeeeeeeeeeeeeec_amd import lib as plib
+from peeeeeeeeeeeeeeeeeeec_amd import meshgen
+from tests import checker as ck
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+ENTRY_RTOL = 1e-12
+
+
+def rel_err(a, b):
+    """Entry-wise relative error, zeros compared absolutely to the scale."""
+    scale = np.abs(b).max()
+    denom = np.maximum(np.abs(b), 1e-300)
+    err = np.where(b != 0, np.abs(a - b) / denom, np.abs(a - b) / scale)
+    return err
+
+
+def report(name, a, b):
+    e = rel_err(a, b)
+    print(f"{name}: max entry rel err {e.max():.3e}, entries > 1e-12: {(e > ENTRY_RTOL).sum()} of {e.size}",
+          flush=True)
+    return e
+
+
+def test_plate_msh_static_matches_golden(require_gpu):
+    g = np.load(os.path.join(GOLD, "plate_msh.npz"))
+    with plib.Model(msh=ck.PLATE_MSH) as m:
+        L = m.inductance()
+        P = m.potential()
+    assert report("L plate.msh", L, g["L"]).max() <= ENTRY_RTOL
+    assert report("P plate.msh", P, g["P"]).max() <= ENTRY_RTOL
+    assert np.array_equal(L[g["L"] == 0], g["L"][g["L"] == 0])
+
+
+def test_plate16_static_matches_golden(require_gpu):
+    g = np.load(os.path.join(GOLD, "plate16.npz"))
+    with plib.Model(*meshgen.plate(16, 16)) as m:
+        L = m.inductance()
+        P = m.potential()
+    iu, ju = np.triu_indices(m.ne), np.triu_indices(m.nv)
+    assert report("L plate16", L[iu], g["L_upper"]).max() <= ENTRY_RTOL
+    assert report("P plate16", P[ju], g["P_upper"]).max() <= ENTRY_RTOL
+    assert np.array_equal(L, L.T) and np.array_equal(P, P.T)
+
+
+def test_box_static_matches_oracle(require_gpu):
+    xyz, quads = meshgen.box(1.0, 0.6, 0.4, 5, 3, 2)
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Lo, Po = o.inductance(), o.potential()
+    finally:
+        o.close()
+    with plib.Model(xyz, quads) as m:
+        L, P = m.inductance(), m.potential()
+    assert report("L box", L, Lo).max() <= ENTRY_RTOL
+    assert report("P box", P, Po).max() <= ENTRY_RTOL
+
+
+def test_piece_pairs_match_golden(require_gpu):
+    import ctypes as C
+    g = np.load(os.path.join(GOLD, "pairs.npz"))
+    lib = plib.load()
+    for a, b, val in zip(g["a"], g["b"], g["value"]):
+        pa, pb = api.QuadPatch(), api.QuadPatch()
+        for i in range(12):
+            pa.points[i] = a[i]
+            pb.points[i] = b[i]
+        same = np.array_equal(a, b)
+        v = lib.quad_patch_kernel_1_over_r(C.byref(pa), C.byref(pa) if same else C.byref(pb), 4, 10, 12, 8, 0.5)
+        assert abs(v - val) <= ENTRY_RTOL * abs(val), (v, val)
+    bad = lib.quad_patch_kernel_1_over_r(C.byref(pa), C.byref(pb), 7, 10, 12, 8, 0.5)
+    assert np.isnan(bad)
+
+
+def test_compute_pair_matches_matrix(require_gpu):
+    import ctypes as C
+    g = np.load(os.path.join(GOLD, "plate_msh.npz"))
+    lib = plib.load()
+    with plib.Model(msh=ck.PLATE_MSH) as m:
+        lo = api.InductanceOptions()
+        lib.inductance_options_default(C.byref(lo), 1)
+        po = api.PotentialOptions()
+        lib.potential_options_default(C.byref(po), 1)
+        for a, b in ((0, 0), (0, 1), (3, 40), (17, 17), (5, 179)):
+            v = C.c_double()
+            assert lib.inductance_compute_pair(C.byref(m.mesh), C.byref(m.dual), a, b, C.byref(lo), C.byref(v)) == 0
+            assert abs(v.value - g["L"][a, b]) <= ENTRY_RTOL * abs(g["L"][a, b]) + 0.0
+        for i, j in ((0, 0), (0, 1), (10, 55), (99, 99)):
+            v = C.c_double()
+            assert lib.potential_compute_pair(C.byref(m.mesh), C.byref(m.dual), i, j, C.byref(po), C.byref(v)) == 0
+            assert abs(v.value - g["P"][i, j]) <= ENTRY_RTOL * abs(g["P"][i, j])
+
+
+@pytest.mark.parametrize("k", [0.0, 0.7, 4.19])
+def test_retarded_matches_oracle(require_gpu, k):
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 3, 2, 2)
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Ls, Ps = o.inductance(), o.potential()
+        Lo, Po = o.retarded(k, 2, Ls, Ps, 4)
+    finally:
+        o.close()
+    with plib.Model(xyz, quads) as m:
+        L, P = m.retarded(k, 2, 4)
+    el = np.abs(L - Lo).max() / np.abs(Lo).max()
+    ep = np.abs(P - Po).max() / np.abs(Po).max()
+    print(f"retarded k={k}: L {el:.3e}, P {ep:.3e}", flush=True)
+    assert el <= 1e-11 and ep <= 1e-11

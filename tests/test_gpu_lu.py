@@ -1,0 +1,182 @@
+"""GPU LU, drop-in harmonic path and the frequency-point engine against the
+reference's known answers, golden vectors and the oracle."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from peeeeeeeeeeeeeeeeeeec_amd import ctypes_api as api
+from peeeeeeeeeeeeeeeeeeec_amd import lib as plib
+from peeeeeeeeeeeeeeeeeeec_amd import meshgen
+from tests import checker as ck
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_complex_lu_known_answer(require_gpu):
+    """test_core.c:41-66 (needs a row swap), error < 1e-11."""
+    A = np.array([[0, 2 + 1j], [1 - 1j, 3 + 0.5j]])
+    x = np.array([1 + 2j, -0.5 + 0.75j])
+    st, X, info = plib.complex_lu_factor_solve(A, A @ x)
+    assert st == 0 and np.abs(X[0] - x).max() < 1e-11
+    assert info["pivots"][0] == 1 and info["swap_count"] == 1
+
+
+def test_complex_lu_singular(require_gpu):
+    """test_core.c:328-337."""
+    A = np.array([[1 + 1j, 2 + 2j], [2 + 2j, 4 + 4j]])
+    st, _, _ = plib.complex_lu_factor_solve(A, np.ones(2))
+    assert st == -2
+
+
+def test_real_lu_known_answer(require_gpu):
+    """test_core.c:17-39 and the determinant (lu.c:732-750)."""
+    A = np.array([[0, 2, 1], [1, -2, -3], [2, 3, 1]], float)
+    x = np.array([1, -2, 0.5])
+    st, X, det = plib.real_lu_factor_solve(A, A @ x)
+    assert st == 0 and np.abs(X[0] - x).max() < 1e-11
+    assert abs(det - np.linalg.det(A)) < 1e-12 * abs(np.linalg.det(A))
+
+
+@pytest.mark.parametrize("n", [7, 130, 515, 1100])
+def test_complex_lu_random_against_oracle(require_gpu, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A *= rng.uniform(1e-3, 1e3, size=(n, 1))
+    B = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    st, X, info = plib.complex_lu_factor_solve(A, B)
+    assert st == 0
+    st_o, fac = ck.oracle_complex_lu(A)
+    assert st_o == 0
+    # equilibration is element-wise: identical scales
+    assert np.array_equal(info["row_scale"], fac[2]) and np.array_equal(info["column_scale"], fac[3])
+    for k in range(3):
+        xo = ck.oracle_complex_solve(fac, B[k])
+        assert np.abs(X[k] - xo).max() <= 1e-10 * np.abs(xo).max()
+        eta = ck.oracle_backward_error(A, X[k], B[k], n // 2)[2]
+        assert eta < 1e-13
+
+
+def test_real_lu_random(require_gpu):
+    rng = np.random.default_rng(5)
+    n = 300
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    st, X, _ = plib.real_lu_factor_solve(A, b)
+    assert st == 0
+    ref = np.linalg.solve(A, b)
+    assert np.abs(X[0] - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def _dropin_harmonic(model, frequency, a, b):
+    """The reference driver's call sequence (main.c:1349-1465) on the GPU library."""
+    lib = plib.load()
+    lo = api.InductanceOptions(); lib.inductance_options_default(C.byref(lo), 1)
+    po = api.PotentialOptions(); lib.potential_options_default(C.byref(po), 1)
+    L, P = api.InductanceMatrix(), api.PotentialMatrix()
+    assert lib.inductance_compute_matrix(C.byref(model.mesh), C.byref(model.dual), C.byref(lo), C.byref(L)) == 0
+    assert lib.potential_compute_matrix(C.byref(model.mesh), C.byref(model.dual), C.byref(po), C.byref(P)) == 0
+    A = api.IncidenceMatrix()
+    assert lib.incidence_build(C.byref(model.mesh), C.byref(A)) == 0
+    PAT = api.Matrix()
+    assert lib.incidence_build_PAT(C.byref(model.mesh), C.byref(P), C.byref(PAT), 1) == 0
+    ro = api.ResistanceOptions(); lib.resistance_options_default(C.byref(ro), 1)
+    R = api.ResistanceMatrix()
+    assert lib.resistance_compute_matrix(C.byref(model.mesh), C.byref(model.dual), C.byref(ro), C.byref(R)) == 0
+    system = api.HarmonicSystem()
+    lib.harmonic_system_init(C.byref(system))
+    assert lib.harmonic_build_system(C.byref(L), C.byref(R), C.byref(A), C.byref(PAT), frequency, 1,
+                                     C.byref(system)) == 0
+    assert lib.harmonic_factorize(C.byref(system), 1) == 0
+    Is = ck.port_current(model.nv, a, b)
+    buf = api.complex_buffer(Is)
+    ex = api.HarmonicExcitation()
+    ex.n_edges, ex.n_nodes = model.ne, model.nv
+    ex.node_current = C.cast(buf, C.POINTER(api.Complex))
+    sol = api.HarmonicSolution()
+    lib.harmonic_solution_init(C.byref(sol))
+    assert lib.harmonic_solve(C.byref(system), C.byref(P), C.byref(ex), C.byref(sol), 1) == 0
+    res = api.HarmonicResidual()
+    assert lib.harmonic_compute_residual(C.byref(system), C.byref(P), C.byref(ex), C.byref(sol), 1,
+                                         C.byref(res)) == 0
+    I = api.complex_array(sol.edge_current, model.ne)
+    V = api.complex_array(sol.node_voltage, model.nv)
+    lib.harmonic_solution_free(C.byref(sol))
+    lib.harmonic_system_free(C.byref(system))
+    for obj, fn in ((L, "inductance_free"), (P, "potential_free"), (A, "incidence_free"), (PAT, "matrix_free"),
+                    (R, "resistance_free")):
+        getattr(lib, fn)(C.byref(obj))
+    return V[a] - V[b], I, V, res.backward_error
+
+
+@pytest.mark.parametrize("frequency", [1.0, 1e3, 1e6])
+def test_dropin_harmonic_plate_msh(require_gpu, frequency):
+    """run_tests.py:95-106: backward error <= 1e-9 at 1 Hz / 1 kHz / 1 MHz."""
+    with plib.Model(msh=ck.PLATE_MSH) as m:
+        zin, I, V, eta = _dropin_harmonic(m, frequency, 0, m.nv - 1)
+    print(f"plate.msh f={frequency:g}: Z_in={zin:.9e} eta={eta:.3e}", flush=True)
+    assert eta <= 1e-9
+
+
+def test_dropin_harmonic_plate16_matches_golden(require_gpu):
+    g = np.load(os.path.join(GOLD, "plate16.npz"))
+    with plib.Model(*meshgen.plate(16, 16)) as m:
+        zin, I, V, eta = _dropin_harmonic(m, float(g["frequency"]), 0, 288)
+    z_ref = g["z_in"][0]
+    print(f"plate16 drop-in: Z_in={zin} ref={z_ref} rel={abs(zin - z_ref) / abs(z_ref):.3e} eta={eta:.3e}")
+    assert abs(zin - z_ref) <= 1e-10 * abs(z_ref)
+    assert np.abs(I - g["I"]).max() <= 1e-10 * np.abs(g["I"]).max()
+    assert np.abs(V - g["V"]).max() <= 1e-10 * np.abs(g["V"]).max()
+    assert eta <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [plib.Engine.FP64, plib.Engine.MIXED])
+def test_engine_plate16_matches_golden(require_gpu, mode):
+    g = np.load(os.path.join(GOLD, "plate16.npz"))
+    with plib.Model(*meshgen.plate(16, 16)) as m:
+        eng = plib.Engine(m)
+        st, z, I, V, rep = eng.solve(float(g["frequency"]), [(0, 288)], mode)
+        eng.close()
+    z_ref = g["z_in"][0]
+    print(f"engine mode={mode}: Z_in={z[0]} rel={abs(z[0] - z_ref) / abs(z_ref):.3e} eta={rep.backward_error:.3e} "
+          f"ir={rep.refinement_steps}", flush=True)
+    assert st == 0
+    assert abs(z[0] - z_ref) <= 1e-10 * abs(z_ref)
+    assert np.abs(I[0] - g["I"]).max() <= 1e-10 * np.abs(g["I"]).max()
+    assert np.abs(V[0] - g["V"]).max() <= 1e-10 * np.abs(g["V"]).max()
+    assert rep.backward_error <= 1e-10
+
+
+@pytest.mark.parametrize("frequency", [1e4, 3e6, 2e8])
+def test_engine_retarded_box_mixed_vs_fp64_vs_oracle(require_gpu, frequency):
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 6, 4, 4)
+    k = 2 * math.pi * frequency / 299792458.0
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Ls, Ps = o.inductance(), o.potential()
+        Lc, Pc = o.retarded(k, 2, Ls, Ps, 4)
+        Z = o.mna(None, None, frequency, Lc=Lc, Pc=Pc)
+        port = (0, o.nv - 1)
+        rhs = ck.oracle_rhs(o.ne, o.nv, Pc=Pc, Is=ck.port_current(o.nv, *port))
+        st, fac = ck.oracle_complex_lu(Z)
+        x = ck.oracle_complex_solve(fac, rhs)
+        z_o = x[o.ne + port[0]] - x[o.ne + port[1]]
+    finally:
+        o.close()
+    with plib.Model(xyz, quads) as m:
+        eng = plib.Engine(m, retarded=True)
+        st64, z64, I64, _, r64 = eng.solve(frequency, [port], plib.Engine.FP64)
+        stm, zm, Im, _, rm = eng.solve(frequency, [port], plib.Engine.MIXED)
+        eng.close()
+    print(f"f={frequency:g}: oracle {z_o:.10e} fp64 {z64[0]:.10e} mixed {zm[0]:.10e} "
+          f"eta64={r64.backward_error:.2e} etam={rm.backward_error:.2e} ir={rm.refinement_steps} "
+          f"fallback={rm.used_fp64_fallback}", flush=True)
+    assert st64 == 0 and stm == 0
+    assert abs(z64[0] - z_o) <= 1e-9 * abs(z_o)
+    assert abs(zm[0] - z64[0]) <= 1e-10 * abs(z64[0])
+    assert r64.backward_error <= 1e-10 and rm.backward_error <= 1e-10
