@@ -94,7 +94,35 @@ size_t peec_engine_nodes(const PeecEngine *engine);
 long long peec_engine_far_evaluations(const PeecEngine *engine);
 long long peec_engine_smooth_evaluations(const PeecEngine *engine);
 
+/* Per-kernel timing: HIP events recorded on the engine's stream around
+ * every launch of the kernel family, summed over launches. flops/bytes are
+ * the algorithmic counts of those launches. */
+typedef enum
+{
+    PEEC_KERNEL_LU_UPDATE = 0,  /* trailing-update GEMM on MFMA: 8mnk (complex) */
+    PEEC_KERNEL_PANEL = 1,      /* cooperative panel factorization */
+    PEEC_KERNEL_SMOOTH_TILE = 2,/* retarded remainder tile kernel: 42 flops/eval */
+    PEEC_KERNEL_FAR_TILE = 3    /* static 1/R tile kernel: 12 flops/eval */
+} PeecKernel;
+
+typedef struct
+{
+    long long launches;
+    double total_ms;
+    double flops;
+    double bytes;
+} PeecKernelStats;
+
+/* Enables (1) or disables (0) the timing and clears the totals. */
+int peec_engine_set_profiling(PeecEngine *engine, int enable);
+int peec_engine_kernel_stats(const PeecEngine *engine, int kernel, PeecKernelStats *stats);
+
 void peec_engine_destroy(PeecEngine *engine);
+
+/* Library self-test: runs the 16x16x4 FP64 MFMA on known operands and
+ * returns every lane's accumulators (FP64 then FP32, 1024 doubles) so the operand layout the
+ * GEMM kernels assume can be checked. */
+int peec_selftest_mfma_layout(double *out1024);
 
 #ifdef __cplusplus
 }

@@ -39,10 +39,11 @@ class RegionSet {
     RegionKind kind() const { return kind_; }
 
     // static matrix (row-major R x R, original numbering) into d_out
-    void assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work);
+    void assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work,
+                         KernelTimer *timer = nullptr);
     // complex retarded matrix: d_out = d_static + scale * smooth remainder
     void assemble_retarded(const double *d_static, double wave_number, double2 *d_out,
-                           hipStream_t s, AssemblyWork *work);
+                           hipStream_t s, AssemblyWork *work, KernelTimer *timer = nullptr);
     // single entries through the near-field kernel (drop-in *_compute_pair)
     void assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,
                         int *d_error);

@@ -480,10 +480,16 @@ __global__ __launch_bounds__(256) void far_tile_kernel(
     }
 }
 
-// sin and cos of a moderate argument (|x| < 2^20): Cody-Waite reduction by
+// sin and cos of a moderate argument (|x| < 2^20, larger ones go to the
+// library sin/cos): Cody-Waite reduction by
 // pi/2 split in three parts (33+33+53 bits), then the Taylor series of sin and
 // cos on [-pi/4, pi/4] to degree 19/20 (truncation < 1e-19).
 __device__ inline void d_sincos_moderate(double x, double &s, double &c) {
+    if (!(fabs(x) < 0x1p20)) {  // outside the reduction's exact range
+        s = sin(x);
+        c = cos(x);
+        return;
+    }
     const double n = rint(x * 0x1.45f306dc9c883p-1);
     double r = __builtin_fma(-n, 0x1.921fb54400000p+0, x);
     r = __builtin_fma(-n, 0x1.0b4611a600000p-34, r);
@@ -825,7 +831,9 @@ void RegionSet::build_points(const std::vector<double> &pv, const std::vector<in
         }
         npts[kr] = n;
         for (int m = n; m < slots; ++m) {
-            const double far[4] = {1.0e30, 1.0e30, 1.0e30, 0.0};
+            // zero weight at a remote but moderate point: 1/R and exp(-jkR)
+            // stay finite, so the slot contributes exactly 0
+            const double far[4] = {1.0e4, 1.0e4, 1.0e4, 0.0};
             std::memcpy(&pi[4 * ((size_t)m * R_ + kr)], far, sizeof far);
             std::memcpy(&pj[4 * ((size_t)kr * slots + m)], far, sizeof far);
         }
@@ -908,10 +916,12 @@ void RegionSet::build_near_list(const Mesh *mesh) {
     }
 }
 
-void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work) {
+void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work,
+                                KernelTimer *timer) {
     const int ntiles = (int)h_tiles_.size() / 2;
     const size_t lds = 4 * (size_t)far_slots_ * sizeof(double4);
     if (ntiles) {
+        const int tk = timer ? timer->start(s) : -1;
         if (kind_ == RegionKind::Node)
             hipLaunchKernelGGL(far_tile_kernel<0>, dim3(ntiles), dim3(256), lds, s, d_far_i_.ptr,
                                d_far_j_.ptr, d_far_n_.ptr, R_, far_slots_, d_aux_k_.ptr,
@@ -921,6 +931,8 @@ void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, Asse
                                d_far_j_.ptr, d_far_n_.ptr, R_, far_slots_, d_aux_k_.ptr,
                                d_dir_k_.ptr, dir_tol_, d_perm_.ptr, d_tiles_.ptr, d_out, (size_t)R_);
         HIP_CHECK(hipGetLastError());
+        // 12 flops per 1/R evaluation; bytes: the matrix written once
+        if (timer) timer->stop(tk, s, kKernelFarTile, 12.0 * far_evals_, 8.0 * R_ * R_);
     }
     const int nn = (int)h_near_.size() / 2;
     if (nn) {
@@ -936,11 +948,12 @@ void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, Asse
 }
 
 void RegionSet::assemble_retarded(const double *d_static, double k, double2 *d_out, hipStream_t s,
-                                  AssemblyWork *work) {
+                                  AssemblyWork *work, KernelTimer *timer) {
     const int ntiles = (int)h_tiles_.size() / 2;
     const size_t lds = 4 * (size_t)smooth_slots_ * sizeof(double4);
     if (!ntiles) return;
     if (smooth_order_ <= 0) throw HipError{"region set built without smooth-rule points"};
+    const int tk = timer ? timer->start(s) : -1;
     if (kind_ == RegionKind::Node)
         hipLaunchKernelGGL(smooth_tile_kernel<0>, dim3(ntiles), dim3(256), lds, s, d_smooth_i_.ptr,
                            d_smooth_j_.ptr, d_smooth_n_.ptr, R_, smooth_slots_, d_aux_k_.ptr,
@@ -952,6 +965,10 @@ void RegionSet::assemble_retarded(const double *d_static, double k, double2 *d_o
                            d_dir_k_.ptr, dir_tol_, d_perm_.ptr, d_tiles_.ptr, k, d_static, d_out,
                            (size_t)R_);
     HIP_CHECK(hipGetLastError());
+    // per exp(-jkR) evaluation: 1/R (12) + half-angle reduction and two
+    // degree-9/10 polynomials (~26) + complex accumulate (4) = 42 flops;
+    // bytes: static matrix read + complex matrix written
+    if (timer) timer->stop(tk, s, kKernelSmoothTile, 42.0 * smooth_evals_, 8.0 * R_ * R_ + 16.0 * R_ * R_);
     const int nn = (int)h_near_.size() / 2;
     if (nn && near_smooth_order_ > 0) {
         if (!d_smooth_err_.ptr) d_smooth_err_.resize(1);

@@ -108,6 +108,71 @@ struct Stream {
     void sync() const { HIP_CHECK(hipStreamSynchronize(s)); }
 };
 
+// Per-kernel timing with HIP events recorded on the launching stream; the
+// elapsed times are resolved after the caller's next synchronisation.
+enum KernelKind { kKernelLuUpdate = 0, kKernelPanel = 1, kKernelSmoothTile = 2, kKernelFarTile = 3, kKernelKinds = 4 };
+
+struct KernelTotals {
+    long long launches = 0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
+};
+
+class KernelTimer {
+  public:
+    bool enabled = false;
+    KernelTotals totals[kKernelKinds];
+    ~KernelTimer() {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    }
+    // returns a ticket, or -1 when disabled
+    int start(hipStream_t s) {
+        if (!enabled) return -1;
+        const size_t need = used_ + 2;
+        while (pool_.size() < need) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            pool_.push_back(e);
+        }
+        HIP_CHECK(hipEventRecord(pool_[used_], s));
+        const int t = (int)used_;
+        used_ += 2;
+        return t;
+    }
+    void stop(int ticket, hipStream_t s, int kind, double flops, double bytes) {
+        if (ticket < 0) return;
+        HIP_CHECK(hipEventRecord(pool_[ticket + 1], s));
+        pending_.push_back({ticket, kind, flops, bytes});
+    }
+    // call after the stream has been synchronised
+    void resolve() {
+        for (const Rec &r : pending_) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, pool_[r.ticket], pool_[r.ticket + 1]));
+            KernelTotals &t = totals[r.kind];
+            t.launches += 1;
+            t.ms += ms;
+            t.flops += r.flops;
+            t.bytes += r.bytes;
+        }
+        pending_.clear();
+        used_ = 0;
+    }
+    void reset() {
+        pending_.clear();
+        used_ = 0;
+        for (KernelTotals &t : totals) t = KernelTotals{};
+    }
+
+  private:
+    struct Rec {
+        int ticket, kind;
+        double flops, bytes;
+    };
+    std::vector<hipEvent_t> pool_;
+    size_t used_ = 0;
+    std::vector<Rec> pending_;
+};
+
 // Fails loudly when no GPU is visible: there is no CPU fallback.
 bool require_device();
 

@@ -834,6 +834,20 @@ void complex_lu_print_info(const ComplexLUFactorization *lu) {
 }
 
 // ---------------------------------------------------------------------------
+// library self-test (no reference counterpart): MFMA operand layout probe
+
+int peec_selftest_mfma_layout(double *out1024) {
+    if (!out1024 || !require_device()) return -1;
+    try {
+        mfma_layout_probe(out1024);
+        return 0;
+    } catch (const HipError &e) {
+        PEEC_ERR("%s", e.what.c_str());
+        return -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // harmonic.h
 
 void harmonic_system_init(HarmonicSystem *s) {

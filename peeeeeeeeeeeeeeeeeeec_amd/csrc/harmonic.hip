@@ -199,6 +199,7 @@ struct PeecEngine {
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
+    KernelTimer timer;
     ~PeecEngine() {
         for (hipEvent_t &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -244,6 +245,7 @@ int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineO
         e->nodes = std::make_unique<RegionSet>(mesh, dual, RegionKind::Node, qp, o->smooth_order, 0.0, s, true,
                                                o->near_smooth_order);
         e->lu = std::make_unique<LUContext>(s);
+        e->lu->timer = &e->timer;
         std::vector<int> edges(mesh->edges, mesh->edges + 2 * e->ne);
         e->d_edges.upload(edges, s);
         e->d_L.resize((size_t)e->ne * e->ne);
@@ -276,6 +278,23 @@ int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineO
 
 void peec_engine_destroy(PeecEngine *e) { delete e; }
 
+int peec_engine_set_profiling(PeecEngine *e, int enable) {
+    if (!e) return -1;
+    e->timer.reset();
+    e->timer.enabled = enable != 0;
+    return 0;
+}
+
+int peec_engine_kernel_stats(const PeecEngine *e, int kernel, PeecKernelStats *out) {
+    if (!e || !out || kernel < 0 || kernel >= kKernelKinds) return -1;
+    const KernelTotals &t = e->timer.totals[kernel];
+    out->launches = t.launches;
+    out->total_ms = t.ms;
+    out->flops = t.flops;
+    out->bytes = t.bytes;
+    return 0;
+}
+
 size_t peec_engine_edges(const PeecEngine *e) { return e ? (size_t)e->ne : 0; }
 size_t peec_engine_nodes(const PeecEngine *e) { return e ? (size_t)e->nv : 0; }
 long long peec_engine_far_evaluations(const PeecEngine *e) { return e ? e->far_evals : 0; }
@@ -298,8 +317,8 @@ static void assemble_retarded(PeecEngine *e, double k) {
     e->d_Lw.resize((size_t)e->ne * e->ne);
     e->d_Pw.resize((size_t)e->nv * e->nv);
     AssemblyWork w;
-    e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w);
-    e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w);
+    e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w, &e->timer);
+    e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w, &e->timer);
 }
 
 int peec_engine_retarded_matrices(PeecEngine *e, double k, Complex *L, Complex *P) {
@@ -428,6 +447,7 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             if (edge_current) std::memcpy(edge_current + (size_t)k * ne, xk, sizeof(Complex) * ne);
             if (node_voltage) std::memcpy(node_voltage + (size_t)k * nv, xk + ne, sizeof(Complex) * nv);
         }
+        e->timer.resolve();
         float ms[5];
         for (int i = 0; i < 5; ++i) HIP_CHECK(hipEventElapsedTime(&ms[i], e->ev[i], e->ev[i + 1]));
         report.ms_assembly = ms[0];

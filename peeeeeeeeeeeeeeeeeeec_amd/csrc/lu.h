@@ -24,6 +24,7 @@ struct LUContext {
     DeviceBuffer<double> cand;        // per-workgroup pivot candidates
     int cus = 0;
     hipStream_t stream = nullptr;
+    KernelTimer *timer = nullptr;      // optional per-kernel timing
     explicit LUContext(hipStream_t s);
 };
 
@@ -55,5 +56,8 @@ void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s
 //   out[i] = (|r_i|, sum_j |A_ij||x_j| + |b_i|, |b_i|) with r = A x - b.
 void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, const double2 *db,
                       double2 *dr, double *dterms, hipStream_t s);
+
+// In-bounds self-test of the MFMA lane layout (1024 doubles out).
+void mfma_layout_probe(double *host_out);
 
 }  // namespace peec

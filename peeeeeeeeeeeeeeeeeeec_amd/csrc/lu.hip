@@ -375,15 +375,17 @@ __global__ __launch_bounds__(256) void trsm_kernel(S *A, size_t lda, int n, int 
 
 // ---------------------------------------------------------------------------
 // trailing update on MFMA. Block tile 64x64, 4 waves of 32x32, each wave a
-// 2x2 grid of 16x16 MFMA blocks. Operand layout of the 16x16x4 MFMAs:
-// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[4(l/16)+v][l%16] in
-// element v of lane l.
+// 2x2 grid of 16x16 MFMA blocks. Operand layout of the 16x16x4 MFMAs (checked
+// on gfx950 by mfma_layout_probe_kernel): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k; accumulator element v of lane l holds row 4v + l/16 for the
+// FP64 instruction and row 4(l/16) + v for the FP32 one, column l % 16.
 
 template <class S> struct Mfma;
 template <> struct Mfma<double2> {
     using acc = v4d;
     using op = double;
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lk, int v) { return 4 * v + lk; }
     __device__ static void load(const double2 *p, op &re, op &im) {
         const double2 v = *p;
         re = v.x;
@@ -394,6 +396,7 @@ template <> struct Mfma<float2> {
     using acc = v4f;
     using op = float;
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lk, int v) { return 4 * lk + v; }
     __device__ static void load(const float2 *p, op &re, op &im) {
         const float2 v = *p;
         re = v.x;
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(256) void gemm_complex_kernel(S *A, size_t lda, int
             const int col = wc + ni * 16 + li;
             if (col >= ncols) continue;
             for (int v = 0; v < 4; ++v) {
-                const int row = wr + mi * 16 + 4 * lk + v;
+                const int row = wr + mi * 16 + M::row(lk, v);
                 if (row >= m) continue;
                 S *c = A + (size_t)(row0 + row) * lda + col0 + col;
                 S cv = *c;
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(256) void gemm_real_kernel(double *A, size_t lda, i
             const int col = wc + ni * 16 + li;
             if (col >= ncols) continue;
             for (int v = 0; v < 4; ++v) {
-                const int row = wr + mi * 16 + 4 * lk + v;
+                const int row = wr + mi * 16 + 4 * v + lk;  // FP64 accumulator layout
                 if (row < m) A[(size_t)(row0 + row) * lda + col0 + col] -= c[mi][ni][v];
             }
         }
@@ -498,6 +501,29 @@ void launch_gemm(S *A, size_t lda, int row0, int col0, int m, int ncols, int k0,
         hipLaunchKernelGGL(gemm_complex_kernel<S>, grid, dim3(256), 0, s, A, lda, row0, col0, m, ncols, k0, kw, status);
     else
         hipLaunchKernelGGL(gemm_real_kernel, grid, dim3(256), 0, s, A, lda, row0, col0, m, ncols, k0, kw, status);
+}
+
+// Self-test of the 16x16x4 MFMA operand layouts used by the GEMM kernels:
+// one wavefront, A[i][k] = i + 16k + 1 in lane i + 16k, B[k][j] = (j == k)
+// and B2[k][j] = (k == 0) in lane j + 16k; D = A padded and D2[i][j] =
+// A[i][0]. Writes every lane's 4 accumulators for both products, first for
+// the FP64 instruction, then for the FP32 one (4 x 256 doubles).
+__global__ void mfma_layout_probe_kernel(double *out) {
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lk = lane >> 4;
+    const double a = (double)(li + 16 * lk + 1);
+    const double b = (li == lk) ? 1.0 : 0.0;
+    const double b2 = (lk == 0) ? 1.0 : 0.0;
+    v4d d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, v4d{0, 0, 0, 0}, 0, 0, 0);
+    v4d d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b2, v4d{0, 0, 0, 0}, 0, 0, 0);
+    v4f f = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, v4f{0, 0, 0, 0}, 0, 0, 0);
+    v4f f2 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b2, v4f{0, 0, 0, 0}, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) {
+        out[lane * 4 + v] = d[v];
+        out[256 + lane * 4 + v] = d2[v];
+        out[512 + lane * 4 + v] = f[v];
+        out[768 + lane * 4 + v] = f2[v];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -538,8 +564,10 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         double lim = pivot_limit;
         void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &ctx.cand.ptr, &d_ipiv, &ctx.barrier.ptr,
                         &base, &lim, &ctx.status.ptr};
+        const int tp = ctx.timer ? ctx.timer->start(s) : -1;
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(panel_kernel<S>), dim3(G), dim3(256),
                                              args, (unsigned)shm, s));
+        if (ctx.timer) ctx.timer->stop(tp, s, kKernelPanel, 0.0, 2.0 * m * w * sizeof(S));
         bar_base += (unsigned)G * w;
         if (n - w > 0)
             hipLaunchKernelGGL(laswp_kernel<S>, dim3((n - w + 255) / 256), dim3(256), 0, s, dA, lda, n, k0, w,
@@ -548,7 +576,13 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         if (rest > 0) {
             hipLaunchKernelGGL(trsm_kernel<S>, dim3((rest + 31) / 32), dim3(256), 0, s, dA, lda, n, k0, w,
                                ctx.status.ptr);
+            const int tg = ctx.timer ? ctx.timer->start(s) : -1;
             launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, s);
+            if (ctx.timer) {
+                const double mnk = (double)rest * rest * w;
+                ctx.timer->stop(tg, s, kKernelLuUpdate, (Elem<S>::cplx ? 8.0 : 2.0) * mnk,
+                                (2.0 * rest * rest + 2.0 * rest * w) * sizeof(S));
+            }
         }
         HIP_CHECK(hipGetLastError());
         k0 += w;
@@ -770,6 +804,13 @@ void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, c
     hipLaunchKernelGGL(residual_kernel, dim3(n), dim3(256), 0, s, dA, lda, n, dx, ax.ptr, db, dr, dterms);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mfma_layout_probe(double *host_out) {
+    DeviceBuffer<double> d(1024);
+    hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, 0, d.ptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(host_out, d.ptr, 1024 * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 }  // namespace peec

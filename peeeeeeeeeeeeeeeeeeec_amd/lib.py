@@ -38,6 +38,14 @@ class PeecPointReport(C.Structure):
                 ("ms_solve", C.c_double), ("ms_check", C.c_double)]
 
 
+class PeecKernelStats(C.Structure):
+    _fields_ = [("launches", C.c_longlong), ("total_ms", C.c_double), ("flops", C.c_double),
+                ("bytes", C.c_double)]
+
+
+KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3}
+
+
 class RetardationOptions(C.Structure):
     _fields_ = [("inductance", api.InductanceOptions), ("potential", api.PotentialOptions),
                 ("smooth_order", C.c_int), ("near_smooth_order", C.c_int)]
@@ -79,7 +87,8 @@ EXPORTED = [
     "peec_engine_options_default", "peec_engine_create", "peec_engine_solve",
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
-    "peec_engine_destroy",
+    "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats",
+    "peec_selftest_mfma_layout",
 ]
 
 
@@ -183,6 +192,9 @@ def load():
     _sig(lib, "peec_engine_far_evaluations", C.c_longlong, C.c_void_p)
     _sig(lib, "peec_engine_smooth_evaluations", C.c_longlong, C.c_void_p)
     _sig(lib, "peec_engine_destroy", None, C.c_void_p)
+    _sig(lib, "peec_engine_set_profiling", C.c_int, C.c_void_p, C.c_int)
+    _sig(lib, "peec_selftest_mfma_layout", C.c_int, _dp)
+    _sig(lib, "peec_engine_kernel_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(PeecKernelStats))
     _lib = lib
     return lib
 
@@ -383,6 +395,14 @@ class Engine:
             I = I.reshape(k, self.ne)
             V = V.reshape(k, self.nv)
         return st, z, I, V, rep
+
+    def set_profiling(self, on=True):
+        load().peec_engine_set_profiling(self.handle, int(on))
+
+    def kernel_stats(self, name):
+        st = PeecKernelStats()
+        load().peec_engine_kernel_stats(self.handle, KERNELS[name], C.byref(st))
+        return {"launches": st.launches, "ms": st.total_ms, "flops": st.flops, "bytes": st.bytes}
 
     @property
     def far_evaluations(self):

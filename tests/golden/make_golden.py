@@ -57,21 +57,21 @@ def ref_LP(m, d, threads=8):
     return L, P
 
 
-def ref_port_solve(m, d, L, P, frequency, a, b):
+def ref_port_solve(m, d, L, P, frequency, a, b, threads=8):
     ref = ck.ref()
     A = api.IncidenceMatrix()
     assert ref.incidence_build(C.byref(m), C.byref(A)) == 0
     PAT = api.Matrix()
-    assert ref.incidence_build_PAT(C.byref(m), C.byref(P), C.byref(PAT), 8) == 0
+    assert ref.incidence_build_PAT(C.byref(m), C.byref(P), C.byref(PAT), threads) == 0
     ro = api.ResistanceOptions()
-    ref.resistance_options_default(C.byref(ro), 8)
+    ref.resistance_options_default(C.byref(ro), threads)
     R = api.ResistanceMatrix()
     assert ref.resistance_compute_matrix(C.byref(m), C.byref(d), C.byref(ro), C.byref(R)) == 0
     sysm = api.HarmonicSystem()
     ref.harmonic_system_init(C.byref(sysm))
-    assert ref.harmonic_build_system(C.byref(L), C.byref(R), C.byref(A), C.byref(PAT), frequency, 8,
+    assert ref.harmonic_build_system(C.byref(L), C.byref(R), C.byref(A), C.byref(PAT), frequency, threads,
                                      C.byref(sysm)) == 0
-    assert ref.harmonic_factorize(C.byref(sysm), 8) == 0
+    assert ref.harmonic_factorize(C.byref(sysm), threads) == 0
     Is = ck.port_current(m.n_nodes, a, b)
     Ib = api.complex_buffer(Is)
     ex = api.HarmonicExcitation()
@@ -80,9 +80,9 @@ def ref_port_solve(m, d, L, P, frequency, a, b):
     ex.node_current = C.cast(Ib, C.POINTER(api.Complex))
     sol = api.HarmonicSolution()
     ref.harmonic_solution_init(C.byref(sol))
-    assert ref.harmonic_solve(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), 8) == 0
+    assert ref.harmonic_solve(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), threads) == 0
     res = api.HarmonicResidual()
-    assert ref.harmonic_compute_residual(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), 8,
+    assert ref.harmonic_compute_residual(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), threads,
                                          C.byref(res)) == 0
     I = api.complex_array(sol.edge_current, m.n_edges)
     V = api.complex_array(sol.node_voltage, m.n_nodes)
