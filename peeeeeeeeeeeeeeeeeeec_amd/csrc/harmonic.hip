@@ -193,7 +193,7 @@ struct PeecEngine {
     Stream stream;
     std::unique_ptr<RegionSet> nodes, edges;
     std::unique_ptr<LUContext> lu;
-    DeviceBuffer<int> d_edges, d_ports, d_ipiv, d_err;
+    DeviceBuffer<int> d_edges, d_ports, d_ipiv, d_perm, d_err;
     DeviceBuffer<double> d_L, d_P, d_row, d_col, d_terms;
     DeviceBuffer<double2> d_Lw, d_Pw, d_Z, d_Z0, d_B, d_X, d_R;
     DeviceBuffer<float2> d_Z32;
@@ -378,6 +378,8 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             else PEEC_ERR("LU failed (code %d).", st.code);
             return st.code == -2 ? -2 : -1;
         }
+        e->d_perm.resize(n);
+        permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
         e->d_ports.upload(ports, 2 * nports, s);
         e->d_B.resize((size_t)n * nr);
         e->d_X.resize((size_t)n * nr);
@@ -399,16 +401,16 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             return summary[2];
         };
         if (mode == PEEC_SOLVE_MIXED) {
-            lu_solve<float2, double2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr, nr,
-                                      *e->lu);
+            lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                               e->d_X.ptr, nr, *e->lu);
             int steps = 0;
             for (int k = 0; k < nr; ++k) {
                 double eta = backward_error(k);
                 double prev = HUGE_VAL;
                 while (eta > e->opt.refinement_target && steps < e->opt.max_refinement_steps && eta < prev) {
                     // x <- x - A^{-1} r with r = A x - b
-                    lu_solve<float2, double2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->d_row.ptr, e->d_col.ptr,
-                                              e->d_R.ptr + (size_t)k * n, 1, *e->lu);
+                    lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr,
+                                                       e->d_col.ptr, e->d_R.ptr + (size_t)k * n, 1, *e->lu);
                     hipLaunchKernelGGL(axpy_sub_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                                        e->d_X.ptr + (size_t)k * n, e->d_R.ptr + (size_t)k * n, (size_t)n);
                     ++steps;
@@ -423,16 +425,17 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
                 report.used_fp64_fallback = 1;
                 st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
                 if (st.code != 0) return st.code == -2 ? -2 : -1;
+                permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
                 HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
-                lu_solve<double2, double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr,
-                                           nr, *e->lu);
+                lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                                    e->d_X.ptr, nr, *e->lu);
                 worst = 0.0;
                 for (int k = 0; k < nr; ++k) worst = std::fmax(worst, backward_error(k));
             }
             HIP_CHECK(hipEventRecord(e->ev[4], s));
         } else {
-            lu_solve<double2, double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr, nr,
-                                       *e->lu);
+            lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                                e->d_X.ptr, nr, *e->lu);
             HIP_CHECK(hipEventRecord(e->ev[4], s));
             for (int k = 0; k < nr; ++k) worst = std::fmax(worst, backward_error(k));
         }

@@ -25,6 +25,10 @@ struct LUContext {
     int cus = 0;
     hipStream_t stream = nullptr;
     KernelTimer *timer = nullptr;      // optional per-kernel timing
+    DeviceBuffer<int> flags;           // triangular-solve chain flags
+    DeviceBuffer<int> solve_bad;
+    DeviceBuffer<double> solve_tmp;
+    int epoch = 0;
     explicit LUContext(hipStream_t s);
 };
 
@@ -47,6 +51,13 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
 template <class S, class V>
 void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *d_row,
               const double *d_col, V *dX, int nrhs, LUContext &ctx);
+
+// Same with a precomputed row permutation (perm[i] = original row of
+// position i after all swaps), see permutation_from_pivots.
+template <class S, class V>
+void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
+                       const double *d_col, V *dX, int nrhs, LUContext &ctx);
+void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s);
 
 // Convert an FP64 complex matrix to FP32 (for the mixed factor).
 void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s);

@@ -27,6 +27,8 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxNB = 128;
 constexpr unsigned kSpinLimit = 1u << 22;  // bounded grid-barrier wait (a few seconds)
+constexpr size_t kBarWords = 64;
+
 
 // ---------------------------------------------------------------------------
 // element helpers: storage S, arithmetic type Elem<S>::A (always FP64)
@@ -158,8 +160,9 @@ LUContext::LUContext(hipStream_t s) : stream(s) {
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    barrier.resize(1);
+    barrier.resize(kBarWords);
     status.resize(2);
+    solve_bad.resize(1);
     slots.resize((size_t)2 * (cus + 1) * kMaxNB * 2);
     cand.resize((size_t)2 * cus * 2);
 }
@@ -188,6 +191,23 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
     return 0;
 }
 
+// Grid barrier for cooperative kernels (called by one thread per
+// workgroup): one release fence, one arrival on a flat counter, polling with
+// relaxed agent-scope loads (no cache maintenance per poll) and a single
+// acquire fence once the count is reached. The counter restarts at every
+// launch; barrier t completes at t * G arrivals.
+__device__ inline bool grid_barrier(unsigned *bar, unsigned target) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit) return false;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // panel factorization (cooperative)
 
@@ -201,7 +221,7 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
     T *rows = reinterpret_cast<T *>(smem_raw);
     __shared__ T u[kMaxNB];
     __shared__ double red_m[256];
-    __shared__ int red_i[256];
+    __shared__ int red_i[256], red_g[256];
     __shared__ int s_p, s_gw, s_fail;
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
     if (status[0] != 0) return;
@@ -257,38 +277,49 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
         target += G;
         __syncthreads();
         if (tid == 0) {
-            __threadfence();
-            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned spins = 0;
-            while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    status[0] = -3;
-                    break;
-                }
+            const bool ok = grid_barrier(bar, target);
+            if (!ok) status[0] = -3;
+            s_fail = !ok;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // winner over all workgroups' candidates, one candidate per thread:
+        // largest magnitude, lowest row on ties
+        {
+            double m = -1.0;
+            int idx = INT_MAX;
+            if (tid < G) {
+                m = __hip_atomic_load(&cand[2 * ((size_t)buf * G + tid)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                idx = (int)__hip_atomic_load(&cand[2 * ((size_t)buf * G + tid) + 1], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                if (idx == INT_MAX) m = -1.0;
             }
-            __threadfence();
-            // winner: largest magnitude, lowest row on ties
-            double best = -1.0;
-            int bidx = INT_MAX, gw = -1;
-            for (int q = 0; q < G; ++q) {
-                const double m = __hip_atomic_load(&cand[2 * ((size_t)buf * G + q)], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                const int idx = (int)__hip_atomic_load(&cand[2 * ((size_t)buf * G + q) + 1],
-                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (idx == INT_MAX) continue;
-                if (m > best || (m == best && idx < bidx)) {
-                    best = m;
-                    bidx = idx;
-                    gw = q;
+            red_m[tid] = m;
+            red_i[tid] = idx;
+            red_g[tid] = tid;
+            __syncthreads();
+            for (int st = 128; st > 0; st >>= 1) {
+                if (tid < st) {
+                    const double om = red_m[tid + st];
+                    const int oi = red_i[tid + st];
+                    if (om > red_m[tid] || (om == red_m[tid] && oi < red_i[tid])) {
+                        red_m[tid] = om;
+                        red_i[tid] = oi;
+                        red_g[tid] = red_g[tid + st];
+                    }
                 }
+                __syncthreads();
             }
-            s_p = bidx;
-            s_gw = gw;
-            s_fail = (spins > kSpinLimit) || !(best > limit) || !isfinite(best) || gw < 0;
-            if (s_fail && g == 0 && spins <= kSpinLimit) {
-                status[0] = -2;
-                status[1] = k;
+            if (tid == 0) {
+                const double best = red_m[0];
+                s_p = red_i[0];
+                s_gw = red_g[0];
+                const bool bad = !(best > limit) || !isfinite(best) || red_i[0] == INT_MAX;
+                if (bad && !s_fail && g == 0) {
+                    status[0] = -2;
+                    status[1] = k;
+                }
+                s_fail = s_fail || bad;
             }
         }
         __syncthreads();
@@ -535,10 +566,10 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
     hipStream_t s = ctx.stream;
     LUStatus out;
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2, 0);
-    hipLaunchKernelGGL(fill_uint_kernel, dim3(1), dim3(256), 0, s, ctx.barrier.ptr, 1, 0u);
+    hipLaunchKernelGGL(fill_uint_kernel, dim3((unsigned)((kBarWords + 255) / 256)), dim3(256), 0, s, ctx.barrier.ptr,
+                       (int)kBarWords, 0u);
     HIP_CHECK(hipGetLastError());
     const size_t lds_budget = 60 * 1024;
-    unsigned bar_base = 0;
     int k0 = 0;
     while (k0 < n) {
         const int m = n - k0;
@@ -560,15 +591,18 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         const size_t shm = (size_t)rpw * w * sizeof(T);
         int nn = n, kk0 = k0, ww = w, rr = rpw;
         size_t ld = lda;
-        unsigned base = bar_base;
+        unsigned base = 0;
         double lim = pivot_limit;
         void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &ctx.cand.ptr, &d_ipiv, &ctx.barrier.ptr,
                         &base, &lim, &ctx.status.ptr};
+        // barrier counters restart with every launch (the group sizes depend on G)
+        hipLaunchKernelGGL(fill_uint_kernel, dim3((unsigned)((kBarWords + 255) / 256)), dim3(256), 0, s, ctx.barrier.ptr,
+                           (int)kBarWords, 0u);
+        base = 0;
         const int tp = ctx.timer ? ctx.timer->start(s) : -1;
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(panel_kernel<S>), dim3(G), dim3(256),
                                              args, (unsigned)shm, s));
         if (ctx.timer) ctx.timer->stop(tp, s, kKernelPanel, 0.0, 2.0 * m * w * sizeof(S));
-        bar_base += (unsigned)G * w;
         if (n - w > 0)
             hipLaunchKernelGGL(laswp_kernel<S>, dim3((n - w + 255) / 256), dim3(256), 0, s, dA, lda, n, k0, w,
                                d_ipiv, ctx.status.ptr);
@@ -692,11 +726,147 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const S *M, size_t lda, V
     }
 }
 
+// Triangular solve in one cooperative launch: workgroup w owns the 32-row
+// block b (b = w forward, b = nblk-1-w backward). It accumulates the
+// off-diagonal products of every block it depends on as soon as that block's
+// flag shows it final, then solves its diagonal block from LDS, writes its
+// rows and raises its flag. Up to 8 right-hand sides per launch; all
+// arithmetic in FP64 (FP32 factors are widened on load).
+constexpr int kChainTB = 32;
+
+template <class S, class V, bool UPPER>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, V *X, int n, int nrhs,
+                                                         int *flags, int epoch, int *bad) {
+    using T = typename VecT<V>::A;
+    constexpr int TB = kChainTB;
+    __shared__ T dblk[TB][TB];
+    __shared__ T z[8][TB];
+    __shared__ T red[8][8][TB];
+    const int nblk = (n + TB - 1) / TB;
+    const int b = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    const int r0 = b * TB, rows = min(TB, n - r0);
+    const int tid = threadIdx.x;
+    const int r = tid % TB, cg = tid / TB;  // row in block, column group (8 groups of 4 columns)
+    T acc[8];
+    for (int q = 0; q < 8; ++q) acc[q] = ezero<T>();
+    // diagonal block into LDS
+    for (int idx = tid; idx < TB * TB; idx += 256) {
+        const int i = idx / TB, j = idx % TB;
+        dblk[i][j] = (i < rows && j < rows) ? ld(M + (size_t)(r0 + i) * lda + r0 + j) : ezero<T>();
+    }
+    // off-diagonal contributions, in the order the blocks become final; each
+    // round takes every dependency already final (at least one)
+    __shared__ int s_ready, s_abort;
+    const int ndeps = UPPER ? nblk - 1 - b : b;
+    int d = 0;
+    while (d < ndeps) {
+        if (tid == 0) {
+            int ready = d, abort = 0;
+            unsigned spins = 0;
+            const int jb0 = UPPER ? nblk - 1 - d : d;
+            while (__hip_atomic_load(&flags[jb0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    atomicOr(bad, 2);
+                    abort = 1;
+                    break;
+                }
+            }
+            if (!abort) {
+                ready = d + 1;
+                while (ready < ndeps && ready < d + 16) {
+                    const int jb = UPPER ? nblk - 1 - ready : ready;
+                    if (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) break;
+                    ++ready;
+                }
+            }
+            s_ready = ready;
+            s_abort = abort;
+        }
+        __syncthreads();
+        if (s_abort) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int upto = s_ready;
+        for (; d < upto; ++d) {
+            const int jb = UPPER ? nblk - 1 - d : d;
+            const int c0 = jb * TB, cols = min(TB, n - c0);
+            if (r < rows) {
+                const S *mrow = M + (size_t)(r0 + r) * lda + c0;
+                for (int c = cg * 4; c < cg * 4 + 4 && c < cols; ++c) {
+                    const T m = ld(mrow + c);
+                    for (int q = 0; q < nrhs; ++q) acc[q] = eadd(acc[q], emul(m, X[(size_t)q * n + c0 + c]));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // reduce the 8 column groups; z = x_b - sum
+    for (int q = 0; q < nrhs; ++q) red[q][cg][r] = acc[q];
+    __syncthreads();
+    if (tid < nrhs * TB) {
+        const int q = tid / TB, i = tid % TB;
+        T sum = ezero<T>();
+        for (int g2 = 0; g2 < 8; ++g2) sum = eadd(sum, red[q][g2][i]);
+        z[q][i] = i < rows ? esub(X[(size_t)q * n + r0 + i], sum) : ezero<T>();
+    }
+    __syncthreads();
+    // diagonal block: unit lower (forward) or upper with division (backward)
+    for (int step = 0; step < rows; ++step) {
+        const int i = UPPER ? rows - 1 - step : step;
+        if (UPPER && tid < nrhs) {
+            const T d = dblk[i][i];
+            if constexpr (Elem<V>::cplx) {
+                const double den = d.x * d.x + d.y * d.y;
+                if (!isfinite(den) || den == 0.0) atomicOr(bad, 1);
+            } else {
+                if (d == 0.0 || !isfinite(d)) atomicOr(bad, 1);
+            }
+            z[tid][i] = ediv(z[tid][i], d);
+        }
+        __syncthreads();
+        if (tid < nrhs * TB) {
+            const int q = tid / TB, k = tid % TB;
+            if (UPPER ? (k < i) : (k > i && k < rows)) z[q][k] = esub(z[q][k], emul(dblk[k][i], z[q][i]));
+        }
+        __syncthreads();
+    }
+    if (tid < nrhs * TB) {
+        const int q = tid / TB, i = tid % TB;
+        if (i < rows) X[(size_t)q * n + r0 + i] = z[q][i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// one triangular solve through the chain kernel; false if the device cannot
+// keep every block resident (caller falls back to the blocked path)
+template <class S, class V, bool UPPER>
+bool chain_solve(const S *dLU, int n, size_t lda, V *X, int nrhs, int *flags, int epoch, int *bad,
+                 const LUContext &ctx) {
+    const int nblk = (n + kChainTB - 1) / kChainTB;
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, UPPER>), 256, 0));
+    if ((long long)per_cu * ctx.cus < nblk) return false;
+    for (int q0 = 0; q0 < nrhs; q0 += 8) {
+        V *Xq = X + (size_t)q0 * n;
+        int nq = std::min(8, nrhs - q0), nn = n, ep = epoch + q0;
+        size_t ld = lda;
+        void *args[] = {(void *)&dLU, &ld, &Xq, &nn, &nq, &flags, &ep, &bad};
+        HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER>), dim3(nblk),
+                                             dim3(256), args, 0, ctx.stream));
+    }
+    return true;
+}
+
 template <class S, class V>
 void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *d_row, const double *d_col, V *dX,
               int nrhs, LUContext &ctx) {
     hipStream_t s = ctx.stream;
-    // permutation from the sequential swaps
+    // row permutation from the sequential swaps
     std::vector<int> ipiv(n), perm(n);
     HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -704,36 +874,56 @@ void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *
     for (int k = 0; k < n; ++k) std::swap(perm[k], perm[ipiv[k]]);
     DeviceBuffer<int> d_perm;
     d_perm.upload(perm, s);
-    DeviceBuffer<V> tmp((size_t)n * nrhs);
+    lu_solve_permuted(dLU, n, lda, d_perm.ptr, d_row, d_col, dX, nrhs, ctx);
+}
+
+template <class S, class V>
+void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
+                       const double *d_col, V *dX, int nrhs, LUContext &ctx) {
+    hipStream_t s = ctx.stream;
     const size_t total = (size_t)n * nrhs;
-    hipLaunchKernelGGL(permute_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp.ptr, dX,
-                       d_perm.ptr, d_row, n, nrhs);
-    DeviceBuffer<int> bad(1);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
-    const int T = 128;
-    const int nblk = (n + T - 1) / T;
-    for (int b = 0; b < nblk; ++b) {
-        const int i0 = b * T, rows = std::min(T, n - i0);
-        if (i0 > 0)
-            hipLaunchKernelGGL((trsv_gemv_kernel<S, V>), dim3(rows, nrhs), dim3(256), 0, s, dLU, lda, tmp.ptr, n, i0,
-                               rows, 0, i0);
-        hipLaunchKernelGGL((trsv_diag_kernel<S, V>), dim3(nrhs), dim3(64), 0, s, dLU, lda, tmp.ptr, n, i0, rows, 0,
-                           bad.ptr);
+    if (ctx.solve_tmp.count < total * sizeof(V) / sizeof(double) + 1) ctx.solve_tmp.resize(total * sizeof(V) / sizeof(double) + 1);
+    V *tmp = reinterpret_cast<V *>(ctx.solve_tmp.ptr);
+    hipLaunchKernelGGL(permute_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp, dX, d_perm, d_row, n,
+                       nrhs);
+    const int nblk = (n + kChainTB - 1) / kChainTB;
+    if ((int)ctx.flags.count < 2 * nblk) {
+        ctx.flags.resize(2 * (size_t)nblk);
+        hipLaunchKernelGGL(fill_int_kernel, dim3((2 * nblk + 255) / 256), dim3(256), 0, s, ctx.flags.ptr, 2 * nblk, 0);
+        ctx.epoch = 0;
     }
-    for (int b = nblk - 1; b >= 0; --b) {
-        const int i0 = b * T, rows = std::min(T, n - i0);
-        if (i0 + rows < n)
-            hipLaunchKernelGGL((trsv_gemv_kernel<S, V>), dim3(rows, nrhs), dim3(256), 0, s, dLU, lda, tmp.ptr, n, i0,
-                               rows, i0 + rows, n);
-        hipLaunchKernelGGL((trsv_diag_kernel<S, V>), dim3(nrhs), dim3(64), 0, s, dLU, lda, tmp.ptr, n, i0, rows, 1,
-                           bad.ptr);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.solve_bad.ptr, 1, 0);
+    const int ep = ctx.epoch + 1;
+    ctx.epoch += 8 * ((nrhs + 7) / 8) + 1;
+    bool chained = chain_solve<S, V, false>(dLU, n, lda, tmp, nrhs, ctx.flags.ptr, ep, ctx.solve_bad.ptr, ctx) &&
+                   chain_solve<S, V, true>(dLU, n, lda, tmp, nrhs, ctx.flags.ptr + nblk, ep, ctx.solve_bad.ptr, ctx);
+    if (!chained) {
+        const int T = 128;
+        const int nb2 = (n + T - 1) / T;
+        for (int b = 0; b < nb2; ++b) {
+            const int i0 = b * T, rows = std::min(T, n - i0);
+            if (i0 > 0)
+                hipLaunchKernelGGL((trsv_gemv_kernel<S, V>), dim3(rows, nrhs), dim3(256), 0, s, dLU, lda, tmp, n, i0,
+                                   rows, 0, i0);
+            hipLaunchKernelGGL((trsv_diag_kernel<S, V>), dim3(nrhs), dim3(64), 0, s, dLU, lda, tmp, n, i0, rows, 0,
+                               ctx.solve_bad.ptr);
+        }
+        for (int b = nb2 - 1; b >= 0; --b) {
+            const int i0 = b * T, rows = std::min(T, n - i0);
+            if (i0 + rows < n)
+                hipLaunchKernelGGL((trsv_gemv_kernel<S, V>), dim3(rows, nrhs), dim3(256), 0, s, dLU, lda, tmp, n, i0,
+                                   rows, i0 + rows, n);
+            hipLaunchKernelGGL((trsv_diag_kernel<S, V>), dim3(nrhs), dim3(64), 0, s, dLU, lda, tmp, n, i0, rows, 1,
+                               ctx.solve_bad.ptr);
+        }
     }
-    if (d_col) hipLaunchKernelGGL(col_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp.ptr, d_col, n, nrhs);
+    if (d_col) hipLaunchKernelGGL(col_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp, d_col, n, nrhs);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(dX, tmp.ptr, sizeof(V) * total, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dX, tmp, sizeof(V) * total, hipMemcpyDeviceToDevice, s));
     int h_bad = 0;
-    bad.download(&h_bad, 1, s);
+    HIP_CHECK(hipMemcpyAsync(&h_bad, ctx.solve_bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    if (h_bad & 2) throw HipError{"triangular solve chain timed out"};
     if (h_bad) throw HipError{"singular U diagonal in solve"};
 }
 
@@ -743,6 +933,22 @@ template void lu_solve<float2, double2>(const float2 *, int, size_t, const int *
                                         double2 *, int, LUContext &);
 template void lu_solve<double, double>(const double *, int, size_t, const int *, const double *, const double *,
                                        double *, int, LUContext &);
+template void lu_solve_permuted<double2, double2>(const double2 *, int, size_t, const int *, const double *,
+                                                  const double *, double2 *, int, LUContext &);
+template void lu_solve_permuted<float2, double2>(const float2 *, int, size_t, const int *, const double *,
+                                                 const double *, double2 *, int, LUContext &);
+template void lu_solve_permuted<double, double>(const double *, int, size_t, const int *, const double *,
+                                                const double *, double *, int, LUContext &);
+
+void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s) {
+    std::vector<int> ipiv(n), perm(n);
+    HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) std::swap(perm[k], perm[ipiv[k]]);
+    HIP_CHECK(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
 
 __global__ void to_f32_kernel(const double2 *src, float2 *dst, size_t count) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
