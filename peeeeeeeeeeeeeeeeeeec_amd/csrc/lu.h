@@ -21,7 +21,9 @@ struct LUContext {
     DeviceBuffer<unsigned> barrier;   // grid barrier counter
     DeviceBuffer<int> status;         // [0] code, [1] column
     DeviceBuffer<double> slots;       // pivot-row exchange for the panel kernel
-    DeviceBuffer<double> cand;        // per-workgroup pivot candidates
+    DeviceBuffer<double> cand;        // per-workgroup pivot candidate magnitudes
+    DeviceBuffer<unsigned long long> tags;  // per-workgroup (round, row) tags
+    unsigned round = 0;               // pivot-exchange round counter
     int cus = 0;
     hipStream_t stream = nullptr;
     KernelTimer *timer = nullptr;      // optional per-kernel timing

@@ -165,6 +165,8 @@ LUContext::LUContext(hipStream_t s) : stream(s) {
     solve_bad.resize(1);
     slots.resize((size_t)2 * (cus + 1) * kMaxNB * 2);
     cand.resize((size_t)2 * cus * 2);
+    tags.resize((size_t)2 * cus);
+    HIP_CHECK(hipMemset(tags.ptr, 0, tags.bytes()));
 }
 
 int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d_col,
@@ -210,13 +212,43 @@ __device__ inline bool grid_barrier(unsigned *bar, unsigned target) {
 
 // ---------------------------------------------------------------------------
 // panel factorization (cooperative)
+//
+// Pivot exchange without fences: workgroup g publishes, for column j, its
+// candidate row into a double-buffered slot and its candidate magnitude with
+// agent-scope atomic (cache-bypassing) stores, waits for those stores to
+// complete, then publishes a 64-bit tag (round << 32 | row). Readers poll all
+// G tags in parallel (one per thread) and then read the winner's row. The
+// owner of row k publishes that row too (needed by the owner of the pivot
+// row for the swap). Rounds increase monotonically per LU context, so tags
+// never need resetting; the double buffer is safe because nobody can write
+// column j+2 before every workgroup has published column j+1, which it does
+// only after reading column j.
+
+__device__ inline void pub(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double sub(const double *p) {
+    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline void pub_elem(double *p, double2 v) {
+    pub(p, v.x);
+    pub(p + 1, v.y);
+}
+__device__ inline void pub_elem(double *p, double v) { pub(p, v); }
+__device__ inline void sub_elem(const double *p, double2 &v) {
+    v.x = sub(p);
+    v.y = sub(p + 1);
+}
+__device__ inline void sub_elem(const double *p, double &v) { v = sub(p); }
 
 template <class S>
-__global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int k0, int nb, int rpw,
-                                                    typename Elem<S>::A *slots, double *cand,
-                                                    int *ipiv, unsigned *bar, unsigned bar_base,
-                                                    double limit, int *status) {
+__global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int k0, int nb, int rpw, double *slots,
+                                                    double *mags, unsigned long long *tags, int *ipiv,
+                                                    unsigned round0, double limit, int *status) {
     using T = typename Elem<S>::A;
+    constexpr int W = sizeof(T) / sizeof(double);  // doubles per element
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *rows = reinterpret_cast<T *>(smem_raw);
     __shared__ T u[kMaxNB];
@@ -233,10 +265,10 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
         rows[idx] = ld(A + (size_t)(r0 + rr) * lda + k0 + c);
     }
     __syncthreads();
-    unsigned target = bar_base;
     for (int j = 0; j < nb; ++j) {
         const int k = k0 + j, buf = j & 1;
-        // local candidate
+        const unsigned round = round0 + (unsigned)j + 1u;
+        // local candidate: largest magnitude, lowest row on ties
         double bm = -1.0;
         int bi = INT_MAX;
         for (int rr = tid; rr < nr; rr += 256) {
@@ -251,10 +283,10 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
         red_m[tid] = bm;
         red_i[tid] = bi;
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                const double om = red_m[tid + s];
-                const int oi = red_i[tid + s];
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st) {
+                const double om = red_m[tid + st];
+                const int oi = red_i[tid + st];
                 if (om > red_m[tid] || (om == red_m[tid] && oi < red_i[tid])) {
                     red_m[tid] = om;
                     red_i[tid] = oi;
@@ -264,40 +296,48 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
         }
         bm = red_m[0];
         bi = red_i[0];
-        T *slot_base = slots + (size_t)buf * (G + 1) * kMaxNB;
-        if (tid == 0) {
-            cand[2 * ((size_t)buf * G + g)] = bm;
-            cand[2 * ((size_t)buf * G + g) + 1] = (double)bi;
-        }
+        // publish candidate row (and row k if owned), then the tag
+        double *slot_base = slots + (size_t)buf * (G + 1) * kMaxNB * W;
         if (bi != INT_MAX)
-            for (int c = tid; c < nb; c += 256) slot_base[(size_t)g * kMaxNB + c] = rows[(bi - r0) * nb + c];
+            for (int c = tid; c < nb; c += 256) pub_elem(slot_base + ((size_t)g * kMaxNB + c) * W, rows[(bi - r0) * nb + c]);
         if (k >= r0 && k < r1)
-            for (int c = tid; c < nb; c += 256) slot_base[(size_t)G * kMaxNB + c] = rows[(k - r0) * nb + c];
-        // grid barrier (bounded wait)
-        target += G;
+            for (int c = tid; c < nb; c += 256) pub_elem(slot_base + ((size_t)G * kMaxNB + c) * W, rows[(k - r0) * nb + c]);
+        if (tid == 0) pub(&mags[(size_t)buf * G + g], bm);
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) {
-            const bool ok = grid_barrier(bar, target);
-            if (!ok) status[0] = -3;
-            s_fail = !ok;
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // winner over all workgroups' candidates, one candidate per thread:
-        // largest magnitude, lowest row on ties
+        if (tid == 0)
+            __hip_atomic_store(&tags[(size_t)buf * G + g],
+                               ((unsigned long long)round << 32) | (unsigned)(bi == INT_MAX ? 0xffffffffu : (unsigned)bi),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // gather every workgroup's candidate (one per thread), bounded wait
         {
             double m = -1.0;
             int idx = INT_MAX;
+            int timeout = 0;
             if (tid < G) {
-                m = __hip_atomic_load(&cand[2 * ((size_t)buf * G + tid)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                idx = (int)__hip_atomic_load(&cand[2 * ((size_t)buf * G + tid) + 1], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-                if (idx == INT_MAX) m = -1.0;
+                unsigned long long tg;
+                unsigned spins = 0;
+                while (((tg = __hip_atomic_load(&tags[(size_t)buf * G + tid], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) >> 32) != round) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSpinLimit) {
+                        timeout = 1;
+                        break;
+                    }
+                }
+                const unsigned low = (unsigned)(tg & 0xffffffffu);
+                if (!timeout && low != 0xffffffffu) {
+                    idx = (int)low;
+                    m = sub(&mags[(size_t)buf * G + tid]);
+                }
             }
             red_m[tid] = m;
             red_i[tid] = idx;
-            red_g[tid] = tid;
+            red_g[tid] = timeout ? -1 : tid;
             __syncthreads();
+            if (tid == 0) s_fail = 0;
+            __syncthreads();
+            if (timeout) atomicOr(&s_fail, 1);
             for (int st = 128; st > 0; st >>= 1) {
                 if (tid < st) {
                     const double om = red_m[tid + st];
@@ -314,22 +354,25 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
                 const double best = red_m[0];
                 s_p = red_i[0];
                 s_gw = red_g[0];
-                const bool bad = !(best > limit) || !isfinite(best) || red_i[0] == INT_MAX;
-                if (bad && !s_fail && g == 0) {
-                    status[0] = -2;
-                    status[1] = k;
+                if (s_fail) {
+                    status[0] = -3;
+                } else if (!(best > limit) || !isfinite(best) || red_i[0] == INT_MAX) {
+                    if (g == 0) {
+                        status[0] = -2;
+                        status[1] = k;
+                    }
+                    s_fail = 1;
                 }
-                s_fail = s_fail || bad;
             }
         }
         __syncthreads();
         if (s_fail) return;
         const int p = s_p, gw = s_gw;
-        for (int c = tid; c < nb; c += 256) u[c] = slot_base[(size_t)gw * kMaxNB + c];
+        for (int c = tid; c < nb; c += 256) sub_elem(slot_base + ((size_t)gw * kMaxNB + c) * W, u[c]);
         __syncthreads();
         if (p != k) {
             if (p >= r0 && p < r1)
-                for (int c = tid; c < nb; c += 256) rows[(p - r0) * nb + c] = slot_base[(size_t)G * kMaxNB + c];
+                for (int c = tid; c < nb; c += 256) sub_elem(slot_base + ((size_t)G * kMaxNB + c) * W, rows[(p - r0) * nb + c]);
             if (k >= r0 && k < r1)
                 for (int c = tid; c < nb; c += 256) rows[(k - r0) * nb + c] = u[c];
         }
@@ -587,18 +630,16 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         }
         const int w = std::min(nb, m);
         rpw = (m + G - 1) / G;
-        T *slots = reinterpret_cast<T *>(ctx.slots.ptr);
+        double *slots = ctx.slots.ptr;
+        double *mags = ctx.cand.ptr;
+        unsigned long long *tags = ctx.tags.ptr;
         const size_t shm = (size_t)rpw * w * sizeof(T);
         int nn = n, kk0 = k0, ww = w, rr = rpw;
         size_t ld = lda;
-        unsigned base = 0;
+        unsigned round0 = ctx.round;
+        ctx.round += (unsigned)w + 1u;
         double lim = pivot_limit;
-        void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &ctx.cand.ptr, &d_ipiv, &ctx.barrier.ptr,
-                        &base, &lim, &ctx.status.ptr};
-        // barrier counters restart with every launch (the group sizes depend on G)
-        hipLaunchKernelGGL(fill_uint_kernel, dim3((unsigned)((kBarWords + 255) / 256)), dim3(256), 0, s, ctx.barrier.ptr,
-                           (int)kBarWords, 0u);
-        base = 0;
+        void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &mags, &tags, &d_ipiv, &round0, &lim, &ctx.status.ptr};
         const int tp = ctx.timer ? ctx.timer->start(s) : -1;
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(panel_kernel<S>), dim3(G), dim3(256),
                                              args, (unsigned)shm, s));
@@ -785,7 +826,6 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
         }
         __syncthreads();
         if (s_abort) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int upto = s_ready;
         for (; d < upto; ++d) {
             const int jb = UPPER ? nblk - 1 - d : d;
@@ -794,7 +834,11 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
                 const S *mrow = M + (size_t)(r0 + r) * lda + c0;
                 for (int c = cg * 4; c < cg * 4 + 4 && c < cols; ++c) {
                     const T m = ld(mrow + c);
-                    for (int q = 0; q < nrhs; ++q) acc[q] = eadd(acc[q], emul(m, X[(size_t)q * n + c0 + c]));
+                    for (int q = 0; q < nrhs; ++q) {
+                        T xv;
+                        sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + c0 + c), xv);
+                        acc[q] = eadd(acc[q], emul(m, xv));
+                    }
                 }
             }
         }
@@ -830,15 +874,15 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
         }
         __syncthreads();
     }
+    // publish the solved rows with cache-bypassing stores, wait for them to
+    // complete, then raise the flag (no fences needed)
     if (tid < nrhs * TB) {
         const int q = tid / TB, i = tid % TB;
-        if (i < rows) X[(size_t)q * n + r0 + i] = z[q][i];
+        if (i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), z[q][i]);
     }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (tid == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // one triangular solve through the chain kernel; false if the device cannot
