@@ -126,6 +126,7 @@ def main():
     torch.cuda.synchronize()
     t_static = time.perf_counter() - t0
     n = model.ne + model.nv
+    n_cells, n_e, n_v = int(model.mesh.n_quads), model.ne, model.nv
     K, W = args.steps, args.warmup
     idx = np.unique(np.round(np.linspace(0, len(freqs) - 1, K)).astype(int)) if K <= len(freqs) else \
         np.arange(K) % len(freqs)
@@ -194,8 +195,8 @@ def main():
         "dtype": "complex fp32 MFMA factor + complex fp64 refinement/residual",
         "data": "synthetic mesh (closed box), no external inputs",
         "config": {
-            "workload": (f"closed box 2x1x1 m, {h*100:.0f} cm quads ({model.mesh.n_quads} cells, Ne={model.ne}, "
-                         f"Nv={model.nv}, N={n}), retarded exp(-jkR)/R, {len(timed)} of the 200 log-spaced points "
+            "workload": (f"closed box 2x1x1 m, {h*100:.0f} cm quads ({n_cells} cells, Ne={n_e}, "
+                         f"Nv={n_v}, N={n}), retarded exp(-jkR)/R, {len(timed)} of the 200 log-spaced points "
                          f"10 kHz-200 MHz, 1 port node {port[0]} -> {port[1]}"),
             "points": [float(f) for f in timed],
         },
