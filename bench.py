@@ -213,6 +213,8 @@ def main():
         "backward_error_max": float(max(r.backward_error for r in reports)),
         "refinement_steps_mean": float(np.mean([r.refinement_steps for r in reports])),
         "fp64_fallbacks": int(sum(r.used_fp64_fallback for r in reports)),
+        "pivot_order_reused": int(sum(r.pivot_reuse and not r.pivot_refactor for r in reports)),
+        "pivot_order_refactors": int(sum(r.pivot_refactor for r in reports)),
         "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) for r in reports]))
                           for k in ("assembly", "fill", "factor", "solve", "check")},
         "static_assembly_s": t_static,

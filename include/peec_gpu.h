@@ -47,6 +47,10 @@ typedef struct
     int max_refinement_steps;   /* mixed mode cap, default 30 */
     double refinement_target;   /* mixed mode stops at this backward error, default 1e-14 */
     double backward_error_gate; /* a point is accepted at or below this, default 1e-10 */
+    int reuse_pivots;           /* mixed mode: factor in the row order of the previous
+                                   point (no pivot search); refactors with partial
+                                   pivoting when refinement does not reach 1e-3 x gate.
+                                   Default 1 */
 } PeecEngineOptions;
 
 typedef struct
@@ -55,6 +59,9 @@ typedef struct
     double backward_error;      /* worst over the point's right-hand sides */
     int refinement_steps;       /* mixed mode */
     int used_fp64_fallback;     /* mixed mode did not reach the gate and refactored in FP64 */
+    int pivot_reuse;            /* mixed mode: the factor used the previous point's row order */
+    int pivot_refactor;         /* mixed mode: that factor was rejected and redone with pivoting */
+    double reuse_backward_error; /* backward error the rejected reused-order factor reached */
     double ms_assembly;         /* retarded Lp(w)/P(w) */
     double ms_fill;             /* MNA fill + equilibration */
     double ms_factor;

@@ -78,6 +78,10 @@ struct DeviceBuffer {
         if (n) HIP_CHECK(hipMalloc(&ptr, n * sizeof(T)));
         count = n;
     }
+    // grow only (keeps the allocation when it is large enough)
+    void reserve(size_t n) {
+        if (n > count) resize(n);
+    }
     void release() {
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;

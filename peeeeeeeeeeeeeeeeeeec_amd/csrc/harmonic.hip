@@ -194,6 +194,8 @@ struct PeecEngine {
     std::unique_ptr<RegionSet> nodes, edges;
     std::unique_ptr<LUContext> lu;
     DeviceBuffer<int> d_edges, d_ports, d_ipiv, d_perm, d_err;
+    DeviceBuffer<int> d_reuse_perm;  // row order of the last partial-pivoting factor
+    bool have_reuse = false;
     DeviceBuffer<double> d_L, d_P, d_row, d_col, d_terms;
     DeviceBuffer<double2> d_Lw, d_Pw, d_Z, d_Z0, d_B, d_X, d_R;
     DeviceBuffer<float2> d_Z32;
@@ -220,6 +222,7 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->max_refinement_steps = 30;
     o->refinement_target = 1e-14;
     o->backward_error_gate = 1e-10;
+    o->reuse_pivots = 1;
 }
 
 int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o, PeecEngine **out) {
@@ -364,22 +367,47 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
         HIP_CHECK(hipEventRecord(e->ev[2], s));
         const double limit = 1.0e-14 * max_scaled;
         e->d_ipiv.resize(n);
-        LUStatus st;
-        if (mode == PEEC_SOLVE_MIXED) {
-            e->d_Z32.resize(nn);
-            convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, nn, s);
-            st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-        } else {
-            st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-        }
-        HIP_CHECK(hipEventRecord(e->ev[3], s));
-        if (st.code != 0) {
+        e->d_perm.resize(n);
+        e->d_reuse_perm.resize(n);
+        auto keep_order = [&]() {
+            // the row order of the partial-pivoting factor just made
+            permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
+            HIP_CHECK(hipMemcpyAsync(e->d_reuse_perm.ptr, e->d_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+            e->have_reuse = true;
+        };
+        auto factor_error = [&](const LUStatus &st) {
             if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
             else PEEC_ERR("LU failed (code %d).", st.code);
             return st.code == -2 ? -2 : -1;
+        };
+        LUStatus st;
+        bool ordered = false;
+        if (mode == PEEC_SOLVE_MIXED) {
+            e->d_Z32.resize(nn);
+            if (e->opt.reuse_pivots && e->have_reuse) {
+                convert_rows_to_f32(e->d_Z.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, n, s);
+                st = lu_factor_ordered(e->d_Z32.ptr, n, n, limit, *e->lu, e->opt.block_size);
+                if (st.code == 0) {
+                    ordered = true;
+                    HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n,
+                                             hipMemcpyDeviceToDevice, s));
+                } else if (st.code != -2) {
+                    return factor_error(st);
+                }
+            }
+            if (!ordered) {
+                convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, nn, s);
+                st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
+                if (st.code != 0) return factor_error(st);
+                keep_order();
+            }
+        } else {
+            st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
+            if (st.code != 0) return factor_error(st);
+            keep_order();
         }
-        e->d_perm.resize(n);
-        permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
+        HIP_CHECK(hipEventRecord(e->ev[3], s));
+        report.pivot_reuse = ordered ? 1 : 0;
         e->d_ports.upload(ports, 2 * nports, s);
         e->d_B.resize((size_t)n * nr);
         e->d_X.resize((size_t)n * nr);
@@ -388,7 +416,6 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
         const size_t tot = (size_t)n * nr;
         hipLaunchKernelGGL(port_rhs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, e->d_B.ptr, ne, nv,
                            e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
-        HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
         std::vector<double> terms((size_t)3 * n);
         double worst = 0.0;
         auto backward_error = [&](int k) {
@@ -400,32 +427,50 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             residual_summary(terms, n, ne, summary);
             return summary[2];
         };
-        if (mode == PEEC_SOLVE_MIXED) {
+        // solve with the FP32 factor and refine against the FP64 matrix:
+        // x <- x - A^{-1} r, r = A x - b, until the target, stagnation or the
+        // step cap; returns the worst backward error over the right-hand sides
+        auto solve_refine = [&]() {
+            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
             lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                                e->d_X.ptr, nr, *e->lu);
-            int steps = 0;
+            double w = 0.0;
             for (int k = 0; k < nr; ++k) {
                 double eta = backward_error(k);
                 double prev = HUGE_VAL;
-                while (eta > e->opt.refinement_target && steps < e->opt.max_refinement_steps && eta < prev) {
-                    // x <- x - A^{-1} r with r = A x - b
+                while (eta > e->opt.refinement_target && report.refinement_steps < e->opt.max_refinement_steps &&
+                       eta < prev) {
                     lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr,
                                                        e->d_col.ptr, e->d_R.ptr + (size_t)k * n, 1, *e->lu);
                     hipLaunchKernelGGL(axpy_sub_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                                        e->d_X.ptr + (size_t)k * n, e->d_R.ptr + (size_t)k * n, (size_t)n);
-                    ++steps;
+                    ++report.refinement_steps;
                     prev = eta;
                     eta = backward_error(k);
                 }
-                worst = std::fmax(worst, eta);
+                w = std::fmax(w, eta);
             }
-            report.refinement_steps = steps;
+            return w;
+        };
+        if (mode == PEEC_SOLVE_MIXED) {
+            worst = solve_refine();
+            if (ordered && !(worst <= 1e-3 * e->opt.backward_error_gate)) {
+                // the reused row order is not good enough at this point:
+                // refactor with partial pivoting and refine again
+                report.pivot_refactor = 1;
+                report.reuse_backward_error = worst;
+                convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, nn, s);
+                st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
+                if (st.code != 0) return factor_error(st);
+                keep_order();
+                worst = solve_refine();
+            }
             if (!(worst <= e->opt.backward_error_gate)) {
                 // refinement stalled above the gate: refactor in FP64
                 report.used_fp64_fallback = 1;
                 st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-                if (st.code != 0) return st.code == -2 ? -2 : -1;
-                permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
+                if (st.code != 0) return factor_error(st);
+                keep_order();
                 HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
                 lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                                     e->d_X.ptr, nr, *e->lu);
@@ -434,6 +479,7 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             }
             HIP_CHECK(hipEventRecord(e->ev[4], s));
         } else {
+            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
             lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                                 e->d_X.ptr, nr, *e->lu);
             HIP_CHECK(hipEventRecord(e->ev[4], s));

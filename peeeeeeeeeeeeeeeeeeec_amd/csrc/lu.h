@@ -30,8 +30,18 @@ struct LUContext {
     DeviceBuffer<int> flags;           // triangular-solve chain flags
     DeviceBuffer<int> solve_bad;
     DeviceBuffer<double> solve_tmp;
+    DeviceBuffer<unsigned char> pack_l, pack_u;  // MFMA-ordered trailing-update operands
+    DeviceBuffer<int> swap_plan;                  // row moves of one panel
+    DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves
+    DeviceBuffer<unsigned char> diag_inv;         // inverses of the 32x32 diagonal blocks (solves)
+    int lds_per_block = 64 * 1024;
+    hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
+    hipEvent_t ev_next = nullptr, ev_diag = nullptr;
     int epoch = 0;
     explicit LUContext(hipStream_t s);
+    ~LUContext();
+    LUContext(const LUContext &) = delete;
+    LUContext &operator=(const LUContext &) = delete;
 };
 
 // Per-row / per-column max equilibration (complex_lu.c:275-320) of a
@@ -45,6 +55,16 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
 template <class S>
 LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LUContext &ctx,
                    int block_size = 0);
+
+// Factor a complex FP32 matrix whose rows are already in pivot order
+// (no pivot search, no row exchange between workgroups); status -2 when a
+// pivot has |u_kk| <= pivot_limit. Used by the mixed-precision path with the
+// row order of an earlier partial-pivoting factorization.
+LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx,
+                           int block_size = 0);
+
+// dst[i][c] = (float2) src[perm[i]][c] for an n x n row-major matrix.
+void convert_rows_to_f32(const double2 *src, float2 *dst, const int *d_perm, int n, hipStream_t s);
 
 // Solve with the factors for `nrhs` right-hand sides stored rhs-major
 // (X[r*n + i]); applies the row scale, pivots, L, U and column scale like

@@ -5,9 +5,9 @@
 //                   reference's strict '>' scan in complex_lu.c:341-360),
 //                   exchanges the pivot row through a small HBM slot and
 //                   applies the rank-1 update to the LDS slab.
-//   laswp_kernel    applies the panel's row swaps to the other columns.
-//   trsm_kernel     U12 = L11^{-1} A12, 8 lanes per column with a shuffle
-//                   reduction.
+//   swap_*_kernel   apply the panel's row swaps to the other columns.
+//   trsm_kernel     U12 = L11^{-1} A12, columns in registers, 8 lanes per
+//                   column.
 //   gemm_kernel     A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 for FP64,
 //                   v_mfma_f32_16x16x4f32 for the mixed-precision factor).
 // Triangular solves are blocked (GEMV + diagonal-block kernels).
@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "lu.h"
@@ -77,6 +78,75 @@ __device__ inline double2 eshfl_xor(double2 v, int m) {
     return make_double2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
 __device__ inline double eshfl_xor(double v, int m) { return __shfl_xor(v, m); }
+// Register array of N arithmetic elements. Complex values are kept as two
+// plain double arrays: arrays of HIP vector types are not split into
+// registers by the compiler and end up in scratch memory.
+template <class T, int N> struct Regs;
+template <int N> struct Regs<double2, N> {
+    double re[N], im[N];
+    __device__ double2 get(int i) const { return make_double2(re[i], im[i]); }
+    __device__ void set(int i, double2 v) {
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+};
+template <int N> struct Regs<double, N> {
+    double v[N];
+    __device__ double get(int i) const { return v[i]; }
+    __device__ void set(int i, double x) { v[i] = x; }
+};
+
+template <int N> struct Regs<float2, N> {
+    float re[N], im[N];
+    __device__ float2 get(int i) const { return make_float2(re[i], im[i]); }
+    __device__ void set(int i, float2 v) {
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+};
+
+// Working precision of the triangular-solve and diagonal-block kernels: the
+// storage precision (the FP32 factor is formed in FP32 arithmetic, like its
+// MFMA trailing update), with fused multiply-adds.
+template <class S> struct Work { using T = S; };
+__device__ inline float2 wld(const float2 *p) { return *p; }
+__device__ inline double2 wld(const double2 *p) { return *p; }
+__device__ inline double wld(const double *p) { return *p; }
+__device__ inline void wst(float2 *p, float2 v) { *p = v; }
+__device__ inline void wst(double2 *p, double2 v) { *p = v; }
+__device__ inline void wst(double *p, double v) { *p = v; }
+template <class T> __device__ inline T wzero();
+template <> __device__ inline float2 wzero<float2>() { return make_float2(0.f, 0.f); }
+template <> __device__ inline double2 wzero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ inline double wzero<double>() { return 0.0; }
+template <class T> __device__ inline T wone();
+template <> __device__ inline float2 wone<float2>() { return make_float2(1.f, 0.f); }
+template <> __device__ inline double2 wone<double2>() { return make_double2(1.0, 0.0); }
+template <> __device__ inline double wone<double>() { return 1.0; }
+// c - a b
+__device__ inline float2 wfms(float2 c, float2 a, float2 b) {
+    return make_float2(fmaf(-a.x, b.x, fmaf(a.y, b.y, c.x)), fmaf(-a.x, b.y, fmaf(-a.y, b.x, c.y)));
+}
+__device__ inline double2 wfms(double2 c, double2 a, double2 b) {
+    return make_double2(fma(-a.x, b.x, fma(a.y, b.y, c.x)), fma(-a.x, b.y, fma(-a.y, b.x, c.y)));
+}
+__device__ inline double wfms(double c, double a, double b) { return fma(-a, b, c); }
+// a / b by the naive formula (complex_lu.c:162-184)
+__device__ inline float2 wdiv(float2 a, float2 b) {
+    const float den = b.x * b.x + b.y * b.y;
+    return make_float2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__device__ inline double2 wdiv(double2 a, double2 b) { return ediv(a, b); }
+__device__ inline double wdiv(double a, double b) { return a / b; }
+__device__ inline double wmag(float2 v) { return hypot((double)v.x, (double)v.y); }
+__device__ inline double wmag(double2 v) { return hypot(v.x, v.y); }
+__device__ inline double wmag(double v) { return fabs(v); }
+__device__ inline float2 wshfl(float2 v, int src) { return make_float2(__shfl(v.x, src), __shfl(v.y, src)); }
+__device__ inline double2 wshfl(double2 v, int src) { return make_double2(__shfl(v.x, src), __shfl(v.y, src)); }
+__device__ inline double wshfl(double v, int src) { return __shfl(v, src); }
+
+__device__ inline double2 eshfl(double2 v, int src) { return make_double2(__shfl(v.x, src), __shfl(v.y, src)); }
+__device__ inline double eshfl(double v, int src) { return __shfl(v, src); }
 
 // ---------------------------------------------------------------------------
 // equilibration (complex_lu.c:275-320)
@@ -167,6 +237,17 @@ LUContext::LUContext(hipStream_t s) : stream(s) {
     cand.resize((size_t)2 * cus * 2);
     tags.resize((size_t)2 * cus);
     HIP_CHECK(hipMemset(tags.ptr, 0, tags.bytes()));
+    swap_plan.resize((size_t)1 + 4 * kMaxNB);
+    HIP_CHECK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
+    HIP_CHECK(hipDeviceGetAttribute(&lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+}
+
+LUContext::~LUContext() {
+    if (ev_next) (void)hipEventDestroy(ev_next);
+    if (ev_diag) (void)hipEventDestroy(ev_diag);
+    if (aux) (void)hipStreamDestroy(aux);
 }
 
 int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d_col,
@@ -243,6 +324,21 @@ __device__ inline void sub_elem(const double *p, double2 &v) {
 }
 __device__ inline void sub_elem(const double *p, double &v) { v = sub(p); }
 
+// Butterfly arg-max over a wavefront: largest magnitude, lowest row on
+// ties (the order of complex_lu.c:341-360); every lane ends with the result.
+__device__ inline void wave_argmax(double &m, int &i, int &g) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double om = __shfl_xor(m, off);
+        const int oi = __shfl_xor(i, off);
+        const int og = __shfl_xor(g, off);
+        if (om > m || (om == m && oi < i)) {
+            m = om;
+            i = oi;
+            g = og;
+        }
+    }
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int k0, int nb, int rpw, double *slots,
                                                     double *mags, unsigned long long *tags, int *ipiv,
@@ -252,10 +348,10 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *rows = reinterpret_cast<T *>(smem_raw);
     __shared__ T u[kMaxNB];
-    __shared__ double red_m[256];
-    __shared__ int red_i[256], red_g[256];
-    __shared__ int s_p, s_gw, s_fail;
+    __shared__ double wl_m[4], wg_m[4];
+    __shared__ int wl_i[4], wg_i[4], wg_g[4];
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     if (status[0] != 0) return;
     const int r0 = k0 + g * rpw;
     const int r1 = min(n, r0 + rpw);
@@ -280,22 +376,22 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
                 bi = row;
             }
         }
-        red_m[tid] = bm;
-        red_i[tid] = bi;
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (tid < st) {
-                const double om = red_m[tid + st];
-                const int oi = red_i[tid + st];
-                if (om > red_m[tid] || (om == red_m[tid] && oi < red_i[tid])) {
-                    red_m[tid] = om;
-                    red_i[tid] = oi;
-                }
+        {
+            int unused = 0;
+            wave_argmax(bm, bi, unused);
+            if (lane == 0) {
+                wl_m[wave] = bm;
+                wl_i[wave] = bi;
             }
             __syncthreads();
+            bm = wl_m[0];
+            bi = wl_i[0];
+            for (int q = 1; q < 4; ++q)
+                if (wl_m[q] > bm || (wl_m[q] == bm && wl_i[q] < bi)) {
+                    bm = wl_m[q];
+                    bi = wl_i[q];
+                }
         }
-        bm = red_m[0];
-        bi = red_i[0];
         // publish candidate row (and row k if owned), then the tag
         double *slot_base = slots + (size_t)buf * (G + 1) * kMaxNB * W;
         if (bi != INT_MAX)
@@ -310,6 +406,8 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
                                ((unsigned long long)round << 32) | (unsigned)(bi == INT_MAX ? 0xffffffffu : (unsigned)bi),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // gather every workgroup's candidate (one per thread), bounded wait
+        double best;
+        int p, gw;
         {
             double m = -1.0;
             int idx = INT_MAX;
@@ -331,43 +429,34 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
                     m = sub(&mags[(size_t)buf * G + tid]);
                 }
             }
-            red_m[tid] = m;
-            red_i[tid] = idx;
-            red_g[tid] = timeout ? -1 : tid;
-            __syncthreads();
-            if (tid == 0) s_fail = 0;
-            __syncthreads();
-            if (timeout) atomicOr(&s_fail, 1);
-            for (int st = 128; st > 0; st >>= 1) {
-                if (tid < st) {
-                    const double om = red_m[tid + st];
-                    const int oi = red_i[tid + st];
-                    if (om > red_m[tid] || (om == red_m[tid] && oi < red_i[tid])) {
-                        red_m[tid] = om;
-                        red_i[tid] = oi;
-                        red_g[tid] = red_g[tid + st];
-                    }
-                }
-                __syncthreads();
+            int src = tid;
+            wave_argmax(m, idx, src);
+            if (lane == 0) {
+                wg_m[wave] = m;
+                wg_i[wave] = idx;
+                wg_g[wave] = src;
             }
-            if (tid == 0) {
-                const double best = red_m[0];
-                s_p = red_i[0];
-                s_gw = red_g[0];
-                if (s_fail) {
-                    status[0] = -3;
-                } else if (!(best > limit) || !isfinite(best) || red_i[0] == INT_MAX) {
-                    if (g == 0) {
-                        status[0] = -2;
-                        status[1] = k;
-                    }
-                    s_fail = 1;
-                }
+            if (__syncthreads_or(timeout)) {
+                if (tid == 0) status[0] = -3;
+                return;
             }
+            best = wg_m[0];
+            p = wg_i[0];
+            gw = wg_g[0];
+            for (int q = 1; q < 4; ++q)
+                if (wg_m[q] > best || (wg_m[q] == best && wg_i[q] < p)) {
+                    best = wg_m[q];
+                    p = wg_i[q];
+                    gw = wg_g[q];
+                }
         }
-        __syncthreads();
-        if (s_fail) return;
-        const int p = s_p, gw = s_gw;
+        if (!(best > limit) || !isfinite(best) || p == INT_MAX) {
+            if (g == 0 && tid == 0) {
+                status[0] = -2;
+                status[1] = k;
+            }
+            return;
+        }
         for (int c = tid; c < nb; c += 256) sub_elem(slot_base + ((size_t)gw * kMaxNB + c) * W, u[c]);
         __syncthreads();
         if (p != k) {
@@ -400,181 +489,409 @@ __global__ __launch_bounds__(256) void panel_kernel(S *A, size_t lda, int n, int
     }
 }
 
-// apply the swaps of one panel to every other column
-template <class S>
-__global__ void laswp_kernel(S *A, size_t lda, int n, int k0, int w, const int *ipiv, const int *status) {
+// Row swaps of one panel on every other column, in two parallel passes.
+// swap_plan_kernel replays the panel's sequential swaps (ipiv[k0 + t] with
+// k0 + t, t = 0..w-1, as complex_lu.c:365-380 applies them) on the at most
+// 2w affected positions and emits the resulting moves (dst <- src);
+// swap_gather_kernel copies every moved row's source to a staging buffer and
+// swap_scatter_kernel writes it to its destination, so no row is read after
+// it has been overwritten.
+__global__ __launch_bounds__(256) void swap_plan_kernel(const int *ipiv, int k0, int w, int *plan,
+                                                        const int *status) {
+    __shared__ int pos[2 * kMaxNB], content[2 * kMaxNB], pslot[kMaxNB], fo[kMaxNB], slot_first[kMaxNB];
+    __shared__ int piv[kMaxNB];
+    __shared__ int s_count, s_extra;
     if (status[0] != 0) return;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n - w) return;
-    const int col = c < k0 ? c : c + w;
-    for (int t = 0; t < w; ++t) {
-        const int p = ipiv[k0 + t];
-        if (p != k0 + t) {
-            S *a = A + (size_t)(k0 + t) * lda + col;
-            S *b = A + (size_t)p * lda + col;
-            const S tmp = *a;
-            *a = *b;
-            *b = tmp;
+    const int tid = threadIdx.x;
+    if (tid < w) piv[tid] = ipiv[k0 + tid];
+    __syncthreads();
+    int p = 0;
+    if (tid < w) {
+        p = piv[tid];
+        int f = -1;
+        if (p >= k0 + w) {
+            f = tid;
+            for (int t = 0; t < tid; ++t)
+                if (piv[t] == p) {
+                    f = t;
+                    break;
+                }
         }
+        fo[tid] = f;
+        pos[tid] = k0 + tid;
     }
+    __syncthreads();
+    if (tid == 0) {
+        int extra = 0;
+        for (int t = 0; t < w; ++t)
+            if (fo[t] == t) {
+                slot_first[t] = w + extra;
+                pos[w + extra] = piv[t];
+                ++extra;
+            }
+        s_extra = extra;
+        s_count = 0;
+    }
+    __syncthreads();
+    if (tid < w) pslot[tid] = p < k0 + w ? p - k0 : slot_first[fo[tid]];
+    const int slots = w + s_extra;
+    for (int q = tid; q < slots; q += 256) content[q] = pos[q];
+    __syncthreads();
+    if (tid == 0)
+        for (int t = 0; t < w; ++t) {
+            const int q = pslot[t];
+            const int c = content[t];
+            content[t] = content[q];
+            content[q] = c;
+        }
+    __syncthreads();
+    for (int q = tid; q < slots; q += 256)
+        if (content[q] != pos[q]) {
+            const int i = atomicAdd(&s_count, 1);
+            plan[1 + 2 * i] = pos[q];
+            plan[2 + 2 * i] = content[q];
+        }
+    __syncthreads();
+    if (tid == 0) plan[0] = s_count;
 }
 
-// U12 = L11^{-1} A12; 8 lanes per column (32 columns per block)
+// column c of the n - w columns outside the panel [k0, k0 + w)
+__device__ inline int swap_column(int c, int k0, int w) { return c < k0 ? c : c + w; }
+
+template <class S>
+__global__ void swap_gather_kernel(const S *A, size_t lda, int n, int k0, int w, const int *plan, S *stage,
+                                   const int *status) {
+    if (status[0] != 0) return;
+    const int q = blockIdx.y;
+    if (q >= plan[0]) return;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n - w) return;
+    stage[(size_t)q * n + c] = A[(size_t)plan[2 + 2 * q] * lda + swap_column(c, k0, w)];
+}
+
+template <class S>
+__global__ void swap_scatter_kernel(S *A, size_t lda, int n, int k0, int w, const int *plan, const S *stage,
+                                    const int *status) {
+    if (status[0] != 0) return;
+    const int q = blockIdx.y;
+    if (q >= plan[0]) return;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n - w) return;
+    A[(size_t)plan[1 + 2 * q] * lda + swap_column(c, k0, w)] = stage[(size_t)q * n + c];
+}
+
+// U12 = L11^{-1} A12 by forward substitution with the unit-lower L11
+// (complex_lu.c:598-632 order per column). The strictly lower part of L11
+// is staged once per workgroup in LDS (packed by rows, in the storage
+// type). Eight lanes share one column, lane `sub` keeping rows
+// 16 sub .. 16 sub + 15 of it in registers; at step r the owner of row r
+// broadcasts x_r with a shuffle and every lane updates its rows below r.
+constexpr int kTrsmTPC = 8;                   // lanes per column
+constexpr int kTrsmRPT = kMaxNB / kTrsmTPC;  // rows per lane
+
+__host__ __device__ constexpr size_t trsm_lds_elems(int w) { return (size_t)w * (w - 1) / 2; }
+
 template <class S>
 __global__ __launch_bounds__(256) void trsm_kernel(S *A, size_t lda, int n, int k0, int w, const int *status) {
-    using T = typename Elem<S>::A;
-    __shared__ T xs[32][kMaxNB];
+    using T = typename Work<S>::T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    S *lw = reinterpret_cast<S *>(smem_raw);  // L11[i][t], t < i, at i(i-1)/2 + t
     if (status[0] != 0) return;
-    const int cl = threadIdx.x >> 3, sub = threadIdx.x & 7;
-    const int col = k0 + w + blockIdx.x * 32 + cl;
-    const bool valid = col < n;
-    for (int r = 0; r < w; ++r) {
-        T part = ezero<T>();
-        const S *lrow = A + (size_t)(k0 + r) * lda + k0;
-        for (int t = sub; t < r; t += 8) part = eadd(part, emul(ld(lrow + t), xs[cl][t]));
-        part = eadd(part, eshfl_xor(part, 1));
-        part = eadd(part, eshfl_xor(part, 2));
-        part = eadd(part, eshfl_xor(part, 4));
-        T x = ezero<T>();
-        if (valid) x = esub(ld(A + (size_t)(k0 + r) * lda + col), part);
-        if (sub == 0) {
-            xs[cl][r] = x;
-            if (valid) st(A + (size_t)(k0 + r) * lda + col, x);
+    {
+        // flat pass over the packed triangle, 8 independent loads in flight
+        const int total = (int)trsm_lds_elems(w);
+        for (int base = threadIdx.x; base < total; base += 256 * 8) {
+            Regs<T, 8> v;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * 256;
+                v.set(u, wzero<T>());
+                if (idx < total) {
+                    int i = (int)((1.0 + sqrt(1.0 + 8.0 * idx)) * 0.5);
+                    while (i * (i - 1) / 2 > idx) --i;
+                    while ((i + 1) * i / 2 <= idx) ++i;
+                    v.set(u, wld(A + (size_t)(k0 + i) * lda + k0 + (idx - i * (i - 1) / 2)));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (base + u * 256 < total) wst(lw + base + u * 256, v.get(u));
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const int lane = threadIdx.x & 63, sub = lane & (kTrsmTPC - 1);
+    const int col = k0 + w + (int)blockIdx.x * (256 / kTrsmTPC) + (int)threadIdx.x / kTrsmTPC;
+    const bool valid = col < n;
+    const int base = lane & ~(kTrsmTPC - 1);
+    Regs<T, kTrsmRPT> a;
+#pragma unroll
+    for (int t = 0; t < kTrsmRPT; ++t) {
+        const int i = sub * kTrsmRPT + t;
+        a.set(t, (valid && i < w) ? wld(A + (size_t)(k0 + i) * lda + col) : wzero<T>());
+    }
+    __syncthreads();
+    const int last = max(0, (int)trsm_lds_elems(w) - 1);
+    const int nblk = (w + kTrsmRPT - 1) / kTrsmRPT;
+    for (int rb = 0; rb < nblk; ++rb) {
+#pragma unroll
+        for (int rr = 0; rr < kTrsmRPT; ++rr) {
+            const int r = min(rb * kTrsmRPT + rr, w - 1);
+            const T x = wshfl(a.get(rr), base + rb);
+            // lanes whose rows all lie above r have nothing left to do
+            if (sub * kTrsmRPT + kTrsmRPT - 1 <= r) continue;
+            // branch-free inside: every lane reads an in-bounds element and
+            // keeps the update only for its rows below r
+#pragma unroll
+            for (int t = 0; t < kTrsmRPT; ++t) {
+                const int i = sub * kTrsmRPT + t;
+                const int ic = min(i, w - 1);
+                const int idx = min(ic * (ic - 1) / 2 + r, last);
+                const T cur = a.get(t);
+                const T upd = wfms(cur, wld(lw + idx), x);
+                a.set(t, (i > r && i < w) ? upd : cur);
+            }
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int t = 0; t < kTrsmRPT; ++t) {
+            const int i = sub * kTrsmRPT + t;
+            if (i < w) wst(A + (size_t)(k0 + i) * lda + col, a.get(t));
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// trailing update on MFMA. Block tile 64x64, 4 waves of 32x32, each wave a
-// 2x2 grid of 16x16 MFMA blocks. Operand layout of the 16x16x4 MFMAs (checked
-// on gfx950 by mfma_layout_probe_kernel): A[i][k] in lane i + 16k, B[k][j] in
-// lane j + 16k; accumulator element v of lane l holds row 4v + l/16 for the
-// FP64 instruction and row 4(l/16) + v for the FP32 one, column l % 16.
+// trailing update A22 -= L21 U12 on MFMA.
+//
+// The operands are first packed in the MFMA lane order (pack_l_kernel,
+// pack_u_kernel): element (i, k) of L21 goes to chunk [k/4][i/16], lane
+// (i%16) + 16(k%4); element (k, j) of U12 to chunk [k/4][j/16], lane
+// (j%16) + 16(k%4). A chunk is one wavefront's operand of one 16x16x4 MFMA,
+// so every operand load is a single contiguous 64-lane access (512 B for
+// complex FP32). Rows and columns beyond the matrix are zero in the packed
+// copies. Accumulator element v of lane l holds row 4v + l/16 for the FP64
+// instruction and row 4(l/16) + v for the FP32 one, column l % 16 (checked
+// on gfx950 by mfma_layout_probe_kernel).
 
 template <class S> struct Mfma;
 template <> struct Mfma<double2> {
     using acc = v4d;
     using op = double;
+    static constexpr bool cplx = true;
+    static constexpr int wave_col_blocks = 2;  // 64 x 32 wave tile
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int lk, int v) { return 4 * v + lk; }
-    __device__ static void load(const double2 *p, op &re, op &im) {
-        const double2 v = *p;
-        re = v.x;
-        im = v.y;
-    }
 };
 template <> struct Mfma<float2> {
     using acc = v4f;
     using op = float;
+    static constexpr bool cplx = true;
+    static constexpr int wave_col_blocks = 4;  // 64 x 64 wave tile
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static int row(int lk, int v) { return 4 * lk + v; }
-    __device__ static void load(const float2 *p, op &re, op &im) {
-        const float2 v = *p;
-        re = v.x;
-        im = v.y;
-    }
 };
+template <> struct Mfma<double> {
+    using acc = v4d;
+    using op = double;
+    static constexpr bool cplx = false;
+    static constexpr int wave_col_blocks = 4;
+    __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lk, int v) { return 4 * v + lk; }
+};
+constexpr int kWaveRowBlocks = 4;  // 64 rows per wave, 128 per workgroup (2 x 2 waves)
+
+template <class S> __device__ inline S szero();
+template <> __device__ inline double2 szero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ inline float2 szero<float2>() { return make_float2(0.f, 0.f); }
+template <> __device__ inline double szero<double>() { return 0.0; }
+
+// L21 rows [row0, row0 + m), columns [k0, k0 + kw) -> chunks [kw/4][rb_pad]
+template <class S>
+__global__ void pack_l_kernel(const S *A, size_t lda, int row0, int m, int k0, int kw, int rb_pad, S *out,
+                              const int *status) {
+    if (status[0] != 0) return;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)((kw + 3) / 4) * rb_pad * 64;
+    if (idx >= total) return;
+    const int lane = (int)(idx & 63);
+    const size_t chunk = idx >> 6;
+    const int rb = (int)(chunk % rb_pad), ks = (int)(chunk / rb_pad);
+    const int row = rb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
+    out[idx] = (row < m && k < kw) ? A[(size_t)(row0 + row) * lda + k0 + k] : szero<S>();
+}
+
+// U12 rows [k0, k0 + kw), columns [col0, col0 + ncols) -> chunks [kw/4][cb_pad]
+template <class S>
+__global__ void pack_u_kernel(const S *A, size_t lda, int k0, int kw, int col0, int ncols, int cb_pad, S *out,
+                              const int *status) {
+    if (status[0] != 0) return;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)((kw + 3) / 4) * cb_pad * 64;
+    if (idx >= total) return;
+    const int lane = (int)(idx & 63);
+    const size_t chunk = idx >> 6;
+    const int cb = (int)(chunk % cb_pad), ks = (int)(chunk / cb_pad);
+    const int col = cb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
+    out[idx] = (col < ncols && k < kw) ? A[(size_t)(k0 + k) * lda + col0 + col] : szero<S>();
+}
+
+// Workgroup tile index -> (tile row, tile column). Workgroups are dispatched
+// round-robin over the 8 XCDs; consecutive ids on one XCD are given a
+// contiguous range of tiles, walked in groups of 8 tile rows (column-major
+// inside a group), so the tiles one XCD runs at a time share L21 rows and
+// U12 columns in that XCD's L2.
+__device__ inline bool gemm_tile(int tiles_m, int tiles_n, int &tm, int &tn) {
+    const int T = tiles_m * tiles_n, per = (T + 7) / 8;
+    const int nid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (nid >= T) return false;
+    constexpr int kGroup = 8;
+    const int in_group = kGroup * tiles_n;
+    const int grp = nid / in_group, first = grp * kGroup;
+    const int gsz = min(tiles_m - first, kGroup);
+    const int loc = nid - grp * in_group;
+    tm = first + loc % gsz;
+    tn = loc / gsz;
+    return true;
+}
 
 template <class S>
-__global__ __launch_bounds__(256) void gemm_complex_kernel(S *A, size_t lda, int row0, int col0, int m,
-                                                           int ncols, int k0, int kw, const int *status) {
+__global__ __launch_bounds__(256, 2) void gemm_packed_kernel(S *A, size_t lda, int row0, int col0, int m, int ncols,
+                                                          int ks_count, const S *__restrict__ Lp,
+                                                          const S *__restrict__ Up, int rb_pad, int cb_pad,
+                                                          const int *status) {
     using M = Mfma<S>;
     using acc_t = typename M::acc;
     using op = typename M::op;
+    constexpr int MB = kWaveRowBlocks, NB = M::wave_col_blocks;
+    constexpr int BM = 32 * MB, BN = 32 * NB;
     if (status[0] != 0) return;
+    int tm, tn;
+    if (!gemm_tile((m + BM - 1) / BM, (ncols + BN - 1) / BN, tm, tn)) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int wc = blockIdx.x * 64 + (wave & 1) * 32;
-    acc_t cre[2][2], cim[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
+    const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
+    const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
+    acc_t cre[MB][NB], cim[MB][NB];
+    for (int a = 0; a < MB; ++a)
+        for (int b = 0; b < NB; ++b) {
             cre[a][b] = acc_t{0, 0, 0, 0};
             cim[a][b] = acc_t{0, 0, 0, 0};
         }
-    const int li = lane & 15, lk = lane >> 4;
-    for (int kk = 0; kk < kw; kk += 4) {
-        const int t = kk + lk;
-        op are[2], aim[2], bre[2], bim[2];
-        for (int mi = 0; mi < 2; ++mi) {
-            const int row = wr + mi * 16 + li;
-            are[mi] = aim[mi] = 0;
-            if (row < m && t < kw) M::load(A + (size_t)(row0 + row) * lda + k0 + t, are[mi], aim[mi]);
-        }
-        for (int ni = 0; ni < 2; ++ni) {
-            const int col = wc + ni * 16 + li;
-            bre[ni] = bim[ni] = 0;
-            if (col < ncols && t < kw) M::load(A + (size_t)(k0 + t) * lda + col0 + col, bre[ni], bim[ni]);
-        }
-        for (int mi = 0; mi < 2; ++mi)
-            for (int ni = 0; ni < 2; ++ni) {
-                cre[mi][ni] = M::mma(are[mi], bre[ni], cre[mi][ni]);
-                cre[mi][ni] = M::mma(-aim[mi], bim[ni], cre[mi][ni]);
-                cim[mi][ni] = M::mma(are[mi], bim[ni], cim[mi][ni]);
-                cim[mi][ni] = M::mma(aim[mi], bre[ni], cim[mi][ni]);
+    const S *lp = Lp + (size_t)rb0 * 64 + lane;
+    const S *up = Up + (size_t)cb0 * 64 + lane;
+    const size_t lstride = (size_t)rb_pad * 64, ustride = (size_t)cb_pad * 64;
+    S a[MB], b[NB];
+    for (int i = 0; i < MB; ++i) a[i] = lp[i * 64];
+    for (int j = 0; j < NB; ++j) b[j] = up[j * 64];
+    for (int ks = 0; ks < ks_count; ++ks) {
+        // prefetch the next k step (the packed copies hold one spare step)
+        lp += lstride;
+        up += ustride;
+        S an[MB], bn[NB];
+        for (int i = 0; i < MB; ++i) an[i] = lp[i * 64];
+        for (int j = 0; j < NB; ++j) bn[j] = up[j * 64];
+        if constexpr (M::cplx) {
+            // (ar + j ai)(br + j bi): the two products of each accumulator
+            // are issued in separate sweeps to keep dependent MFMAs apart
+            for (int i = 0; i < MB; ++i)
+                for (int j = 0; j < NB; ++j) {
+                    cre[i][j] = M::mma((op)a[i].x, (op)b[j].x, cre[i][j]);
+                    cim[i][j] = M::mma((op)a[i].x, (op)b[j].y, cim[i][j]);
+                }
+            for (int i = 0; i < MB; ++i) {
+                const op nai = -(op)a[i].y;
+                for (int j = 0; j < NB; ++j) {
+                    cre[i][j] = M::mma(nai, (op)b[j].y, cre[i][j]);
+                    cim[i][j] = M::mma((op)a[i].y, (op)b[j].x, cim[i][j]);
+                }
             }
+        } else {
+            for (int i = 0; i < MB; ++i)
+                for (int j = 0; j < NB; ++j) cre[i][j] = M::mma((op)a[i], (op)b[j], cre[i][j]);
+        }
+        for (int i = 0; i < MB; ++i) a[i] = an[i];
+        for (int j = 0; j < NB; ++j) b[j] = bn[j];
     }
-    for (int mi = 0; mi < 2; ++mi)
-        for (int ni = 0; ni < 2; ++ni) {
-            const int col = wc + ni * 16 + li;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int i = 0; i < MB; ++i)
+        for (int j = 0; j < NB; ++j) {
+            const int col = (cb0 + j) * 16 + li;
             if (col >= ncols) continue;
             for (int v = 0; v < 4; ++v) {
-                const int row = wr + mi * 16 + M::row(lk, v);
+                const int row = (rb0 + i) * 16 + M::row(lk, v);
                 if (row >= m) continue;
                 S *c = A + (size_t)(row0 + row) * lda + col0 + col;
-                S cv = *c;
-                cv.x -= cre[mi][ni][v];
-                cv.y -= cim[mi][ni][v];
-                *c = cv;
+                if constexpr (M::cplx) {
+                    S cv = *c;
+                    cv.x -= cre[i][j][v];
+                    cv.y -= cim[i][j][v];
+                    *c = cv;
+                } else {
+                    *c -= cre[i][j][v];
+                }
             }
         }
 }
 
-__global__ __launch_bounds__(256) void gemm_real_kernel(double *A, size_t lda, int row0, int col0, int m,
-                                                        int ncols, int k0, int kw, const int *status) {
-    if (status[0] != 0) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int wc = blockIdx.x * 64 + (wave & 1) * 32;
-    v4d c[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) c[a][b] = v4d{0, 0, 0, 0};
-    const int li = lane & 15, lk = lane >> 4;
-    for (int kk = 0; kk < kw; kk += 4) {
-        const int t = kk + lk;
-        double a[2], b[2];
-        for (int mi = 0; mi < 2; ++mi) {
-            const int row = wr + mi * 16 + li;
-            a[mi] = (row < m && t < kw) ? A[(size_t)(row0 + row) * lda + k0 + t] : 0.0;
-        }
-        for (int ni = 0; ni < 2; ++ni) {
-            const int col = wc + ni * 16 + li;
-            b[ni] = (col < ncols && t < kw) ? A[(size_t)(k0 + t) * lda + col0 + col] : 0.0;
-        }
-        for (int mi = 0; mi < 2; ++mi)
-            for (int ni = 0; ni < 2; ++ni) c[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[ni], c[mi][ni], 0, 0, 0);
+// Packed-operand buffer sizes (elements) for a trailing update of m rows
+// and ncols columns with k width kw (one spare k step for the prefetch).
+template <class S>
+void gemm_pads(int m, int ncols, int &rb_pad, int &cb_pad) {
+    constexpr int BM = 32 * kWaveRowBlocks;
+    int nbk = 4;
+    if constexpr (std::is_same<S, double2>::value) nbk = 2;
+    const int BN = 32 * nbk;
+    rb_pad = (m + BM - 1) / BM * (BM / 16);
+    cb_pad = (ncols + BN - 1) / BN * (BN / 16);
+}
+
+// Pack the operands of A[row0.., col0..] -= L21 U12 (m rows, ncols columns,
+// k width kw) into the context's buffers; returns the chunk counts.
+template <class S>
+void pack_operands(S *A, size_t lda, int row0, int col0, int m, int ncols, int k0, int kw, const int *status,
+                   LUContext &ctx, hipStream_t s, int &rb_pad, int &cb_pad) {
+    gemm_pads<S>(m, ncols, rb_pad, cb_pad);
+    const int ks_count = (kw + 3) / 4;
+    const size_t nl = (size_t)(ks_count + 1) * rb_pad * 64, nu = (size_t)(ks_count + 1) * cb_pad * 64;
+    ctx.pack_l.reserve(nl * sizeof(S));
+    ctx.pack_u.reserve(nu * sizeof(S));
+    S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
+    S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
+    const size_t tl = (size_t)ks_count * rb_pad * 64, tu = (size_t)ks_count * cb_pad * 64;
+    hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, A, lda, row0, m, k0,
+                       kw, rb_pad, Lp, status);
+    hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu + 255) / 256)), dim3(256), 0, s, A, lda, k0, kw, col0,
+                       ncols, cb_pad, Up, status);
+}
+
+// C = A[row0.., col0 + c_off ..] (m x nc) -= L21 U12[:, c_off .. c_off + nc)
+// from the packed operands; c_off is a multiple of the tile width.
+template <class S>
+void gemm_packed(S *A, size_t lda, int row0, int col0, int m, int c_off, int nc, int kw, int rb_pad, int cb_pad,
+                 const int *status, LUContext &ctx, hipStream_t s) {
+    if (m <= 0 || nc <= 0) return;
+    const int ks_count = (kw + 3) / 4;
+    const S *Lp = reinterpret_cast<const S *>(ctx.pack_l.ptr);
+    const S *Up = reinterpret_cast<const S *>(ctx.pack_u.ptr) + (size_t)(c_off / 16) * 64;
+    int rb_t, cb_t;
+    gemm_pads<S>(m, nc, rb_t, cb_t);
+    const int tiles = (rb_t / (2 * kWaveRowBlocks)) * (cb_t / (std::is_same<S, double2>::value ? 4 : 8));
+    const int grid = (tiles + 7) / 8 * 8;
+    const int tg = ctx.timer ? ctx.timer->start(s) : -1;
+    hipLaunchKernelGGL(gemm_packed_kernel<S>, dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
+                       ks_count, Lp, Up, rb_pad, cb_pad, status);
+    if (ctx.timer) {
+        const double mnk = (double)m * nc * kw;
+        ctx.timer->stop(tg, s, kKernelLuUpdate, (Elem<S>::cplx ? 8.0 : 2.0) * mnk,
+                        (2.0 * m * nc + (double)(m + nc) * kw) * sizeof(S));
     }
-    for (int mi = 0; mi < 2; ++mi)
-        for (int ni = 0; ni < 2; ++ni) {
-            const int col = wc + ni * 16 + li;
-            if (col >= ncols) continue;
-            for (int v = 0; v < 4; ++v) {
-                const int row = wr + mi * 16 + 4 * v + lk;  // FP64 accumulator layout
-                if (row < m) A[(size_t)(row0 + row) * lda + col0 + col] -= c[mi][ni][v];
-            }
-        }
 }
 
 template <class S>
 void launch_gemm(S *A, size_t lda, int row0, int col0, int m, int ncols, int k0, int kw, const int *status,
-                 hipStream_t s) {
-    dim3 grid((ncols + 63) / 64, (m + 63) / 64);
-    if constexpr (Elem<S>::cplx)
-        hipLaunchKernelGGL(gemm_complex_kernel<S>, grid, dim3(256), 0, s, A, lda, row0, col0, m, ncols, k0, kw, status);
-    else
-        hipLaunchKernelGGL(gemm_real_kernel, grid, dim3(256), 0, s, A, lda, row0, col0, m, ncols, k0, kw, status);
+                 LUContext &ctx, hipStream_t s) {
+    int rb_pad, cb_pad;
+    pack_operands<S>(A, lda, row0, col0, m, ncols, k0, kw, status, ctx, s, rb_pad, cb_pad);
+    gemm_packed<S>(A, lda, row0, col0, m, 0, ncols, kw, rb_pad, cb_pad, status, ctx, s);
 }
 
 // Self-test of the 16x16x4 MFMA operand layouts used by the GEMM kernels:
@@ -612,7 +929,9 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
     hipLaunchKernelGGL(fill_uint_kernel, dim3((unsigned)((kBarWords + 255) / 256)), dim3(256), 0, s, ctx.barrier.ptr,
                        (int)kBarWords, 0u);
     HIP_CHECK(hipGetLastError());
-    const size_t lds_budget = 60 * 1024;
+    // dynamic LDS of the panel kernel: the device's per-workgroup limit less
+    // the kernel's static arrays
+    const size_t lds_budget = (size_t)ctx.lds_per_block - 8 * 1024;
     int k0 = 0;
     while (k0 < n) {
         const int m = n - k0;
@@ -640,24 +959,38 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         ctx.round += (unsigned)w + 1u;
         double lim = pivot_limit;
         void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &mags, &tags, &d_ipiv, &round0, &lim, &ctx.status.ptr};
+        if (shm > 64 * 1024)
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(panel_kernel<S>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         const int tp = ctx.timer ? ctx.timer->start(s) : -1;
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(panel_kernel<S>), dim3(G), dim3(256),
                                              args, (unsigned)shm, s));
         if (ctx.timer) ctx.timer->stop(tp, s, kKernelPanel, 0.0, 2.0 * m * w * sizeof(S));
-        if (n - w > 0)
-            hipLaunchKernelGGL(laswp_kernel<S>, dim3((n - w + 255) / 256), dim3(256), 0, s, dA, lda, n, k0, w,
-                               d_ipiv, ctx.status.ptr);
+        if (n - w > 0) {
+            ctx.swap_stage.reserve((size_t)2 * w * n * sizeof(S));
+            S *stage = reinterpret_cast<S *>(ctx.swap_stage.ptr);
+            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(256), 0, s, d_ipiv, k0, w, ctx.swap_plan.ptr,
+                               ctx.status.ptr);
+            const dim3 sg((unsigned)((n - w + 255) / 256), (unsigned)(2 * w));
+            hipLaunchKernelGGL(swap_gather_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, k0, w, ctx.swap_plan.ptr,
+                               stage, ctx.status.ptr);
+            hipLaunchKernelGGL(swap_scatter_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, k0, w, ctx.swap_plan.ptr,
+                               stage, ctx.status.ptr);
+        }
         const int rest = n - k0 - w;
         if (rest > 0) {
-            hipLaunchKernelGGL(trsm_kernel<S>, dim3((rest + 31) / 32), dim3(256), 0, s, dA, lda, n, k0, w,
-                               ctx.status.ptr);
-            const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-            launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, s);
-            if (ctx.timer) {
-                const double mnk = (double)rest * rest * w;
-                ctx.timer->stop(tg, s, kKernelLuUpdate, (Elem<S>::cplx ? 8.0 : 2.0) * mnk,
-                                (2.0 * rest * rest + 2.0 * rest * w) * sizeof(S));
+            const size_t tshm = trsm_lds_elems(w) * sizeof(S);
+            if (tshm > (size_t)ctx.lds_per_block) {
+                out.code = -1;
+                PEEC_ERR("LU block of %d columns needs %zu bytes of LDS per workgroup.", w, tshm);
+                return out;
             }
+            if (tshm > 64 * 1024)
+                HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(trsm_kernel<S>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)tshm));
+            hipLaunchKernelGGL(trsm_kernel<S>, dim3((rest + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC)), dim3(256),
+                               tshm, s, dA, lda, n, k0, w, ctx.status.ptr);
+            launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s);
         }
         HIP_CHECK(hipGetLastError());
         k0 += w;
@@ -673,6 +1006,229 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
 template LUStatus lu_factor<double2>(double2 *, int, size_t, int *, double, LUContext &, int);
 template LUStatus lu_factor<float2>(float2 *, int, size_t, int *, double, LUContext &, int);
 template LUStatus lu_factor<double>(double *, int, size_t, int *, double, LUContext &, int);
+
+// ---------------------------------------------------------------------------
+// factorization with a given row order (no pivot search)
+//
+// The rows have been permuted beforehand (convert_rows_to_f32) into the order
+// a previous partial-pivoting factorization chose; the factor is then a
+// plain blocked LU: the w x w diagonal block is factored by one workgroup in
+// LDS, L21 = A21 U11^{-1} and U12 = L11^{-1} A12 are formed in parallel, and
+// the trailing update is the same MFMA GEMM. There is no cross-workgroup
+// exchange. A pivot with |u_kk| <= limit stops it with status -2.
+// Block width w <= 128 (the 32 x 32 tile grid of diag_lu_kernel).
+
+// One workgroup of 1024 threads; thread (ti, tc) keeps the 4 x 4 tile at
+// rows 4 ti.., columns 4 tc.. of the w x w block in registers (FP64). Per
+// column j the owners of row j publish it to LDS, the owners of column j
+// form the multipliers, and every thread updates its tile: two barriers per
+// column, double-buffered LDS rows.
+template <class S>
+__global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0, int w, double limit, int *status) {
+    using T = typename Work<S>::T;
+    __shared__ T urow[2][kMaxNB], lcol[2][kMaxNB];
+    if (status[0] != 0) return;
+    const int tid = threadIdx.x, ti = tid >> 5, tc = tid & 31;
+    Regs<T, 16> a;  // element (r, c) at 4 r + c
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * ti + r, cc = 4 * tc + c;
+            a.set(4 * r + c, (i < w && cc < w) ? wld(A + (size_t)(k0 + i) * lda + k0 + cc) : wzero<T>());
+        }
+    for (int j = 0; j < w; ++j) {
+        const int buf = j & 1, jt = j >> 2, jr = j & 3;
+        if (ti == jt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (r == jr) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) urow[buf][4 * tc + c] = a.get(4 * r + c);
+                }
+        }
+        __syncthreads();
+        const T u = urow[buf][j];
+        const double m = wmag(u);
+        if (!(m > limit) || !isfinite(m)) {
+            if (tid == 0) {
+                status[0] = -2;
+                status[1] = k0 + j;
+            }
+            return;
+        }
+        if (tc == jt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * ti + r;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c == jr) {
+                        const T l = wdiv(a.get(4 * r + c), u);
+                        if (i > j) a.set(4 * r + c, l);
+                        lcol[buf][i] = l;
+                    }
+            }
+        }
+        __syncthreads();
+        // tiles entirely above row j or left of column j are final
+        if (4 * ti + 3 <= j || 4 * tc + 3 <= j) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 4 * ti + r;
+            const T l = lcol[buf][i];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int cc = 4 * tc + c;
+                const T cur = a.get(4 * r + c);
+                const T upd = wfms(cur, l, urow[buf][cc]);
+                a.set(4 * r + c, (i > j && cc > j) ? upd : cur);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * ti + r, cc = 4 * tc + c;
+            if (i < w && cc < w) wst(A + (size_t)(k0 + i) * lda + k0 + cc, a.get(4 * r + c));
+        }
+}
+
+// L21 = A21 U11^{-1}: eight lanes per row, lane `sub` keeping columns
+// 16 sub .. 16 sub + 15 in registers; U11 staged in LDS (full block).
+template <class S>
+__global__ __launch_bounds__(256) void trsm_right_kernel(S *A, size_t lda, int n, int k0, int w, const int *status) {
+    using T = typename Work<S>::T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    S *blk = reinterpret_cast<S *>(smem_raw);
+    if (status[0] != 0) return;
+    {
+        const int total = w * w;
+        for (int base = threadIdx.x; base < total; base += 256 * 8) {
+            Regs<T, 8> v;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * 256;
+                v.set(u, idx < total ? wld(A + (size_t)(k0 + idx / w) * lda + k0 + idx % w) : wzero<T>());
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (base + u * 256 < total) wst(blk + base + u * 256, v.get(u));
+        }
+    }
+    const int lane = threadIdx.x & 63, sub = lane & (kTrsmTPC - 1);
+    const int row = k0 + w + (int)blockIdx.x * (256 / kTrsmTPC) + (int)threadIdx.x / kTrsmTPC;
+    const bool valid = row < n;
+    const int base = lane & ~(kTrsmTPC - 1);
+    Regs<T, kTrsmRPT> a;
+#pragma unroll
+    for (int t = 0; t < kTrsmRPT; ++t) {
+        const int c = sub * kTrsmRPT + t;
+        a.set(t, (valid && c < w) ? wld(A + (size_t)row * lda + k0 + c) : wzero<T>());
+    }
+    __syncthreads();
+    const int nblk = (w + kTrsmRPT - 1) / kTrsmRPT;
+    for (int cb = 0; cb < nblk; ++cb) {
+#pragma unroll
+        for (int cc = 0; cc < kTrsmRPT; ++cc) {
+            const int jj = cb * kTrsmRPT + cc;
+            const int j = min(jj, w - 1);
+            const T x = wshfl(wdiv(a.get(cc), wld(blk + j * w + j)), base + cb);
+            if (sub == cb && jj < w) a.set(cc, x);
+            if (sub * kTrsmRPT + kTrsmRPT - 1 <= j) continue;
+            // branch-free inside, in-bounds reads; the update is kept for
+            // columns right of j
+#pragma unroll
+            for (int t = 0; t < kTrsmRPT; ++t) {
+                const int c = sub * kTrsmRPT + t;
+                const T cur = a.get(t);
+                const T upd = wfms(cur, x, wld(blk + j * w + min(c, w - 1)));
+                a.set(t, (c > j && c < w && jj < w) ? upd : cur);
+            }
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int t = 0; t < kTrsmRPT; ++t) {
+            const int c = sub * kTrsmRPT + t;
+            if (c < w) wst(A + (size_t)row * lda + k0 + c, a.get(t));
+        }
+    }
+}
+
+static void set_dynamic_lds(const void *fn, size_t bytes) {
+    if (bytes > 64 * 1024) HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+// Look-ahead: after the panel's triangular solves, the next block column is
+// updated first and its diagonal block is factored on the context's second
+// stream while the rest of the trailing update runs on the first; the next
+// panel waits for that factor through an event.
+LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx, int block_size) {
+    using S = float2;
+    hipStream_t s = ctx.stream, aux = ctx.aux;
+    LUStatus out;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2, 0);
+    int nb = block_size > 0 ? std::min(block_size, kMaxNB) : kMaxNB;
+    while (nb > 16 && (size_t)nb * nb * sizeof(S) > (size_t)ctx.lds_per_block) nb /= 2;
+    set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_kernel<S>), (size_t)nb * nb * sizeof(S));
+    set_dynamic_lds(reinterpret_cast<const void *>(trsm_kernel<S>), trsm_lds_elems(nb) * sizeof(S));
+    auto diag = [&](int k0, int w, hipStream_t st) {
+        const int tp = ctx.timer ? ctx.timer->start(st) : -1;
+        hipLaunchKernelGGL(diag_lu_kernel<S>, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
+                           ctx.status.ptr);
+        if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
+    };
+    diag(0, std::min(nb, n), s);
+    bool pending = false;  // a diagonal factor is in flight on aux
+    for (int k0 = 0; k0 < n; k0 += nb) {
+        const int w = std::min(nb, n - k0), rest = n - k0 - w;
+        if (pending) {
+            HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
+            pending = false;
+        }
+        if (rest <= 0) break;
+        const unsigned rgrid = (unsigned)((rest + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC));
+        hipLaunchKernelGGL(trsm_right_kernel<S>, dim3(rgrid), dim3(256), (size_t)w * w * sizeof(S), s, dA, lda, n,
+                           k0, w, ctx.status.ptr);
+        hipLaunchKernelGGL(trsm_kernel<S>, dim3(rgrid), dim3(256), trsm_lds_elems(w) * sizeof(S), s, dA, lda, n, k0,
+                           w, ctx.status.ptr);
+        int rb_pad, cb_pad;
+        pack_operands<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s, rb_pad, cb_pad);
+        const int w2 = std::min(nb, rest);
+        gemm_packed<S>(dA, lda, k0 + w, k0 + w, rest, 0, w2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+        HIP_CHECK(hipEventRecord(ctx.ev_next, s));
+        HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
+        diag(k0 + w, w2, aux);
+        HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
+        pending = true;
+        gemm_packed<S>(dA, lda, k0 + w, k0 + w, rest, w2, rest - w2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (pending) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
+    int st[2] = {0, 0};
+    ctx.status.download(st, 2, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    out.code = st[0];
+    out.column = st[1];
+    return out;
+}
+
+__global__ void to_f32_rows_kernel(const double2 *src, float2 *dst, const int *perm, int n) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)n * n) return;
+    const int i = (int)(idx / n), c = (int)(idx % n);
+    const double2 v = src[(size_t)perm[i] * n + c];
+    dst[idx] = make_float2((float)v.x, (float)v.y);
+}
+
+void convert_rows_to_f32(const double2 *src, float2 *dst, const int *d_perm, int n, hipStream_t s) {
+    const size_t total = (size_t)n * n;
+    hipLaunchKernelGGL(to_f32_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, d_perm,
+                       n);
+    HIP_CHECK(hipGetLastError());
+}
 
 // ---------------------------------------------------------------------------
 // solves
@@ -767,17 +1323,73 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const S *M, size_t lda, V
     }
 }
 
+constexpr int kChainTB = 32;
+
+// Inverses of the 32 x 32 diagonal blocks of a factor, in FP64: the
+// unit-lower L part (to linv) and the upper U part (to uinv), both
+// identity-padded past row n. One workgroup per block; lanes 0-31 form the
+// columns of L^{-1}, lanes 32-63 those of U^{-1}. A zero or non-finite
+// diagonal of U sets bad |= 1.
+template <class S, class V>
+__global__ __launch_bounds__(64) void diag_inverse_kernel(const S *M, size_t lda, int n,
+                                                          typename VecT<V>::A *linv, typename VecT<V>::A *uinv,
+                                                          int *bad) {
+    using T = typename VecT<V>::A;
+    constexpr int TB = kChainTB;
+    __shared__ T blk[TB][TB];
+    __shared__ T inv[2][TB][TB];
+    const int b = blockIdx.x, r0 = b * TB, rows = min(TB, n - r0);
+    T one = ezero<T>();
+    if constexpr (Elem<V>::cplx) one.x = 1.0; else one = 1.0;
+    for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
+        const int i = idx / TB, j = idx % TB;
+        blk[i][j] = (i < rows && j < rows) ? ld(M + (size_t)(r0 + i) * lda + r0 + j) : (i == j ? one : ezero<T>());
+    }
+    __syncthreads();
+    const int j = threadIdx.x & (TB - 1);
+    if (threadIdx.x < TB) {
+        for (int i = 0; i < j; ++i) inv[0][i][j] = ezero<T>();
+        inv[0][j][j] = one;
+        for (int i = j + 1; i < TB; ++i) {
+            T acc = ezero<T>();
+            for (int t = j; t < i; ++t) acc = eadd(acc, emul(blk[i][t], inv[0][t][j]));
+            inv[0][i][j] = esub(ezero<T>(), acc);
+        }
+    } else {
+        const T d = blk[j][j];
+        if constexpr (Elem<V>::cplx) {
+            const double den = d.x * d.x + d.y * d.y;
+            if (!isfinite(den) || den == 0.0) atomicOr(bad, 1);
+        } else {
+            if (d == 0.0 || !isfinite(d)) atomicOr(bad, 1);
+        }
+        for (int i = j + 1; i < TB; ++i) inv[1][i][j] = ezero<T>();
+        inv[1][j][j] = ediv(one, d);
+        for (int i = j - 1; i >= 0; --i) {
+            T acc = ezero<T>();
+            for (int t = i + 1; t <= j; ++t) acc = eadd(acc, emul(blk[i][t], inv[1][t][j]));
+            inv[1][i][j] = ediv(esub(ezero<T>(), acc), blk[i][i]);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
+        linv[(size_t)b * TB * TB + idx] = inv[0][idx / TB][idx % TB];
+        uinv[(size_t)b * TB * TB + idx] = inv[1][idx / TB][idx % TB];
+    }
+}
+
 // Triangular solve in one cooperative launch: workgroup w owns the 32-row
 // block b (b = w forward, b = nblk-1-w backward). It accumulates the
 // off-diagonal products of every block it depends on as soon as that block's
-// flag shows it final, then solves its diagonal block from LDS, writes its
-// rows and raises its flag. Up to 8 right-hand sides per launch; all
-// arithmetic in FP64 (FP32 factors are widened on load).
-constexpr int kChainTB = 32;
+// flag shows it final, then applies the precomputed inverse of its diagonal
+// block (one 32 x 32 product instead of 32 dependent substitution steps),
+// writes its rows and raises its flag. Up to 8 right-hand sides per launch;
+// all arithmetic in FP64 (FP32 factors are widened on load).
 
 template <class S, class V, bool UPPER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, V *X, int n, int nrhs,
-                                                         int *flags, int epoch, int *bad) {
+                                                         const typename VecT<V>::A *dinv, int *flags, int epoch,
+                                                         int *bad) {
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
     __shared__ T dblk[TB][TB];
@@ -788,13 +1400,11 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
     const int r0 = b * TB, rows = min(TB, n - r0);
     const int tid = threadIdx.x;
     const int r = tid % TB, cg = tid / TB;  // row in block, column group (8 groups of 4 columns)
-    T acc[8];
-    for (int q = 0; q < 8; ++q) acc[q] = ezero<T>();
-    // diagonal block into LDS
-    for (int idx = tid; idx < TB * TB; idx += 256) {
-        const int i = idx / TB, j = idx % TB;
-        dblk[i][j] = (i < rows && j < rows) ? ld(M + (size_t)(r0 + i) * lda + r0 + j) : ezero<T>();
-    }
+    Regs<T, 8> acc;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc.set(q, ezero<T>());
+    // inverse of the diagonal block into LDS
+    for (int idx = tid; idx < TB * TB; idx += 256) dblk[idx / TB][idx % TB] = dinv[(size_t)b * TB * TB + idx];
     // off-diagonal contributions, in the order the blocks become final; each
     // round takes every dependency already final (at least one)
     __shared__ int s_ready, s_abort;
@@ -834,18 +1444,22 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
                 const S *mrow = M + (size_t)(r0 + r) * lda + c0;
                 for (int c = cg * 4; c < cg * 4 + 4 && c < cols; ++c) {
                     const T m = ld(mrow + c);
-                    for (int q = 0; q < nrhs; ++q) {
-                        T xv;
-                        sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + c0 + c), xv);
-                        acc[q] = eadd(acc[q], emul(m, xv));
-                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q < nrhs) {
+                            T xv;
+                            sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + c0 + c), xv);
+                            acc.set(q, eadd(acc.get(q), emul(m, xv)));
+                        }
                 }
             }
         }
         __syncthreads();
     }
     // reduce the 8 column groups; z = x_b - sum
-    for (int q = 0; q < nrhs; ++q) red[q][cg][r] = acc[q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (q < nrhs) red[q][cg][r] = acc.get(q);
     __syncthreads();
     if (tid < nrhs * TB) {
         const int q = tid / TB, i = tid % TB;
@@ -854,31 +1468,13 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
         z[q][i] = i < rows ? esub(X[(size_t)q * n + r0 + i], sum) : ezero<T>();
     }
     __syncthreads();
-    // diagonal block: unit lower (forward) or upper with division (backward)
-    for (int step = 0; step < rows; ++step) {
-        const int i = UPPER ? rows - 1 - step : step;
-        if (UPPER && tid < nrhs) {
-            const T d = dblk[i][i];
-            if constexpr (Elem<V>::cplx) {
-                const double den = d.x * d.x + d.y * d.y;
-                if (!isfinite(den) || den == 0.0) atomicOr(bad, 1);
-            } else {
-                if (d == 0.0 || !isfinite(d)) atomicOr(bad, 1);
-            }
-            z[tid][i] = ediv(z[tid][i], d);
-        }
-        __syncthreads();
-        if (tid < nrhs * TB) {
-            const int q = tid / TB, k = tid % TB;
-            if (UPPER ? (k < i) : (k > i && k < rows)) z[q][k] = esub(z[q][k], emul(dblk[k][i], z[q][i]));
-        }
-        __syncthreads();
-    }
-    // publish the solved rows with cache-bypassing stores, wait for them to
-    // complete, then raise the flag (no fences needed)
+    // x_b = D^{-1} z; publish the solved rows with cache-bypassing stores,
+    // wait for them to complete, then raise the flag (no fences needed)
     if (tid < nrhs * TB) {
         const int q = tid / TB, i = tid % TB;
-        if (i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), z[q][i]);
+        T xv = ezero<T>();
+        for (int k = 0; k < TB; ++k) xv = eadd(xv, emul(dblk[i][k], z[q][k]));
+        if (i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), xv);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -888,8 +1484,8 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
 // one triangular solve through the chain kernel; false if the device cannot
 // keep every block resident (caller falls back to the blocked path)
 template <class S, class V, bool UPPER>
-bool chain_solve(const S *dLU, int n, size_t lda, V *X, int nrhs, int *flags, int epoch, int *bad,
-                 const LUContext &ctx) {
+bool chain_solve(const S *dLU, int n, size_t lda, V *X, int nrhs, const typename VecT<V>::A *dinv, int *flags,
+                 int epoch, int *bad, const LUContext &ctx) {
     const int nblk = (n + kChainTB - 1) / kChainTB;
     int per_cu = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -899,7 +1495,7 @@ bool chain_solve(const S *dLU, int n, size_t lda, V *X, int nrhs, int *flags, in
         V *Xq = X + (size_t)q0 * n;
         int nq = std::min(8, nrhs - q0), nn = n, ep = epoch + q0;
         size_t ld = lda;
-        void *args[] = {(void *)&dLU, &ld, &Xq, &nn, &nq, &flags, &ep, &bad};
+        void *args[] = {(void *)&dLU, &ld, &Xq, &nn, &nq, (void *)&dinv, &flags, &ep, &bad};
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER>), dim3(nblk),
                                              dim3(256), args, 0, ctx.stream));
     }
@@ -939,8 +1535,15 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.solve_bad.ptr, 1, 0);
     const int ep = ctx.epoch + 1;
     ctx.epoch += 8 * ((nrhs + 7) / 8) + 1;
-    bool chained = chain_solve<S, V, false>(dLU, n, lda, tmp, nrhs, ctx.flags.ptr, ep, ctx.solve_bad.ptr, ctx) &&
-                   chain_solve<S, V, true>(dLU, n, lda, tmp, nrhs, ctx.flags.ptr + nblk, ep, ctx.solve_bad.ptr, ctx);
+    using T = typename VecT<V>::A;
+    ctx.diag_inv.reserve((size_t)2 * nblk * kChainTB * kChainTB * sizeof(T));
+    T *linv = reinterpret_cast<T *>(ctx.diag_inv.ptr);
+    T *uinv = linv + (size_t)nblk * kChainTB * kChainTB;
+    hipLaunchKernelGGL((diag_inverse_kernel<S, V>), dim3(nblk), dim3(64), 0, s, dLU, lda, n, linv, uinv,
+                       ctx.solve_bad.ptr);
+    bool chained =
+        chain_solve<S, V, false>(dLU, n, lda, tmp, nrhs, linv, ctx.flags.ptr, ep, ctx.solve_bad.ptr, ctx) &&
+        chain_solve<S, V, true>(dLU, n, lda, tmp, nrhs, uinv, ctx.flags.ptr + nblk, ep, ctx.solve_bad.ptr, ctx);
     if (!chained) {
         const int T = 128;
         const int nb2 = (n + T - 1) / T;

@@ -28,12 +28,13 @@ class PeecEngineOptions(C.Structure):
                 ("retarded", C.c_int), ("smooth_order", C.c_int), ("near_smooth_order", C.c_int),
                 ("block_size", C.c_int),
                 ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
-                ("backward_error_gate", C.c_double)]
+                ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int)]
 
 
 class PeecPointReport(C.Structure):
     _fields_ = [("frequency", C.c_double), ("backward_error", C.c_double),
                 ("refinement_steps", C.c_int), ("used_fp64_fallback", C.c_int),
+                ("pivot_reuse", C.c_int), ("pivot_refactor", C.c_int), ("reuse_backward_error", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("ms_check", C.c_double)]
 

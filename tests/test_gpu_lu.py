@@ -81,9 +81,9 @@ def test_complex_lu_random_against_oracle(require_gpu, n):
         assert eta < 1e-13
 
 
-def test_real_lu_random(require_gpu):
+@pytest.mark.parametrize("n", [300, 1030])
+def test_real_lu_random(require_gpu, n):
     rng = np.random.default_rng(5)
-    n = 300
     A = rng.standard_normal((n, n))
     b = rng.standard_normal(n)
     st, X, _ = plib.real_lu_factor_solve(A, b)
@@ -199,3 +199,35 @@ def test_engine_retarded_box_mixed_vs_fp64_vs_oracle(require_gpu, frequency):
     assert abs(z64[0] - z_o) <= 1e-9 * abs(z_o)
     assert abs(zm[0] - z64[0]) <= 1e-10 * abs(z64[0])
     assert r64.backward_error <= 1e-10 and rm.backward_error <= 1e-10
+
+
+def test_engine_mixed_sweep_reuses_row_order(require_gpu):
+    """Mixed-precision points after the first factor in the row order of the
+    previous partial-pivoting factor (no pivot search); every point's
+    impedance must still match the FP64 LU to 1e-10 with the backward error
+    at or below 1e-10, whether the reused order was accepted or redone."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 10, 6, 6)
+    port = (0, len(xyz) - 1)
+    freqs = [1e4, 1.1e4, 1e5, 1e6, 1e7, 1e8, 2e8]
+    with plib.Model(xyz, quads) as m:
+        eng = plib.Engine(m, retarded=True)
+        z64 = [eng.solve(f, [port], plib.Engine.FP64)[1][0] for f in freqs]
+        eng.close()
+        eng = plib.Engine(m, retarded=True)
+        reports, zm = [], []
+        for f in freqs:
+            st, z, _, _, rep = eng.solve(f, [port], plib.Engine.MIXED)
+            assert st == 0
+            zm.append(z[0])
+            reports.append((rep.pivot_reuse, rep.pivot_refactor, rep.refinement_steps, rep.backward_error,
+                            rep.used_fp64_fallback, rep.reuse_backward_error))
+        eng.close()
+    for f, a, b, r in zip(freqs, zm, z64, reports):
+        print(f"f={f:g} mixed {a:.12e} fp64 {b:.12e} rel={abs(a - b) / abs(b):.2e} reuse/refactor/ir/eta/fb/eta_reused={r}",
+              flush=True)
+        assert abs(a - b) <= 1e-10 * abs(b)
+        assert r[3] <= 1e-10
+    assert reports[0][0] == 0  # the first point has no order to reuse
+    assert all(r[0] == 1 for r in reports[1:])
+    # the reused order is kept at most points of the sweep
+    assert sum(1 for r in reports[1:] if not r[1]) >= len(freqs) // 2
