@@ -197,7 +197,7 @@ struct PeecEngine {
     DeviceBuffer<int> d_reuse_perm;  // row order of the last partial-pivoting factor
     bool have_reuse = false;
     DeviceBuffer<double> d_L, d_P, d_row, d_col, d_terms;
-    DeviceBuffer<double2> d_Lw, d_Pw, d_Z, d_Z0, d_B, d_X, d_R;
+    DeviceBuffer<double2> d_Lw, d_Pw, d_Z, d_Z0, d_B, d_X, d_R, d_Xbest;
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
@@ -428,25 +428,46 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             return summary[2];
         };
         // solve with the FP32 factor and refine against the FP64 matrix:
-        // x <- x - A^{-1} r, r = A x - b, until the target, stagnation or the
-        // step cap; returns the worst backward error over the right-hand sides
+        // x <- x - A^{-1} r, r = A x - b, until the target or the step cap.
+        // The component-wise backward error need not fall at every step (the
+        // first correction often barely changes it at low frequencies), so
+        // refinement stops only after two steps without a new best, and the
+        // best iterate is kept. Returns the worst backward error over the
+        // right-hand sides.
+        e->d_Xbest.resize(n);
         auto solve_refine = [&]() {
             HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
             lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                                e->d_X.ptr, nr, *e->lu);
             double w = 0.0;
             for (int k = 0; k < nr; ++k) {
-                double eta = backward_error(k);
-                double prev = HUGE_VAL;
-                while (eta > e->opt.refinement_target && report.refinement_steps < e->opt.max_refinement_steps &&
-                       eta < prev) {
+                double2 *xk = e->d_X.ptr + (size_t)k * n;
+                double eta = backward_error(k), best = eta;
+                int since_best = 0;
+                bool saved = false;
+                while (eta > e->opt.refinement_target && report.refinement_steps < e->opt.max_refinement_steps) {
                     lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr,
                                                        e->d_col.ptr, e->d_R.ptr + (size_t)k * n, 1, *e->lu);
-                    hipLaunchKernelGGL(axpy_sub_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
-                                       e->d_X.ptr + (size_t)k * n, e->d_R.ptr + (size_t)k * n, (size_t)n);
+                    if (since_best == 0 && !saved) {
+                        // keep the current best before stepping away from it
+                        HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, xk, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+                        saved = true;
+                    }
+                    hipLaunchKernelGGL(axpy_sub_kernel, dim3((n + 255) / 256), dim3(256), 0, s, xk,
+                                       e->d_R.ptr + (size_t)k * n, (size_t)n);
                     ++report.refinement_steps;
-                    prev = eta;
                     eta = backward_error(k);
+                    if (eta < best) {
+                        best = eta;
+                        since_best = 0;
+                        saved = false;
+                    } else if (++since_best >= 2) {
+                        break;
+                    }
+                }
+                if (eta > best) {
+                    HIP_CHECK(hipMemcpyAsync(xk, e->d_Xbest.ptr, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+                    eta = best;
                 }
                 w = std::fmax(w, eta);
             }

@@ -131,6 +131,13 @@ __device__ inline double2 wfms(double2 c, double2 a, double2 b) {
     return make_double2(fma(-a.x, b.x, fma(a.y, b.y, c.x)), fma(-a.x, b.y, fma(-a.y, b.x, c.y)));
 }
 __device__ inline double wfms(double c, double a, double b) { return fma(-a, b, c); }
+__device__ inline float2 wmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ inline double2 wmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ inline double wmul(double a, double b) { return a * b; }
 // a / b by the naive formula (complex_lu.c:162-184)
 __device__ inline float2 wdiv(float2 a, float2 b) {
     const float den = b.x * b.x + b.y * b.y;
@@ -583,16 +590,17 @@ __global__ void swap_scatter_kernel(S *A, size_t lda, int n, int k0, int w, cons
 // U12 = L11^{-1} A12 by forward substitution with the unit-lower L11
 // (complex_lu.c:598-632 order per column). The strictly lower part of L11
 // is staged once per workgroup in LDS (packed by rows, in the storage
-// type). Eight lanes share one column, lane `sub` keeping rows
-// 16 sub .. 16 sub + 15 of it in registers; at step r the owner of row r
+// type). Sixteen lanes share one column, lane `sub` keeping rows
+// 8 sub .. 8 sub + 7 of it in registers; at step r the owner of row r
 // broadcasts x_r with a shuffle and every lane updates its rows below r.
-constexpr int kTrsmTPC = 8;                   // lanes per column
+constexpr int kTrsmTPC = 16;                  // lanes per column
 constexpr int kTrsmRPT = kMaxNB / kTrsmTPC;  // rows per lane
 
 __host__ __device__ constexpr size_t trsm_lds_elems(int w) { return (size_t)w * (w - 1) / 2; }
 
 template <class S>
-__global__ __launch_bounds__(256) void trsm_kernel(S *A, size_t lda, int n, int k0, int w, const int *status) {
+__global__ __launch_bounds__(256) void trsm_kernel(S *A, size_t lda, int c0, int c1, int k0, int w,
+                                                   const int *status) {
     using T = typename Work<S>::T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     S *lw = reinterpret_cast<S *>(smem_raw);  // L11[i][t], t < i, at i(i-1)/2 + t
@@ -619,8 +627,8 @@ __global__ __launch_bounds__(256) void trsm_kernel(S *A, size_t lda, int n, int 
         }
     }
     const int lane = threadIdx.x & 63, sub = lane & (kTrsmTPC - 1);
-    const int col = k0 + w + (int)blockIdx.x * (256 / kTrsmTPC) + (int)threadIdx.x / kTrsmTPC;
-    const bool valid = col < n;
+    const int col = c0 + (int)blockIdx.x * (256 / kTrsmTPC) + (int)threadIdx.x / kTrsmTPC;
+    const bool valid = col < c1;
     const int base = lane & ~(kTrsmTPC - 1);
     Regs<T, kTrsmRPT> a;
 #pragma unroll
@@ -720,19 +728,21 @@ __global__ void pack_l_kernel(const S *A, size_t lda, int row0, int m, int k0, i
     out[idx] = (row < m && k < kw) ? A[(size_t)(row0 + row) * lda + k0 + k] : szero<S>();
 }
 
-// U12 rows [k0, k0 + kw), columns [col0, col0 + ncols) -> chunks [kw/4][cb_pad]
+// U12 rows [k0, k0 + kw), columns [col0, col0 + ncols) -> chunks
+// [kw/4][cb_count] of a buffer whose k steps are cb_stride chunks apart
 template <class S>
-__global__ void pack_u_kernel(const S *A, size_t lda, int k0, int kw, int col0, int ncols, int cb_pad, S *out,
-                              const int *status) {
+__global__ void pack_u_kernel(const S *A, size_t lda, int k0, int kw, int col0, int ncols, int cb_count,
+                              int cb_stride, S *out, const int *status) {
     if (status[0] != 0) return;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = (size_t)((kw + 3) / 4) * cb_pad * 64;
+    const size_t total = (size_t)((kw + 3) / 4) * cb_count * 64;
     if (idx >= total) return;
     const int lane = (int)(idx & 63);
     const size_t chunk = idx >> 6;
-    const int cb = (int)(chunk % cb_pad), ks = (int)(chunk / cb_pad);
+    const int cb = (int)(chunk % cb_count), ks = (int)(chunk / cb_count);
     const int col = cb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
-    out[idx] = (col < ncols && k < kw) ? A[(size_t)(k0 + k) * lda + col0 + col] : szero<S>();
+    out[((size_t)ks * cb_stride + cb) * 64 + lane] =
+        (col < ncols && k < kw) ? A[(size_t)(k0 + k) * lda + col0 + col] : szero<S>();
 }
 
 // Workgroup tile index -> (tile row, tile column). Workgroups are dispatched
@@ -832,6 +842,88 @@ __global__ __launch_bounds__(256, 2) void gemm_packed_kernel(S *A, size_t lda, i
         }
 }
 
+// Narrow update on the look-ahead path: 64 x 64 tiles, the k range split
+// over the four waves of a workgroup and the partial sums added through
+// LDS, so one tile takes a quarter of the sequential MFMA chain.
+template <class S>
+__global__ __launch_bounds__(256) void gemm_splitk_kernel(S *A, size_t lda, int row0, int col0, int m, int ncols,
+                                                          int ks_count, const S *__restrict__ Lp,
+                                                          const S *__restrict__ Up, int rb_pad, int cb_pad,
+                                                          const int *status) {
+    using M = Mfma<S>;
+    using acc_t = typename M::acc;
+    using op = typename M::op;
+    static_assert(M::cplx, "complex storage only");
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    op *part = reinterpret_cast<op *>(smem_raw);  // [wave][re/im][16 blocks][4][64 lanes]
+    if (status[0] != 0) return;
+    const int tiles_n = (ncols + 63) / 64;
+    const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rb0 = tm * 4, cb0 = tn * 4;
+    const int per = (ks_count + 3) / 4, ks0 = wave * per, ks1 = min(ks_count, ks0 + per);
+    acc_t cre[4][4], cim[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            cre[i][j] = acc_t{0, 0, 0, 0};
+            cim[i][j] = acc_t{0, 0, 0, 0};
+        }
+    if (ks0 < ks1) {
+        const S *lp = Lp + ((size_t)ks0 * rb_pad + rb0) * 64 + lane;
+        const S *up = Up + ((size_t)ks0 * cb_pad + cb0) * 64 + lane;
+        const size_t lstride = (size_t)rb_pad * 64, ustride = (size_t)cb_pad * 64;
+        S a[4], b[4];
+        for (int i = 0; i < 4; ++i) a[i] = lp[i * 64];
+        for (int j = 0; j < 4; ++j) b[j] = up[j * 64];
+        for (int ks = ks0; ks < ks1; ++ks) {
+            lp += lstride;
+            up += ustride;
+            S an[4], bn[4];
+            for (int i = 0; i < 4; ++i) an[i] = lp[i * 64];
+            for (int j = 0; j < 4; ++j) bn[j] = up[j * 64];
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    cre[i][j] = M::mma((op)a[i].x, (op)b[j].x, cre[i][j]);
+                    cim[i][j] = M::mma((op)a[i].x, (op)b[j].y, cim[i][j]);
+                }
+            for (int i = 0; i < 4; ++i) {
+                const op nai = -(op)a[i].y;
+                for (int j = 0; j < 4; ++j) {
+                    cre[i][j] = M::mma(nai, (op)b[j].y, cre[i][j]);
+                    cim[i][j] = M::mma((op)a[i].y, (op)b[j].x, cim[i][j]);
+                }
+            }
+            for (int i = 0; i < 4; ++i) a[i] = an[i];
+            for (int j = 0; j < 4; ++j) b[j] = bn[j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int v = 0; v < 4; ++v) {
+                const int e = ((i * 4 + j) * 4 + v) * 64 + lane;
+                part[(size_t)(wave * 2 + 0) * 4096 + e] = cre[i][j][v];
+                part[(size_t)(wave * 2 + 1) * 4096 + e] = cim[i][j][v];
+            }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4096; e += 256) {
+        op re = 0, im = 0;
+        for (int q = 0; q < 4; ++q) {
+            re += part[(size_t)(q * 2 + 0) * 4096 + e];
+            im += part[(size_t)(q * 2 + 1) * 4096 + e];
+        }
+        const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+        const int row = (rb0 + (blk >> 2)) * 16 + M::row(ln >> 4, v);
+        const int col = (cb0 + (blk & 3)) * 16 + (ln & 15);
+        if (row < m && col < ncols) {
+            S *c = A + (size_t)(row0 + row) * lda + col0 + col;
+            S cv = *c;
+            cv.x -= re;
+            cv.y -= im;
+            *c = cv;
+        }
+    }
+}
+
 // Packed-operand buffer sizes (elements) for a trailing update of m rows
 // and ncols columns with k width kw (one spare k step for the prefetch).
 template <class S>
@@ -860,7 +952,7 @@ void pack_operands(S *A, size_t lda, int row0, int col0, int m, int ncols, int k
     hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, A, lda, row0, m, k0,
                        kw, rb_pad, Lp, status);
     hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu + 255) / 256)), dim3(256), 0, s, A, lda, k0, kw, col0,
-                       ncols, cb_pad, Up, status);
+                       ncols, cb_pad, cb_pad, Up, status);
 }
 
 // C = A[row0.., col0 + c_off ..] (m x nc) -= L21 U12[:, c_off .. c_off + nc)
@@ -989,7 +1081,7 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
                 HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(trsm_kernel<S>),
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)tshm));
             hipLaunchKernelGGL(trsm_kernel<S>, dim3((rest + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC)), dim3(256),
-                               tshm, s, dA, lda, n, k0, w, ctx.status.ptr);
+                               tshm, s, dA, lda, k0 + w, n, k0, w, ctx.status.ptr);
             launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s);
         }
         HIP_CHECK(hipGetLastError());
@@ -1026,9 +1118,17 @@ template LUStatus lu_factor<double>(double *, int, size_t, int *, double, LUCont
 template <class S>
 __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0, int w, double limit, int *status) {
     using T = typename Work<S>::T;
-    __shared__ T urow[2][kMaxNB], lcol[2][kMaxNB];
+    // row j and the raw column j, double-buffered by the parity of j: one
+    // barrier per column (a thread passing barrier j + 1 knows that every
+    // thread has finished reading buffer j & 1)
+    __shared__ T urow[2][kMaxNB], acol[2][kMaxNB];
+    __shared__ T s_rcp[2];
+    __shared__ int s_ok[2];
     if (status[0] != 0) return;
-    const int tid = threadIdx.x, ti = tid >> 5, tc = tid & 31;
+    // wave v covers the 32 x 32 region (v / 4, v % 4); lane l the 4 x 4 tile
+    // (l / 8, l % 8) inside it, so whole waves fall idle as j advances
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+    const int ti = 8 * (wv >> 2) + (ln >> 3), tc = 8 * (wv & 3) + (ln & 7);
     Regs<T, 16> a;  // element (r, c) at 4 r + c
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -1047,42 +1147,45 @@ __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0,
                     for (int c = 0; c < 4; ++c) urow[buf][4 * tc + c] = a.get(4 * r + c);
                 }
         }
+        if (tc == jt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c == jr) {
+                        const T v = a.get(4 * r + c);
+                        acol[buf][4 * ti + r] = v;
+                        if (4 * ti + r == j) {
+                            // pivot owner: magnitude check and 1 / u
+                            const double m = wmag(v);
+                            s_ok[buf] = (m > limit) && isfinite(m);
+                            s_rcp[buf] = wdiv(wone<T>(), v);
+                        }
+                    }
+        }
         __syncthreads();
-        const T u = urow[buf][j];
-        const double m = wmag(u);
-        if (!(m > limit) || !isfinite(m)) {
+        if (!s_ok[buf]) {
             if (tid == 0) {
                 status[0] = -2;
                 status[1] = k0 + j;
             }
             return;
         }
-        if (tc == jt) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 4 * ti + r;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (c == jr) {
-                        const T l = wdiv(a.get(4 * r + c), u);
-                        if (i > j) a.set(4 * r + c, l);
-                        lcol[buf][i] = l;
-                    }
-            }
-        }
-        __syncthreads();
         // tiles entirely above row j or left of column j are final
-        if (4 * ti + 3 <= j || 4 * tc + 3 <= j) continue;
+        if (4 * ti + 3 <= j || 4 * tc + 3 < j) continue;
+        const T rcp = s_rcp[buf];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 4 * ti + r;
-            const T l = lcol[buf][i];
+            const T l = wmul(acol[buf][i], rcp);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int cc = 4 * tc + c;
                 const T cur = a.get(4 * r + c);
-                const T upd = wfms(cur, l, urow[buf][cc]);
-                a.set(4 * r + c, (i > j && cc > j) ? upd : cur);
+                // column j below the diagonal becomes the multiplier, the
+                // columns right of j get the rank-1 update
+                const T upd = cc == j ? l : wfms(cur, l, urow[buf][cc]);
+                a.set(4 * r + c, (i > j && cc >= j) ? upd : cur);
             }
         }
     }
@@ -1095,13 +1198,15 @@ __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0,
         }
 }
 
-// L21 = A21 U11^{-1}: eight lanes per row, lane `sub` keeping columns
-// 16 sub .. 16 sub + 15 in registers; U11 staged in LDS (full block).
+// L21 = A21 U11^{-1}: sixteen lanes per row, lane `sub` keeping columns
+// 8 sub .. 8 sub + 7 in registers; U11 staged in LDS (full block) with the
+// reciprocals of its diagonal.
 template <class S>
 __global__ __launch_bounds__(256) void trsm_right_kernel(S *A, size_t lda, int n, int k0, int w, const int *status) {
     using T = typename Work<S>::T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     S *blk = reinterpret_cast<S *>(smem_raw);
+    __shared__ T rcp[kMaxNB];
     if (status[0] != 0) return;
     {
         const int total = w * w;
@@ -1128,13 +1233,15 @@ __global__ __launch_bounds__(256) void trsm_right_kernel(S *A, size_t lda, int n
         a.set(t, (valid && c < w) ? wld(A + (size_t)row * lda + k0 + c) : wzero<T>());
     }
     __syncthreads();
+    if ((int)threadIdx.x < w) rcp[threadIdx.x] = wdiv(wone<T>(), wld(blk + threadIdx.x * w + threadIdx.x));
+    __syncthreads();
     const int nblk = (w + kTrsmRPT - 1) / kTrsmRPT;
     for (int cb = 0; cb < nblk; ++cb) {
 #pragma unroll
         for (int cc = 0; cc < kTrsmRPT; ++cc) {
             const int jj = cb * kTrsmRPT + cc;
             const int j = min(jj, w - 1);
-            const T x = wshfl(wdiv(a.get(cc), wld(blk + j * w + j)), base + cb);
+            const T x = wshfl(wmul(a.get(cc), rcp[j]), base + cb);
             if (sub == cb && jj < w) a.set(cc, x);
             if (sub * kTrsmRPT + kTrsmRPT - 1 <= j) continue;
             // branch-free inside, in-bounds reads; the update is kept for
@@ -1172,8 +1279,16 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2, 0);
     int nb = block_size > 0 ? std::min(block_size, kMaxNB) : kMaxNB;
     while (nb > 16 && (size_t)nb * nb * sizeof(S) > (size_t)ctx.lds_per_block) nb /= 2;
+    const size_t split_lds = (size_t)4 * 2 * 4096 * sizeof(float);
+    if (split_lds > (size_t)ctx.lds_per_block) {
+        PEEC_ERR("the ordered factor needs %zu bytes of LDS per workgroup.", split_lds);
+        out.code = -1;
+        return out;
+    }
     set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_kernel<S>), (size_t)nb * nb * sizeof(S));
     set_dynamic_lds(reinterpret_cast<const void *>(trsm_kernel<S>), trsm_lds_elems(nb) * sizeof(S));
+    set_dynamic_lds(reinterpret_cast<const void *>(gemm_splitk_kernel<S>), split_lds);
+    const int cpb = 256 / kTrsmTPC;  // columns (rows) per triangular-solve workgroup
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
         hipLaunchKernelGGL(diag_lu_kernel<S>, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
@@ -1189,21 +1304,51 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             pending = false;
         }
         if (rest <= 0) break;
-        const unsigned rgrid = (unsigned)((rest + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC));
-        hipLaunchKernelGGL(trsm_right_kernel<S>, dim3(rgrid), dim3(256), (size_t)w * w * sizeof(S), s, dA, lda, n,
-                           k0, w, ctx.status.ptr);
-        hipLaunchKernelGGL(trsm_kernel<S>, dim3(rgrid), dim3(256), trsm_lds_elems(w) * sizeof(S), s, dA, lda, n, k0,
-                           w, ctx.status.ptr);
+        const int w2 = std::min(nb, rest), r0 = k0 + w;
+        const int ks_count = (w + 3) / 4;
         int rb_pad, cb_pad;
-        pack_operands<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s, rb_pad, cb_pad);
-        const int w2 = std::min(nb, rest);
-        gemm_packed<S>(dA, lda, k0 + w, k0 + w, rest, 0, w2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+        gemm_pads<S>(rest, rest, rb_pad, cb_pad);
+        ctx.pack_l.reserve((size_t)(ks_count + 1) * rb_pad * 64 * sizeof(S));
+        ctx.pack_u.reserve((size_t)(ks_count + 1) * cb_pad * 64 * sizeof(S));
+        S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
+        S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
+        // critical path: L21, U12 of the next block column, its update
+        hipLaunchKernelGGL(trsm_right_kernel<S>, dim3((unsigned)((rest + cpb - 1) / cpb)), dim3(256),
+                           (size_t)w * w * sizeof(S), s, dA, lda, n, k0, w, ctx.status.ptr);
+        hipLaunchKernelGGL(trsm_kernel<S>, dim3((unsigned)((w2 + cpb - 1) / cpb)), dim3(256),
+                           trsm_lds_elems(w) * sizeof(S), s, dA, lda, r0, r0 + w2, k0, w, ctx.status.ptr);
+        const size_t tl = (size_t)ks_count * rb_pad * 64;
+        hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, dA, lda, r0, rest,
+                           k0, w, rb_pad, Lp, ctx.status.ptr);
+        const int cb_next = 4 * ((w2 + 63) / 64);
+        const size_t tu1 = (size_t)ks_count * cb_next * 64;
+        hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu1 + 255) / 256)), dim3(256), 0, s, dA, lda, k0, w,
+                           r0, w2, cb_next, cb_pad, Up, ctx.status.ptr);
+        {
+            const int tiles = ((rest + 63) / 64) * ((w2 + 63) / 64);
+            const int tg = ctx.timer ? ctx.timer->start(s) : -1;
+            hipLaunchKernelGGL(gemm_splitk_kernel<S>, dim3((unsigned)tiles), dim3(256), split_lds, s, dA, lda, r0, r0,
+                               rest, w2, ks_count, Lp, Up, rb_pad, cb_pad, ctx.status.ptr);
+            if (ctx.timer)
+                ctx.timer->stop(tg, s, kKernelLuUpdate, 8.0 * rest * w2 * w,
+                                (2.0 * rest * w2 + (double)(rest + w2) * w) * sizeof(S));
+        }
         HIP_CHECK(hipEventRecord(ctx.ev_next, s));
         HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
-        diag(k0 + w, w2, aux);
+        diag(r0, w2, aux);
         HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
         pending = true;
-        gemm_packed<S>(dA, lda, k0 + w, k0 + w, rest, w2, rest - w2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+        // off the critical path: the rest of U12 and of the trailing update
+        const int rest2 = rest - w2;
+        if (rest2 > 0) {
+            hipLaunchKernelGGL(trsm_kernel<S>, dim3((unsigned)((rest2 + cpb - 1) / cpb)), dim3(256),
+                               trsm_lds_elems(w) * sizeof(S), s, dA, lda, r0 + w2, n, k0, w, ctx.status.ptr);
+            const int cb_rest = cb_pad - w2 / 16;
+            const size_t tu2 = (size_t)ks_count * cb_rest * 64;
+            hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu2 + 255) / 256)), dim3(256), 0, s, dA, lda, k0,
+                               w, r0 + w2, rest2, cb_rest, cb_pad, Up + (size_t)(w2 / 16) * 64, ctx.status.ptr);
+            gemm_packed<S>(dA, lda, r0, r0, rest, w2, rest2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+        }
         HIP_CHECK(hipGetLastError());
     }
     if (pending) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
