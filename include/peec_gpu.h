@@ -25,6 +25,9 @@
 #include "inductance.h"
 #include "mesh.h"
 #include "potential.h"
+#include "incident_field.h"
+#include "rcs.h"
+#include "scattering.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -86,6 +89,32 @@ int peec_engine_solve(PeecEngine *engine, double frequency, int mode,
                       const int *port_nodes, size_t n_ports, Complex *z_in,
                       Complex *edge_current, Complex *node_voltage,
                       PeecPointReport *report);
+
+/* Monostatic radar cross section on one factor (the workload of
+ * main_rcs.c:1380-1500 for many incidences at once). For incidence i the
+ * plane wave arrives from (theta_deg[i], phi_deg[i]) and excites the edges as
+ * scattering_build_harmonic_excitation does; the right-hand side is
+ * [-U ; 0]. All incidences are solved on one factor (mixed mode refines each
+ * right-hand side); results[i] is rcs_compute_direction of that solution in
+ * the backscatter direction -k_hat (angles in results[i].theta_deg/phi_deg).
+ * edge_current: optional, n_angles x Ne, rhs-major. Returns as
+ * peec_engine_solve. */
+typedef struct
+{
+    IncidentWaveType wave_type;           /* COMPLEX (default), COS or SIN */
+    IncidentPolarization polarization;    /* default VERTICAL (e_theta) */
+    double E_amp;                         /* V/m, default 1 */
+    double phase;                         /* rad, default 0 */
+    ScatteringEdgeIntegration excitation; /* default ANALYTIC_PLANE_WAVE */
+    RcsOptions rcs;                       /* default DUAL_PATCH_QUADRATURE, order 6 */
+} PeecRcsOptions;
+
+void peec_rcs_options_default(PeecRcsOptions *options);
+
+int peec_engine_monostatic_rcs(PeecEngine *engine, double frequency, int mode,
+                               const PeecRcsOptions *options, const double *theta_deg,
+                               const double *phi_deg, size_t n_angles, RcsResult *results,
+                               Complex *edge_current, PeecPointReport *report);
 
 /* Copies the static matrices (Ne x Ne, Nv x Nv, row-major) to the host. */
 int peec_engine_static_matrices(const PeecEngine *engine, double *L, double *P);

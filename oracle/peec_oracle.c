@@ -5,6 +5,7 @@
  * reference bit for bit. Citations are to the reference sources. */
 #include "peec_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -1220,5 +1221,225 @@ int oracle_retarded_matrices(const Mesh *m, const DualMesh *dm,
             }
         }
     }
+    return 0;
+}
+
+
+/* ======================================================================
+ * plane-wave excitation and radar cross section
+ * ====================================================================== */
+
+static const double ORACLE_C0 = 299792458.0;       /* incident_field.c:18 */
+static const double ORACLE_MU0 = 1.25663706212e-6; /* rcs.c:23 */
+static const double ORACLE_EPS0 = 8.8541878128e-12; /* rcs.c:26 */
+
+static double o_dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double o_norm3(const double a[3]) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+static int o_normalize(double a[3])
+{
+    double nrm = o_norm3(a);
+    if (!isfinite(nrm) || nrm <= 1.0e-14) return -1;
+    a[0] /= nrm;
+    a[1] /= nrm;
+    a[2] /= nrm;
+    return 0;
+}
+
+/* incident_field.c:110-283,312-420 */
+int oracle_plane_wave(IncidentField *f, int wave_type, int polarization, double E_amp, double frequency,
+                      double theta_deg, double phi_deg, double phase)
+{
+    if (f == NULL || wave_type < 0 || wave_type > 2 || !(frequency > 0.0)) return -1;
+    memset(f, 0, sizeof(*f));
+    f->wave_type = (IncidentWaveType)wave_type;
+    f->polarization = (IncidentPolarization)polarization;
+    f->E_amp = E_amp;
+    f->frequency = frequency;
+    f->theta_deg = theta_deg;
+    f->phi_deg = phi_deg;
+    f->phase = phase;
+    f->theta = theta_deg * M_PI / 180.0;
+    f->phi = phi_deg * M_PI / 180.0;
+    f->omega = 2.0 * M_PI * frequency;
+    f->wave_number = f->omega / ORACLE_C0;
+    const double t = f->theta, p = f->phi;
+    f->k_hat[0] = -sin(t) * cos(p);
+    f->k_hat[1] = -sin(t) * sin(p);
+    f->k_hat[2] = -cos(t);
+    f->e_theta[0] = cos(t) * cos(p);
+    f->e_theta[1] = cos(t) * sin(p);
+    f->e_theta[2] = -sin(t);
+    f->e_phi[0] = -sin(p);
+    f->e_phi[1] = cos(p);
+    f->e_phi[2] = 0.0;
+    if (o_normalize(f->k_hat) || o_normalize(f->e_theta) || o_normalize(f->e_phi)) return -1;
+    if (fabs(o_dot3(f->k_hat, f->e_theta)) > 1.0e-12 || fabs(o_dot3(f->k_hat, f->e_phi)) > 1.0e-12) return -1;
+    const double *pol;
+    if (polarization == 1) pol = f->e_theta;
+    else if (polarization == 0) pol = f->e_phi;
+    else return -1;
+    for (int c = 0; c < 3; ++c) f->E0[c] = E_amp * pol[c];
+    return 0;
+}
+
+/* incident_field.c:754-858 */
+int oracle_phasor(const IncidentField *f, const double r[3], Complex E[3])
+{
+    if (f->wave_type > 2) return -1;
+    double gamma = -f->wave_number * o_dot3(f->k_hat, r) + f->phase;
+    if (f->wave_type == 1) gamma -= M_PI / 2.0;
+    const double c = cos(gamma), s = sin(gamma);
+    for (int k = 0; k < 3; ++k) {
+        E[k].re = f->E0[k] * c;
+        E[k].im = f->E0[k] * s;
+    }
+    return 0;
+}
+
+/* scattering.c:23-43 */
+static double o_sinc(double x)
+{
+    double ax = fabs(x);
+    if (ax < 1.0e-4) {
+        double x2 = x * x, x4 = x2 * x2;
+        return 1.0 - x2 / 6.0 + x4 / 120.0;
+    }
+    return sin(x) / x;
+}
+
+/* scattering.c:180-337 (edge loop of 338-489) */
+int oracle_excitation(const Mesh *mesh, const IncidentField *f, int integration, Complex *U)
+{
+    if (mesh == NULL || f == NULL || U == NULL || mesh->n_edges == 0) return -1;
+    if (f->wave_type == 3 || (integration != 0 && integration != 1)) return -1;
+    for (size_t e = 0; e < mesh->n_edges; ++e) {
+        const double *rc = &mesh->edge_centers[3 * e];
+        const double *l = &mesh->edge_vectors[3 * e];
+        Complex E[3];
+        if (oracle_phasor(f, rc, E)) return -1;
+        Complex mid;
+        mid.re = E[0].re * l[0] + E[1].re * l[1] + E[2].re * l[2];
+        mid.im = E[0].im * l[0] + E[1].im * l[1] + E[2].im * l[2];
+        if (integration == 1) {
+            double q = f->wave_number * o_dot3(f->k_hat, l);
+            double factor = o_sinc(0.5 * q);
+            mid.re *= factor;
+            mid.im *= factor;
+        }
+        if (!isfinite(mid.re) || !isfinite(mid.im)) return -1;
+        U[e] = mid;
+    }
+    return 0;
+}
+
+/* rcs.c:157-362: G_T = integral over triangle (p0,p1,p2) of exp(j k d.r) dS */
+static void o_triangle_phase(const double *p0, const double *p1, const double *p2, double k, const double d[3],
+                             int order, const double *x, const double *w, double *gre, double *gim)
+{
+    double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    *gre = 0.0;
+    *gim = 0.0;
+    for (int iu = 0; iu < order; ++iu) {
+        double u = 0.5 * (x[iu] + 1.0), wu = 0.5 * w[iu];
+        for (int iv = 0; iv < order; ++iv) {
+            double v = 0.5 * (x[iv] + 1.0), wv = 0.5 * w[iv];
+            double xi = u, eta = (1.0 - u) * v;
+            double r[3] = {p0[0] + xi * a[0] + eta * b[0], p0[1] + xi * a[1] + eta * b[1],
+                           p0[2] + xi * a[2] + eta * b[2]};
+            double jac = o_norm3(cr) * (1.0 - u);
+            double phase = k * o_dot3(d, r);
+            double weight = wu * wv * jac;
+            *gre += weight * cos(phase);
+            *gim += weight * sin(phase);
+        }
+    }
+}
+
+/* rcs.c:701-995 (sequential edge order) */
+int oracle_radiation_vector(const Mesh *mesh, const DualMesh *dual, const Complex *I, double k,
+                            const double d[3], int integration, int order, Complex F[3])
+{
+    double x[16], w[16];
+    if (integration == 1 && oracle_gauss_legendre(order, x, w) != 0) return -1;
+    double fr[3] = {0, 0, 0}, fi[3] = {0, 0, 0};
+    for (size_t e = 0; e < mesh->n_edges; ++e) {
+        const double *lv = &mesh->edge_vectors[3 * e];
+        double len = o_norm3(lv);
+        if (len <= 0.0) continue;
+        double ed[3] = {lv[0] / len, lv[1] / len, lv[2] / len};
+        double sre, sim;
+        if (integration == 0) {
+            double phase = k * o_dot3(d, &mesh->edge_centers[3 * e]);
+            double er = cos(phase), ei = sin(phase);
+            sre = I[e].re * er - I[e].im * ei;
+            sim = I[e].re * ei + I[e].im * er;
+            sre *= len;
+            sim *= len;
+        } else {
+            double area = dual->edge_region_areas[e];
+            if (area <= 0.0) continue;
+            double width = area / len;
+            double gre = 0.0, gim = 0.0;
+            const PatchList *list = &dual->edge_patches[e];
+            for (size_t p = 0; p < list->count; ++p) {
+                const double *pt = list->patches[p].points;
+                double r0, i0, r1, i1;
+                o_triangle_phase(pt, pt + 3, pt + 6, k, d, order, x, w, &r0, &i0);
+                o_triangle_phase(pt, pt + 6, pt + 9, k, d, order, x, w, &r1, &i1);
+                gre += r0 + r1;
+                gim += i0 + i1;
+            }
+            double cre = I[e].re / width, cim = I[e].im / width;
+            sre = cre * gre - cim * gim;
+            sim = cre * gim + cim * gre;
+        }
+        for (int c = 0; c < 3; ++c) {
+            fr[c] += sre * ed[c];
+            fi[c] += sim * ed[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        F[c].re = fr[c];
+        F[c].im = fi[c];
+    }
+    return 0;
+}
+
+/* rcs.c:536-598,638-700,1000-1165 */
+int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentField *f, const Complex *I,
+                         int integration, int order, double theta_deg, double phi_deg, RcsResult *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->theta_deg = theta_deg;
+    out->phi_deg = phi_deg;
+    double t = theta_deg * M_PI / 180.0, p = phi_deg * M_PI / 180.0;
+    out->direction[0] = sin(t) * cos(p);
+    out->direction[1] = sin(t) * sin(p);
+    out->direction[2] = cos(t);
+    double nrm = o_norm3(out->direction);
+    if (!isfinite(nrm) || nrm <= 0.0) return -1;
+    for (int c = 0; c < 3; ++c) out->direction[c] /= nrm;
+    if (oracle_radiation_vector(mesh, dual, I, f->wave_number, out->direction, integration, order,
+                                out->radiation_vector))
+        return -1;
+    const double *d = out->direction;
+    const Complex *F = out->radiation_vector;
+    double dre = d[0] * F[0].re + d[1] * F[1].re + d[2] * F[2].re;
+    double dim = d[0] * F[0].im + d[1] * F[1].im + d[2] * F[2].im;
+    double t2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        out->transverse_vector[c].re = d[c] * dre - F[c].re;
+        out->transverse_vector[c].im = d[c] * dim - F[c].im;
+    }
+    for (int c = 0; c < 3; ++c)
+        t2 += out->transverse_vector[c].re * out->transverse_vector[c].re +
+              out->transverse_vector[c].im * out->transverse_vector[c].im;
+    double eta0 = sqrt(ORACLE_MU0 / ORACLE_EPS0);
+    double k = f->wave_number, E0 = f->E_amp;
+    out->sigma = k * k * eta0 * eta0 * t2 / (4.0 * M_PI * E0 * E0);
+    out->sigma_dbsm = out->sigma > DBL_MIN ? 10.0 * log10(out->sigma) : -INFINITY;
     return 0;
 }

@@ -31,6 +31,25 @@
 
 #ifdef __cplusplus
 extern "C" {
+/* ---- plane-wave excitation and RCS (physics/incident_field.c,
+ *      physics/scattering.c, physics/rcs.c) ---- */
+#include "../include/incident_field.h"
+#include "../include/rcs.h"
+
+/* incident_field.c:28-283,312-420: harmonic set-up (COS/SIN/COMPLEX) */
+int oracle_plane_wave(IncidentField *field, int wave_type, int polarization, double E_amp, double frequency,
+                      double theta_deg, double phi_deg, double phase);
+/* incident_field.c:754-858 */
+int oracle_phasor(const IncidentField *field, const double r[3], Complex E[3]);
+/* scattering.c:23-43,180-337,338-489: integration 0 midpoint, 1 analytic */
+int oracle_excitation(const Mesh *mesh, const IncidentField *field, int integration, Complex *U);
+/* rcs.c:117-531,701-995 */
+int oracle_radiation_vector(const Mesh *mesh, const DualMesh *dual, const Complex *I, double k,
+                            const double direction[3], int integration, int order, Complex F[3]);
+/* rcs.c:536-598,638-700,1000-1165 */
+int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentField *field, const Complex *I,
+                         int integration, int order, double theta_deg, double phi_deg, RcsResult *result);
+
 #endif
 
 /* Work counters filled by the matrix routines (may be NULL). */
@@ -102,6 +121,44 @@ int oracle_retarded_matrices(const Mesh *mesh, const DualMesh *dual,
 
 #ifdef __cplusplus
 }
+/* ---- plane-wave excitation and RCS (physics/incident_field.c,
+ *      physics/scattering.c, physics/rcs.c) ---- */
+#include "../include/incident_field.h"
+#include "../include/rcs.h"
+
+/* incident_field.c:28-283,312-420: harmonic set-up (COS/SIN/COMPLEX) */
+int oracle_plane_wave(IncidentField *field, int wave_type, int polarization, double E_amp, double frequency,
+                      double theta_deg, double phi_deg, double phase);
+/* incident_field.c:754-858 */
+int oracle_phasor(const IncidentField *field, const double r[3], Complex E[3]);
+/* scattering.c:23-43,180-337,338-489: integration 0 midpoint, 1 analytic */
+int oracle_excitation(const Mesh *mesh, const IncidentField *field, int integration, Complex *U);
+/* rcs.c:117-531,701-995 */
+int oracle_radiation_vector(const Mesh *mesh, const DualMesh *dual, const Complex *I, double k,
+                            const double direction[3], int integration, int order, Complex F[3]);
+/* rcs.c:536-598,638-700,1000-1165 */
+int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentField *field, const Complex *I,
+                         int integration, int order, double theta_deg, double phi_deg, RcsResult *result);
+
 #endif
+
+/* ---- plane-wave excitation and RCS (physics/incident_field.c,
+ *      physics/scattering.c, physics/rcs.c) ---- */
+#include "../include/incident_field.h"
+#include "../include/rcs.h"
+
+/* incident_field.c:28-283,312-420: harmonic set-up (COS/SIN/COMPLEX) */
+int oracle_plane_wave(IncidentField *field, int wave_type, int polarization, double E_amp, double frequency,
+                      double theta_deg, double phi_deg, double phase);
+/* incident_field.c:754-858 */
+int oracle_phasor(const IncidentField *field, const double r[3], Complex E[3]);
+/* scattering.c:23-43,180-337,338-489: integration 0 midpoint, 1 analytic */
+int oracle_excitation(const Mesh *mesh, const IncidentField *field, int integration, Complex *U);
+/* rcs.c:117-531,701-995 */
+int oracle_radiation_vector(const Mesh *mesh, const DualMesh *dual, const Complex *I, double k,
+                            const double direction[3], int integration, int order, Complex F[3]);
+/* rcs.c:536-598,638-700,1000-1165 */
+int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentField *field, const Complex *I,
+                         int integration, int order, double theta_deg, double phi_deg, RcsResult *result);
 
 #endif

@@ -12,6 +12,7 @@
 #include "assembly.h"
 #include "harmonic_kernels.h"
 #include "lu.h"
+#include "rcs_kernels.h"
 
 namespace peec {
 
@@ -198,6 +199,13 @@ struct PeecEngine {
     bool have_reuse = false;
     DeviceBuffer<double> d_L, d_P, d_row, d_col, d_terms;
     DeviceBuffer<double2> d_Lw, d_Pw, d_Z, d_Z0, d_B, d_X, d_R, d_Xbest;
+    DeviceBuffer<double> d_ax;                  // |x| scratch of the residuals
+    DeviceBuffer<unsigned long long> d_eta;     // per right-hand side backward error (bits)
+    DeviceBuffer<int> d_flags;                  // per right-hand side masks
+    std::unique_ptr<RcsGeometry> rcs;           // edge geometry for excitation and far field
+    double limit = 0.0;                         // pivot limit of the current point
+    bool ordered = false;                       // current FP32 factor uses the reused row order
+    bool retarded_now = false;
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
@@ -207,6 +215,248 @@ struct PeecEngine {
             if (e) (void)hipEventDestroy(e);
     }
 };
+
+// ---------------------------------------------------------------------------
+// one frequency point: factor, then solves with refinement
+
+__global__ void fill_zero2_kernel(double2 *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+static void fill_zero(double2 *p, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_zero2_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+// X[q] -= D[q] (or dst[q] = src[q] for the copy) for the right-hand sides
+// whose flag is set
+__global__ void masked_axpy_kernel(double2 *X, const double2 *D, const int *flags, int n, int nrhs) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * nrhs || !flags[i / n]) return;
+    X[i].x -= D[i].x;
+    X[i].y -= D[i].y;
+}
+__global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *flags, int n, int nrhs) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * nrhs || !flags[i / n]) return;
+    dst[i] = src[i];
+}
+
+static void assemble_retarded(PeecEngine *e, double k);
+
+static int factor_error(const LUStatus &st) {
+    if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
+    else PEEC_ERR("LU failed (code %d).", st.code);
+    return st.code == -2 ? -2 : -1;
+}
+
+// Partial-pivoting factor (FP64 in d_Z, or FP32 in d_Z32 from the
+// equilibrated d_Z); its row order becomes the order reused by the next
+// mixed-precision point.
+static int factor_pivoted(PeecEngine *e, bool fp64) {
+    hipStream_t s = e->stream;
+    const int n = e->n;
+    LUStatus st;
+    if (fp64) {
+        st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
+    } else {
+        convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, (size_t)n * n, s);
+        st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
+    }
+    if (st.code != 0) return factor_error(st);
+    permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
+    HIP_CHECK(hipMemcpyAsync(e->d_reuse_perm.ptr, e->d_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    e->have_reuse = true;
+    e->ordered = false;
+    return 0;
+}
+
+// Retarded assembly, MNA fill, equilibration and the factor (FP64, or FP32
+// in the previous point's row order / with partial pivoting). 0, -1 or -2.
+static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &report) {
+    hipStream_t s = e->stream;
+    const int n = e->n, ne = e->ne, nv = e->nv;
+    const size_t nn = (size_t)n * n;
+    const double omega = kTwoPi * f;
+    HIP_CHECK(hipEventRecord(e->ev[0], s));
+    const bool ret = e->opt.retarded != 0;
+    e->retarded_now = ret;
+    if (ret) assemble_retarded(e, omega / kC0);
+    HIP_CHECK(hipEventRecord(e->ev[1], s));
+    e->d_Z.resize(nn);
+    e->d_Z0.resize(nn);
+    launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr, e->d_P.ptr,
+                    ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
+    e->d_row.resize(n);
+    e->d_col.resize(n);
+    double max_scaled = 0.0;
+    if (equilibrate_complex(e->d_Z.ptr, n, n, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) != 0) return -2;
+    HIP_CHECK(hipEventRecord(e->ev[2], s));
+    e->limit = 1.0e-14 * max_scaled;
+    e->d_ipiv.resize(n);
+    e->d_perm.resize(n);
+    e->d_reuse_perm.resize(n);
+    LUStatus st;
+    e->ordered = false;
+    if (mode == PEEC_SOLVE_MIXED) {
+        e->d_Z32.resize(nn);
+        if (e->opt.reuse_pivots && e->have_reuse) {
+            convert_rows_to_f32(e->d_Z.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, n, s);
+            st = lu_factor_ordered(e->d_Z32.ptr, n, n, e->limit, *e->lu, e->opt.block_size);
+            if (st.code == 0) {
+                e->ordered = true;
+                HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                                         s));
+            } else if (st.code != -2) {
+                return factor_error(st);
+            }
+        }
+        if (!e->ordered) {
+            const int code = factor_pivoted(e, false);
+            if (code) return code;
+        }
+    } else {
+        const int code = factor_pivoted(e, true);
+        if (code) return code;
+    }
+    HIP_CHECK(hipEventRecord(e->ev[3], s));
+    report.pivot_reuse = e->ordered ? 1 : 0;
+    return 0;
+}
+
+// Solve the nrhs right-hand sides in d_B into d_X: FP64 directly, or the
+// FP32 factor with refinement against the FP64 matrix (x <- x - A^{-1} r)
+// and the fallbacks (a rejected reused order is refactored with pivoting; a
+// point that stays above the gate is refactored in FP64). The component-wise
+// backward error need not fall at every step (the first correction often
+// barely changes it at low frequencies), so a right-hand side stops after
+// two steps without a new best, and its best iterate is kept.
+// 0, -1, -2, or -3 (backward error above the gate).
+static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &report) {
+    hipStream_t s = e->stream;
+    const int n = e->n;
+    const size_t tot = (size_t)n * nrhs;
+    e->d_X.resize(tot);
+    e->d_R.resize(tot);
+    e->d_Xbest.resize(tot);
+    e->d_ax.resize(tot);
+    e->d_eta.resize((size_t)nrhs);
+    e->d_flags.resize((size_t)nrhs);
+    std::vector<double> eta(nrhs);
+    std::vector<unsigned long long> bits(nrhs);
+    auto measure = [&]() {
+        residual_multi(e->d_Z0.ptr, n, n, e->d_X.ptr, e->d_B.ptr, e->d_R.ptr, nrhs, e->d_ax.ptr, e->d_eta.ptr, s);
+        e->d_eta.download(bits.data(), nrhs, s);
+        e->stream.sync();
+        for (int q = 0; q < nrhs; ++q) {
+            double v;
+            std::memcpy(&v, &bits[q], sizeof v);
+            eta[q] = v;
+        }
+    };
+    const unsigned grid = (unsigned)((tot + 255) / 256);
+    auto refine = [&]() {
+        HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+        lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr,
+                                           nrhs, *e->lu);
+        measure();
+        std::vector<double> best = eta;
+        std::vector<int> since(nrhs, 0), active(nrhs), flags(nrhs);
+        for (int q = 0; q < nrhs; ++q) active[q] = eta[q] > e->opt.refinement_target;
+        HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, e->d_X.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+        while (report.refinement_steps < e->opt.max_refinement_steps) {
+            bool any = false;
+            for (int q = 0; q < nrhs; ++q) any = any || active[q];
+            if (!any) break;
+            lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                               e->d_R.ptr, nrhs, *e->lu);
+            e->d_flags.upload(active.data(), nrhs, s);
+            hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr,
+                               e->d_flags.ptr, n, nrhs);
+            ++report.refinement_steps;
+            measure();
+            bool improved = false;
+            for (int q = 0; q < nrhs; ++q) {
+                flags[q] = 0;
+                if (!active[q]) continue;
+                if (eta[q] < best[q]) {
+                    best[q] = eta[q];
+                    since[q] = 0;
+                    flags[q] = 1;
+                    improved = true;
+                } else if (++since[q] >= 2) {
+                    active[q] = 0;
+                }
+                if (best[q] <= e->opt.refinement_target) active[q] = 0;
+            }
+            if (improved) {
+                e->d_flags.upload(flags.data(), nrhs, s);
+                hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_Xbest.ptr, e->d_X.ptr,
+                                   e->d_flags.ptr, n, nrhs);
+            }
+        }
+        bool restore = false;
+        double w = 0.0;
+        for (int q = 0; q < nrhs; ++q) {
+            flags[q] = eta[q] > best[q];
+            restore = restore || flags[q];
+            w = std::fmax(w, best[q]);
+        }
+        if (restore) {
+            e->d_flags.upload(flags.data(), nrhs, s);
+            hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_Xbest.ptr,
+                               e->d_flags.ptr, n, nrhs);
+        }
+        return w;
+    };
+    double worst = 0.0;
+    if (mode == PEEC_SOLVE_MIXED) {
+        worst = refine();
+        if (e->ordered && !(worst <= 1e-3 * e->opt.backward_error_gate)) {
+            // the reused row order is not good enough at this point:
+            // refactor with partial pivoting and refine again
+            report.pivot_refactor = 1;
+            report.reuse_backward_error = worst;
+            const int code = factor_pivoted(e, false);
+            if (code) return code;
+            worst = refine();
+        }
+        if (!(worst <= e->opt.backward_error_gate)) {
+            // refinement stalled above the gate: refactor in FP64
+            report.used_fp64_fallback = 1;
+            const int code = factor_pivoted(e, true);
+            if (code) return code;
+            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+            lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                                e->d_X.ptr, nrhs, *e->lu);
+            measure();
+            worst = 0.0;
+            for (int q = 0; q < nrhs; ++q) worst = std::fmax(worst, eta[q]);
+        }
+        HIP_CHECK(hipEventRecord(e->ev[4], s));
+    } else {
+        HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+        lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr,
+                                            nrhs, *e->lu);
+        HIP_CHECK(hipEventRecord(e->ev[4], s));
+        measure();
+        for (int q = 0; q < nrhs; ++q) worst = std::fmax(worst, eta[q]);
+    }
+    HIP_CHECK(hipEventRecord(e->ev[5], s));
+    report.backward_error = worst;
+    return worst <= e->opt.backward_error_gate ? 0 : -3;
+}
+
+static void engine_timings(PeecEngine *e, PeecPointReport &report) {
+    e->stream.sync();
+    e->timer.resolve();
+    float ms[5];
+    for (int i = 0; i < 5; ++i) HIP_CHECK(hipEventElapsedTime(&ms[i], e->ev[i], e->ev[i + 1]));
+    report.ms_assembly = ms[0];
+    report.ms_fill = ms[1];
+    report.ms_factor = ms[2];
+    report.ms_solve = ms[3];
+    report.ms_check = ms[4];
+}
 
 extern "C" {
 
@@ -249,6 +499,7 @@ int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineO
                                                o->near_smooth_order);
         e->lu = std::make_unique<LUContext>(s);
         e->lu->timer = &e->timer;
+        if (mesh->edge_centers && mesh->edge_vectors) e->rcs = std::make_unique<RcsGeometry>(mesh, dual, s);
         std::vector<int> edges(mesh->edges, mesh->edges + 2 * e->ne);
         e->d_edges.upload(edges, s);
         e->d_L.resize((size_t)e->ne * e->ne);
@@ -349,164 +600,21 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
     try {
         hipStream_t s = e->stream;
         const int n = e->n, ne = e->ne, nv = e->nv, nr = (int)nports;
-        const size_t nn = (size_t)n * n;
-        const double omega = kTwoPi * f;
-        HIP_CHECK(hipEventRecord(e->ev[0], s));
-        const bool ret = e->opt.retarded != 0;
-        if (ret) assemble_retarded(e, omega / kC0);
-        HIP_CHECK(hipEventRecord(e->ev[1], s));
-        e->d_Z.resize(nn);
-        e->d_Z0.resize(nn);
-        launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
-                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
-        e->d_row.resize(n);
-        e->d_col.resize(n);
-        double max_scaled = 0.0;
-        if (equilibrate_complex(e->d_Z.ptr, n, n, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) != 0)
-            return -2;
-        HIP_CHECK(hipEventRecord(e->ev[2], s));
-        const double limit = 1.0e-14 * max_scaled;
-        e->d_ipiv.resize(n);
-        e->d_perm.resize(n);
-        e->d_reuse_perm.resize(n);
-        auto keep_order = [&]() {
-            // the row order of the partial-pivoting factor just made
-            permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
-            HIP_CHECK(hipMemcpyAsync(e->d_reuse_perm.ptr, e->d_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-            e->have_reuse = true;
-        };
-        auto factor_error = [&](const LUStatus &st) {
-            if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
-            else PEEC_ERR("LU failed (code %d).", st.code);
-            return st.code == -2 ? -2 : -1;
-        };
-        LUStatus st;
-        bool ordered = false;
-        if (mode == PEEC_SOLVE_MIXED) {
-            e->d_Z32.resize(nn);
-            if (e->opt.reuse_pivots && e->have_reuse) {
-                convert_rows_to_f32(e->d_Z.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, n, s);
-                st = lu_factor_ordered(e->d_Z32.ptr, n, n, limit, *e->lu, e->opt.block_size);
-                if (st.code == 0) {
-                    ordered = true;
-                    HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n,
-                                             hipMemcpyDeviceToDevice, s));
-                } else if (st.code != -2) {
-                    return factor_error(st);
-                }
-            }
-            if (!ordered) {
-                convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, nn, s);
-                st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-                if (st.code != 0) return factor_error(st);
-                keep_order();
-            }
-        } else {
-            st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-            if (st.code != 0) return factor_error(st);
-            keep_order();
+        int code = engine_factor(e, f, mode, report);
+        if (code != 0) {
+            if (rep) *rep = report;
+            return code;
         }
-        HIP_CHECK(hipEventRecord(e->ev[3], s));
-        report.pivot_reuse = ordered ? 1 : 0;
         e->d_ports.upload(ports, 2 * nports, s);
         e->d_B.resize((size_t)n * nr);
-        e->d_X.resize((size_t)n * nr);
-        e->d_R.resize((size_t)n * nr);
-        e->d_terms.resize((size_t)3 * n);
         const size_t tot = (size_t)n * nr;
         hipLaunchKernelGGL(port_rhs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, e->d_B.ptr, ne, nv,
-                           e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
-        std::vector<double> terms((size_t)3 * n);
-        double worst = 0.0;
-        auto backward_error = [&](int k) {
-            complex_residual(e->d_Z0.ptr, n, n, e->d_X.ptr + (size_t)k * n, e->d_B.ptr + (size_t)k * n,
-                             e->d_R.ptr + (size_t)k * n, e->d_terms.ptr, s);
-            e->d_terms.download(terms.data(), terms.size(), s);
-            e->stream.sync();
-            double summary[7];
-            residual_summary(terms, n, ne, summary);
-            return summary[2];
-        };
-        // solve with the FP32 factor and refine against the FP64 matrix:
-        // x <- x - A^{-1} r, r = A x - b, until the target or the step cap.
-        // The component-wise backward error need not fall at every step (the
-        // first correction often barely changes it at low frequencies), so
-        // refinement stops only after two steps without a new best, and the
-        // best iterate is kept. Returns the worst backward error over the
-        // right-hand sides.
-        e->d_Xbest.resize(n);
-        auto solve_refine = [&]() {
-            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
-            lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
-                                               e->d_X.ptr, nr, *e->lu);
-            double w = 0.0;
-            for (int k = 0; k < nr; ++k) {
-                double2 *xk = e->d_X.ptr + (size_t)k * n;
-                double eta = backward_error(k), best = eta;
-                int since_best = 0;
-                bool saved = false;
-                while (eta > e->opt.refinement_target && report.refinement_steps < e->opt.max_refinement_steps) {
-                    lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr,
-                                                       e->d_col.ptr, e->d_R.ptr + (size_t)k * n, 1, *e->lu);
-                    if (since_best == 0 && !saved) {
-                        // keep the current best before stepping away from it
-                        HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, xk, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
-                        saved = true;
-                    }
-                    hipLaunchKernelGGL(axpy_sub_kernel, dim3((n + 255) / 256), dim3(256), 0, s, xk,
-                                       e->d_R.ptr + (size_t)k * n, (size_t)n);
-                    ++report.refinement_steps;
-                    eta = backward_error(k);
-                    if (eta < best) {
-                        best = eta;
-                        since_best = 0;
-                        saved = false;
-                    } else if (++since_best >= 2) {
-                        break;
-                    }
-                }
-                if (eta > best) {
-                    HIP_CHECK(hipMemcpyAsync(xk, e->d_Xbest.ptr, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
-                    eta = best;
-                }
-                w = std::fmax(w, eta);
-            }
-            return w;
-        };
-        if (mode == PEEC_SOLVE_MIXED) {
-            worst = solve_refine();
-            if (ordered && !(worst <= 1e-3 * e->opt.backward_error_gate)) {
-                // the reused row order is not good enough at this point:
-                // refactor with partial pivoting and refine again
-                report.pivot_refactor = 1;
-                report.reuse_backward_error = worst;
-                convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, nn, s);
-                st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-                if (st.code != 0) return factor_error(st);
-                keep_order();
-                worst = solve_refine();
-            }
-            if (!(worst <= e->opt.backward_error_gate)) {
-                // refinement stalled above the gate: refactor in FP64
-                report.used_fp64_fallback = 1;
-                st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, limit, *e->lu, e->opt.block_size);
-                if (st.code != 0) return factor_error(st);
-                keep_order();
-                HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
-                lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
-                                                    e->d_X.ptr, nr, *e->lu);
-                worst = 0.0;
-                for (int k = 0; k < nr; ++k) worst = std::fmax(worst, backward_error(k));
-            }
-            HIP_CHECK(hipEventRecord(e->ev[4], s));
-        } else {
-            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
-            lu_solve_permuted<double2, double2>(e->d_Z.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
-                                                e->d_X.ptr, nr, *e->lu);
-            HIP_CHECK(hipEventRecord(e->ev[4], s));
-            for (int k = 0; k < nr; ++k) worst = std::fmax(worst, backward_error(k));
+                           e->d_P.ptr, e->retarded_now ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
+        code = engine_solve(e, mode, nr, report);
+        if (code == -1 || code == -2) {
+            if (rep) *rep = report;
+            return code;
         }
-        HIP_CHECK(hipEventRecord(e->ev[5], s));
         std::vector<double2> x(tot);
         e->d_X.download(x.data(), tot, s);
         e->stream.sync();
@@ -517,21 +625,121 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
             if (edge_current) std::memcpy(edge_current + (size_t)k * ne, xk, sizeof(Complex) * ne);
             if (node_voltage) std::memcpy(node_voltage + (size_t)k * nv, xk + ne, sizeof(Complex) * nv);
         }
-        e->timer.resolve();
-        float ms[5];
-        for (int i = 0; i < 5; ++i) HIP_CHECK(hipEventElapsedTime(&ms[i], e->ev[i], e->ev[i + 1]));
-        report.ms_assembly = ms[0];
-        report.ms_fill = ms[1];
-        report.ms_factor = ms[2];
-        report.ms_solve = ms[3];
-        report.ms_check = ms[4];
-        report.backward_error = worst;
+        engine_timings(e, report);
         if (rep) *rep = report;
-        if (!(worst <= e->opt.backward_error_gate)) {
-            PEEC_ERR("backward error %.3e above the gate %.1e at f = %.6g Hz.", worst, e->opt.backward_error_gate, f);
-            return -3;
+        if (code == -3)
+            PEEC_ERR("backward error %.3e above the gate %.1e at f = %.6g Hz.", report.backward_error,
+                     e->opt.backward_error_gate, f);
+        return code;
+    } catch (const HipError &err) {
+        PEEC_ERR("%s", err.what.c_str());
+        if (rep) *rep = report;
+        return -1;
+    }
+}
+
+void peec_rcs_options_default(PeecRcsOptions *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->wave_type = INCIDENT_WAVE_COMPLEX;
+    o->polarization = INCIDENT_POLARIZATION_VERTICAL;
+    o->E_amp = 1.0;
+    o->phase = 0.0;
+    o->excitation = SCATTERING_EDGE_ANALYTIC_PLANE_WAVE;
+    rcs_options_default(&o->rcs, 1);
+}
+
+int peec_engine_monostatic_rcs(PeecEngine *e, double f, int mode, const PeecRcsOptions *o, const double *theta_deg,
+                               const double *phi_deg, size_t n_angles, RcsResult *results, Complex *edge_current,
+                               PeecPointReport *rep) {
+    if (!e || !o || !theta_deg || !phi_deg || n_angles == 0 || !results || !(f > 0.0) || !std::isfinite(f))
+        return -1;
+    if (mode != PEEC_SOLVE_FP64 && mode != PEEC_SOLVE_MIXED) return -1;
+    if (o->wave_type != INCIDENT_WAVE_COMPLEX && o->wave_type != INCIDENT_WAVE_COS &&
+        o->wave_type != INCIDENT_WAVE_SIN)
+        return -1;
+    if ((o->excitation != SCATTERING_EDGE_MIDPOINT && o->excitation != SCATTERING_EDGE_ANALYTIC_PLANE_WAVE) ||
+        (o->rcs.integration != RCS_CURRENT_MIDPOINT && o->rcs.integration != RCS_CURRENT_DUAL_PATCH_QUADRATURE) ||
+        !gl_order_supported(o->rcs.surface_order) || !(o->E_amp > 0.0) || !std::isfinite(o->E_amp))
+        return -1;
+    if (!e->rcs) {
+        PEEC_ERR("the engine was created without the mesh data RCS needs.");
+        return -1;
+    }
+    const int na = (int)n_angles;
+    // incident waves and the monostatic observation directions (back toward
+    // the source, computed through the angles as the reference driver does,
+    // main_rcs.c:1380-1440)
+    std::vector<PlaneWave> waves(na);
+    std::vector<double> dirs(3 * (size_t)na);
+    std::vector<double> obs_theta(na), obs_phi(na);
+    double k = 0.0;
+    for (int a = 0; a < na; ++a) {
+        IncidentField fld;
+        if (incident_field_configure_harmonic(&fld, o->wave_type, o->polarization, o->E_amp, f, theta_deg[a],
+                                              phi_deg[a], o->phase) != 0)
+            return -1;
+        waves[a] = plane_wave_of(fld);
+        k = fld.wave_number;
+        const double bx = -fld.k_hat[0], by = -fld.k_hat[1], bz = -fld.k_hat[2];
+        double bt = std::acos(std::fmax(-1.0, std::fmin(1.0, bz))) * 180.0 / M_PI;
+        double bp = std::atan2(by, bx) * 180.0 / M_PI;
+        if (bp < 0.0) bp += 360.0;
+        obs_theta[a] = bt;
+        obs_phi[a] = bp;
+        if (rcs_build_observation_direction(bt, bp, &dirs[3 * (size_t)a]) != 0) return -1;
+    }
+    PeecPointReport report{};
+    report.frequency = f;
+    try {
+        hipStream_t s = e->stream;
+        const int n = e->n, ne = e->ne;
+        int code = engine_factor(e, f, mode, report);
+        if (code != 0) {
+            if (rep) *rep = report;
+            return code;
         }
-        return 0;
+        // right-hand sides [-U ; 0] for every incidence
+        e->d_B.resize((size_t)n * na);
+        fill_zero(e->d_B.ptr, (size_t)n * na, s);
+        DeviceBuffer<PlaneWave> dw;
+        dw.upload(waves.data(), (size_t)na, s);
+        launch_excitation(*e->rcs, dw.ptr, na, o->excitation, -1.0, e->d_B.ptr, (size_t)n, s);
+        code = engine_solve(e, mode, na, report);
+        if (code == -1 || code == -2) {
+            if (rep) *rep = report;
+            return code;
+        }
+        DeviceBuffer<double> dd;
+        dd.upload(dirs, s);
+        DeviceBuffer<double> work(radiation_work_size(ne, na));
+        DeviceBuffer<double2> dF(3 * (size_t)na);
+        launch_radiation(*e->rcs, e->d_X.ptr, (size_t)n, dd.ptr, na, k, o->rcs.integration, o->rcs.surface_order,
+                         work.ptr, dF.ptr, s);
+        std::vector<double2> F(3 * (size_t)na);
+        dF.download(F.data(), F.size(), s);
+        if (edge_current)
+            for (int a = 0; a < na; ++a)
+                HIP_CHECK(hipMemcpyAsync(edge_current + (size_t)a * ne, e->d_X.ptr + (size_t)a * n,
+                                         sizeof(Complex) * ne, hipMemcpyDeviceToHost, s));
+        e->stream.sync();
+        for (int a = 0; a < na; ++a) {
+            RcsResult &r = results[a];
+            std::memset(&r, 0, sizeof(r));
+            r.theta_deg = obs_theta[a];
+            r.phi_deg = obs_phi[a];
+            for (int c = 0; c < 3; ++c) {
+                r.direction[c] = dirs[3 * (size_t)a + c];
+                r.radiation_vector[c] = Complex{F[3 * (size_t)a + c].x, F[3 * (size_t)a + c].y};
+            }
+            rcs_result_finish(r.direction, k, o->E_amp, &r);
+        }
+        engine_timings(e, report);
+        if (rep) *rep = report;
+        if (code == -3)
+            PEEC_ERR("backward error %.3e above the gate %.1e at f = %.6g Hz.", report.backward_error,
+                     e->opt.backward_error_gate, f);
+        return code;
     } catch (const HipError &err) {
         PEEC_ERR("%s", err.what.c_str());
         if (rep) *rep = report;

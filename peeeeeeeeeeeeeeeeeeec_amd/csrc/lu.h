@@ -90,6 +90,12 @@ void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s
 void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, const double2 *db,
                       double2 *dr, double *dterms, hipStream_t s);
 
+// R = A X - B for nrhs right-hand sides (rhs-major) and each one's
+// component-wise backward error (harmonic.c:932-969) as the bit pattern of
+// a non-negative double in d_eta_bits[q]. d_ax: n * nrhs doubles of scratch.
+void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, const double2 *dB, double2 *dR,
+                    int nrhs, double *d_ax, unsigned long long *d_eta_bits, hipStream_t s);
+
 // In-bounds self-test of the MFMA lane layout (1024 doubles out).
 void mfma_layout_probe(double *host_out);
 

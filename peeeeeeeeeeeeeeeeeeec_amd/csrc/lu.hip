@@ -1804,6 +1804,82 @@ void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, c
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Residuals of up to kResidualRhs right-hand sides per pass, one wavefront
+// per row: R[q] = A X[q] - B[q] and the component-wise backward error
+// eta[q] = max_i |r_qi| / (sum_j |a_ij| |x_qj| + |b_qi|) (harmonic.c:932-969),
+// reduced with an integer max on the bit patterns of the non-negative
+// values, so the result does not depend on the order of the rows.
+constexpr int kResidualRhs = 8;
+
+__global__ void abs_multi_kernel(const double2 *x, double *ax, size_t count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) ax[i] = hypot(x[i].x, x[i].y);
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, size_t lda, int n, const double2 *X,
+                                                             const double *AX, const double2 *B, double2 *R,
+                                                             int nq, unsigned long long *eta_bits) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = (int)blockIdx.x * 4 + wave;
+    if (i >= n) return;
+    double yr[kResidualRhs], yi[kResidualRhs], den[kResidualRhs];
+#pragma unroll
+    for (int q = 0; q < kResidualRhs; ++q) yr[q] = yi[q] = den[q] = 0.0;
+    const double2 *row = A + (size_t)i * lda;
+    for (int j = lane; j < n; j += 64) {
+        const double2 z = row[j];
+        const double za = hypot(z.x, z.y);
+#pragma unroll
+        for (int q = 0; q < kResidualRhs; ++q)
+            if (q < nq) {
+                const double2 v = X[(size_t)q * n + j];
+                yr[q] += z.x * v.x - z.y * v.y;
+                yi[q] += z.x * v.y + z.y * v.x;
+                den[q] += za * AX[(size_t)q * n + j];
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < kResidualRhs; ++q)
+        if (q < nq) {
+            for (int off = 32; off > 0; off >>= 1) {
+                yr[q] += __shfl_xor(yr[q], off);
+                yi[q] += __shfl_xor(yi[q], off);
+                den[q] += __shfl_xor(den[q], off);
+            }
+        }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kResidualRhs; ++q)
+            if (q < nq) {
+                const double2 bi = B[(size_t)q * n + i];
+                const double rr = yr[q] - bi.x, ri = yi[q] - bi.y;
+                R[(size_t)q * n + i] = make_double2(rr, ri);
+                const double ra = hypot(rr, ri), d = den[q] + hypot(bi.x, bi.y);
+                const double eta = d > 0.0 ? ra / d : ra;
+                atomicMax(&eta_bits[q], (unsigned long long)__double_as_longlong(eta));
+            }
+    }
+}
+
+void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, const double2 *dB, double2 *dR,
+                    int nrhs, double *d_ax, unsigned long long *d_eta_bits, hipStream_t s) {
+    const size_t total = (size_t)n * nrhs;
+    hipLaunchKernelGGL(abs_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dX, d_ax, total);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, s, d_eta_bits, nrhs);
+    for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
+        const int nq = std::min(kResidualRhs, nrhs - q0);
+        const size_t off = (size_t)q0 * n;
+        hipLaunchKernelGGL(residual_multi_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, dA, lda, n,
+                           dX + off, d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 void mfma_layout_probe(double *host_out) {
     DeviceBuffer<double> d(1024);
     hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, 0, d.ptr);

@@ -152,6 +152,30 @@ _libc.malloc.restype = C.c_void_p
 _libc.malloc.argtypes = [C.c_size_t]
 
 
+class IncidentField(C.Structure):
+    """incident_field.h"""
+    _fields_ = [("wave_type", C.c_int), ("polarization", C.c_int), ("theta_deg", C.c_double),
+                ("phi_deg", C.c_double), ("theta", C.c_double), ("phi", C.c_double),
+                ("k_hat", C.c_double * 3), ("e_theta", C.c_double * 3), ("e_phi", C.c_double * 3),
+                ("E_amp", C.c_double), ("E0", C.c_double * 3), ("frequency", C.c_double),
+                ("omega", C.c_double), ("wave_number", C.c_double), ("phase", C.c_double),
+                ("pulse_k", C.c_double), ("pulse_alpha", C.c_double), ("pulse_beta", C.c_double)]
+
+
+class ScatteringOptions(C.Structure):
+    _fields_ = [("integration", C.c_int), ("parallel_threads", C.c_int)]
+
+
+class RcsOptions(C.Structure):
+    _fields_ = [("integration", C.c_int), ("surface_order", C.c_int), ("parallel_threads", C.c_int)]
+
+
+class RcsResult(C.Structure):
+    _fields_ = [("theta_deg", C.c_double), ("phi_deg", C.c_double), ("direction", C.c_double * 3),
+                ("radiation_vector", Complex * 3), ("transverse_vector", Complex * 3), ("sigma", C.c_double),
+                ("sigma_dbsm", C.c_double)]
+
+
 def c_malloc_copy(arr: np.ndarray, ctype):
     """malloc() a copy of a numpy array (the C side frees it with free())."""
     arr = np.ascontiguousarray(arr)
@@ -207,6 +231,8 @@ def mesh_arrays(m: Mesh) -> dict:
         "cell_edge_signs": as_array(m.cell_edge_signs, 4 * nq, np.int32).reshape(-1, 4),
         "edge_lengths": as_array(m.edge_lengths, ne),
         "edge_directions": as_array(m.edge_directions, 3 * ne).reshape(-1, 3),
+        "edge_vectors": as_array(m.edge_vectors, 3 * ne).reshape(-1, 3),
+        "edge_centers": as_array(m.edge_centers, 3 * ne).reshape(-1, 3),
         "cell_areas": as_array(m.cell_areas, nq),
     }
 

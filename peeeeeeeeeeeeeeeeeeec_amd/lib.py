@@ -39,6 +39,11 @@ class PeecPointReport(C.Structure):
                 ("ms_solve", C.c_double), ("ms_check", C.c_double)]
 
 
+class PeecRcsOptions(C.Structure):
+    _fields_ = [("wave_type", C.c_int), ("polarization", C.c_int), ("E_amp", C.c_double), ("phase", C.c_double),
+                ("excitation", C.c_int), ("rcs", api.RcsOptions)]
+
+
 class PeecKernelStats(C.Structure):
     _fields_ = [("launches", C.c_longlong), ("total_ms", C.c_double), ("flops", C.c_double),
                 ("bytes", C.c_double)]
@@ -89,7 +94,15 @@ EXPORTED = [
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
     "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats",
-    "peec_selftest_mfma_layout",
+    "peec_selftest_mfma_layout", "peec_rcs_options_default", "peec_engine_monostatic_rcs",
+    # incident_field.h / scattering.h / rcs.h
+    "incident_field_init", "incident_field_configure_harmonic", "incident_field_configure_pulse",
+    "incident_field_evaluate_time", "incident_field_evaluate_phasor", "incident_field_print_info",
+    "scattering_options_default", "scattering_build_harmonic_excitation", "scattering_build_time_excitation",
+    "scattering_zero_node_current_real", "scattering_zero_node_current_complex",
+    "scattering_print_harmonic_excitation_info", "scattering_print_time_excitation_info",
+    "rcs_options_default", "rcs_build_observation_direction", "rcs_compute_radiation_vector",
+    "rcs_compute_direction", "rcs_compute_far_electric_field", "rcs_print_result",
 ]
 
 
@@ -196,6 +209,27 @@ def load():
     _sig(lib, "peec_engine_set_profiling", C.c_int, C.c_void_p, C.c_int)
     _sig(lib, "peec_selftest_mfma_layout", C.c_int, _dp)
     _sig(lib, "peec_engine_kernel_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(PeecKernelStats))
+    _F = C.POINTER(api.IncidentField)
+    _sig(lib, "incident_field_configure_harmonic", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double,
+         C.c_double, C.c_double, C.c_double)
+    _sig(lib, "incident_field_configure_pulse", C.c_int, _F, C.c_int, C.c_double, C.c_double, C.c_double,
+         C.c_double, C.c_double)
+    _sig(lib, "incident_field_evaluate_phasor", C.c_int, _F, _dp, _cp)
+    _sig(lib, "incident_field_evaluate_time", C.c_int, _F, _dp, C.c_double, _dp)
+    _sig(lib, "scattering_options_default", None, C.POINTER(api.ScatteringOptions), C.c_int)
+    _sig(lib, "scattering_build_harmonic_excitation", C.c_int, _M, _F, C.POINTER(api.ScatteringOptions), _cp)
+    _sig(lib, "scattering_build_time_excitation", C.c_int, _M, _F, C.POINTER(api.ScatteringOptions),
+         C.c_double, _dp)
+    _sig(lib, "rcs_options_default", None, C.POINTER(api.RcsOptions), C.c_int)
+    _sig(lib, "rcs_build_observation_direction", C.c_int, C.c_double, C.c_double, _dp)
+    _sig(lib, "rcs_compute_radiation_vector", C.c_int, _M, _D, _cp, C.c_double, _dp,
+         C.POINTER(api.RcsOptions), _cp)
+    _sig(lib, "rcs_compute_direction", C.c_int, _M, _D, _F, C.POINTER(api.HarmonicSolution),
+         C.POINTER(api.RcsOptions), C.c_double, C.c_double, C.POINTER(api.RcsResult))
+    _sig(lib, "rcs_compute_far_electric_field", C.c_int, C.POINTER(api.RcsResult), C.c_double, C.c_double, _cp)
+    _sig(lib, "peec_rcs_options_default", None, C.POINTER(PeecRcsOptions))
+    _sig(lib, "peec_engine_monostatic_rcs", C.c_int, C.c_void_p, C.c_double, C.c_int, C.POINTER(PeecRcsOptions),
+         _dp, _dp, C.c_size_t, C.POINTER(api.RcsResult), _cp, C.POINTER(PeecPointReport))
     _lib = lib
     return lib
 
@@ -280,6 +314,65 @@ class Model:
         lib.complex_matrix_free(C.byref(L))
         lib.complex_matrix_free(C.byref(P))
         return Lc, Pc
+
+
+def plane_wave(frequency, theta_deg, phi_deg, polarization=1, E_amp=1.0, phase=0.0, wave_type=2):
+    """incident_field_configure_harmonic (wave_type 2 = COMPLEX)."""
+    lib = load()
+    f = api.IncidentField()
+    st = lib.incident_field_configure_harmonic(C.byref(f), wave_type, polarization, E_amp, frequency, theta_deg,
+                                               phi_deg, phase)
+    if st != 0:
+        raise RuntimeError("incident_field_configure_harmonic failed")
+    return f
+
+
+def excitation(model, field, integration=1):
+    """scattering_build_harmonic_excitation -> U (Ne complex)."""
+    lib = load()
+    o = api.ScatteringOptions()
+    lib.scattering_options_default(C.byref(o), 1)
+    o.integration = integration
+    U = np.zeros(model.ne, np.complex128)
+    st = lib.scattering_build_harmonic_excitation(C.byref(model.mesh), C.byref(field), C.byref(o),
+                                                  U.ctypes.data_as(_cp))
+    if st != 0:
+        raise RuntimeError("scattering_build_harmonic_excitation failed")
+    return U
+
+
+def radiation_vector(model, currents, wave_number, direction, integration=1, order=6):
+    """rcs_compute_radiation_vector -> F (3 complex)."""
+    lib = load()
+    o = api.RcsOptions()
+    lib.rcs_options_default(C.byref(o), 1)
+    o.integration, o.surface_order = integration, order
+    I = np.ascontiguousarray(currents, np.complex128)
+    d = np.ascontiguousarray(direction, np.float64)
+    F = np.zeros(3, np.complex128)
+    st = lib.rcs_compute_radiation_vector(C.byref(model.mesh), C.byref(model.dual), I.ctypes.data_as(_cp),
+                                          wave_number, d.ctypes.data_as(_dp), C.byref(o), F.ctypes.data_as(_cp))
+    if st != 0:
+        raise RuntimeError("rcs_compute_radiation_vector failed")
+    return F
+
+
+def rcs_direction(model, field, currents, theta_deg, phi_deg, integration=1, order=6):
+    """rcs_compute_direction on a solution with the given edge currents."""
+    lib = load()
+    o = api.RcsOptions()
+    lib.rcs_options_default(C.byref(o), 1)
+    o.integration, o.surface_order = integration, order
+    I = np.ascontiguousarray(currents, np.complex128)
+    sol = api.HarmonicSolution()
+    sol.n_edges, sol.n_nodes = model.ne, model.nv
+    sol.edge_current = C.cast(I.ctypes.data, C.POINTER(api.Complex))
+    r = api.RcsResult()
+    st = lib.rcs_compute_direction(C.byref(model.mesh), C.byref(model.dual), C.byref(field), C.byref(sol),
+                                   C.byref(o), theta_deg, phi_deg, C.byref(r))
+    if st != 0:
+        raise RuntimeError("rcs_compute_direction failed")
+    return r
 
 
 def complex_lu_factor_solve(Z, B):
@@ -396,6 +489,25 @@ class Engine:
             I = I.reshape(k, self.ne)
             V = V.reshape(k, self.nv)
         return st, z, I, V, rep
+
+    def monostatic_rcs(self, frequency, theta_deg, phi_deg, mode=0, polarization=1, E_amp=1.0, phase=0.0,
+                       excitation=1, integration=1, order=6, want_currents=False):
+        """peec_engine_monostatic_rcs. Returns (status, results, currents, report)."""
+        lib = load()
+        th = np.ascontiguousarray(np.atleast_1d(theta_deg), np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi_deg), th.shape), np.float64)
+        na = th.size
+        o = PeecRcsOptions()
+        lib.peec_rcs_options_default(C.byref(o))
+        o.polarization, o.E_amp, o.phase, o.excitation = polarization, E_amp, phase, excitation
+        o.rcs.integration, o.rcs.surface_order = integration, order
+        res = (api.RcsResult * na)()
+        I = np.zeros(na * self.ne, np.complex128) if want_currents else None
+        rep = PeecPointReport()
+        st = lib.peec_engine_monostatic_rcs(self.handle, float(frequency), int(mode), C.byref(o),
+                                            th.ctypes.data_as(_dp), ph.ctypes.data_as(_dp), na, res,
+                                            I.ctypes.data_as(_cp) if I is not None else None, C.byref(rep))
+        return st, list(res), (I.reshape(na, self.ne) if I is not None else None), rep
 
     def set_profiling(self, on=True):
         load().peec_engine_set_profiling(self.handle, int(on))

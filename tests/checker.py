@@ -10,6 +10,7 @@ Both take the same Mesh/DualMesh structs as the product library.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -75,6 +76,14 @@ def oracle():
         _sig(lib, "oracle_retarded_matrices", C.c_int, _M, _D,
              C.POINTER(api.InductanceOptions), C.POINTER(api.PotentialOptions), C.c_double,
              C.c_int, C.c_int, _dp, _dp, _cp, _cp)
+        _F = C.POINTER(api.IncidentField)
+        _sig(lib, "oracle_plane_wave", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+             C.c_double, C.c_double)
+        _sig(lib, "oracle_phasor", C.c_int, _F, _dp, _cp)
+        _sig(lib, "oracle_excitation", C.c_int, _M, _F, C.c_int, _cp)
+        _sig(lib, "oracle_radiation_vector", C.c_int, _M, _D, _cp, C.c_double, _dp, C.c_int, C.c_int, _cp)
+        _sig(lib, "oracle_rcs_direction", C.c_int, _M, _D, _F, _cp, C.c_int, C.c_int, C.c_double, C.c_double,
+             C.POINTER(api.RcsResult))
         _oracle = lib
     return _oracle
 
@@ -140,6 +149,16 @@ def ref():
         _sig(lib, "lu_factorize", C.c_int, _dp, C.c_size_t, C.POINTER(api.LUOptions),
              C.POINTER(api.LUFactorization))
         _sig(lib, "lu_solve", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp)
+        _F = C.POINTER(api.IncidentField)
+        _sig(lib, "incident_field_configure_harmonic", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double,
+             C.c_double, C.c_double, C.c_double)
+        _sig(lib, "scattering_options_default", None, C.POINTER(api.ScatteringOptions), C.c_int)
+        _sig(lib, "scattering_build_harmonic_excitation", C.c_int, _M, _F, C.POINTER(api.ScatteringOptions), _cp)
+        _sig(lib, "rcs_options_default", None, C.POINTER(api.RcsOptions), C.c_int)
+        _sig(lib, "rcs_compute_radiation_vector", C.c_int, _M, _D, _cp, C.c_double, _dp,
+             C.POINTER(api.RcsOptions), _cp)
+        _sig(lib, "rcs_compute_direction", C.c_int, _M, _D, _F, C.POINTER(api.HarmonicSolution),
+             C.POINTER(api.RcsOptions), C.c_double, C.c_double, C.POINTER(api.RcsResult))
         _ref = lib
     return _ref
 
@@ -208,6 +227,30 @@ class OracleModel:
 
     def edges(self):
         return api.as_array(self.mesh.edges, 2 * self.ne, np.int32).reshape(-1, 2)
+
+    def excitation(self, field, integration=1):
+        U = np.zeros(self.ne, np.complex128)
+        if oracle().oracle_excitation(C.byref(self.mesh), C.byref(field), integration, U.ctypes.data_as(_cp)):
+            raise RuntimeError("oracle excitation failed")
+        return U
+
+    def radiation(self, I, k, direction, integration=1, order=6):
+        I = np.ascontiguousarray(I, np.complex128)
+        d = np.ascontiguousarray(direction, np.float64)
+        F = np.zeros(3, np.complex128)
+        if oracle().oracle_radiation_vector(C.byref(self.mesh), C.byref(self.dual), I.ctypes.data_as(_cp), k,
+                                            d.ctypes.data_as(_dp), integration, order, F.ctypes.data_as(_cp)):
+            raise RuntimeError("oracle radiation vector failed")
+        return F
+
+    def rcs_direction(self, field, I, theta_deg, phi_deg, integration=1, order=6):
+        I = np.ascontiguousarray(I, np.complex128)
+        r = api.RcsResult()
+        if oracle().oracle_rcs_direction(C.byref(self.mesh), C.byref(self.dual), C.byref(field),
+                                         I.ctypes.data_as(_cp), integration, order, theta_deg, phi_deg,
+                                         C.byref(r)):
+            raise RuntimeError("oracle rcs failed")
+        return r
 
     def pat(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)
@@ -316,3 +359,22 @@ def oracle_port_solve(model: OracleModel, L, P, frequency, a, b):
     eta = oracle_backward_error(Z, x, rhs, model.ne)
     I, V = x[:model.ne], x[model.ne:]
     return V[a] - V[b], I, V, eta
+
+
+def oracle_plane_wave(frequency, theta_deg, phi_deg, polarization=1, E_amp=1.0, phase=0.0, wave_type=2):
+    """incident_field_configure_harmonic restated (wave_type 2 = COMPLEX)."""
+    f = api.IncidentField()
+    if oracle().oracle_plane_wave(C.byref(f), wave_type, polarization, E_amp, frequency, theta_deg, phi_deg, phase):
+        raise RuntimeError("oracle plane wave failed")
+    return f
+
+
+def oracle_backscatter_angles(field):
+    """Observation angles back toward the source, as main_rcs.c:1380-1440
+    derives them from -k_hat."""
+    bx, by, bz = -field.k_hat[0], -field.k_hat[1], -field.k_hat[2]
+    bt = math.acos(max(-1.0, min(1.0, bz))) * 180.0 / math.pi
+    bp = math.atan2(by, bx) * 180.0 / math.pi
+    if bp < 0.0:
+        bp += 360.0
+    return bt, bp

@@ -46,7 +46,8 @@ def test_struct_sizes_match_reference_layout():
     LUFactorization); the ctypes mirrors and include/*.h must agree."""
     expected = {"Mesh": 120, "QuadPatch": 104, "DualMesh": 64, "InductanceOptions": 40,
                 "PotentialOptions": 32, "ComplexLUFactorization": 64, "HarmonicSystem": 136,
-                "LUFactorization": 48}
+                "LUFactorization": 48, "IncidentField": 200, "ScatteringOptions": 8, "RcsOptions": 12,
+                "RcsResult": 152, "HarmonicSolution": 32}
     for name, size in expected.items():
         assert C.sizeof(getattr(api, name)) == size, name
 
@@ -63,6 +64,23 @@ def test_headers_compile_to_reference_layout(tmp_path):
     subprocess.run(["gcc", "-std=c17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     assert out == ["120", "104", "64", "40", "32", "64", "136", "48"]
+
+
+def test_rcs_headers_compile_to_reference_layout(tmp_path):
+    """incident_field.h / scattering.h / rcs.h: sizes and two field offsets as
+    gcc lays out the reference's headers (200, 8, 12, 152; E0 at 120,
+    pulse_beta at 192, sigma at 136)."""
+    src = tmp_path / "sz.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "rcs.h"\n#include "scattering.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu", sizeof(IncidentField), sizeof(ScatteringOptions),'
+        'sizeof(RcsOptions), sizeof(RcsResult), offsetof(IncidentField, E0), offsetof(IncidentField, pulse_beta),'
+        'offsetof(RcsResult, sigma));return 0;}')
+    exe = tmp_path / "sz"
+    import subprocess
+    subprocess.run(["gcc", "-std=c17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["200", "8", "12", "152", "120", "192", "136"]
 
 
 @pytest.mark.parametrize("which", ["plate_msh", "box"])

@@ -431,3 +431,114 @@ def test_retardation_remainder_brute_force():
         assert abs(got - brute) / abs(brute) < 2e-3
     finally:
         m.close()
+
+
+# ------------------------------------------------ plane wave, excitation, RCS
+
+RCS_CASES = [  # (frequency, theta, phi, polarization, wave type)
+    (1.0e8, 90.0, 0.0, 1, 2), (3.0e8, 37.0, 211.0, 0, 2), (5.0e7, 120.0, 45.0, 1, 0), (2.0e8, 10.0, 300.0, 0, 1)]
+
+
+@pytest.mark.skipif(not ck.ref_available(), reason="oracle/_ref not built")
+def test_rcs_oracle_bit_exact_against_reference_build():
+    """incident_field.c / scattering.c / rcs.c compiled from the reference:
+    the restated plane wave, both excitation rules, both far-field rules and
+    the RCS of one direction agree bit for bit (one OpenMP thread, so the
+    reference sums the edges in order)."""
+    import tests.golden.make_golden as mg
+    ref = ck.ref()
+    xyz, quads = meshgen.box(1.0, 0.6, 0.4, 5, 3, 2)
+    m, d = mg.ref_model(xyz=xyz, quads=quads)
+    o = ck.OracleModel(xyz, quads)
+    rng = np.random.default_rng(11)
+    I = rng.standard_normal(o.ne) + 1j * rng.standard_normal(o.ne)
+    try:
+        for f, th, ph, pol, wt in RCS_CASES:
+            fr = api.IncidentField()
+            assert ref.incident_field_configure_harmonic(C.byref(fr), wt, pol, 2.5, f, th, ph, 0.3) == 0
+            fo = ck.oracle_plane_wave(f, th, ph, pol, 2.5, 0.3, wt)
+            assert bytes(fr) == bytes(fo)
+            for integ in (0, 1):
+                so = api.ScatteringOptions()
+                ref.scattering_options_default(C.byref(so), 1)
+                so.integration = integ
+                Ur = np.zeros(o.ne, np.complex128)
+                assert ref.scattering_build_harmonic_excitation(C.byref(m), C.byref(fr), C.byref(so),
+                                                                Ur.ctypes.data_as(ck._cp)) == 0
+                assert np.array_equal(Ur, o.excitation(fo, integ))
+            dvec = np.array([math.sin(0.7) * math.cos(2.0), math.sin(0.7) * math.sin(2.0), math.cos(0.7)])
+            for integ, order in ((0, 6), (1, 6), (1, 3)):
+                ro = api.RcsOptions()
+                ref.rcs_options_default(C.byref(ro), 1)
+                ro.integration, ro.surface_order = integ, order
+                Fr = np.zeros(3, np.complex128)
+                assert ref.rcs_compute_radiation_vector(C.byref(m), C.byref(d), I.ctypes.data_as(ck._cp),
+                                                        fr.wave_number, dvec.ctypes.data_as(ck._dp), C.byref(ro),
+                                                        Fr.ctypes.data_as(ck._cp)) == 0
+                assert np.array_equal(Fr, o.radiation(I, fo.wave_number, dvec, integ, order))
+                sol = api.HarmonicSolution()
+                sol.n_edges, sol.n_nodes = o.ne, o.nv
+                sol.edge_current = C.cast(I.ctypes.data, C.POINTER(api.Complex))
+                rr = api.RcsResult()
+                assert ref.rcs_compute_direction(C.byref(m), C.byref(d), C.byref(fr), C.byref(sol), C.byref(ro),
+                                                 th + 180.0, ph, C.byref(rr)) == 0
+                assert bytes(rr) == bytes(o.rcs_direction(fo, I, th + 180.0, ph, integ, order))
+    finally:
+        o.close()
+
+
+def test_excitation_is_the_edge_integral_of_the_phasor():
+    """Analytic rule = Gauss-Legendre (order 16) integral of E_hat . dl along
+    the edge; it reduces to the midpoint rule when k_hat is normal to the
+    edge (scattering.c:263-337)."""
+    xyz, quads = meshgen.plate(3, 2, 0.9, 0.6)
+    o = ck.OracleModel(xyz, quads)
+    try:
+        fo = ck.oracle_plane_wave(9.0e8, 63.0, 28.0, 1, 1.7, 0.2)
+        U = o.excitation(fo, 1)
+        x, w = np.polynomial.legendre.leggauss(16)
+        arr = api.mesh_arrays(o.mesh)
+        cen, vec = arr["edge_centers"].reshape(-1, 3), arr["edge_vectors"].reshape(-1, 3)
+        k_hat, E0 = np.array(fo.k_hat), np.array(fo.E0)
+        for e in range(o.ne):
+            pts = cen[e] + 0.5 * x[:, None] * vec[e]
+            ph = np.exp(1j * (-fo.wave_number * pts @ k_hat + fo.phase))
+            ref_val = 0.5 * np.sum(w * ph) * (E0 @ vec[e])
+            assert abs(U[e] - ref_val) <= 1e-13 * abs(E0 @ vec[e]) + 1e-300
+        # k_hat normal to the plate's edges: theta = 0 -> k_hat = -z
+        fz = ck.oracle_plane_wave(9.0e8, 0.0, 0.0, 1, 1.0, 0.0)
+        assert np.array_equal(o.excitation(fz, 1), o.excitation(fz, 0))
+    finally:
+        o.close()
+
+
+def test_radiation_vector_closed_forms():
+    """Observed normal to a flat plate the phase is constant on the surface,
+    so the dual-region rule integrates exp(j k r_hat.r') = 1 exactly and
+    equals the midpoint rule (F = sum I_e l_e); the transverse vector is
+    orthogonal to r_hat and sigma = k^2 eta0^2 |T|^2 / (4 pi E0^2) with
+    |T|^2 = |F|^2 - |r_hat.F|^2 (rcs.c:536-598,1100-1140)."""
+    xyz, quads = meshgen.plate(4, 3, 0.8, 0.6)
+    o = ck.OracleModel(xyz, quads)
+    rng = np.random.default_rng(3)
+    I = rng.standard_normal(o.ne) + 1j * rng.standard_normal(o.ne)
+    try:
+        k = 2 * math.pi * 4.0e8 / 299792458.0
+        z = np.array([0.0, 0.0, 1.0])
+        Fq = o.radiation(I, k, z, 1, 6)
+        Fm = o.radiation(I, k, z, 0, 6)
+        vec = api.mesh_arrays(o.mesh)["edge_vectors"].reshape(-1, 3)
+        assert np.abs(Fq - Fm).max() <= 1e-13 * np.abs(Fm).max()
+        assert np.abs(Fm - I @ vec).max() <= 1e-13 * np.abs(Fm).max()
+        fo = ck.oracle_plane_wave(4.0e8, 30.0, 40.0, 0, 2.0, 0.0)
+        r = o.rcs_direction(fo, I, 150.0, 220.0, 1, 6)
+        d = np.array(r.direction)
+        F = np.array([c.re + 1j * c.im for c in r.radiation_vector])
+        T = np.array([c.re + 1j * c.im for c in r.transverse_vector])
+        assert abs(d @ T) <= 1e-13 * np.abs(F).max()
+        t2 = np.vdot(F, F).real - abs(d @ F) ** 2
+        eta0 = math.sqrt(1.25663706212e-6 / 8.8541878128e-12)
+        assert r.sigma == pytest.approx(k * k * eta0 * eta0 * t2 / (4 * math.pi * 4.0), rel=1e-12)
+        assert r.sigma_dbsm == pytest.approx(10 * math.log10(r.sigma), rel=1e-14)
+    finally:
+        o.close()
