@@ -26,6 +26,7 @@
 #include "mesh.h"
 #include "potential.h"
 #include "incident_field.h"
+#include "lightning.h"
 #include "rcs.h"
 #include "scattering.h"
 
@@ -81,6 +82,12 @@ void peec_engine_options_default(PeecEngineOptions *options);
 int peec_engine_create(const Mesh *mesh, const DualMesh *dual,
                        const PeecEngineOptions *options, PeecEngine **engine);
 
+/* Same, with the static Lp (Ne x Ne) and P (Nv x Nv) given by the caller,
+ * row-major on the host (e.g. from matrix_import_dense_mtx): no assembly. */
+int peec_engine_create_with_matrices(const Mesh *mesh, const DualMesh *dual,
+                                     const PeecEngineOptions *options, const double *L,
+                                     const double *P, PeecEngine **engine);
+
 /* Solves one frequency point for n_ports port excitations on one factor.
  * z_in: n_ports values; edge_current / node_voltage: optional, rhs-major
  * (n_ports x Ne, n_ports x Nv). Returns 0, -1 (bad input / GPU failure),
@@ -115,6 +122,36 @@ int peec_engine_monostatic_rcs(PeecEngine *engine, double frequency, int mode,
                                const PeecRcsOptions *options, const double *theta_deg,
                                const double *phi_deg, size_t n_angles, RcsResult *results,
                                Complex *edge_current, PeecPointReport *report);
+
+/* Port impedance sweep: z_in[k] = Z_in(frequencies[k]) for one port
+ * (port_nodes[0] -> port_nodes[1]); reports[k] optional. Mixed mode reuses
+ * the row order along the sweep. Stops at the first failing point and
+ * returns its status. */
+int peec_engine_port_sweep(PeecEngine *engine, const double *frequencies, size_t n, int mode,
+                           const int port_nodes[2], Complex *z_in, PeecPointReport *reports);
+
+/* Fourier transform of the lightning current, I(f) = int i(t) e^{-j 2 pi f t} dt
+ * = K e^{-j w delay} (1 / (alpha + j w) - 1 / (beta + j w)). */
+int peec_lightning_spectrum(const LightningSource *source, double frequency, Complex *current);
+
+/* Time response of a port to the lightning current by frequency-domain
+ * superposition: V(f_k) = Z_in(f_k) I(f_k) at f_k = k df, k = 1..n_freq
+ * (a power of two), Z_in(0) extrapolated linearly from the two lowest
+ * points (f > 0 is required by the MNA system), and a real inverse FFT:
+ * 2 n_freq samples t_i = i / (2 n_freq df); voltage[i] = v(t_i) [V],
+ * current[i] = lightning_current(t_i). The window 1 / df must exceed the
+ * response's decay (wrap-around otherwise). Host-only. */
+int peec_port_transient_from_spectrum(const LightningSource *source, size_t n_freq, double df,
+                                      const Complex *z_in, double *time, double *current,
+                                      double *voltage);
+
+/* The frequency sweep on the engine (strike -> return node) followed by
+ * peec_port_transient_from_spectrum: the time-domain lightning workload.
+ * time/current/voltage: 2 n_freq values; z_in (n_freq) and reports (n_freq)
+ * optional. */
+int peec_engine_lightning_response(PeecEngine *engine, const LightningSource *source, size_t n_freq,
+                                   double df, int mode, double *time, double *current, double *voltage,
+                                   Complex *z_in, PeecPointReport *reports);
 
 /* Copies the static matrices (Ne x Ne, Nv x Nv, row-major) to the host. */
 int peec_engine_static_matrices(const PeecEngine *engine, double *L, double *P);

@@ -475,7 +475,21 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->reuse_pivots = 1;
 }
 
+static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o, const double *L_host,
+                         const double *P_host, PeecEngine **out);
+
 int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o, PeecEngine **out) {
+    return engine_create(mesh, dual, o, nullptr, nullptr, out);
+}
+
+int peec_engine_create_with_matrices(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o,
+                                     const double *L, const double *P, PeecEngine **out) {
+    if (!L || !P) return -1;
+    return engine_create(mesh, dual, o, L, P, out);
+}
+
+static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o, const double *L_host,
+                         const double *P_host, PeecEngine **out) {
     if (!mesh || !dual || !o || !out || !mesh->edges || mesh->n_edges == 0 || mesh->n_nodes == 0) return -1;
     if (!gl_order_supported(o->smooth_order) || !gl_order_supported(o->near_smooth_order)) {
         PEEC_ERR("unsupported smooth_order %d / near_smooth_order %d.", o->smooth_order, o->near_smooth_order);
@@ -507,8 +521,15 @@ int peec_engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineO
         e->d_err.resize(1);
         HIP_CHECK(hipMemsetAsync(e->d_err.ptr, 0, sizeof(int), s));
         AssemblyWork w;
-        e->edges->assemble_static(e->d_L.ptr, s, e->d_err.ptr, &w);
-        e->nodes->assemble_static(e->d_P.ptr, s, e->d_err.ptr, &w);
+        if (L_host && P_host) {
+            // static matrices supplied by the caller (e.g. read back from
+            // Matrix Market files): no assembly
+            HIP_CHECK(hipMemcpyAsync(e->d_L.ptr, L_host, sizeof(double) * e->ne * e->ne, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(e->d_P.ptr, P_host, sizeof(double) * e->nv * e->nv, hipMemcpyHostToDevice, s));
+        } else {
+            e->edges->assemble_static(e->d_L.ptr, s, e->d_err.ptr, &w);
+            e->nodes->assemble_static(e->d_P.ptr, s, e->d_err.ptr, &w);
+        }
         int err = 0;
         e->d_err.download(&err, 1, s);
         e->stream.sync();

@@ -176,6 +176,12 @@ class RcsResult(C.Structure):
                 ("sigma_dbsm", C.c_double)]
 
 
+class LightningSource(C.Structure):
+    """lightning.h"""
+    _fields_ = [("wave_type", C.c_int), ("K", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
+                ("delay", C.c_double), ("strike_node", C.c_size_t), ("return_node", C.c_size_t)]
+
+
 def c_malloc_copy(arr: np.ndarray, ctype):
     """malloc() a copy of a numpy array (the C side frees it with free())."""
     arr = np.ascontiguousarray(arr)

@@ -90,7 +90,7 @@ EXPORTED = [
     "harmonic_solution_free", "harmonic_build_system", "harmonic_factorize", "harmonic_build_rhs",
     "harmonic_solve", "harmonic_compute_residual", "harmonic_print_info",
     # peec_gpu.h
-    "peec_engine_options_default", "peec_engine_create", "peec_engine_solve",
+    "peec_engine_options_default", "peec_engine_create", "peec_engine_create_with_matrices", "peec_engine_solve",
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
     "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats",
@@ -103,6 +103,14 @@ EXPORTED = [
     "scattering_print_harmonic_excitation_info", "scattering_print_time_excitation_info",
     "rcs_options_default", "rcs_build_observation_direction", "rcs_compute_radiation_vector",
     "rcs_compute_direction", "rcs_compute_far_electric_field", "rcs_print_result",
+    # lightning.h and the frequency-domain transient (peec_gpu.h)
+    "lightning_source_default", "lightning_source_validate", "lightning_current", "lightning_peak",
+    "lightning_build_node_current", "lightning_find_nearest_node", "lightning_print_info",
+    "peec_engine_port_sweep", "peec_lightning_spectrum", "peec_port_transient_from_spectrum",
+    "peec_engine_lightning_response",
+    # matrix_export.h
+    "matrix_export_dense_mtx", "matrix_export_incidence_mtx", "matrix_export_resistance_mtx",
+    "matrix_export_peec_set", "matrix_import_dense_mtx",
 ]
 
 
@@ -197,6 +205,8 @@ def load():
          C.POINTER(api.HarmonicSolution), C.c_int, C.POINTER(api.HarmonicResidual))
     _sig(lib, "peec_engine_options_default", None, C.POINTER(PeecEngineOptions))
     _sig(lib, "peec_engine_create", C.c_int, _M, _D, C.POINTER(PeecEngineOptions), C.POINTER(C.c_void_p))
+    _sig(lib, "peec_engine_create_with_matrices", C.c_int, _M, _D, C.POINTER(PeecEngineOptions), _dp, _dp,
+         C.POINTER(C.c_void_p))
     _sig(lib, "peec_engine_solve", C.c_int, C.c_void_p, C.c_double, C.c_int, _ip, C.c_size_t, _cp, _cp,
          _cp, C.POINTER(PeecPointReport))
     _sig(lib, "peec_engine_static_matrices", C.c_int, C.c_void_p, _dp, _dp)
@@ -230,6 +240,26 @@ def load():
     _sig(lib, "peec_rcs_options_default", None, C.POINTER(PeecRcsOptions))
     _sig(lib, "peec_engine_monostatic_rcs", C.c_int, C.c_void_p, C.c_double, C.c_int, C.POINTER(PeecRcsOptions),
          _dp, _dp, C.c_size_t, C.POINTER(api.RcsResult), _cp, C.POINTER(PeecPointReport))
+    _sig(lib, "matrix_export_dense_mtx", C.c_int, C.c_char_p, _dp, C.c_size_t, C.c_size_t)
+    _sig(lib, "matrix_export_incidence_mtx", C.c_int, C.c_char_p, _M)
+    _sig(lib, "matrix_export_resistance_mtx", C.c_int, C.c_char_p, C.POINTER(api.ResistanceMatrix))
+    _sig(lib, "matrix_export_peec_set", C.c_int, C.c_char_p, _M, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.PotentialMatrix), C.POINTER(api.Matrix), C.POINTER(api.ResistanceMatrix))
+    _sig(lib, "matrix_import_dense_mtx", C.c_int, C.c_char_p, C.POINTER(_dp), C.POINTER(C.c_size_t),
+         C.POINTER(C.c_size_t))
+    _S = C.POINTER(api.LightningSource)
+    _sig(lib, "lightning_source_default", None, _S)
+    _sig(lib, "lightning_source_validate", C.c_int, _S, C.c_size_t)
+    _sig(lib, "lightning_current", C.c_double, _S, C.c_double)
+    _sig(lib, "lightning_peak", C.c_int, _S, _dp, _dp)
+    _sig(lib, "lightning_build_node_current", C.c_int, _S, C.c_size_t, C.c_double, _dp)
+    _sig(lib, "lightning_find_nearest_node", C.c_int, _M, _dp, C.POINTER(C.c_size_t), _dp)
+    _sig(lib, "peec_engine_port_sweep", C.c_int, C.c_void_p, _dp, C.c_size_t, C.c_int, _ip, _cp,
+         C.POINTER(PeecPointReport))
+    _sig(lib, "peec_lightning_spectrum", C.c_int, _S, C.c_double, _cp)
+    _sig(lib, "peec_port_transient_from_spectrum", C.c_int, _S, C.c_size_t, C.c_double, _cp, _dp, _dp, _dp)
+    _sig(lib, "peec_engine_lightning_response", C.c_int, C.c_void_p, _S, C.c_size_t, C.c_double, C.c_int, _dp,
+         _dp, _dp, _cp, C.POINTER(PeecPointReport))
     _lib = lib
     return lib
 
@@ -375,6 +405,26 @@ def rcs_direction(model, field, currents, theta_deg, phi_deg, integration=1, ord
     return r
 
 
+def lightning_source(K=102900.0, alpha=1500.0, beta=1.0e6, delay=0.0, strike=0, ret=0):
+    s = api.LightningSource()
+    load().lightning_source_default(C.byref(s))
+    s.K, s.alpha, s.beta, s.delay, s.strike_node, s.return_node = K, alpha, beta, delay, strike, ret
+    return s
+
+
+def transient_from_spectrum(source, df, z_in):
+    """peec_port_transient_from_spectrum: (time, current, voltage)."""
+    z = np.ascontiguousarray(z_in, np.complex128)
+    n = z.size
+    t, i, v = np.zeros(2 * n), np.zeros(2 * n), np.zeros(2 * n)
+    st = load().peec_port_transient_from_spectrum(C.byref(source), n, df, z.ctypes.data_as(_cp),
+                                                  t.ctypes.data_as(_dp), i.ctypes.data_as(_dp),
+                                                  v.ctypes.data_as(_dp))
+    if st != 0:
+        raise RuntimeError("peec_port_transient_from_spectrum failed")
+    return t, i, v
+
+
 def complex_lu_factor_solve(Z, B):
     """Factor Z with complex_lu_factorize and solve the columns of B (n x k)."""
     lib = load()
@@ -428,7 +478,7 @@ class Engine:
     MIXED = 1
 
     def __init__(self, model: Model, retarded=False, smooth_order=2, near_smooth_order=4, block_size=0,
-                 **opts):
+                 static_matrices=None, **opts):
         lib = load()
         self.model = model
         o = PeecEngineOptions()
@@ -441,7 +491,14 @@ class Engine:
             setattr(o, k, v)
         self.options = o
         h = C.c_void_p()
-        st = lib.peec_engine_create(C.byref(model.mesh), C.byref(model.dual), C.byref(o), C.byref(h))
+        if static_matrices is not None:
+            Lh = np.ascontiguousarray(static_matrices[0], np.float64)
+            Ph = np.ascontiguousarray(static_matrices[1], np.float64)
+            assert Lh.shape == (model.ne, model.ne) and Ph.shape == (model.nv, model.nv)
+            st = lib.peec_engine_create_with_matrices(C.byref(model.mesh), C.byref(model.dual), C.byref(o),
+                                                      Lh.ctypes.data_as(_dp), Ph.ctypes.data_as(_dp), C.byref(h))
+        else:
+            st = lib.peec_engine_create(C.byref(model.mesh), C.byref(model.dual), C.byref(o), C.byref(h))
         if st != 0:
             raise RuntimeError(f"peec_engine_create failed ({st})")
         self.handle = h
@@ -509,6 +566,26 @@ class Engine:
                                             I.ctypes.data_as(_cp) if I is not None else None, C.byref(rep))
         return st, list(res), (I.reshape(na, self.ne) if I is not None else None), rep
 
+    def port_sweep(self, frequencies, port, mode=0):
+        """peec_engine_port_sweep: (status, z_in, reports)."""
+        f = np.ascontiguousarray(frequencies, np.float64)
+        z = np.zeros(f.size, np.complex128)
+        reps = (PeecPointReport * f.size)()
+        p = np.ascontiguousarray(port, np.int32)
+        st = load().peec_engine_port_sweep(self.handle, f.ctypes.data_as(_dp), f.size, int(mode), p.ctypes.data_as(_ip),
+                                           z.ctypes.data_as(_cp), reps)
+        return st, z, list(reps)
+
+    def lightning_response(self, source, n_freq, df, mode=0):
+        """peec_engine_lightning_response: (status, time, current, voltage, z_in, reports)."""
+        t, i, v = np.zeros(2 * n_freq), np.zeros(2 * n_freq), np.zeros(2 * n_freq)
+        z = np.zeros(n_freq, np.complex128)
+        reps = (PeecPointReport * n_freq)()
+        st = load().peec_engine_lightning_response(self.handle, C.byref(source), n_freq, df, int(mode),
+                                                   t.ctypes.data_as(_dp), i.ctypes.data_as(_dp),
+                                                   v.ctypes.data_as(_dp), z.ctypes.data_as(_cp), reps)
+        return st, t, i, v, z, list(reps)
+
     def set_profiling(self, on=True):
         load().peec_engine_set_profiling(self.handle, int(on))
 
@@ -524,3 +601,21 @@ class Engine:
     @property
     def smooth_evaluations(self):
         return load().peec_engine_smooth_evaluations(self.handle)
+
+
+def export_dense_mtx(path, M):
+    M = np.ascontiguousarray(M, np.float64)
+    if load().matrix_export_dense_mtx(path.encode(), M.ctypes.data_as(_dp), M.shape[0], M.shape[1]) != 0:
+        raise RuntimeError("matrix_export_dense_mtx failed")
+
+
+def import_dense_mtx(path):
+    """matrix_import_dense_mtx -> (rows x cols) array."""
+    lib = load()
+    p = _dp()
+    r, c = C.c_size_t(), C.c_size_t()
+    if lib.matrix_import_dense_mtx(path.encode(), C.byref(p), C.byref(r), C.byref(c)) != 0:
+        raise RuntimeError("matrix_import_dense_mtx failed")
+    out = np.ctypeslib.as_array(p, shape=(r.value * c.value,)).copy().reshape(r.value, c.value)
+    C.CDLL(None).free(p)
+    return out

@@ -231,3 +231,21 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
     assert all(r[0] == 1 for r in reports[1:])
     # the reused order is kept at most points of the sweep
     assert sum(1 for r in reports[1:] if not r[1]) >= len(freqs) // 2
+
+
+def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path):
+    """The reference's own Lp/P of the 16 x 16 plate (golden), written and read
+    back as Matrix Market files, drive the engine without assembly: the port
+    solution matches the reference's to 1e-10."""
+    g = np.load(os.path.join(GOLD, "plate16.npz"))
+    plib.export_dense_mtx(str(tmp_path / "L.mtx"), g["L"])
+    plib.export_dense_mtx(str(tmp_path / "P.mtx"), g["P"])
+    L, P = plib.import_dense_mtx(str(tmp_path / "L.mtx")), plib.import_dense_mtx(str(tmp_path / "P.mtx"))
+    assert np.array_equal(L, g["L"]) and np.array_equal(P, g["P"])
+    with plib.Model(*meshgen.plate(16, 16)) as m:
+        eng = plib.Engine(m, static_matrices=(L, P))
+        st, z, I, V, rep = eng.solve(float(g["frequency"]), [(0, 288)], plib.Engine.FP64)
+        eng.close()
+    assert st == 0
+    assert abs(z[0] - g["z_in"][0]) <= 1e-10 * abs(g["z_in"][0])
+    assert np.abs(I[0] - g["I"]).max() <= 1e-10 * np.abs(g["I"]).max()
