@@ -238,10 +238,16 @@ def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path)
     back as Matrix Market files, drive the engine without assembly: the port
     solution matches the reference's to 1e-10."""
     g = np.load(os.path.join(GOLD, "plate16.npz"))
-    plib.export_dense_mtx(str(tmp_path / "L.mtx"), g["L"])
-    plib.export_dense_mtx(str(tmp_path / "P.mtx"), g["P"])
+    ne, nv = int(g["ne"]), int(g["nv"])
+    Lg, Pg = np.zeros((ne, ne)), np.zeros((nv, nv))
+    Lg[np.triu_indices(ne)] = g["L_upper"]
+    Pg[np.triu_indices(nv)] = g["P_upper"]
+    Lg = np.triu(Lg) + np.triu(Lg, 1).T
+    Pg = np.triu(Pg) + np.triu(Pg, 1).T
+    plib.export_dense_mtx(str(tmp_path / "L.mtx"), Lg)
+    plib.export_dense_mtx(str(tmp_path / "P.mtx"), Pg)
     L, P = plib.import_dense_mtx(str(tmp_path / "L.mtx")), plib.import_dense_mtx(str(tmp_path / "P.mtx"))
-    assert np.array_equal(L, g["L"]) and np.array_equal(P, g["P"])
+    assert np.array_equal(L, Lg) and np.array_equal(P, Pg)
     with plib.Model(*meshgen.plate(16, 16)) as m:
         eng = plib.Engine(m, static_matrices=(L, P))
         st, z, I, V, rep = eng.solve(float(g["frequency"]), [(0, 288)], plib.Engine.FP64)
