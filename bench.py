@@ -95,6 +95,34 @@ def cpu_baseline():
     }
 
 
+def calibrate_copy():
+    """A plain device-to-device copy of 1 GiB, 5 times: the known-size kernel
+    whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the LU
+    update (profiles/lu_update_traffic_c2.md)."""
+    import torch
+
+    assert torch.cuda.is_available(), "needs a GPU"
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 5}))
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the LU update measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration), read
+    from the committed summary; None when absent."""
+    path = os.path.join(ROOT, "profiles", "lu_update_traffic_c2.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -102,7 +130,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(independent frequency points per GPU)", "n_gpus": args.gpus}))
@@ -225,7 +258,7 @@ def main():
             "peak": PEAKS["fp32_mfma"] / 1e12,
             "unit": "TFLOP/s",
             "frac": gemm_tf / PEAKS["fp32_mfma"],
-            "traffic": None,
+            "traffic": traffic_from_profile(),
             "launches": gemm["launches"],
             "avg_launch_ms": gemm["ms"] / max(1, gemm["launches"]),
             "algorithmic_flops_per_launch": gemm["flops"] / max(1, gemm["launches"]),
