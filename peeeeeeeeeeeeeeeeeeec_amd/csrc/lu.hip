@@ -17,6 +17,7 @@
 #include <climits>
 #include <cmath>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "lu.h"
@@ -689,6 +690,8 @@ template <> struct Mfma<double2> {
     static constexpr int wave_col_blocks = 2;  // 64 x 32 wave tile
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int lk, int v) { return 4 * v + lk; }
+    __device__ static op re(const double2 &v) { return v.x; }
+    __device__ static op im(const double2 &v) { return v.y; }
 };
 template <> struct Mfma<float2> {
     using acc = v4f;
@@ -697,6 +700,8 @@ template <> struct Mfma<float2> {
     static constexpr int wave_col_blocks = 4;  // 64 x 64 wave tile
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static int row(int lk, int v) { return 4 * lk + v; }
+    __device__ static op re(const float2 &v) { return v.x; }
+    __device__ static op im(const float2 &v) { return v.y; }
 };
 template <> struct Mfma<double> {
     using acc = v4d;
@@ -705,6 +710,8 @@ template <> struct Mfma<double> {
     static constexpr int wave_col_blocks = 4;
     __device__ static acc mma(op a, op b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int lk, int v) { return 4 * v + lk; }
+    __device__ static op re(const double &v) { return v; }
+    __device__ static op im(const double &) { return 0.0; }
 };
 constexpr int kWaveRowBlocks = 4;  // 64 rows per wave, 128 per workgroup (2 x 2 waves)
 
@@ -743,6 +750,14 @@ __global__ void pack_u_kernel(const S *A, size_t lda, int k0, int kw, int col0, 
     const int col = cb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
     out[((size_t)ks * cb_stride + cb) * 64 + lane] =
         (col < ncols && k < kw) ? A[(size_t)(k0 + k) * lda + col0 + col] : szero<S>();
+}
+
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>)
+template <class F, int... I> __device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ inline void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // Workgroup tile index -> (tile row, tile column). Workgroups are dispatched
@@ -789,38 +804,65 @@ __global__ __launch_bounds__(256, 2) void gemm_packed_kernel(S *A, size_t lda, i
     const S *lp = Lp + (size_t)rb0 * 64 + lane;
     const S *up = Up + (size_t)cb0 * 64 + lane;
     const size_t lstride = (size_t)rb_pad * 64, ustride = (size_t)cb_pad * 64;
-    S a[MB], b[NB];
-    for (int i = 0; i < MB; ++i) a[i] = lp[i * 64];
-    for (int j = 0; j < NB; ++j) b[j] = up[j * 64];
-    for (int ks = 0; ks < ks_count; ++ks) {
-        // prefetch the next k step (the packed copies hold one spare step)
-        lp += lstride;
-        up += ustride;
-        S an[MB], bn[NB];
-        for (int i = 0; i < MB; ++i) an[i] = lp[i * 64];
-        for (int j = 0; j < NB; ++j) bn[j] = up[j * 64];
-        if constexpr (M::cplx) {
-            // (ar + j ai)(br + j bi): the two products of each accumulator
-            // are issued in separate sweeps to keep dependent MFMAs apart
-            for (int i = 0; i < MB; ++i)
-                for (int j = 0; j < NB; ++j) {
-                    cre[i][j] = M::mma((op)a[i].x, (op)b[j].x, cre[i][j]);
-                    cim[i][j] = M::mma((op)a[i].x, (op)b[j].y, cim[i][j]);
-                }
-            for (int i = 0; i < MB; ++i) {
-                const op nai = -(op)a[i].y;
-                for (int j = 0; j < NB; ++j) {
-                    cre[i][j] = M::mma(nai, (op)b[j].y, cre[i][j]);
-                    cim[i][j] = M::mma((op)a[i].y, (op)b[j].x, cim[i][j]);
-                }
-            }
-        } else {
-            for (int i = 0; i < MB; ++i)
-                for (int j = 0; j < NB; ++j) cre[i][j] = M::mma((op)a[i], (op)b[j], cre[i][j]);
-        }
-        for (int i = 0; i < MB; ++i) a[i] = an[i];
-        for (int j = 0; j < NB; ++j) b[j] = bn[j];
+    // the loads of step ks + 1 are issued before the MFMAs of step ks (the
+    // packed copies hold spare k steps, so the load past the last step stays
+    // in bounds). Operands are kept as plain scalars (arrays of vector types
+    // would go to scratch).
+    op ar0[MB], ai0[MB], br0[NB], bi0[NB], ar1[MB], ai1[MB], br1[NB], bi1[NB];
+#define PEEC_GEMM_LOAD(AR, AI, BR, BI, KS)                                      \
+    {                                                                          \
+        const S *l_ = lp + (size_t)(KS) * lstride, *u_ = up + (size_t)(KS) * ustride; \
+        for (int i = 0; i < MB; ++i) {                                         \
+            const S v_ = l_[i * 64];                                           \
+            AR[i] = Mfma<S>::re(v_);                                           \
+            AI[i] = Mfma<S>::im(v_);                                           \
+        }                                                                      \
+        for (int j = 0; j < NB; ++j) {                                         \
+            const S v_ = u_[j * 64];                                           \
+            BR[j] = Mfma<S>::re(v_);                                           \
+            BI[j] = Mfma<S>::im(v_);                                           \
+        }                                                                      \
     }
+#define PEEC_GEMM_STEP(AR, AI, BR, BI)                                         \
+    {                                                                          \
+        if constexpr (M::cplx) {                                               \
+            for (int i = 0; i < MB; ++i)                                       \
+                for (int j = 0; j < NB; ++j) {                                 \
+                    cre[i][j] = M::mma(AR[i], BR[j], cre[i][j]);               \
+                    cim[i][j] = M::mma(AR[i], BI[j], cim[i][j]);               \
+                }                                                              \
+            for (int i = 0; i < MB; ++i) {                                     \
+                const op nai_ = -AI[i];                                        \
+                for (int j = 0; j < NB; ++j) {                                 \
+                    cre[i][j] = M::mma(nai_, BI[j], cre[i][j]);                \
+                    cim[i][j] = M::mma(AI[i], BR[j], cim[i][j]);               \
+                }                                                              \
+            }                                                                  \
+        } else {                                                               \
+            for (int i = 0; i < MB; ++i)                                       \
+                for (int j = 0; j < NB; ++j) cre[i][j] = M::mma(AR[i], BR[j], cre[i][j]); \
+        }                                                                      \
+    }
+    PEEC_GEMM_LOAD(ar0, ai0, br0, bi0, 0);
+    for (int ks = 0; ks < ks_count; ++ks) {
+        PEEC_GEMM_LOAD(ar1, ai1, br1, bi1, ks + 1);
+        PEEC_GEMM_STEP(ar0, ai0, br0, bi0);
+        for (int i = 0; i < MB; ++i) {
+            ar0[i] = ar1[i];
+            ai0[i] = ai1[i];
+        }
+        for (int j = 0; j < NB; ++j) {
+            br0[j] = br1[j];
+            bi0[j] = bi1[j];
+        }
+        // issue order for the scheduler: the next step's loads first, then
+        // this step's MFMAs (otherwise the loads sink behind the MFMAs and
+        // every step waits for a full memory latency)
+        __builtin_amdgcn_sched_group_barrier(0x020, MB + NB, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, (M::cplx ? 4 : 1) * MB * NB, 0);
+    }
+#undef PEEC_GEMM_LOAD
+#undef PEEC_GEMM_STEP
     const int li = lane & 15, lk = lane >> 4;
     for (int i = 0; i < MB; ++i)
         for (int j = 0; j < NB; ++j) {
@@ -838,6 +880,126 @@ __global__ __launch_bounds__(256, 2) void gemm_packed_kernel(S *A, size_t lda, i
                 } else {
                     *c -= cre[i][j][v];
                 }
+            }
+        }
+}
+
+// The same update for the block widths the factorizations use (KS k steps of
+// 4, i.e. 64 or 128 columns of L21), one workgroup per CU. With every
+// workgroup running the same number of steps, a kernel that reads the C tile
+// only after its last MFMA makes all CUs wait on HBM together; here the C
+// tile is read into registers a few elements per k step, so its traffic runs
+// under the MFMAs and the end of the tile only subtracts and stores. The k
+// loop is unrolled, operands are loaded two steps ahead, and per step
+// scheduling barriers keep the loads (operands of step s + 2, then the
+// step's share of C) ahead of the step's MFMAs.
+template <class S, int KS>
+__global__ __launch_bounds__(256, 1) void gemm_stream_kernel(S *A, size_t lda, int row0, int col0, int m, int ncols,
+                                                             const S *__restrict__ Lp, const S *__restrict__ Up,
+                                                             int rb_pad, int cb_pad, const int *status) {
+    using M = Mfma<S>;
+    using acc_t = typename M::acc;
+    using op = typename M::op;
+    constexpr int MB = kWaveRowBlocks, NB = M::wave_col_blocks;
+    constexpr int BM = 32 * MB, BN = 32 * NB;
+    constexpr int CL = MB * NB * 4;  // C elements per lane
+    constexpr int PD = 2;            // operand prefetch distance (k steps)
+    if (status[0] != 0) return;
+    int tm, tn;
+    if (!gemm_tile((m + BM - 1) / BM, (ncols + BN - 1) / BN, tm, tn)) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
+    const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t cre[MB][NB], cim[MB][NB];
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            cre[a][b] = acc_t{0, 0, 0, 0};
+            cim[a][b] = acc_t{0, 0, 0, 0};
+        }
+    // C element e = (i * NB + j) * 4 + v of this lane: row (rb0 + i) * 16 +
+    // row(lk, v), column (cb0 + j) * 16 + li; rows and columns past the
+    // matrix read a clamped in-bounds address and are not stored
+    S cv[CL];
+    auto c_ptr = [&](int e) -> S * {
+        const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
+        const int row = min((rb0 + i) * 16 + M::row(lk, v), m - 1);
+        const int col = min((cb0 + j) * 16 + li, ncols - 1);
+        return A + (size_t)(row0 + row) * lda + col0 + col;
+    };
+    const S *lp = Lp + (size_t)rb0 * 64 + lane;
+    const S *up = Up + (size_t)cb0 * 64 + lane;
+    const size_t lstride = (size_t)rb_pad * 64, ustride = (size_t)cb_pad * 64;
+    op ar[PD + 1][MB], ai[PD + 1][MB], br[PD + 1][NB], bi[PD + 1][NB];
+    auto load_ops = [&](int buf, int ks) {
+        const S *l_ = lp + (size_t)ks * lstride, *u_ = up + (size_t)ks * ustride;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const S v_ = l_[i * 64];
+            ar[buf][i] = M::re(v_);
+            ai[buf][i] = M::im(v_);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const S v_ = u_[j * 64];
+            br[buf][j] = M::re(v_);
+            bi[buf][j] = M::im(v_);
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < PD && s < KS; ++s) load_ops(s, s);
+    static_for<KS>([&](auto step) {
+        constexpr int s = decltype(step)::value;
+        constexpr int e0 = s * CL / KS, e1 = (s + 1) * CL / KS;
+        if constexpr (s + PD < KS) load_ops((s + PD) % (PD + 1), s + PD);
+#pragma unroll
+        for (int e = e0; e < e1; ++e) cv[e] = *c_ptr(e);
+        __builtin_amdgcn_sched_barrier(0);
+        constexpr int b = s % (PD + 1);
+        if constexpr (M::cplx) {
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
+                    cim[i][j] = M::mma(ar[b][i], bi[b][j], cim[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < MB; ++i) {
+                const op nai = -ai[b][i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    cre[i][j] = M::mma(nai, bi[b][j], cre[i][j]);
+                    cim[i][j] = M::mma(ai[b][i], br[b][j], cim[i][j]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int col = (cb0 + j) * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = (rb0 + i) * 16 + M::row(lk, v);
+                const int e = (i * NB + j) * 4 + v;
+                S r = cv[e];
+                if constexpr (M::cplx) {
+                    r.x -= cre[i][j][v];
+                    r.y -= cim[i][j][v];
+                } else {
+                    r -= cre[i][j][v];
+                }
+                if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
             }
         }
 }
@@ -925,7 +1087,7 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(S *A, size_t lda, int 
 }
 
 // Packed-operand buffer sizes (elements) for a trailing update of m rows
-// and ncols columns with k width kw (one spare k step for the prefetch).
+// and ncols columns with k width kw (two spare k steps for the prefetch).
 template <class S>
 void gemm_pads(int m, int ncols, int &rb_pad, int &cb_pad) {
     constexpr int BM = 32 * kWaveRowBlocks;
@@ -943,7 +1105,7 @@ void pack_operands(S *A, size_t lda, int row0, int col0, int m, int ncols, int k
                    LUContext &ctx, hipStream_t s, int &rb_pad, int &cb_pad) {
     gemm_pads<S>(m, ncols, rb_pad, cb_pad);
     const int ks_count = (kw + 3) / 4;
-    const size_t nl = (size_t)(ks_count + 1) * rb_pad * 64, nu = (size_t)(ks_count + 1) * cb_pad * 64;
+    const size_t nl = (size_t)(ks_count + 2) * rb_pad * 64, nu = (size_t)(ks_count + 2) * cb_pad * 64;
     ctx.pack_l.reserve(nl * sizeof(S));
     ctx.pack_u.reserve(nu * sizeof(S));
     S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
@@ -969,8 +1131,15 @@ void gemm_packed(S *A, size_t lda, int row0, int col0, int m, int c_off, int nc,
     const int tiles = (rb_t / (2 * kWaveRowBlocks)) * (cb_t / (std::is_same<S, double2>::value ? 4 : 8));
     const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-    hipLaunchKernelGGL(gemm_packed_kernel<S>, dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
-                       ks_count, Lp, Up, rb_pad, cb_pad, status);
+    if (ks_count == 32)
+        hipLaunchKernelGGL((gemm_stream_kernel<S, 32>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
+                           Lp, Up, rb_pad, cb_pad, status);
+    else if (ks_count == 16)
+        hipLaunchKernelGGL((gemm_stream_kernel<S, 16>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
+                           Lp, Up, rb_pad, cb_pad, status);
+    else
+        hipLaunchKernelGGL(gemm_packed_kernel<S>, dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
+                           ks_count, Lp, Up, rb_pad, cb_pad, status);
     if (ctx.timer) {
         const double mnk = (double)m * nc * kw;
         ctx.timer->stop(tg, s, kKernelLuUpdate, (Elem<S>::cplx ? 8.0 : 2.0) * mnk,
@@ -1308,8 +1477,8 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         const int ks_count = (w + 3) / 4;
         int rb_pad, cb_pad;
         gemm_pads<S>(rest, rest, rb_pad, cb_pad);
-        ctx.pack_l.reserve((size_t)(ks_count + 1) * rb_pad * 64 * sizeof(S));
-        ctx.pack_u.reserve((size_t)(ks_count + 1) * cb_pad * 64 * sizeof(S));
+        ctx.pack_l.reserve((size_t)(ks_count + 2) * rb_pad * 64 * sizeof(S));
+        ctx.pack_u.reserve((size_t)(ks_count + 2) * cb_pad * 64 * sizeof(S));
         S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
         S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
         // critical path: L21, U12 of the next block column, its update
