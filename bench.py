@@ -252,7 +252,8 @@ def main():
                           for k in ("assembly", "fill", "factor", "solve", "check")},
         "static_assembly_s": t_static,
         "roofline": {
-            "kernel": "gemm_complex_kernel<float2> (LU trailing update, v_mfma_f32_16x16x4f32)",
+            "kernel": ("LU trailing update A22 -= L21 U12, complex FP32 on v_mfma_f32_16x16x4f32 "
+                       "(gemm_stream_kernel<float2, 32> + gemm_splitk_kernel<float2> of the look-ahead column)"),
             "bound": "mfma",
             "achieved": gemm_tf / 1e12,
             "peak": PEAKS["fp32_mfma"] / 1e12,

@@ -368,7 +368,7 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             for (int q = 0; q < nrhs; ++q) any = any || active[q];
             if (!any) break;
             lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
-                                               e->d_R.ptr, nrhs, *e->lu);
+                                               e->d_R.ptr, nrhs, *e->lu, true);
             e->d_flags.upload(active.data(), nrhs, s);
             hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr,
                                e->d_flags.ptr, n, nrhs);

@@ -33,7 +33,9 @@ struct LUContext {
     DeviceBuffer<unsigned char> pack_l, pack_u;  // MFMA-ordered trailing-update operands
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves
-    DeviceBuffer<unsigned char> diag_inv;         // inverses of the 32x32 diagonal blocks (solves)
+    DeviceBuffer<unsigned char> diag_inv;         // inverses of the 64x64 diagonal blocks (solves)
+    const void *dinv_factor = nullptr;            // factor whose inverses diag_inv holds
+    int dinv_n = 0;
     int lds_per_block = 64 * 1024;
     hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
     hipEvent_t ev_next = nullptr, ev_diag = nullptr;
@@ -75,10 +77,12 @@ void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *
               const double *d_col, V *dX, int nrhs, LUContext &ctx);
 
 // Same with a precomputed row permutation (perm[i] = original row of
-// position i after all swaps), see permutation_from_pivots.
+// position i after all swaps), see permutation_from_pivots. same_factor:
+// dLU holds the same factor as in the previous call on this context (its
+// diagonal-block inverses are reused).
 template <class S, class V>
 void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
-                       const double *d_col, V *dX, int nrhs, LUContext &ctx);
+                       const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor = false);
 void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s);
 
 // Convert an FP64 complex matrix to FP32 (for the mixed factor).

@@ -1637,99 +1637,115 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const S *M, size_t lda, V
     }
 }
 
-constexpr int kChainTB = 32;
+constexpr int kChainTB = 64;
+constexpr int kChainPad = kChainTB + 1;  // LDS row stride of the inverse block
 
-// Inverses of the 32 x 32 diagonal blocks of a factor, in FP64: the
-// unit-lower L part (to linv) and the upper U part (to uinv), both
-// identity-padded past row n. One workgroup per block; lanes 0-31 form the
-// columns of L^{-1}, lanes 32-63 those of U^{-1}. A zero or non-finite
-// diagonal of U sets bad |= 1.
+// Inverses of the 64 x 64 diagonal blocks of a factor, in FP64: the
+// unit-lower L part to linv, then the upper U part to uinv, both
+// identity-padded past row n. One workgroup per block; thread t owns row
+// t / 4 and columns 16 (t % 4) .. + 16 of X, and the block is inverted by
+// substitution against the identity (L X = I forward, U X = I backward) in
+// LDS. A zero or non-finite diagonal of U sets bad |= 1.
 template <class S, class V>
-__global__ __launch_bounds__(64) void diag_inverse_kernel(const S *M, size_t lda, int n,
-                                                          typename VecT<V>::A *linv, typename VecT<V>::A *uinv,
-                                                          int *bad) {
+__global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t lda, int n,
+                                                           typename VecT<V>::A *linv, typename VecT<V>::A *uinv,
+                                                           int *bad) {
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
     __shared__ T blk[TB][TB];
-    __shared__ T inv[2][TB][TB];
+    __shared__ T X[TB][TB];
     const int b = blockIdx.x, r0 = b * TB, rows = min(TB, n - r0);
     T one = ezero<T>();
     if constexpr (Elem<V>::cplx) one.x = 1.0; else one = 1.0;
-    for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
-        const int i = idx / TB, j = idx % TB;
-        blk[i][j] = (i < rows && j < rows) ? ld(M + (size_t)(r0 + i) * lda + r0 + j) : (i == j ? one : ezero<T>());
+    const int tid = threadIdx.x, i = tid >> 2, c0 = (tid & 3) * 16;
+    for (int idx = tid; idx < TB * TB; idx += 256) {
+        const int r = idx / TB, c = idx % TB;
+        blk[r][c] = (r < rows && c < rows) ? ld(M + (size_t)(r0 + r) * lda + r0 + c) : (r == c ? one : ezero<T>());
     }
+    for (int c = c0; c < c0 + 16; ++c) X[i][c] = (i == c) ? one : ezero<T>();
     __syncthreads();
-    const int j = threadIdx.x & (TB - 1);
-    if (threadIdx.x < TB) {
-        for (int i = 0; i < j; ++i) inv[0][i][j] = ezero<T>();
-        inv[0][j][j] = one;
-        for (int i = j + 1; i < TB; ++i) {
-            T acc = ezero<T>();
-            for (int t = j; t < i; ++t) acc = eadd(acc, emul(blk[i][t], inv[0][t][j]));
-            inv[0][i][j] = esub(ezero<T>(), acc);
+    // L X = I: row k is final at step k; rows below subtract L[i][k] X[k][:]
+    for (int k = 0; k < TB - 1; ++k) {
+        if (i > k) {
+            const T l = blk[i][k];
+            for (int c = c0; c < c0 + 16; ++c) X[i][c] = esub(X[i][c], emul(l, X[k][c]));
         }
-    } else {
-        const T d = blk[j][j];
+        __syncthreads();
+    }
+    for (int c = c0; c < c0 + 16; ++c) linv[((size_t)b * TB + i) * TB + c] = X[i][c];
+    __syncthreads();
+    for (int c = c0; c < c0 + 16; ++c) X[i][c] = (i == c) ? one : ezero<T>();
+    if (tid < TB) {
+        const T d = blk[tid][tid];
         if constexpr (Elem<V>::cplx) {
             const double den = d.x * d.x + d.y * d.y;
             if (!isfinite(den) || den == 0.0) atomicOr(bad, 1);
         } else {
             if (d == 0.0 || !isfinite(d)) atomicOr(bad, 1);
         }
-        for (int i = j + 1; i < TB; ++i) inv[1][i][j] = ezero<T>();
-        inv[1][j][j] = ediv(one, d);
-        for (int i = j - 1; i >= 0; --i) {
-            T acc = ezero<T>();
-            for (int t = i + 1; t <= j; ++t) acc = eadd(acc, emul(blk[i][t], inv[1][t][j]));
-            inv[1][i][j] = ediv(esub(ezero<T>(), acc), blk[i][i]);
-        }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
-        linv[(size_t)b * TB * TB + idx] = inv[0][idx / TB][idx % TB];
-        uinv[(size_t)b * TB * TB + idx] = inv[1][idx / TB][idx % TB];
+    // U X = I backward: scale row k by 1 / U[k][k], then rows above subtract
+    // U[i][k] X[k][:]
+    for (int k = TB - 1; k >= 0; --k) {
+        if (i == k) {
+            const T r = ediv(one, blk[k][k]);
+            for (int c = c0; c < c0 + 16; ++c) X[k][c] = emul(r, X[k][c]);
+        }
+        __syncthreads();
+        if (i < k) {
+            const T u = blk[i][k];
+            for (int c = c0; c < c0 + 16; ++c) X[i][c] = esub(X[i][c], emul(u, X[k][c]));
+        }
+        __syncthreads();
     }
+    for (int c = c0; c < c0 + 16; ++c) uinv[((size_t)b * TB + i) * TB + c] = X[i][c];
 }
 
-// Triangular solve in one cooperative launch: workgroup w owns the 32-row
+// Triangular solve in one cooperative launch: workgroup w owns the 64-row
 // block b (b = w forward, b = nblk-1-w backward). It accumulates the
 // off-diagonal products of every block it depends on as soon as that block's
 // flag shows it final, then applies the precomputed inverse of its diagonal
-// block (one 32 x 32 product instead of 32 dependent substitution steps),
-// writes its rows and raises its flag. Up to 8 right-hand sides per launch;
+// block (one 64 x 64 product instead of 64 dependent substitution steps),
+// writes its rows and raises its flag. Thread t works on row t / 4 and the
+// 16 columns 16 (t % 4) .. + 16 of each block, so the four threads of a row
+// read 64 consecutive elements; the factor entries of the next dependency
+// are loaded before its flag is polled. Up to 8 right-hand sides per launch;
 // all arithmetic in FP64 (FP32 factors are widened on load).
-
 template <class S, class V, bool UPPER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, V *X, int n, int nrhs,
                                                          const typename VecT<V>::A *dinv, int *flags, int epoch,
                                                          int *bad) {
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
-    __shared__ T dblk[TB][TB];
-    __shared__ T z[8][TB];
-    __shared__ T red[8][8][TB];
+    __shared__ T dblk[TB * kChainPad];
+    __shared__ T xs[8][TB];
+    __shared__ int s_abort;
     const int nblk = (n + TB - 1) / TB;
     const int b = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x;
     const int r0 = b * TB, rows = min(TB, n - r0);
     const int tid = threadIdx.x;
-    const int r = tid % TB, cg = tid / TB;  // row in block, column group (8 groups of 4 columns)
+    const int i = tid >> 2, qt = tid & 3, c0 = qt * 16;
+    const int ri = min(i, rows - 1);  // clamped row for loads
     Regs<T, 8> acc;
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc.set(q, ezero<T>());
-    // inverse of the diagonal block into LDS
-    for (int idx = tid; idx < TB * TB; idx += 256) dblk[idx / TB][idx % TB] = dinv[(size_t)b * TB * TB + idx];
-    // off-diagonal contributions, in the order the blocks become final; each
-    // round takes every dependency already final (at least one)
-    __shared__ int s_ready, s_abort;
+    for (int idx = tid; idx < TB * TB; idx += 256)
+        dblk[(idx / TB) * kChainPad + idx % TB] = dinv[(size_t)b * TB * TB + idx];
     const int ndeps = UPPER ? nblk - 1 - b : b;
-    int d = 0;
-    while (d < ndeps) {
+    const S *mrow = M + (size_t)(r0 + ri) * lda;
+    for (int d = 0; d < ndeps; ++d) {
+        const int jb = UPPER ? nblk - 1 - d : d;
+        const int cb = jb * TB, cols = min(TB, n - cb);
+        // this thread's 16 factor entries of the dependency (clamped columns
+        // past n are multiplied by zero below)
+        Regs<T, 16> m;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) m.set(c, ld(mrow + cb + min(c0 + c, cols - 1)));
         if (tid == 0) {
-            int ready = d, abort = 0;
             unsigned spins = 0;
-            const int jb0 = UPPER ? nblk - 1 - d : d;
-            while (__hip_atomic_load(&flags[jb0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            int abort = 0;
+            while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
                     atomicOr(bad, 2);
@@ -1737,59 +1753,48 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
                     break;
                 }
             }
-            if (!abort) {
-                ready = d + 1;
-                while (ready < ndeps && ready < d + 16) {
-                    const int jb = UPPER ? nblk - 1 - ready : ready;
-                    if (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) break;
-                    ++ready;
-                }
-            }
-            s_ready = ready;
             s_abort = abort;
         }
         __syncthreads();
         if (s_abort) return;
-        const int upto = s_ready;
-        for (; d < upto; ++d) {
-            const int jb = UPPER ? nblk - 1 - d : d;
-            const int c0 = jb * TB, cols = min(TB, n - c0);
-            if (r < rows) {
-                const S *mrow = M + (size_t)(r0 + r) * lda + c0;
-                for (int c = cg * 4; c < cg * 4 + 4 && c < cols; ++c) {
-                    const T m = ld(mrow + c);
+        for (int idx = tid; idx < nrhs * TB; idx += 256) {
+            const int q = idx / TB, c = idx % TB;
+            T xv = ezero<T>();
+            if (c < cols) sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + cb + c), xv);
+            xs[q][c] = xv;
+        }
+        __syncthreads();
 #pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (q < nrhs) {
-                            T xv;
-                            sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + c0 + c), xv);
-                            acc.set(q, eadd(acc.get(q), emul(m, xv)));
-                        }
-                }
-            }
+        for (int c = 0; c < 16; ++c) {
+            const T mv = m.get(c);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < nrhs) acc.set(q, eadd(acc.get(q), emul(mv, xs[q][c0 + c])));
         }
         __syncthreads();
     }
-    // reduce the 8 column groups; z = x_b - sum
+    // row sums over the four threads of a row; z = x_b - sum into xs
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-        if (q < nrhs) red[q][cg][r] = acc.get(q);
-    __syncthreads();
-    if (tid < nrhs * TB) {
-        const int q = tid / TB, i = tid % TB;
-        T sum = ezero<T>();
-        for (int g2 = 0; g2 < 8; ++g2) sum = eadd(sum, red[q][g2][i]);
-        z[q][i] = i < rows ? esub(X[(size_t)q * n + r0 + i], sum) : ezero<T>();
-    }
+        if (q < nrhs) {
+            T v = acc.get(q);
+            v = eadd(v, eshfl_xor(v, 1));
+            v = eadd(v, eshfl_xor(v, 2));
+            if (qt == 0) xs[q][i] = i < rows ? esub(X[(size_t)q * n + r0 + i], v) : ezero<T>();
+        }
     __syncthreads();
     // x_b = D^{-1} z; publish the solved rows with cache-bypassing stores,
     // wait for them to complete, then raise the flag (no fences needed)
-    if (tid < nrhs * TB) {
-        const int q = tid / TB, i = tid % TB;
-        T xv = ezero<T>();
-        for (int k = 0; k < TB; ++k) xv = eadd(xv, emul(dblk[i][k], z[q][k]));
-        if (i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), xv);
-    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (q < nrhs) {
+            T v = ezero<T>();
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v = eadd(v, emul(dblk[i * kChainPad + c0 + k], xs[q][c0 + k]));
+            v = eadd(v, eshfl_xor(v, 1));
+            v = eadd(v, eshfl_xor(v, 2));
+            if (qt == (q & 3) && i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), v);
+        }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1828,12 +1833,12 @@ void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *
     for (int k = 0; k < n; ++k) std::swap(perm[k], perm[ipiv[k]]);
     DeviceBuffer<int> d_perm;
     d_perm.upload(perm, s);
-    lu_solve_permuted(dLU, n, lda, d_perm.ptr, d_row, d_col, dX, nrhs, ctx);
+    lu_solve_permuted(dLU, n, lda, d_perm.ptr, d_row, d_col, dX, nrhs, ctx, false);
 }
 
 template <class S, class V>
 void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
-                       const double *d_col, V *dX, int nrhs, LUContext &ctx) {
+                       const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor) {
     hipStream_t s = ctx.stream;
     const size_t total = (size_t)n * nrhs;
     if (ctx.solve_tmp.count < total * sizeof(V) / sizeof(double) + 1) ctx.solve_tmp.resize(total * sizeof(V) / sizeof(double) + 1);
@@ -1850,11 +1855,18 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
     const int ep = ctx.epoch + 1;
     ctx.epoch += 8 * ((nrhs + 7) / 8) + 1;
     using T = typename VecT<V>::A;
+    if (ctx.diag_inv.count < (size_t)2 * nblk * kChainTB * kChainTB * sizeof(T)) ctx.dinv_factor = nullptr;
     ctx.diag_inv.reserve((size_t)2 * nblk * kChainTB * kChainTB * sizeof(T));
     T *linv = reinterpret_cast<T *>(ctx.diag_inv.ptr);
     T *uinv = linv + (size_t)nblk * kChainTB * kChainTB;
-    hipLaunchKernelGGL((diag_inverse_kernel<S, V>), dim3(nblk), dim3(64), 0, s, dLU, lda, n, linv, uinv,
-                       ctx.solve_bad.ptr);
+    // the block inverses depend on the factor only: computed again unless the
+    // caller solves with the factor of the previous call
+    if (!(same_factor && ctx.dinv_factor == dLU && ctx.dinv_n == n)) {
+        hipLaunchKernelGGL((diag_inverse_kernel<S, V>), dim3(nblk), dim3(256), 0, s, dLU, lda, n, linv, uinv,
+                           ctx.solve_bad.ptr);
+        ctx.dinv_factor = dLU;
+        ctx.dinv_n = n;
+    }
     bool chained =
         chain_solve<S, V, false>(dLU, n, lda, tmp, nrhs, linv, ctx.flags.ptr, ep, ctx.solve_bad.ptr, ctx) &&
         chain_solve<S, V, true>(dLU, n, lda, tmp, nrhs, uinv, ctx.flags.ptr + nblk, ep, ctx.solve_bad.ptr, ctx);
@@ -1895,11 +1907,11 @@ template void lu_solve<float2, double2>(const float2 *, int, size_t, const int *
 template void lu_solve<double, double>(const double *, int, size_t, const int *, const double *, const double *,
                                        double *, int, LUContext &);
 template void lu_solve_permuted<double2, double2>(const double2 *, int, size_t, const int *, const double *,
-                                                  const double *, double2 *, int, LUContext &);
+                                                  const double *, double2 *, int, LUContext &, bool);
 template void lu_solve_permuted<float2, double2>(const float2 *, int, size_t, const int *, const double *,
-                                                 const double *, double2 *, int, LUContext &);
+                                                 const double *, double2 *, int, LUContext &, bool);
 template void lu_solve_permuted<double, double>(const double *, int, size_t, const int *, const double *,
-                                                const double *, double *, int, LUContext &);
+                                                const double *, double *, int, LUContext &, bool);
 
 void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s) {
     std::vector<int> ipiv(n), perm(n);
