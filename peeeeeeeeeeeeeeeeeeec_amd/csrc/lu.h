@@ -27,7 +27,6 @@ struct LUContext {
     int cus = 0;
     hipStream_t stream = nullptr;
     KernelTimer *timer = nullptr;      // optional per-kernel timing
-    DeviceBuffer<int> flags;           // triangular-solve chain flags
     DeviceBuffer<int> solve_bad;
     DeviceBuffer<double> solve_tmp;
     DeviceBuffer<unsigned char> pack_l, pack_u;  // MFMA-ordered trailing-update operands
@@ -39,7 +38,6 @@ struct LUContext {
     int lds_per_block = 64 * 1024;
     hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
     hipEvent_t ev_next = nullptr, ev_diag = nullptr;
-    int epoch = 0;
     explicit LUContext(hipStream_t s);
     ~LUContext();
     LUContext(const LUContext &) = delete;

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -152,6 +153,19 @@ __device__ inline double wmag(double v) { return fabs(v); }
 __device__ inline float2 wshfl(float2 v, int src) { return make_float2(__shfl(v.x, src), __shfl(v.y, src)); }
 __device__ inline double2 wshfl(double2 v, int src) { return make_double2(__shfl(v.x, src), __shfl(v.y, src)); }
 __device__ inline double wshfl(double v, int src) { return __shfl(v, src); }
+// value of lane src (wave-uniform) of a wavefront
+__device__ inline float rdlane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ inline double rdlane(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline float2 wreadlane(float2 v, int src) { return make_float2(rdlane(v.x, src), rdlane(v.y, src)); }
+__device__ inline double2 wreadlane(double2 v, int src) { return make_double2(rdlane(v.x, src), rdlane(v.y, src)); }
+__device__ inline double wreadlane(double v, int src) { return rdlane(v, src); }
 
 __device__ inline double2 eshfl(double2 v, int src) { return make_double2(__shfl(v.x, src), __shfl(v.y, src)); }
 __device__ inline double eshfl(double v, int src) { return __shfl(v, src); }
@@ -1702,123 +1716,168 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t ld
     for (int c = c0; c < c0 + 16; ++c) uinv[((size_t)b * TB + i) * TB + c] = X[i][c];
 }
 
-// Triangular solve in one cooperative launch: workgroup w owns the 64-row
-// block b (b = w forward, b = nblk-1-w backward). It accumulates the
-// off-diagonal products of every block it depends on as soon as that block's
-// flag shows it final, then applies the precomputed inverse of its diagonal
-// block (one 64 x 64 product instead of 64 dependent substitution steps),
-// writes its rows and raises its flag. Thread t works on row t / 4 and the
-// 16 columns 16 (t % 4) .. + 16 of each block, so the four threads of a row
-// read 64 consecutive elements; the factor entries of the next dependency
-// are loaded before its flag is polled. Up to 8 right-hand sides per launch;
-// all arithmetic in FP64 (FP32 factors are widened on load).
+// Marker of a solved value not yet written: a signalling NaN, which no
+// arithmetic result can be (operations on NaNs return quiet NaNs).
+constexpr unsigned long long kChainEmpty = 0x7FF4DEADBEEF0001ull;
+
+__global__ void fill_u64_kernel(unsigned long long *p, size_t n, unsigned long long v) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ inline bool chain_empty(const double2 &v) {
+    return (unsigned long long)__double_as_longlong(v.x) == kChainEmpty ||
+           (unsigned long long)__double_as_longlong(v.y) == kChainEmpty;
+}
+__device__ inline bool chain_empty(double v) { return (unsigned long long)__double_as_longlong(v) == kChainEmpty; }
+
+// Triangular solve in one cooperative launch over 64-row blocks, processed
+// in solve order (b = w forward, b = nblk-1-w backward, w = 0, 1, ...);
+// workgroup g takes w = g, g + G, g + 2G, ... (G = grid size, at most what
+// fits on the device at once). Every block depends only on blocks of lower
+// w, each owned by a workgroup that reaches it after its own lower blocks,
+// so the chain always completes. The right-hand side comes from B; the
+// solution goes to Y, which holds kChainEmpty everywhere when the launch
+// starts. For its block, a workgroup accumulates the off-diagonal products
+// of the blocks it depends on in order, each as soon as that block's values
+// in Y are no longer empty (the values are their own ready flags: one memory
+// round trip per dependency, no fences), then applies the precomputed
+// inverse of its diagonal block (one 64 x 64 product instead of 64 dependent
+// substitution steps) and writes its rows to Y with cache-bypassing stores.
+// Thread t works on row t / 4 and the 16 columns 16 (t % 4) .. + 16 of each
+// block, so the four threads of a row read 64 consecutive elements; the
+// factor entries of a dependency are loaded before its values are polled.
+// Up to 8 right-hand sides per launch; all arithmetic in FP64 (FP32 factors
+// are widened on load).
 template <class S, class V, bool UPPER>
-__global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, V *X, int n, int nrhs,
-                                                         const typename VecT<V>::A *dinv, int *flags, int epoch,
-                                                         int *bad) {
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, const V *B, V *Y, int n, int nrhs,
+                                                         const typename VecT<V>::A *dinv, int *bad) {
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
     __shared__ T dblk[TB * kChainPad];
     __shared__ T xs[8][TB];
+    __shared__ T bs[8][TB];
     __shared__ int s_abort;
     const int nblk = (n + TB - 1) / TB;
-    const int b = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x;
-    const int r0 = b * TB, rows = min(TB, n - r0);
     const int tid = threadIdx.x;
     const int i = tid >> 2, qt = tid & 3, c0 = qt * 16;
-    const int ri = min(i, rows - 1);  // clamped row for loads
-    Regs<T, 8> acc;
+    if (tid == 0) s_abort = 0;
+    for (int w = blockIdx.x; w < nblk; w += gridDim.x) {
+        const int b = UPPER ? nblk - 1 - w : w;
+        const int r0 = b * TB, rows = min(TB, n - r0);
+        const int ri = min(i, rows - 1);  // clamped row for loads
+        Regs<T, 8> acc;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc.set(q, ezero<T>());
-    for (int idx = tid; idx < TB * TB; idx += 256)
-        dblk[(idx / TB) * kChainPad + idx % TB] = dinv[(size_t)b * TB * TB + idx];
-    const int ndeps = UPPER ? nblk - 1 - b : b;
-    const S *mrow = M + (size_t)(r0 + ri) * lda;
-    for (int d = 0; d < ndeps; ++d) {
-        const int jb = UPPER ? nblk - 1 - d : d;
-        const int cb = jb * TB, cols = min(TB, n - cb);
-        // this thread's 16 factor entries of the dependency (clamped columns
-        // past n are multiplied by zero below)
-        Regs<T, 16> m;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) m.set(c, ld(mrow + cb + min(c0 + c, cols - 1)));
-        if (tid == 0) {
-            unsigned spins = 0;
-            int abort = 0;
-            while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    atomicOr(bad, 2);
-                    abort = 1;
-                    break;
-                }
-            }
-            s_abort = abort;
-        }
-        __syncthreads();
-        if (s_abort) return;
+        for (int q = 0; q < 8; ++q) acc.set(q, ezero<T>());
+        __syncthreads();  // the previous block is done with dblk, xs, bs
+        for (int idx = tid; idx < TB * TB; idx += 256)
+            dblk[(idx / TB) * kChainPad + idx % TB] = dinv[(size_t)b * TB * TB + idx];
         for (int idx = tid; idx < nrhs * TB; idx += 256) {
             const int q = idx / TB, c = idx % TB;
-            T xv = ezero<T>();
-            if (c < cols) sub_elem(reinterpret_cast<const double *>(X + (size_t)q * n + cb + c), xv);
-            xs[q][c] = xv;
+            bs[q][c] = c < rows ? B[(size_t)q * n + r0 + c] : ezero<T>();
         }
-        __syncthreads();
+        const S *mrow = M + (size_t)(r0 + ri) * lda;
+        for (int d = 0; d < w; ++d) {
+            const int jb = UPPER ? nblk - 1 - d : d;
+            const int cb = jb * TB, cols = min(TB, n - cb);
+            // this thread's 16 factor entries of the dependency (clamped
+            // columns past n are multiplied by zero below)
+            Regs<T, 16> m;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const T mv = m.get(c);
+            for (int c = 0; c < 16; ++c) m.set(c, ld(mrow + cb + min(c0 + c, cols - 1)));
+            for (int idx = tid; idx < nrhs * TB; idx += 256) {
+                const int q = idx / TB, c = idx % TB;
+                T xv = ezero<T>();
+                if (c < cols) {
+                    const double *p = reinterpret_cast<const double *>(Y + (size_t)q * n + cb + c);
+                    unsigned spins = 0;
+                    for (;;) {
+                        sub_elem(p, xv);
+                        if (!chain_empty(xv)) break;
+                        if (++spins > kSpinLimit) {
+                            atomicOr(bad, 2);
+                            s_abort = 1;
+                            xv = ezero<T>();
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                xs[q][c] = xv;
+            }
+            __syncthreads();
+            if (s_abort) return;
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (q < nrhs) acc.set(q, eadd(acc.get(q), emul(mv, xs[q][c0 + c])));
+            for (int c = 0; c < 16; ++c) {
+                const T mv = m.get(c);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (q < nrhs) acc.set(q, eadd(acc.get(q), emul(mv, xs[q][c0 + c])));
+            }
+            __syncthreads();
         }
+        __syncthreads();  // bs and dblk of a block without dependencies
+        // row sums over the four threads of a row; z = b - sum into xs
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q < nrhs) {
+                T v = acc.get(q);
+                v = eadd(v, eshfl_xor(v, 1));
+                v = eadd(v, eshfl_xor(v, 2));
+                if (qt == 0) xs[q][i] = esub(bs[q][i], v);
+            }
         __syncthreads();
+        // x_b = D^{-1} z, published with cache-bypassing stores
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q < nrhs) {
+                T v = ezero<T>();
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v = eadd(v, emul(dblk[i * kChainPad + c0 + k], xs[q][c0 + k]));
+                v = eadd(v, eshfl_xor(v, 1));
+                v = eadd(v, eshfl_xor(v, 2));
+                if (qt == (q & 3) && i < rows) pub_elem(reinterpret_cast<double *>(Y + (size_t)q * n + r0 + i), v);
+            }
     }
-    // row sums over the four threads of a row; z = x_b - sum into xs
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-        if (q < nrhs) {
-            T v = acc.get(q);
-            v = eadd(v, eshfl_xor(v, 1));
-            v = eadd(v, eshfl_xor(v, 2));
-            if (qt == 0) xs[q][i] = i < rows ? esub(X[(size_t)q * n + r0 + i], v) : ezero<T>();
-        }
-    __syncthreads();
-    // x_b = D^{-1} z; publish the solved rows with cache-bypassing stores,
-    // wait for them to complete, then raise the flag (no fences needed)
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-        if (q < nrhs) {
-            T v = ezero<T>();
-#pragma unroll
-            for (int k = 0; k < 16; ++k) v = eadd(v, emul(dblk[i * kChainPad + c0 + k], xs[q][c0 + k]));
-            v = eadd(v, eshfl_xor(v, 1));
-            v = eadd(v, eshfl_xor(v, 2));
-            if (qt == (q & 3) && i < rows) pub_elem(reinterpret_cast<double *>(X + (size_t)q * n + r0 + i), v);
-        }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// one triangular solve through the chain kernel; false if the device cannot
-// keep every block resident (caller falls back to the blocked path)
-template <class S, class V, bool UPPER>
-bool chain_solve(const S *dLU, int n, size_t lda, V *X, int nrhs, const typename VecT<V>::A *dinv, int *flags,
-                 int epoch, int *bad, const LUContext &ctx) {
+// number of chain workgroups that can be resident at once (the chain
+// kernel's workgroups wait on each other)
+template <class S, class V>
+int chain_grid(int n, const LUContext &ctx) {
     const int nblk = (n + kChainTB - 1) / kChainTB;
-    int per_cu = 0;
+    int f = 0, bw = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, UPPER>), 256, 0));
-    if ((long long)per_cu * ctx.cus < nblk) return false;
+        &f, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, false>), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &bw, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, true>), 256, 0));
+    long long g = std::min<long long>(nblk, (long long)std::min(f, bw) * ctx.cus);
+    // PEEC_CHAIN_WORKGROUPS caps the grid (tests use it to run several
+    // blocks per workgroup on small systems)
+    if (const char *cap = std::getenv("PEEC_CHAIN_WORKGROUPS")) {
+        const long long c = std::atoll(cap);
+        if (c > 0) g = std::min(g, c);
+    }
+    return (int)g;
+}
+
+// one triangular solve through the chain kernel: B -> Y (Y is first filled
+// with kChainEmpty)
+template <class S, class V, bool UPPER>
+void chain_solve(const S *dLU, int n, size_t lda, const V *B, V *Y, int nrhs, const typename VecT<V>::A *dinv,
+                 int *bad, int grid, const LUContext &ctx) {
+    const size_t words = (size_t)n * nrhs * sizeof(V) / sizeof(double);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, ctx.stream,
+                       reinterpret_cast<unsigned long long *>(Y), words, kChainEmpty);
     for (int q0 = 0; q0 < nrhs; q0 += 8) {
-        V *Xq = X + (size_t)q0 * n;
-        int nq = std::min(8, nrhs - q0), nn = n, ep = epoch + q0;
+        const V *Bq = B + (size_t)q0 * n;
+        V *Yq = Y + (size_t)q0 * n;
+        int nq = std::min(8, nrhs - q0), nn = n;
         size_t ld = lda;
-        void *args[] = {(void *)&dLU, &ld, &Xq, &nn, &nq, (void *)&dinv, &flags, &ep, &bad};
-        HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER>), dim3(nblk),
+        void *args[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, &nq, (void *)&dinv, &bad};
+        HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER>), dim3(grid),
                                              dim3(256), args, 0, ctx.stream));
     }
-    return true;
 }
 
 template <class S, class V>
@@ -1841,19 +1900,16 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
                        const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor) {
     hipStream_t s = ctx.stream;
     const size_t total = (size_t)n * nrhs;
-    if (ctx.solve_tmp.count < total * sizeof(V) / sizeof(double) + 1) ctx.solve_tmp.resize(total * sizeof(V) / sizeof(double) + 1);
+    // two n x nrhs vectors: the permuted right-hand side and the forward
+    // solve's output
+    const size_t words = total * sizeof(V) / sizeof(double);
+    if (ctx.solve_tmp.count < 2 * words) ctx.solve_tmp.resize(2 * words);
     V *tmp = reinterpret_cast<V *>(ctx.solve_tmp.ptr);
+    V *ybuf = tmp + total;
     hipLaunchKernelGGL(permute_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp, dX, d_perm, d_row, n,
                        nrhs);
     const int nblk = (n + kChainTB - 1) / kChainTB;
-    if ((int)ctx.flags.count < 2 * nblk) {
-        ctx.flags.resize(2 * (size_t)nblk);
-        hipLaunchKernelGGL(fill_int_kernel, dim3((2 * nblk + 255) / 256), dim3(256), 0, s, ctx.flags.ptr, 2 * nblk, 0);
-        ctx.epoch = 0;
-    }
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.solve_bad.ptr, 1, 0);
-    const int ep = ctx.epoch + 1;
-    ctx.epoch += 8 * ((nrhs + 7) / 8) + 1;
     using T = typename VecT<V>::A;
     if (ctx.diag_inv.count < (size_t)2 * nblk * kChainTB * kChainTB * sizeof(T)) ctx.dinv_factor = nullptr;
     ctx.diag_inv.reserve((size_t)2 * nblk * kChainTB * kChainTB * sizeof(T));
@@ -1867,10 +1923,11 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
         ctx.dinv_factor = dLU;
         ctx.dinv_n = n;
     }
-    bool chained =
-        chain_solve<S, V, false>(dLU, n, lda, tmp, nrhs, linv, ctx.flags.ptr, ep, ctx.solve_bad.ptr, ctx) &&
-        chain_solve<S, V, true>(dLU, n, lda, tmp, nrhs, uinv, ctx.flags.ptr + nblk, ep, ctx.solve_bad.ptr, ctx);
-    if (!chained) {
+    const int grid = chain_grid<S, V>(n, ctx);
+    if (grid > 0) {
+        chain_solve<S, V, false>(dLU, n, lda, tmp, ybuf, nrhs, linv, ctx.solve_bad.ptr, grid, ctx);
+        chain_solve<S, V, true>(dLU, n, lda, ybuf, tmp, nrhs, uinv, ctx.solve_bad.ptr, grid, ctx);
+    } else {
         const int T = 128;
         const int nb2 = (n + T - 1) / T;
         for (int b = 0; b < nb2; ++b) {

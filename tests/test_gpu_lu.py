@@ -81,6 +81,24 @@ def test_complex_lu_random_against_oracle(require_gpu, n):
         assert eta < 1e-13
 
 
+@pytest.mark.parametrize("workgroups", [1, 3])
+def test_complex_lu_solve_chain_with_few_workgroups(require_gpu, workgroups, monkeypatch):
+    """The triangular-solve chain with fewer workgroups than 64-row blocks
+    (each workgroup takes blocks w, w + G, ...), as on systems too large for
+    one workgroup per block."""
+    monkeypatch.setenv("PEEC_CHAIN_WORKGROUPS", str(workgroups))
+    n = 515
+    rng = np.random.default_rng(77)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((10, n)) + 1j * rng.standard_normal((10, n))
+    st, X, _ = plib.complex_lu_factor_solve(A, B)
+    assert st == 0
+    st_o, fac = ck.oracle_complex_lu(A)
+    for k in range(10):
+        xo = ck.oracle_complex_solve(fac, B[k])
+        assert np.abs(X[k] - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
 @pytest.mark.parametrize("n", [300, 1030])
 def test_real_lu_random(require_gpu, n):
     rng = np.random.default_rng(5)
