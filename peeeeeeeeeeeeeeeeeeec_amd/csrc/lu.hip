@@ -1381,70 +1381,127 @@ __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0,
         }
 }
 
-// L21 = A21 U11^{-1}: sixteen lanes per row, lane `sub` keeping columns
-// 8 sub .. 8 sub + 7 in registers; U11 staged in LDS (full block) with the
-// reciprocals of its diagonal.
+// Triangular solves of the ordered factor, one wavefront per row of
+// L21 = A21 U11^{-1} or per column of U12 = L11^{-1} A12 (column-by-column
+// substitution in the order of trsm_kernel: x_j = a_j / u_jj once the
+// updates of every earlier j are in, then a_c -= x_j u_jc for c > j, and
+// likewise a_i -= l_ir x_r for i > r). Lane l keeps entries l and l + 64 of
+// the vector; the finished entry is broadcast with a lane read (a scalar
+// register, no LDS round trip) and every lane updates its entries past it
+// against row j of U11 (column r of L11, staged transposed). Workgroups of
+// 16 wavefronts stage the triangle once in LDS and walk their share of the
+// rows (columns).
 template <class S>
-__global__ __launch_bounds__(256) void trsm_right_kernel(S *A, size_t lda, int n, int k0, int w, const int *status) {
+__global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda, int n, int k0, int w,
+                                                               const int *status) {
     using T = typename Work<S>::T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    S *blk = reinterpret_cast<S *>(smem_raw);
+    S *blk = reinterpret_cast<S *>(smem_raw);  // U11 row-major, w x w
     __shared__ T rcp[kMaxNB];
     if (status[0] != 0) return;
     {
-        const int total = w * w;
-        for (int base = threadIdx.x; base < total; base += 256 * 8) {
-            Regs<T, 8> v;
+        constexpr int kPer = kMaxNB * kMaxNB / 1024;
+        Regs<S, kPer> v;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int idx = base + u * 256;
-                v.set(u, idx < total ? wld(A + (size_t)(k0 + idx / w) * lda + k0 + idx % w) : wzero<T>());
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (base + u * 256 < total) wst(blk + base + u * 256, v.get(u));
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024;
+            v.set(u, idx < w * w ? A[(size_t)(k0 + idx / w) * lda + k0 + idx % w] : S{});
         }
-    }
-    const int lane = threadIdx.x & 63, sub = lane & (kTrsmTPC - 1);
-    const int row = k0 + w + (int)blockIdx.x * (256 / kTrsmTPC) + (int)threadIdx.x / kTrsmTPC;
-    const bool valid = row < n;
-    const int base = lane & ~(kTrsmTPC - 1);
-    Regs<T, kTrsmRPT> a;
 #pragma unroll
-    for (int t = 0; t < kTrsmRPT; ++t) {
-        const int c = sub * kTrsmRPT + t;
-        a.set(t, (valid && c < w) ? wld(A + (size_t)row * lda + k0 + c) : wzero<T>());
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024;
+            if (idx < w * w) blk[idx] = v.get(u);
+        }
     }
     __syncthreads();
     if ((int)threadIdx.x < w) rcp[threadIdx.x] = wdiv(wone<T>(), wld(blk + threadIdx.x * w + threadIdx.x));
     __syncthreads();
-    const int nblk = (w + kTrsmRPT - 1) / kTrsmRPT;
-    for (int cb = 0; cb < nblk; ++cb) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ca = lane, cb = lane + 64;
+    const int cla = min(ca, w - 1), clb = min(cb, w - 1);
+    const int nrows = n - k0 - w;
+    for (int rr = (int)blockIdx.x * 16 + wave; rr < nrows; rr += (int)gridDim.x * 16) {
+        S *row = A + (size_t)(k0 + w + rr) * lda + k0;
+        T x0 = ca < w ? wld(row + ca) : wzero<T>();
+        T x1 = cb < w ? wld(row + cb) : wzero<T>();
+        for (int j = 0; j < w; ++j) {
+            const T xj = wmul(wreadlane(j < 64 ? x0 : x1, j & 63), rcp[j]);
+            x0 = (ca == j) ? xj : x0;
+            x1 = (cb == j) ? xj : x1;
+            const T u0 = wld(blk + j * w + cla), u1 = wld(blk + j * w + clb);
+            const T n0 = wfms(x0, xj, u0), n1 = wfms(x1, xj, u1);
+            x0 = (ca > j && ca < w) ? n0 : x0;
+            x1 = (cb > j && cb < w) ? n1 : x1;
+        }
+        if (ca < w) wst(row + ca, x0);
+        if (cb < w) wst(row + cb, x1);
+    }
+}
+
+constexpr int kTrsmWaveCols = 16;  // columns of A12 per workgroup pass
+
+template <class S>
+__global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c0, int c1, int k0, int w,
+                                                         const int *status) {
+    using T = typename Work<S>::T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int wp = w + 1;                      // padded rows: transposed writes spread over the banks
+    S *lt = reinterpret_cast<S *>(smem_raw);   // lt[r * wp + i] = L11[i][r]
+    S *tb = lt + (size_t)w * wp;               // tb[c * wp + i] = A12[i][c] of the current tile
+    if (status[0] != 0) return;
+    {
+        constexpr int kPer = kMaxNB * kMaxNB / 1024;
+        Regs<S, kPer> v;
 #pragma unroll
-        for (int cc = 0; cc < kTrsmRPT; ++cc) {
-            const int jj = cb * kTrsmRPT + cc;
-            const int j = min(jj, w - 1);
-            const T x = wshfl(wmul(a.get(cc), rcp[j]), base + cb);
-            if (sub == cb && jj < w) a.set(cc, x);
-            if (sub * kTrsmRPT + kTrsmRPT - 1 <= j) continue;
-            // branch-free inside, in-bounds reads; the update is kept for
-            // columns right of j
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024;
+            v.set(u, idx < w * w ? A[(size_t)(k0 + idx / w) * lda + k0 + idx % w] : S{});
+        }
 #pragma unroll
-            for (int t = 0; t < kTrsmRPT; ++t) {
-                const int c = sub * kTrsmRPT + t;
-                const T cur = a.get(t);
-                const T upd = wfms(cur, x, wld(blk + j * w + min(c, w - 1)));
-                a.set(t, (c > j && c < w && jj < w) ? upd : cur);
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024;
+            if (idx < w * w) lt[(idx % w) * wp + idx / w] = v.get(u);
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ra = lane, rb = lane + 64;
+    const int rla = min(ra, w - 1), rlb = min(rb, w - 1);
+    const int ncols = c1 - c0, tiles = (ncols + kTrsmWaveCols - 1) / kTrsmWaveCols;
+    constexpr int kTileElems = kMaxNB * kTrsmWaveCols / 1024;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int cbase = c0 + tile * kTrsmWaveCols;
+        __syncthreads();  // the triangle is staged; the previous tile is stored
+#pragma unroll
+        for (int u = 0; u < kTileElems; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024, i = idx / kTrsmWaveCols, c = idx % kTrsmWaveCols;
+            if (i < w && cbase + c < c1) tb[c * wp + i] = A[(size_t)(k0 + i) * lda + cbase + c];
+        }
+        __syncthreads();
+        if (cbase + wave < c1) {
+            S *col = tb + wave * wp;
+            T x0 = ra < w ? wld(col + ra) : wzero<T>();
+            T x1 = rb < w ? wld(col + rb) : wzero<T>();
+            for (int r = 0; r + 1 < w; ++r) {
+                const T xr = wreadlane(r < 64 ? x0 : x1, r & 63);
+                const T l0 = wld(lt + r * wp + rla), l1 = wld(lt + r * wp + rlb);
+                const T n0 = wfms(x0, l0, xr), n1 = wfms(x1, l1, xr);
+                x0 = (ra > r && ra < w) ? n0 : x0;
+                x1 = (rb > r && rb < w) ? n1 : x1;
             }
+            if (ra < w) wst(col + ra, x0);
+            if (rb < w) wst(col + rb, x1);
         }
-    }
-    if (valid) {
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < kTrsmRPT; ++t) {
-            const int c = sub * kTrsmRPT + t;
-            if (c < w) wst(A + (size_t)row * lda + k0 + c, a.get(t));
+        for (int u = 0; u < kTileElems; ++u) {
+            const int idx = (int)threadIdx.x + u * 1024, i = idx / kTrsmWaveCols, c = idx % kTrsmWaveCols;
+            if (i < w && cbase + c < c1) A[(size_t)(k0 + i) * lda + cbase + c] = tb[c * wp + i];
         }
     }
+}
+
+__host__ __device__ constexpr size_t trsm_wave_lds(int w) {
+    return (size_t)w * (w + 1) + (size_t)kTrsmWaveCols * (w + 1);
 }
 
 static void set_dynamic_lds(const void *fn, size_t bytes) {
@@ -1468,10 +1525,22 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         out.code = -1;
         return out;
     }
-    set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_kernel<S>), (size_t)nb * nb * sizeof(S));
-    set_dynamic_lds(reinterpret_cast<const void *>(trsm_kernel<S>), trsm_lds_elems(nb) * sizeof(S));
+    set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_wave_kernel<S>), (size_t)nb * nb * sizeof(S));
+    set_dynamic_lds(reinterpret_cast<const void *>(trsm_wave_kernel<S>), trsm_wave_lds(nb) * sizeof(S));
     set_dynamic_lds(reinterpret_cast<const void *>(gemm_splitk_kernel<S>), split_lds);
-    const int cpb = 256 / kTrsmTPC;  // columns (rows) per triangular-solve workgroup
+    // triangular solves: one wavefront per row of L21 / column of U12, at
+    // most one workgroup of 16 per CU
+    auto solve_l21 = [&](int k0, int w, int rest) {
+        const int grid = std::max(1, std::min((rest + 15) / 16, ctx.cus));
+        hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
+                           dA, lda, n, k0, w, ctx.status.ptr);
+    };
+    auto solve_u12 = [&](int k0, int w, int c0, int c1) {
+        const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
+        const int grid = std::max(1, std::min(tiles, ctx.cus));
+        hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), s, dA,
+                           lda, c0, c1, k0, w, ctx.status.ptr);
+    };
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
         hipLaunchKernelGGL(diag_lu_kernel<S>, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
@@ -1496,10 +1565,8 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
         S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
         // critical path: L21, U12 of the next block column, its update
-        hipLaunchKernelGGL(trsm_right_kernel<S>, dim3((unsigned)((rest + cpb - 1) / cpb)), dim3(256),
-                           (size_t)w * w * sizeof(S), s, dA, lda, n, k0, w, ctx.status.ptr);
-        hipLaunchKernelGGL(trsm_kernel<S>, dim3((unsigned)((w2 + cpb - 1) / cpb)), dim3(256),
-                           trsm_lds_elems(w) * sizeof(S), s, dA, lda, r0, r0 + w2, k0, w, ctx.status.ptr);
+        solve_l21(k0, w, rest);
+        solve_u12(k0, w, r0, r0 + w2);
         const size_t tl = (size_t)ks_count * rb_pad * 64;
         hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, dA, lda, r0, rest,
                            k0, w, rb_pad, Lp, ctx.status.ptr);
@@ -1524,8 +1591,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         // off the critical path: the rest of U12 and of the trailing update
         const int rest2 = rest - w2;
         if (rest2 > 0) {
-            hipLaunchKernelGGL(trsm_kernel<S>, dim3((unsigned)((rest2 + cpb - 1) / cpb)), dim3(256),
-                               trsm_lds_elems(w) * sizeof(S), s, dA, lda, r0 + w2, n, k0, w, ctx.status.ptr);
+            solve_u12(k0, w, r0 + w2, n);
             const int cb_rest = cb_pad - w2 / 16;
             const size_t tu2 = (size_t)ks_count * cb_rest * 64;
             hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu2 + 255) / 256)), dim3(256), 0, s, dA, lda, k0,
