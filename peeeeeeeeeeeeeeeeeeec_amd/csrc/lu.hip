@@ -1424,14 +1424,22 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
         S *row = A + (size_t)(k0 + w + rr) * lda + k0;
         T x0 = ca < w ? wld(row + ca) : wzero<T>();
         T x1 = cb < w ? wld(row + cb) : wzero<T>();
-        for (int j = 0; j < w; ++j) {
-            const T xj = wmul(wreadlane(j < 64 ? x0 : x1, j & 63), rcp[j]);
-            x0 = (ca == j) ? xj : x0;
-            x1 = (cb == j) ? xj : x1;
-            const T u0 = wld(blk + j * w + cla), u1 = wld(blk + j * w + clb);
-            const T n0 = wfms(x0, xj, u0), n1 = wfms(x1, xj, u1);
-            x0 = (ca > j && ca < w) ? n0 : x0;
-            x1 = (cb > j && cb < w) ? n1 : x1;
+        // four steps per iteration (unrolled by hand: the lane reads are
+        // convergent, which keeps the compiler from unrolling), so that their
+        // LDS reads, independent of x, are in flight ahead of the dependent
+        // lane-read chain; steps past w - 1 change nothing
+        for (int j0 = 0; j0 < w; j0 += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u, jc = min(j, w - 1);
+                const T xj = wmul(wreadlane(j < 64 ? x0 : x1, j & 63), rcp[jc]);
+                x0 = (ca == j) ? xj : x0;
+                x1 = (cb == j) ? xj : x1;
+                const T u0 = wld(blk + jc * w + cla), u1 = wld(blk + jc * w + clb);
+                const T n0 = wfms(x0, xj, u0), n1 = wfms(x1, xj, u1);
+                x0 = (ca > j && ca < w) ? n0 : x0;
+                x1 = (cb > j && cb < w) ? n1 : x1;
+            }
         }
         if (ca < w) wst(row + ca, x0);
         if (cb < w) wst(row + cb, x1);
@@ -1481,12 +1489,18 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
             S *col = tb + wave * wp;
             T x0 = ra < w ? wld(col + ra) : wzero<T>();
             T x1 = rb < w ? wld(col + rb) : wzero<T>();
-            for (int r = 0; r + 1 < w; ++r) {
-                const T xr = wreadlane(r < 64 ? x0 : x1, r & 63);
-                const T l0 = wld(lt + r * wp + rla), l1 = wld(lt + r * wp + rlb);
-                const T n0 = wfms(x0, l0, xr), n1 = wfms(x1, l1, xr);
-                x0 = (ra > r && ra < w) ? n0 : x0;
-                x1 = (rb > r && rb < w) ? n1 : x1;
+#pragma unroll 4
+            // four steps per iteration, as in trsm_right_wave_kernel
+            for (int r0 = 0; r0 + 1 < w; r0 += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + u, rc = min(r, w - 1);
+                    const T xr = wreadlane(r < 64 ? x0 : x1, r & 63);
+                    const T l0 = wld(lt + rc * wp + rla), l1 = wld(lt + rc * wp + rlb);
+                    const T n0 = wfms(x0, l0, xr), n1 = wfms(x1, l1, xr);
+                    x0 = (ra > r && ra < w) ? n0 : x0;
+                    x1 = (rb > r && rb < w) ? n1 : x1;
+                }
             }
             if (ra < w) wst(col + ra, x0);
             if (rb < w) wst(col + rb, x1);
