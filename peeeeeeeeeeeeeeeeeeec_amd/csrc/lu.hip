@@ -898,15 +898,34 @@ __global__ __launch_bounds__(256, 2) void gemm_packed_kernel(S *A, size_t lda, i
         }
 }
 
+// Tile of virtual id vid under the mapping of gemm_tile (vid plays the role
+// of blockIdx.x); false for the padding ids past the tile count.
+__device__ inline bool gemm_tile_at(int vid, int tiles_m, int tiles_n, int &tm, int &tn) {
+    const int T = tiles_m * tiles_n, per = (T + 7) / 8;
+    const int nid = (vid & 7) * per + (vid >> 3);
+    if (nid >= T) return false;
+    constexpr int kGroup = 8;
+    const int in_group = kGroup * tiles_n;
+    const int grp = nid / in_group, first = grp * kGroup;
+    const int gsz = min(tiles_m - first, kGroup);
+    const int loc = nid - grp * in_group;
+    tm = first + loc % gsz;
+    tn = loc / gsz;
+    return true;
+}
+
 // The same update for the block widths the factorizations use (KS k steps of
-// 4, i.e. 64 or 128 columns of L21), one workgroup per CU. With every
-// workgroup running the same number of steps, a kernel that reads the C tile
-// only after its last MFMA makes all CUs wait on HBM together; here the C
-// tile is read into registers a few elements per k step, so its traffic runs
-// under the MFMAs and the end of the tile only subtracts and stores. The k
-// loop is unrolled, operands are loaded two steps ahead, and per step
-// scheduling barriers keep the loads (operands of step s + 2, then the
-// step's share of C) ahead of the step's MFMAs.
+// 4, i.e. 64 or 128 columns of L21), persistent: one workgroup per CU walks
+// the tiles vid = blockIdx.x, + gridDim.x, ... (gridDim.x a multiple of 8,
+// so every tile of a workgroup stays on its XCD). With every tile taking the
+// same number of steps, reading the C tile only after the last MFMA would
+// make all CUs wait on HBM together; here the C tile is read into registers
+// a few elements per k step, so its traffic runs under the MFMAs and the end
+// of a tile only subtracts and stores. The k loop is unrolled; operands are
+// loaded PD = 3 steps ahead into four register sets, and the last steps of a
+// tile load the first steps of the workgroup's next tile, so the stores of
+// one tile overlap the start of the next. Per step scheduling barriers keep
+// the loads ahead of the step's MFMAs.
 template <class S, int KS>
 __global__ __launch_bounds__(256, 1) void gemm_stream_kernel(S *A, size_t lda, int row0, int col0, int m, int ncols,
                                                              const S *__restrict__ Lp, const S *__restrict__ Up,
@@ -917,38 +936,22 @@ __global__ __launch_bounds__(256, 1) void gemm_stream_kernel(S *A, size_t lda, i
     constexpr int MB = kWaveRowBlocks, NB = M::wave_col_blocks;
     constexpr int BM = 32 * MB, BN = 32 * NB;
     constexpr int CL = MB * NB * 4;  // C elements per lane
-    constexpr int PD = 2;            // operand prefetch distance (k steps)
+    constexpr int PD = 3, NBUF = 4;  // operand prefetch distance (k steps), register sets
+    static_assert(KS % NBUF == 0, "a tile's steps must cycle the register sets exactly");
     if (status[0] != 0) return;
-    int tm, tn;
-    if (!gemm_tile((m + BM - 1) / BM, (ncols + BN - 1) / BN, tm, tn)) return;
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (ncols + BN - 1) / BN;
+    const int V = 8 * ((tiles_m * tiles_n + 7) / 8);
+    int vt = blockIdx.x, tm = 0, tn = 0;
+    while (vt < V && !gemm_tile_at(vt, tiles_m, tiles_n, tm, tn)) vt += gridDim.x;
+    if (vt >= V) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
-    const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
     const int li = lane & 15, lk = lane >> 4;
-    acc_t cre[MB][NB], cim[MB][NB];
-#pragma unroll
-    for (int a = 0; a < MB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            cre[a][b] = acc_t{0, 0, 0, 0};
-            cim[a][b] = acc_t{0, 0, 0, 0};
-        }
-    // C element e = (i * NB + j) * 4 + v of this lane: row (rb0 + i) * 16 +
-    // row(lk, v), column (cb0 + j) * 16 + li; rows and columns past the
-    // matrix read a clamped in-bounds address and are not stored
-    S cv[CL];
-    auto c_ptr = [&](int e) -> S * {
-        const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
-        const int row = min((rb0 + i) * 16 + M::row(lk, v), m - 1);
-        const int col = min((cb0 + j) * 16 + li, ncols - 1);
-        return A + (size_t)(row0 + row) * lda + col0 + col;
-    };
-    const S *lp = Lp + (size_t)rb0 * 64 + lane;
-    const S *up = Up + (size_t)cb0 * 64 + lane;
     const size_t lstride = (size_t)rb_pad * 64, ustride = (size_t)cb_pad * 64;
-    op ar[PD + 1][MB], ai[PD + 1][MB], br[PD + 1][NB], bi[PD + 1][NB];
-    auto load_ops = [&](int buf, int ks) {
-        const S *l_ = lp + (size_t)ks * lstride, *u_ = up + (size_t)ks * ustride;
+    auto lp_of = [&](int tm_) { return Lp + (size_t)(tm_ * (2 * MB) + (wave >> 1) * MB) * 64 + lane; };
+    auto up_of = [&](int tn_) { return Up + (size_t)(tn_ * (2 * NB) + (wave & 1) * NB) * 64 + lane; };
+    op ar[NBUF][MB], ai[NBUF][MB], br[NBUF][NB], bi[NBUF][NB];
+    auto load_ops = [&](const S *lp_, const S *up_, int buf, int ks) {
+        const S *l_ = lp_ + (size_t)ks * lstride, *u_ = up_ + (size_t)ks * ustride;
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
             const S v_ = l_[i * 64];
@@ -962,60 +965,95 @@ __global__ __launch_bounds__(256, 1) void gemm_stream_kernel(S *A, size_t lda, i
             bi[buf][j] = M::im(v_);
         }
     };
+    const S *lp = lp_of(tm), *up = up_of(tn);
 #pragma unroll
-    for (int s = 0; s < PD && s < KS; ++s) load_ops(s, s);
-    static_for<KS>([&](auto step) {
-        constexpr int s = decltype(step)::value;
-        constexpr int e0 = s * CL / KS, e1 = (s + 1) * CL / KS;
-        if constexpr (s + PD < KS) load_ops((s + PD) % (PD + 1), s + PD);
+    for (int q = 0; q < PD && q < KS; ++q) load_ops(lp, up, q, q);
+    for (;;) {
+        // the workgroup's next tile (its first steps are loaded during this one)
+        int vn = vt + gridDim.x, tmn = tm, tnn = tn;
+        while (vn < V && !gemm_tile_at(vn, tiles_m, tiles_n, tmn, tnn)) vn += gridDim.x;
+        const bool more = vn < V;
+        const S *lpn = more ? lp_of(tmn) : lp, *upn = more ? up_of(tnn) : up;
+        const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
+        const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
+        acc_t cre[MB][NB], cim[MB][NB];
 #pragma unroll
-        for (int e = e0; e < e1; ++e) cv[e] = *c_ptr(e);
-        __builtin_amdgcn_sched_barrier(0);
-        constexpr int b = s % (PD + 1);
-        if constexpr (M::cplx) {
+        for (int a = 0; a < MB; ++a)
 #pragma unroll
-            for (int i = 0; i < MB; ++i)
+            for (int b = 0; b < NB; ++b) {
+                cre[a][b] = acc_t{0, 0, 0, 0};
+                cim[a][b] = acc_t{0, 0, 0, 0};
+            }
+        // C element e = (i * NB + j) * 4 + v of this lane: row (rb0 + i) * 16 +
+        // row(lk, v), column (cb0 + j) * 16 + li; rows and columns past the
+        // matrix read a clamped in-bounds address and are not stored
+        S cv[CL];
+        auto c_ptr = [&](int e) -> S * {
+            const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
+            const int row = min((rb0 + i) * 16 + M::row(lk, v), m - 1);
+            const int col = min((cb0 + j) * 16 + li, ncols - 1);
+            return A + (size_t)(row0 + row) * lda + col0 + col;
+        };
+        static_for<KS>([&](auto step) {
+            constexpr int s = decltype(step)::value;
+            constexpr int e0 = s * CL / KS, e1 = (s + 1) * CL / KS;
+            if constexpr (s + PD < KS) load_ops(lp, up, (s + PD) % NBUF, s + PD);
+            else load_ops(lpn, upn, (s + PD) % NBUF, s + PD - KS);
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
-                    cim[i][j] = M::mma(ar[b][i], bi[b][j], cim[i][j]);
+            for (int e = e0; e < e1; ++e) cv[e] = *c_ptr(e);
+            __builtin_amdgcn_sched_barrier(0);
+            constexpr int b = s % NBUF;
+            if constexpr (M::cplx) {
+#pragma unroll
+                for (int i = 0; i < MB; ++i)
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
+                        cim[i][j] = M::mma(ar[b][i], bi[b][j], cim[i][j]);
+                    }
+#pragma unroll
+                for (int i = 0; i < MB; ++i) {
+                    const op nai = -ai[b][i];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        cre[i][j] = M::mma(nai, bi[b][j], cre[i][j]);
+                        cim[i][j] = M::mma(ai[b][i], br[b][j], cim[i][j]);
+                    }
                 }
+            } else {
 #pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                const op nai = -ai[b][i];
+                for (int i = 0; i < MB; ++i)
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    cre[i][j] = M::mma(nai, bi[b][j], cre[i][j]);
-                    cim[i][j] = M::mma(ai[b][i], br[b][j], cim[i][j]);
+                    for (int j = 0; j < NB; ++j) cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int col = (cb0 + j) * 16 + li;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = (rb0 + i) * 16 + M::row(lk, v);
+                    const int e = (i * NB + j) * 4 + v;
+                    S r = cv[e];
+                    if constexpr (M::cplx) {
+                        r.x -= cre[i][j][v];
+                        r.y -= cim[i][j][v];
+                    } else {
+                        r -= cre[i][j][v];
+                    }
+                    if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
                 }
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < MB; ++i)
-#pragma unroll
-                for (int j = 0; j < NB; ++j) cre[i][j] = M::mma(ar[b][i], br[b][j], cre[i][j]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    });
-#pragma unroll
-    for (int i = 0; i < MB; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const int col = (cb0 + j) * 16 + li;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = (rb0 + i) * 16 + M::row(lk, v);
-                const int e = (i * NB + j) * 4 + v;
-                S r = cv[e];
-                if constexpr (M::cplx) {
-                    r.x -= cre[i][j][v];
-                    r.y -= cim[i][j][v];
-                } else {
-                    r -= cre[i][j][v];
-                }
-                if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
-            }
-        }
+        if (!more) break;
+        vt = vn;
+        tm = tmn;
+        tn = tnn;
+        lp = lpn;
+        up = upn;
+    }
 }
 
 // Narrow update on the look-ahead path: 64 x 64 tiles, the k range split
@@ -1135,7 +1173,7 @@ void pack_operands(S *A, size_t lda, int row0, int col0, int m, int ncols, int k
 // from the packed operands; c_off is a multiple of the tile width.
 template <class S>
 void gemm_packed(S *A, size_t lda, int row0, int col0, int m, int c_off, int nc, int kw, int rb_pad, int cb_pad,
-                 const int *status, LUContext &ctx, hipStream_t s) {
+                 const int *status, LUContext &ctx, hipStream_t s, bool leave_cus = false) {
     if (m <= 0 || nc <= 0) return;
     const int ks_count = (kw + 3) / 4;
     const S *Lp = reinterpret_cast<const S *>(ctx.pack_l.ptr);
@@ -1145,12 +1183,16 @@ void gemm_packed(S *A, size_t lda, int row0, int col0, int m, int c_off, int nc,
     const int tiles = (rb_t / (2 * kWaveRowBlocks)) * (cb_t / (std::is_same<S, double2>::value ? 4 : 8));
     const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
+    // persistent streamed kernel: one workgroup per CU (with leave_cus, one
+    // CU per XCD stays free for the look-ahead's diagonal factor), a
+    // multiple of 8 so that each workgroup keeps its XCD
+    const int pgrid = std::min(grid, std::max(8, (ctx.cus - (leave_cus ? 8 : 0)) / 8 * 8));
     if (ks_count == 32)
-        hipLaunchKernelGGL((gemm_stream_kernel<S, 32>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
-                           Lp, Up, rb_pad, cb_pad, status);
+        hipLaunchKernelGGL((gemm_stream_kernel<S, 32>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m,
+                           nc, Lp, Up, rb_pad, cb_pad, status);
     else if (ks_count == 16)
-        hipLaunchKernelGGL((gemm_stream_kernel<S, 16>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
-                           Lp, Up, rb_pad, cb_pad, status);
+        hipLaunchKernelGGL((gemm_stream_kernel<S, 16>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m,
+                           nc, Lp, Up, rb_pad, cb_pad, status);
     else
         hipLaunchKernelGGL(gemm_packed_kernel<S>, dim3(grid), dim3(256), 0, s, A, lda, row0, col0 + c_off, m, nc,
                            ks_count, Lp, Up, rb_pad, cb_pad, status);
@@ -1610,7 +1652,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             const size_t tu2 = (size_t)ks_count * cb_rest * 64;
             hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu2 + 255) / 256)), dim3(256), 0, s, dA, lda, k0,
                                w, r0 + w2, rest2, cb_rest, cb_pad, Up + (size_t)(w2 / 16) * 64, ctx.status.ptr);
-            gemm_packed<S>(dA, lda, r0, r0, rest, w2, rest2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+            gemm_packed<S>(dA, lda, r0, r0, rest, w2, rest2, w, rb_pad, cb_pad, ctx.status.ptr, ctx, s, true);
         }
         HIP_CHECK(hipGetLastError());
     }
