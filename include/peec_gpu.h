@@ -71,6 +71,9 @@ typedef struct
     double ms_factor;
     double ms_solve;            /* right-hand sides, solves and refinement */
     double ms_check;            /* backward error */
+    int factor_cached;          /* the engine still held this point's factor (same
+                                   frequency and mode as its previous call): no
+                                   assembly, fill or factor */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;
@@ -89,6 +92,8 @@ int peec_engine_create_with_matrices(const Mesh *mesh, const DualMesh *dual,
                                      const double *P, PeecEngine **engine);
 
 /* Solves one frequency point for n_ports port excitations on one factor.
+ * A call at the frequency and mode of the engine's previous successful call
+ * reuses the factor it still holds (report->factor_cached).
  * z_in: n_ports values; edge_current / node_voltage: optional, rhs-major
  * (n_ports x Ne, n_ports x Nv). Returns 0, -1 (bad input / GPU failure),
  * -2 (singular) or -3 (backward error above the gate). */

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpeec_mi355x.so")
-SOURCES = ["geometry.cpp", "assembly.hip", "lu.hip", "harmonic.hip", "rcs.hip", "lightning.cpp", "mtx.cpp", "dropin.cpp"]
+SOURCES = ["geometry.cpp", "assembly.hip", "lu.hip", "harmonic.hip", "rcs.hip", "lightning.cpp", "mtx.cpp", "dropin.cpp",
+           "serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-x", "hip", "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

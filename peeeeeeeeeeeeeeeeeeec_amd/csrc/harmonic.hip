@@ -206,6 +206,11 @@ struct PeecEngine {
     double limit = 0.0;                         // pivot limit of the current point
     bool ordered = false;                       // current FP32 factor uses the reused row order
     bool retarded_now = false;
+    // the factor held for (cached_f, cached_mode) may serve further
+    // right-hand sides at that point without a new assembly or factor
+    bool cache_ok = false;
+    double cached_f = 0.0;
+    int cached_mode = -1;
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
@@ -278,6 +283,15 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
     const size_t nn = (size_t)n * n;
     const double omega = kTwoPi * f;
     HIP_CHECK(hipEventRecord(e->ev[0], s));
+    if (e->cache_ok && f == e->cached_f && mode == e->cached_mode) {
+        // the factor of this point is still held: nothing to assemble,
+        // fill or factor
+        for (int i = 1; i <= 3; ++i) HIP_CHECK(hipEventRecord(e->ev[i], s));
+        report.pivot_reuse = e->ordered ? 1 : 0;
+        report.factor_cached = 1;
+        return 0;
+    }
+    e->cache_ok = false;
     const bool ret = e->opt.retarded != 0;
     e->retarded_now = ret;
     if (ret) assemble_retarded(e, omega / kC0);
@@ -633,9 +647,14 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
                            e->d_P.ptr, e->retarded_now ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
         code = engine_solve(e, mode, nr, report);
         if (code == -1 || code == -2) {
+            e->cache_ok = false;
             if (rep) *rep = report;
             return code;
         }
+        // a point that needed the FP64 fallback keeps no mixed factor
+        e->cache_ok = code == 0 && !report.used_fp64_fallback;
+        e->cached_f = f;
+        e->cached_mode = mode;
         std::vector<double2> x(tot);
         e->d_X.download(x.data(), tot, s);
         e->stream.sync();

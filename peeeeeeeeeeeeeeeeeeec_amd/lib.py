@@ -36,7 +36,16 @@ class PeecPointReport(C.Structure):
                 ("refinement_steps", C.c_int), ("used_fp64_fallback", C.c_int),
                 ("pivot_reuse", C.c_int), ("pivot_refactor", C.c_int), ("reuse_backward_error", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
-                ("ms_solve", C.c_double), ("ms_check", C.c_double)]
+                ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int)]
+
+
+class PeecServerOptions(C.Structure):
+    _fields_ = [("engine", PeecEngineOptions), ("slots", C.c_int), ("hbm_budget_bytes", C.c_size_t)]
+
+
+class PeecRequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("report", PeecPointReport), ("queue_s", C.c_double),
+                ("latency_s", C.c_double), ("engine", C.c_int)]
 
 
 class PeecRcsOptions(C.Structure):
@@ -111,6 +120,9 @@ EXPORTED = [
     # matrix_export.h
     "matrix_export_dense_mtx", "matrix_export_incidence_mtx", "matrix_export_resistance_mtx",
     "matrix_export_peec_set", "matrix_import_dense_mtx",
+    # peec_serve.h
+    "peec_server_options_default", "peec_server_create", "peec_server_add_mesh", "peec_server_submit",
+    "peec_server_wait", "peec_server_memory", "peec_server_destroy", "peec_server_engine_bytes",
 ]
 
 
@@ -216,6 +228,15 @@ def load():
     _sig(lib, "peec_engine_far_evaluations", C.c_longlong, C.c_void_p)
     _sig(lib, "peec_engine_smooth_evaluations", C.c_longlong, C.c_void_p)
     _sig(lib, "peec_engine_destroy", None, C.c_void_p)
+    _sig(lib, "peec_server_options_default", None, C.POINTER(PeecServerOptions))
+    _sig(lib, "peec_server_create", C.c_int, C.POINTER(PeecServerOptions), C.POINTER(C.c_void_p))
+    _sig(lib, "peec_server_add_mesh", C.c_int, C.c_void_p, _M, _D, C.POINTER(C.c_int))
+    _sig(lib, "peec_server_submit", C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int, _ip, C.c_size_t,
+         C.POINTER(C.c_longlong))
+    _sig(lib, "peec_server_wait", C.c_int, C.c_void_p, C.c_longlong, _cp, C.POINTER(PeecRequestResult))
+    _sig(lib, "peec_server_memory", C.c_size_t, C.c_void_p, C.POINTER(C.c_int))
+    _sig(lib, "peec_server_destroy", None, C.c_void_p)
+    _sig(lib, "peec_server_engine_bytes", C.c_size_t, C.c_size_t, C.c_size_t, C.c_int)
     _sig(lib, "peec_engine_set_profiling", C.c_int, C.c_void_p, C.c_int)
     _sig(lib, "peec_selftest_mfma_layout", C.c_int, _dp)
     _sig(lib, "peec_engine_kernel_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(PeecKernelStats))
@@ -619,3 +640,73 @@ def import_dense_mtx(path):
     out = np.ctypeslib.as_array(p, shape=(r.value * c.value,)).copy().reshape(r.value, c.value)
     C.CDLL(None).free(p)
     return out
+
+
+class Server:
+    """Request loop over device-resident engines (include/peec_serve.h):
+    admission by HBM budget, factor reuse keyed by (mesh, frequency, mode),
+    up to `slots` requests running at once on their own streams."""
+
+    FP64 = 0
+    MIXED = 1
+
+    def __init__(self, slots=4, hbm_budget_bytes=0, retarded=False, **engine_opts):
+        lib = load()
+        o = PeecServerOptions()
+        lib.peec_server_options_default(C.byref(o))
+        o.slots = slots
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        o.engine.retarded = int(bool(retarded))
+        for k, v in engine_opts.items():
+            setattr(o.engine, k, v)
+        h = C.c_void_p()
+        if lib.peec_server_create(C.byref(o), C.byref(h)) != 0:
+            raise RuntimeError("peec_server_create failed")
+        self.handle = h
+        self._models = []  # the meshes must outlive the server
+        self._nports = {}
+
+    @staticmethod
+    def engine_bytes(n_edges, n_nodes, retarded):
+        """Estimated device bytes of one engine (the unit of admission)."""
+        return load().peec_server_engine_bytes(n_edges, n_nodes, int(bool(retarded)))
+
+    def add_mesh(self, model: Model) -> int:
+        mid = C.c_int()
+        if load().peec_server_add_mesh(self.handle, C.byref(model.mesh), C.byref(model.dual), C.byref(mid)) != 0:
+            raise RuntimeError("peec_server_add_mesh failed")
+        self._models.append(model)
+        return mid.value
+
+    def submit(self, mesh_id, frequency, ports, mode=MIXED) -> int:
+        p = np.ascontiguousarray(np.asarray(ports, dtype=np.int32).reshape(-1))
+        t = C.c_longlong()
+        if load().peec_server_submit(self.handle, mesh_id, frequency, mode, p.ctypes.data_as(_ip), p.size // 2,
+                                     C.byref(t)) != 0:
+            raise RuntimeError("peec_server_submit rejected the request")
+        self._nports[t.value] = p.size // 2
+        return t.value
+
+    def wait(self, ticket):
+        """(status, z_in array, PeecRequestResult) of a submitted request."""
+        n = self._nports.pop(ticket)
+        z = np.zeros(n, dtype=np.complex128)
+        r = PeecRequestResult()
+        st = load().peec_server_wait(self.handle, ticket, z.ctypes.data_as(_cp), C.byref(r))
+        return st, z, r
+
+    def memory(self):
+        n = C.c_int()
+        b = load().peec_server_memory(self.handle, C.byref(n))
+        return b, n.value
+
+    def close(self):
+        if self.handle:
+            load().peec_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
