@@ -12,6 +12,7 @@ it. The static (frequency-independent) Lp/P assembly happens once per mesh
 and is reported separately.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --workload serving|ragged [--meshes M] [--slots S]
 """
 from __future__ import annotations
 
@@ -123,6 +124,156 @@ def traffic_from_profile():
     return None
 
 
+def _extreme_corner_port(xyz):
+    s = xyz.sum(axis=1)
+    return int(np.argmin(s)), int(np.argmax(s))
+
+
+def _pctl(v, q):
+    return 1e3 * float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def bench_serving(args):
+    """Serving stream: seeded box/plate meshes with 500-12000 unknowns
+    registered on the request server (static Lp/P assembled once each);
+    requests arrive as a Poisson stream at each swept rate, each with a
+    frequency on a per-mesh grid of 16 log-spaced points in 10 kHz - 200 MHz
+    (so repeated points reuse cached factors) and 1-64 random port pairs.
+    Reports throughput and p50/p99 latency (submission to completion) per
+    rate, and the answers of a sample of requests against FP64."""
+    from peeeeeeeeeeeeeeeeeeec_amd import lib, meshgen
+
+    rng = meshgen.Rng(args.seed)
+    models = []
+    while len(models) < args.meshes:
+        # unknowns N = Ne + Nv ~ 3 * cells for a box, ~ 3 * cells for a plate
+        target_n = rng.uniform(500, 12000)
+        kind = "plate" if rng.uniform() < 0.5 else "box"
+        a = rng.uniform(0.3, 3.0)
+        b = a / rng.uniform(1.0, 4.0)
+        c = a / rng.uniform(1.0, 4.0)
+        cells = target_n / 3.0
+        area = a * b if kind == "plate" else 2 * (a * b + a * c + b * c)
+        h = math.sqrt(area / cells)
+        if kind == "plate":
+            xyz, quads = meshgen.plate(max(1, int(round(a / h))), max(1, int(round(b / h))), a, b)
+        else:
+            xyz, quads = meshgen.box_by_size(a, b, c, h)
+        m = lib.Model(xyz, quads)
+        if 500 <= m.ne + m.nv <= 12000:
+            models.append(m)
+    sv = lib.Server(slots=args.slots, retarded=True)
+    t0 = time.perf_counter()
+    ids = [sv.add_mesh(m) for m in models]
+    t_static = time.perf_counter() - t0
+    grids = [[1e4 * (2e4) ** (k / 15.0) for k in range(16)] for _ in models]
+    print(f"[bench] serving: {len(models)} meshes, N = {[m.ne + m.nv for m in models]}, "
+          f"static assembly {t_static:.1f} s", flush=True)
+    per_rate = []
+    sample = []
+    for rate in args.rates:
+        reqs, t = [], 0.0
+        for _ in range(args.requests):
+            t += -math.log(1.0 - rng.uniform()) / rate
+            mi = int(rng.uniform() * len(models))
+            f = grids[mi][int(rng.uniform() * 16)]
+            nrhs = 1 + int(rng.uniform() * 64)
+            nv = models[mi].nv
+            ports = []
+            for _ in range(nrhs):
+                p = int(rng.uniform() * nv)
+                q = (p + 1 + int(rng.uniform() * (nv - 1))) % nv
+                ports.append((p, q))
+            reqs.append((t, mi, f, ports))
+        tickets = []
+        start = time.perf_counter()
+        for t_arr, mi, f, ports in reqs:
+            delay = start + t_arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(sv.submit(ids[mi], f, ports, lib.Server.MIXED))
+        lat, cached, bad = [], 0, 0
+        for tk, (t_arr, mi, f, ports) in zip(tickets, reqs):
+            st, z, r = sv.wait(tk)
+            bad += st != 0
+            lat.append(r.latency_s)
+            cached += r.report.factor_cached
+            if len(sample) < args.fp64_sample and st == 0:
+                sample.append((mi, f, ports, z))
+        elapsed = time.perf_counter() - start
+        rhs = sum(len(r[3]) for r in reqs)
+        per_rate.append({"rate_req_s": rate, "requests": len(reqs), "rhs": rhs, "elapsed_s": elapsed,
+                         "throughput_req_s": len(reqs) / elapsed, "throughput_rhs_s": rhs / elapsed,
+                         "p50_latency_ms": _pctl(lat, 50), "p99_latency_ms": _pctl(lat, 99),
+                         "factor_cache_hits": cached, "failed": bad})
+        print(f"[bench] rate {rate}/s: {len(reqs) / elapsed:.2f} req/s, p50 {_pctl(lat, 50):.0f} ms, "
+              f"p99 {_pctl(lat, 99):.0f} ms, cache hits {cached}", flush=True)
+    errs = []
+    for mi, f, ports, z in sample:
+        st, z64, _ = sv.wait(sv.submit(ids[mi], f, ports, lib.Server.FP64))
+        if st == 0:
+            errs.append(float(np.max(np.abs(z - z64) / np.abs(z64))))
+    sv.close()
+    best = max(per_rate, key=lambda r: r["throughput_req_s"])
+    print(json.dumps({
+        "metric": "serving: solve requests/s with p50/p99 latency (mixed+IR, <=1e-10 backward error)",
+        "value": best["throughput_req_s"], "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+        "data": "synthetic seeded meshes and Poisson arrivals",
+        "config": {"workload": f"serving stream: {len(models)} box/plate meshes, N 500-12000, "
+                               f"retarded, 1-64 RHS per request, {args.slots} slots",
+                   "meshes_N": [m.ne + m.nv for m in models], "static_assembly_s": t_static},
+        "per_rate": per_rate,
+        "answer_rel_error_vs_fp64": {"max": max(errs) if errs else None, "count": len(errs)},
+    }))
+
+
+def bench_ragged(args):
+    """Ragged batch: `--meshes` seeded plates/boxes (cells 300-3000, sides
+    0.3-3 m, aspect 1-4), each with 8 log-uniform frequencies in 1-100 MHz
+    and one port between extreme corners, all queued at once on the request
+    server (engines overlap on the GPU, admission by HBM budget). Mixed and
+    FP64 forms are timed on the same points and compared."""
+    from peeeeeeeeeeeeeeeeeeec_amd import lib, meshgen
+
+    batch = meshgen.ragged_batch(count=args.meshes, seed=args.seed)
+    sv = lib.Server(slots=args.slots, retarded=True)
+    t0 = time.perf_counter()
+    models = [lib.Model(b["xyz"], b["quads"]) for b in batch]
+    ids = [sv.add_mesh(m) for m in models]
+    t_static = time.perf_counter() - t0
+    work = [(ids[i], f, [_extreme_corner_port(batch[i]["xyz"])]) for i in range(len(batch))
+            for f in batch[i]["frequencies"]]
+    print(f"[bench] ragged: {len(models)} meshes, N {min(m.ne + m.nv for m in models)}-"
+          f"{max(m.ne + m.nv for m in models)}, static assembly {t_static:.1f} s, {len(work)} points", flush=True)
+    out = {}
+    for mode, name in ((lib.Server.MIXED, "mixed"), (lib.Server.FP64, "fp64")):
+        start = time.perf_counter()
+        tickets = [sv.submit(m, f, p, mode) for m, f, p in work]
+        res = [sv.wait(t) for t in tickets]
+        el = time.perf_counter() - start
+        out[name] = {"elapsed_s": el, "points_s": len(work) / el, "failed": sum(r[0] != 0 for r in res),
+                     "z": np.array([r[1][0] for r in res]),
+                     "p50_latency_ms": _pctl([r[2].latency_s for r in res], 50),
+                     "p99_latency_ms": _pctl([r[2].latency_s for r in res], 99),
+                     "worst_backward_error": max(r[2].report.backward_error for r in res)}
+        print(f"[bench] ragged {name}: {len(work) / el:.2f} points/s", flush=True)
+    sv.close()
+    err = np.abs(out["mixed"]["z"] - out["fp64"]["z"]) / np.abs(out["fp64"]["z"])
+    for v in out.values():
+        v.pop("z")
+    print(json.dumps({
+        "metric": "ragged batch: frequency-point solves/s (mixed+IR vs FP64, <=1e-10 backward error)",
+        "value": out["mixed"]["points_s"], "unit": "freq-point solves/s", "n_gpus": 1, "higher_is_better": True,
+        "data": "synthetic seeded meshes (splitmix64)",
+        "config": {"workload": f"ragged batch: {len(models)} plates/boxes, 8 frequencies each, retarded, "
+                               f"{args.slots} slots", "static_assembly_s": t_static,
+                   "N": [m.ne + m.nv for m in models]},
+        "mixed": out["mixed"], "fp64": out["fp64"],
+        "speedup_mixed_vs_fp64": out["mixed"]["points_s"] / out["fp64"]["points_s"],
+        "impedance_rel_error_vs_fp64": {"max": float(err.max()), "median": float(np.median(err))},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -132,9 +283,26 @@ def main():
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--workload", choices=("sweep", "serving", "ragged"), default="sweep",
+                    help="sweep: the headline frequency sweep (default); serving: Poisson request stream on the "
+                         "request server; ragged: batch of seeded meshes")
+    ap.add_argument("--meshes", type=int, default=None, help="serving: 6 meshes; ragged: 16 (64 for the full batch)")
+    ap.add_argument("--slots", type=int, default=4, help="serving/ragged: requests running at once")
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 5.0, 10.0], help="serving: arrival rates (req/s)")
+    ap.add_argument("--requests", type=int, default=24, help="serving: requests per rate")
+    ap.add_argument("--fp64-sample", type=int, default=8, help="serving: requests re-solved in FP64")
+    ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload != "sweep":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU", "n_gpus": args.gpus}))
+            sys.exit(2)
+        if args.meshes is None:
+            args.meshes = 6 if args.workload == "serving" else 16
+        (bench_serving if args.workload == "serving" else bench_ragged)(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
