@@ -274,6 +274,64 @@ def bench_ragged(args):
     }))
 
 
+def bench_c5(args):
+    """Large body: closed PEC cylinder r = 1.5 m, length 20 m, flat caps,
+    ~19000 quads (N ~ 57000 MNA unknowns), retarded. (b) monostatic RCS at
+    100 MHz for 360 incidences (theta = 90 deg, phi = 0..359 deg, e_theta)
+    on one factor, mixed and FP64, sigma error in dB; (a) a sample of the
+    1024-point lightning sweep (port from one cap centre to the other; the
+    full sweep time is the per-point time x 1024)."""
+    from peeeeeeeeeeeeeeeeeeec_amd import lib, meshgen
+
+    xyz, quads = meshgen.cylinder(1.5, 20.0, args.c5_m, args.c5_nz)
+    model = lib.Model(xyz, quads)
+    n = model.ne + model.nv
+    print(f"[bench] c5: {len(quads)} quads, Ne={model.ne}, Nv={model.nv}, N={n}", flush=True)
+    t0 = time.perf_counter()
+    eng = lib.Engine(model, retarded=True)
+    t_static = time.perf_counter() - t0
+    print(f"[bench] c5: static assembly {t_static:.1f} s", flush=True)
+    th = np.full(360, 90.0)
+    ph = np.arange(360.0)
+    rcs = {}
+    for mode, name in ((lib.Engine.MIXED, "mixed"), (lib.Engine.FP64, "fp64")):
+        t1 = time.perf_counter()
+        st, res, _, rep = eng.monostatic_rcs(1e8, th, ph, mode)
+        el = time.perf_counter() - t1
+        if st != 0:
+            raise SystemExit(f"c5 rcs {name} failed ({st})")
+        rcs[name] = {"seconds": el, "sigma_dbsm": np.array([r.sigma_dbsm for r in res]),
+                     "backward_error": rep.backward_error, "refinement_steps": rep.refinement_steps,
+                     "ms_assembly": rep.ms_assembly, "ms_factor": rep.ms_factor, "ms_solve": rep.ms_solve}
+        print(f"[bench] c5 rcs {name}: {el:.1f} s for 360 incidences (factor {rep.ms_factor / 1e3:.1f} s, "
+              f"solves {rep.ms_solve / 1e3:.1f} s)", flush=True)
+    err_db = float(np.max(np.abs(rcs["mixed"]["sigma_dbsm"] - rcs["fp64"]["sigma_dbsm"])))
+    a = int(np.argmin(np.linalg.norm(xyz - np.array([0.0, 0.0, 0.0]), axis=1)))
+    b = int(np.argmin(np.linalg.norm(xyz - np.array([0.0, 0.0, 20.0]), axis=1)))
+    df = 2.0e3  # 1024 points up to 2.048 MHz
+    sample = [df, 512 * df, 1024 * df]
+    t1 = time.perf_counter()
+    st, z, reps = eng.port_sweep(sample, (a, b), lib.Engine.MIXED)
+    el = time.perf_counter() - t1
+    if st != 0:
+        raise SystemExit(f"c5 sweep sample failed ({st})")
+    eng.close()
+    for v in rcs.values():
+        v["sigma_dbsm_range"] = [float(v["sigma_dbsm"].min()), float(v["sigma_dbsm"].max())]
+        v.pop("sigma_dbsm")
+    print(json.dumps({
+        "metric": "C5 large body: monostatic RCS, 360 incidences on one factor (s); lightning sweep s/point",
+        "value": rcs["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "data": "synthetic cylinder mesh",
+        "config": {"workload": f"cylinder r=1.5 m, L=20 m, {len(quads)} quads, N={n}, retarded, "
+                               f"100 MHz, theta=90, phi=0..359, e_theta", "static_assembly_s": t_static},
+        "rcs_mixed": rcs["mixed"], "rcs_fp64": rcs["fp64"], "rcs_error_db_max": err_db,
+        "lightning_sample": {"frequencies": sample, "seconds_per_point": el / len(sample),
+                             "full_1024_point_sweep_s_estimate": 1024 * el / len(sample),
+                             "worst_backward_error": max(r.backward_error for r in reps)},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -283,9 +341,11 @@ def main():
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=("sweep", "serving", "ragged"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "serving", "ragged", "c5"), default="sweep",
                     help="sweep: the headline frequency sweep (default); serving: Poisson request stream on the "
-                         "request server; ragged: batch of seeded meshes")
+                         "request server; ragged: batch of seeded meshes; c5: 20 m cylinder RCS + sweep sample")
+    ap.add_argument("--c5-m", type=int, default=23, help="c5: cap lattice cells per side (rim = 4m cells)")
+    ap.add_argument("--c5-nz", type=int, default=195, help="c5: cells along the length")
     ap.add_argument("--meshes", type=int, default=None, help="serving: 6 meshes; ragged: 16 (64 for the full batch)")
     ap.add_argument("--slots", type=int, default=4, help="serving/ragged: requests running at once")
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 5.0, 10.0], help="serving: arrival rates (req/s)")
@@ -302,7 +362,7 @@ def main():
             sys.exit(2)
         if args.meshes is None:
             args.meshes = 6 if args.workload == "serving" else 16
-        (bench_serving if args.workload == "serving" else bench_ragged)(args)
+        {"serving": bench_serving, "ragged": bench_ragged, "c5": bench_c5}[args.workload](args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
