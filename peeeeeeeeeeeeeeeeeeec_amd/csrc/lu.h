@@ -29,6 +29,7 @@ struct LUContext {
     KernelTimer *timer = nullptr;      // optional per-kernel timing
     DeviceBuffer<int> solve_bad;
     DeviceBuffer<double> solve_tmp;
+    DeviceBuffer<float2> solve_f32;              // FP32 right-hand sides of the blocked solves
     DeviceBuffer<unsigned char> pack_l, pack_u;  // MFMA-ordered trailing-update operands
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves

@@ -2002,6 +2002,90 @@ void chain_solve(const S *dLU, int n, size_t lda, const V *B, V *Y, int nrhs, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// many right-hand sides: blocked triangular solves whose off-diagonal work
+// is the MFMA trailing-update GEMM
+//
+// With X the n x nrhs right-hand sides (rhs-major), the forward solve takes
+// the 64-row blocks in order: X_j <- L_jj^{-1} X_j (the diagonal-block
+// inverse), then X_i -= L_ij X_j for every block i below. Transposed, that
+// update is C (nrhs x rows below) -= X_j^T (nrhs x 64) L_.j^T (64 x rows
+// below): the packed-operand GEMM with X_j^T as its left operand and the
+// block column of L as its right one (element (k, i) of L_.j^T is the
+// element (i, k) of L_.j, so the left-operand packing of L_.j is already the
+// right-operand layout). The backward solve is the same with U from the
+// bottom block up. The arithmetic is in the factor's precision.
+constexpr int kBlockedSolveMinRhs = 16;
+
+// X_j <- D_j X_j for the 64-row block j0 of each right-hand side (one
+// workgroup per right-hand side); D_j is the block's FP64 inverse
+template <class S, class T>
+__global__ __launch_bounds__(64) void block_apply_kernel(S *X, int n, int j0, int rows, const T *D) {
+    __shared__ T xs[kChainTB];
+    const int q = blockIdx.x, i = threadIdx.x;
+    S *x = X + (size_t)q * n + j0;
+    xs[i] = i < rows ? ld(x + i) : ezero<T>();
+    __syncthreads();
+    if (i >= rows) return;
+    T y = ezero<T>();
+    for (int k = 0; k < rows; ++k) y = eadd(y, emul(D[i * kChainTB + k], xs[k]));
+    if constexpr (Elem<S>::cplx) {
+        S o;
+        o.x = y.x;
+        o.y = y.y;
+        x[i] = o;
+    } else {
+        x[i] = y;
+    }
+}
+
+__global__ void narrow_kernel(const double2 *src, float2 *dst, size_t count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+__global__ void widen_kernel(const float2 *src, double2 *dst, size_t count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = make_double2(src[i].x, src[i].y);
+}
+
+template <class S, class T>
+void blocked_solve(const S *dLU, int n, size_t lda, S *X, int nrhs, const T *linv, const T *uinv, LUContext &ctx) {
+    hipStream_t s = ctx.stream;
+    const int TB = kChainTB, nblk = (n + TB - 1) / TB;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2, 0);
+    // C (nrhs x nc at column c0 of X^T) -= X_j^T (k0 = j0, kw rows) x (rows
+    // [r0, r0 + nc) of the factor, columns [j0, j0 + kw))^T
+    auto update = [&](int j0, int kw, int r0, int c0, int nc) {
+        if (nc <= 0) return;
+        int rb_pad, cb_pad;
+        gemm_pads<S>(nrhs, nc, rb_pad, cb_pad);
+        const int ks = (kw + 3) / 4;
+        ctx.pack_l.reserve((size_t)(ks + 2) * rb_pad * 64 * sizeof(S));
+        ctx.pack_u.reserve((size_t)(ks + 2) * cb_pad * 64 * sizeof(S));
+        S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
+        S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
+        const size_t tl = (size_t)ks * rb_pad * 64, tu = (size_t)ks * cb_pad * 64;
+        hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, X, (size_t)n, 0,
+                           nrhs, j0, kw, rb_pad, Lp, ctx.status.ptr);
+        hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tu + 255) / 256)), dim3(256), 0, s, dLU, lda, r0, nc,
+                           j0, kw, cb_pad, Up, ctx.status.ptr);
+        gemm_packed<S>(X, (size_t)n, 0, c0, nrhs, 0, nc, kw, rb_pad, cb_pad, ctx.status.ptr, ctx, s);
+    };
+    for (int b = 0; b < nblk; ++b) {
+        const int j0 = b * TB, rows = std::min(TB, n - j0);
+        hipLaunchKernelGGL((block_apply_kernel<S, T>), dim3(nrhs), dim3(64), 0, s, X, n, j0, rows,
+                           linv + (size_t)b * TB * TB);
+        update(j0, rows, j0 + rows, j0 + rows, n - j0 - rows);
+    }
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int j0 = b * TB, rows = std::min(TB, n - j0);
+        hipLaunchKernelGGL((block_apply_kernel<S, T>), dim3(nrhs), dim3(64), 0, s, X, n, j0, rows,
+                           uinv + (size_t)b * TB * TB);
+        update(j0, rows, 0, 0, j0);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 template <class S, class V>
 void lu_solve(const S *dLU, int n, size_t lda, const int *d_ipiv, const double *d_row, const double *d_col, V *dX,
               int nrhs, LUContext &ctx) {
@@ -2046,7 +2130,19 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
         ctx.dinv_n = n;
     }
     const int grid = chain_grid<S, V>(n, ctx);
-    if (grid > 0) {
+    if (nrhs >= kBlockedSolveMinRhs) {
+        if constexpr (std::is_same<S, float2>::value) {
+            // the FP32 factor's solves in FP32 (refinement corrects them)
+            DeviceBuffer<float2> &xs = ctx.solve_f32;
+            xs.reserve(total);
+            const unsigned g = (unsigned)((total + 255) / 256);
+            hipLaunchKernelGGL(narrow_kernel, dim3(g), dim3(256), 0, s, tmp, xs.ptr, total);
+            blocked_solve<S, T>(dLU, n, lda, xs.ptr, nrhs, linv, uinv, ctx);
+            hipLaunchKernelGGL(widen_kernel, dim3(g), dim3(256), 0, s, xs.ptr, tmp, total);
+        } else {
+            blocked_solve<S, T>(dLU, n, lda, tmp, nrhs, linv, uinv, ctx);
+        }
+    } else if (grid > 0) {
         chain_solve<S, V, false>(dLU, n, lda, tmp, ybuf, nrhs, linv, ctx.solve_bad.ptr, grid, ctx);
         chain_solve<S, V, true>(dLU, n, lda, ybuf, tmp, nrhs, uinv, ctx.solve_bad.ptr, grid, ctx);
     } else {

@@ -99,6 +99,47 @@ def test_complex_lu_solve_chain_with_few_workgroups(require_gpu, workgroups, mon
         assert np.abs(X[k] - xo).max() <= 1e-10 * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("n", [70, 515])
+def test_complex_lu_many_rhs_blocked_solve(require_gpu, n):
+    """40 right-hand sides take the blocked solve (diagonal-block inverses +
+    MFMA updates on the transposed right-hand sides) instead of the chain."""
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((40, n)) + 1j * rng.standard_normal((40, n))
+    st, X, _ = plib.complex_lu_factor_solve(A, B)
+    assert st == 0
+    st_o, fac = ck.oracle_complex_lu(A)
+    for k in (0, 17, 39):
+        xo = ck.oracle_complex_solve(fac, B[k])
+        assert np.abs(X[k] - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_real_lu_many_rhs_blocked_solve(require_gpu):
+    rng = np.random.default_rng(9)
+    n = 300
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((20, n))
+    st, X, _ = plib.real_lu_factor_solve(A, B)
+    assert st == 0
+    ref = np.linalg.solve(A, B.T).T
+    assert np.abs(X - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_engine_many_ports_mixed_vs_fp64(require_gpu):
+    """A mixed point with 24 port pairs (FP32 blocked solves inside the
+    refinement) against the FP64 point."""
+    model = plib.Model(*meshgen.box_by_size(0.6, 0.4, 0.3, 0.1))
+    eng = plib.Engine(model, retarded=True)
+    rng = np.random.default_rng(3)
+    ports = [tuple(int(x) for x in rng.choice(model.nv, 2, replace=False)) for _ in range(24)]
+    st1, z1, _, _, r1 = eng.solve(4e7, ports, plib.Engine.MIXED, want_solution=False)
+    st2, z2, _, _, r2 = eng.solve(4e7, ports, plib.Engine.FP64, want_solution=False)
+    eng.close()
+    assert st1 == 0 and st2 == 0
+    assert r1.backward_error <= 1e-10 and r2.backward_error <= 1e-10
+    assert np.max(np.abs(z1 - z2) / np.abs(z2)) < 1e-9
+
+
 @pytest.mark.parametrize("n", [300, 1030])
 def test_real_lu_random(require_gpu, n):
     rng = np.random.default_rng(5)
