@@ -2322,11 +2322,109 @@ __global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, s
     }
 }
 
+// The same for many right-hand sides on FP64 MFMA: Z X (complex, four real
+// products per block pair) and |Z| |X| (the denominators, a fifth) in one
+// pass over Z, which is read directly (16 rows x 4 consecutive columns per
+// operand load) rather than re-read by every group of right-hand sides
+// through the caches. Workgroup tile 128 rows x 64 right-hand sides (2 x 2
+// waves of 64 x 32); the epilogue forms r = Z x - b, stores it, and takes
+// the maximum of |r_i| / (den_i + |b_i|) over the wave's rows per right-hand
+// side with one atomic each.
+constexpr int kResidualMfmaMinRhs = 16;
+
+__global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, size_t lda, int n, const double2 *X,
+                                                            const double2 *B, double2 *R, int nrhs,
+                                                            unsigned long long *eta_bits) {
+    constexpr int MB = 4, NB = 2;
+    const int tiles_q = (nrhs + 63) / 64;
+    const int tm = (int)blockIdx.x / tiles_q, tq = (int)blockIdx.x % tiles_q;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int r0 = tm * 128 + (wave >> 1) * 64, q0 = tq * 64 + (wave & 1) * 32;
+    v4d cre[MB][NB], cim[MB][NB], cden[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cre[i][j] = cim[i][j] = cden[i][j] = v4d{0, 0, 0, 0};
+    // operand rows / right-hand sides of this lane (clamped in bounds)
+    const double2 *arow[MB];
+    const double2 *xcol[NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) arow[i] = A + (size_t)min(r0 + 16 * i + li, n - 1) * lda;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) xcol[j] = X + (size_t)min(q0 + 16 * j + li, nrhs - 1) * n;
+    double ar[MB], ai[MB], am[MB], br[NB], bi[NB], bm[NB];
+    auto load = [&](int k0) {
+        const int k = k0 + lk;
+        const bool in = k < n;
+        const int kc = min(k, n - 1);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const double2 z = arow[i][kc];
+            ar[i] = in ? z.x : 0.0;
+            ai[i] = in ? z.y : 0.0;
+            am[i] = sqrt(fma(ar[i], ar[i], ai[i] * ai[i]));
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const double2 v = xcol[j][kc];
+            br[j] = in ? v.x : 0.0;
+            bi[j] = in ? v.y : 0.0;
+            bm[j] = sqrt(fma(br[j], br[j], bi[j] * bi[j]));
+        }
+    };
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        load(k0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const double nai = -ai[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], cim[i][j], 0, 0, 0);
+                cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], cim[i][j], 0, 0, 0);
+                cden[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[i], bm[j], cden[i][j], 0, 0, 0);
+            }
+        }
+    }
+    // accumulator element v of lane (li, lk): row 4 v + lk of the 16-row
+    // block, right-hand side li of the 16-column block
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int q = q0 + 16 * j + li;
+        double emax = 0.0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + 16 * i + 4 * v + lk;
+                if (row < n && q < nrhs) {
+                    const double2 b = B[(size_t)q * n + row];
+                    const double rr = cre[i][j][v] - b.x, ri = cim[i][j][v] - b.y;
+                    R[(size_t)q * n + row] = make_double2(rr, ri);
+                    const double ra = hypot(rr, ri), d = cden[i][j][v] + hypot(b.x, b.y);
+                    emax = fmax(emax, d > 0.0 ? ra / d : ra);
+                }
+            }
+        emax = fmax(emax, __shfl_xor(emax, 16));
+        emax = fmax(emax, __shfl_xor(emax, 32));
+        if (lk == 0 && q < nrhs) atomicMax(&eta_bits[q], (unsigned long long)__double_as_longlong(emax));
+    }
+}
+
 void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, const double2 *dB, double2 *dR,
                     int nrhs, double *d_ax, unsigned long long *d_eta_bits, hipStream_t s) {
     const size_t total = (size_t)n * nrhs;
-    hipLaunchKernelGGL(abs_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dX, d_ax, total);
     hipLaunchKernelGGL(zero_u64_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, s, d_eta_bits, nrhs);
+    if (nrhs >= kResidualMfmaMinRhs) {
+        const int grid = ((n + 127) / 128) * ((nrhs + 63) / 64);
+        hipLaunchKernelGGL(residual_mfma_kernel, dim3((unsigned)grid), dim3(256), 0, s, dA, lda, n, dX, dB, dR, nrhs,
+                           d_eta_bits);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    hipLaunchKernelGGL(abs_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dX, d_ax, total);
     for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
         const int nq = std::min(kResidualRhs, nrhs - q0);
         const size_t off = (size_t)q0 * n;

@@ -140,6 +140,35 @@ def test_engine_many_ports_mixed_vs_fp64(require_gpu):
     assert np.max(np.abs(z1 - z2) / np.abs(z2)) < 1e-9
 
 
+@pytest.mark.parametrize("nports", [8, 20])
+def test_engine_backward_error_matches_oracle(require_gpu, nports):
+    """The engine's reported backward error (the row-wise residual kernel for
+    8 right-hand sides, the MFMA residual for 20) against the oracle's
+    harmonic.c:932-969 restatement evaluated on the returned solutions. The
+    solutions are the unrefined FP32-factor ones (no refinement, gate lifted),
+    so their residuals (~1e-7) stand far above the rounding of the sums and
+    the two evaluations must agree closely."""
+    xyz, quads = meshgen.box_by_size(0.6, 0.4, 0.3, 0.1)
+    model = plib.Model(xyz, quads)
+    om = ck.OracleModel(xyz, quads)
+    eng = plib.Engine(model, max_refinement_steps=0, backward_error_gate=1.0)
+    L, P = eng.static_matrices()
+    rng = np.random.default_rng(nports)
+    ports = [tuple(int(x) for x in rng.choice(model.nv, 2, replace=False)) for _ in range(nports)]
+    f = 3e6
+    st, z, I, V, rep = eng.solve(f, ports, plib.Engine.MIXED)
+    eng.close()
+    assert st == 0 and rep.refinement_steps == 0 and not rep.used_fp64_fallback
+    assert rep.backward_error > 1e-12
+    Z = om.mna(L, om.pat(P), f)
+    etas = []
+    for k, (a, b) in enumerate(ports):
+        x = np.concatenate([I[k], V[k]])
+        rhs = ck.oracle_rhs(model.ne, model.nv, P=P, Is=ck.port_current(model.nv, a, b))
+        etas.append(ck.oracle_backward_error(Z, x, rhs, model.ne)[2])
+    np.testing.assert_allclose(rep.backward_error, max(etas), rtol=1e-8)
+
+
 @pytest.mark.parametrize("n", [300, 1030])
 def test_real_lu_random(require_gpu, n):
     rng = np.random.default_rng(5)
