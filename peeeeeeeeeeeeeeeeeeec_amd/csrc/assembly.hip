@@ -480,14 +480,17 @@ __global__ __launch_bounds__(256) void far_tile_kernel(
     }
 }
 
-// sin and cos of a moderate argument (|x| < 2^20, larger ones go to the
-// library sin/cos): Cody-Waite reduction by
-// pi/2 split in three parts (33+33+53 bits), then the Taylor series of sin and
-// cos on [-pi/4, pi/4] to degree 19/20 (truncation < 1e-19).
-__device__ inline void d_sincos_moderate(double x, double &s, double &c) {
+// sin^2 x and sin x cos x for a moderate argument (|x| < 2^20, larger ones
+// go to the library sin/cos): Cody-Waite reduction by pi/2 split in three
+// parts (33+33+53 bits) to r in [-pi/4, pi/4], then the Taylor series of
+// sin r to degree 15 and cos r to degree 16 (truncation below 5e-17 and
+// 3e-18 of the value). With n the quadrant, sin^2 x is sin^2 r or cos^2 r
+// and sin x cos x is +-sin r cos r by the parity of n.
+__device__ inline void d_half_angle_terms(double x, double &s2, double &sc) {
     if (!(fabs(x) < 0x1p20)) {  // outside the reduction's exact range
-        s = sin(x);
-        c = cos(x);
+        const double s = sin(x), c = cos(x);
+        s2 = s * s;
+        sc = s * c;
         return;
     }
     const double n = rint(x * 0x1.45f306dc9c883p-1);
@@ -496,9 +499,7 @@ __device__ inline void d_sincos_moderate(double x, double &s, double &c) {
     r = __builtin_fma(-n, 0x1.3198a2e037073p-69, r);
     const double r2 = r * r;
     // sin r = r (1 - r2/3! + r2^2/5! - ...)
-    double ps = -1.0 / 121645100408832000.0;               // -1/19!
-    ps = __builtin_fma(ps, r2, 1.0 / 355687428096000.0);  // 1/17!
-    ps = __builtin_fma(ps, r2, -1.0 / 1307674368000.0);   // -1/15!
+    double ps = -1.0 / 1307674368000.0;                   // -1/15!
     ps = __builtin_fma(ps, r2, 1.0 / 6227020800.0);       // 1/13!
     ps = __builtin_fma(ps, r2, -1.0 / 39916800.0);        // -1/11!
     ps = __builtin_fma(ps, r2, 1.0 / 362880.0);           // 1/9!
@@ -506,9 +507,7 @@ __device__ inline void d_sincos_moderate(double x, double &s, double &c) {
     ps = __builtin_fma(ps, r2, 1.0 / 120.0);              // 1/5!
     ps = __builtin_fma(ps, r2, -1.0 / 6.0);               // -1/3!
     const double sr = __builtin_fma(ps * r2, r, r);
-    double pc = 1.0 / 2432902008176640000.0;               // 1/20!
-    pc = __builtin_fma(pc, r2, -1.0 / 6402373705728000.0); // -1/18!
-    pc = __builtin_fma(pc, r2, 1.0 / 20922789888000.0);   // 1/16!
+    double pc = 1.0 / 20922789888000.0;                   // 1/16!
     pc = __builtin_fma(pc, r2, -1.0 / 87178291200.0);     // -1/14!
     pc = __builtin_fma(pc, r2, 1.0 / 479001600.0);        // 1/12!
     pc = __builtin_fma(pc, r2, -1.0 / 3628800.0);         // -1/10!
@@ -517,9 +516,10 @@ __device__ inline void d_sincos_moderate(double x, double &s, double &c) {
     pc = __builtin_fma(pc, r2, 1.0 / 24.0);               // 1/4!
     pc = __builtin_fma(pc, r2, -0.5);                      // -1/2!
     const double cr = __builtin_fma(pc, r2, 1.0);
-    const int quadrant = (int)(((long long)n) & 3);
-    s = quadrant == 0 ? sr : quadrant == 1 ? cr : quadrant == 2 ? -sr : -cr;
-    c = quadrant == 0 ? cr : quadrant == 1 ? -sr : quadrant == 2 ? -cr : sr;
+    const bool odd = (((long long)n) & 1) != 0;
+    const double p = sr * cr;
+    s2 = odd ? cr * cr : sr * sr;
+    sc = odd ? -p : p;
 }
 
 // g(R) = (exp(-jkR) - 1)/R accumulated as  re: -2 sin^2(kR/2)/R,
@@ -566,11 +566,11 @@ __global__ __launch_bounds__(256) void smooth_tile_kernel(
                     const bool zero = r2 == 0.0;
                     const double rinv = zero ? 0.0 : rsqrt_nr(r2);
                     const double rr = r2 * rinv;
-                    double sh, ch;
-                    d_sincos_moderate(half_k * rr, sh, ch);
+                    double s2, sc;
+                    d_half_angle_terms(half_k * rr, s2, sc);
                     const double wr = Q.w * rinv;
-                    sre = __builtin_fma(wr * sh, sh, sre);
-                    sim = zero ? __builtin_fma(Q.w, half_k, sim) : __builtin_fma(wr * sh, ch, sim);
+                    sre = __builtin_fma(wr, s2, sre);
+                    sim = zero ? __builtin_fma(Q.w, half_k, sim) : __builtin_fma(wr, sc, sim);
                 }
                 acc_re = __builtin_fma(P.w, sre, acc_re);
                 acc_im = __builtin_fma(P.w, sim, acc_im);
@@ -633,10 +633,10 @@ __global__ __launch_bounds__(64) void near_smooth_kernel(
                         const double wb = 0.25 * c_gl_w[order][x] * c_gl_w[order][y] * jb;
                         const double R = d_dist3(ra, rb);
                         if (R > 0.0) {
-                            double sh, ch;
-                            d_sincos_moderate(half_k * R, sh, ch);
-                            pre += wb * (sh * sh) / R;
-                            pim += wb * (sh * ch) / R;
+                            double s2, sc;
+                            d_half_angle_terms(half_k * R, s2, sc);
+                            pre += wb * s2 / R;
+                            pim += wb * sc / R;
                         } else {
                             pim += wb * half_k;
                         }
@@ -965,10 +965,11 @@ void RegionSet::assemble_retarded(const double *d_static, double k, double2 *d_o
                            d_dir_k_.ptr, dir_tol_, d_perm_.ptr, d_tiles_.ptr, k, d_static, d_out,
                            (size_t)R_);
     HIP_CHECK(hipGetLastError());
-    // per exp(-jkR) evaluation: 1/R (12) + half-angle reduction and two
-    // degree-9/10 polynomials (~26) + complex accumulate (4) = 42 flops;
-    // bytes: static matrix read + complex matrix written
-    if (timer) timer->stop(tk, s, kKernelSmoothTile, 42.0 * smooth_evals_, 8.0 * R_ * R_ + 16.0 * R_ * R_);
+    // per exp(-jkR) evaluation: 1/R (12) + half-angle reduction and the
+    // sin (degree 15) and cos (degree 16) polynomials (~22) + complex
+    // accumulate (4) = 38 flops; bytes: static matrix read + complex matrix
+    // written
+    if (timer) timer->stop(tk, s, kKernelSmoothTile, 38.0 * smooth_evals_, 8.0 * R_ * R_ + 16.0 * R_ * R_);
     const int nn = (int)h_near_.size() / 2;
     if (nn && near_smooth_order_ > 0) {
         if (!d_smooth_err_.ptr) d_smooth_err_.resize(1);
