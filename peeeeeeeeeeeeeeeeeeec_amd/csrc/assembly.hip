@@ -328,24 +328,30 @@ __device__ inline void d_piece_triangle(const double *pv, int which, double (&t)
     }
 }
 
-// One wavefront per entry (i, j); see the file header.
+// PER_WAVE entries (i, j) per wavefront, 64 / PER_WAVE lanes each; see the
+// file header. Edge regions have at most 2 x 2 pieces (16 tasks), so four
+// Lp entries share a wavefront; a wavefront runs as long as its slowest
+// adaptive lane, and one Lp entry alone would leave 48 lanes idle.
+template <int PER_WAVE>
 __global__ __launch_bounds__(64) void near_entry_kernel(
     const int *__restrict__ pairs, int n_pairs, const int *__restrict__ ptr,
     const double *__restrict__ pv, QuadratureOrders q, int kind, const double *__restrict__ aux,
     const double *__restrict__ dir, double *__restrict__ out, size_t ld, int mirror,
     int *__restrict__ err) {
+    constexpr int G = 64 / PER_WAVE, CAP = kMaxTasks / PER_WAVE;
     __shared__ double vals[kMaxTasks];
-    const int e = blockIdx.x;
+    const int slot = threadIdx.x / G, sub = threadIdx.x % G;
+    const int e = blockIdx.x * PER_WAVE + slot;
     if (e >= n_pairs) return;
-    const int lane = threadIdx.x;
+    double *v = vals + slot * CAP;
     const int i = pairs[2 * e], j = pairs[2 * e + 1];
     const int a0 = ptr[i], na = ptr[i + 1] - a0, b0 = ptr[j], nb = ptr[j + 1] - b0;
     const int tasks = na * nb * 4;
-    if (tasks > kMaxTasks) {
-        if (lane == 0) atomicOr(err, 2);
+    if (tasks > CAP) {
+        if (sub == 0) atomicOr(err, 2);
         return;
     }
-    for (int t = lane; t < tasks; t += 64) {
+    for (int t = sub; t < tasks; t += G) {
         const int pp = t >> 2, tp = t & 3;
         const int ia = pp / nb, ib = pp - ia * nb;
         const double *pa = pv + 12 * (size_t)(a0 + ia);
@@ -354,18 +360,20 @@ __global__ __launch_bounds__(64) void near_entry_kernel(
         double ta[3][3], tb[3][3];
         d_piece_triangle(pa, tp >> 1, ta);
         d_piece_triangle(pb, tp & 1, tb);
-        vals[t] = cls == 3 ? d_tri_pair_gl(ta, tb, q.far_order) : d_tri_pair_near(ta, tb, cls, q);
+        v[t] = cls == 3 ? d_tri_pair_gl(ta, tb, q.far_order) : d_tri_pair_near(ta, tb, cls, q);
     }
+    // the workgroup is one wavefront: lanes that returned above are simply
+    // inactive at the barrier
     __syncthreads();
-    if (lane != 0) return;
+    if (sub != 0) return;
     double integral = 0.0;
     bool finite = true;
     for (int pp = 0; pp < na * nb; ++pp) {
         double s = 0.0;
         for (int tp = 0; tp < 4; ++tp) {
-            const double v = vals[4 * pp + tp];
-            if (!isfinite(v)) finite = false;
-            s += v;
+            const double x = v[4 * pp + tp];
+            if (!isfinite(x)) finite = false;
+            s += x;
         }
         integral += s;
     }
@@ -936,9 +944,15 @@ void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, Asse
     }
     const int nn = (int)h_near_.size() / 2;
     if (nn) {
-        hipLaunchKernelGGL(near_entry_kernel, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr,
-                           d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
-                           d_error);
+        // edge regions (at most 2 x 2 pieces): four entries per wavefront
+        if (kind_ == RegionKind::Edge && maxp_ <= 2)
+            hipLaunchKernelGGL(near_entry_kernel<4>, dim3((nn + 3) / 4), dim3(64), 0, s, d_near_.ptr, nn,
+                               d_ptr_.ptr, d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
+                               d_error);
+        else
+            hipLaunchKernelGGL(near_entry_kernel<1>, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr,
+                               d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
+                               d_error);
         HIP_CHECK(hipGetLastError());
     }
     if (work) {
@@ -987,7 +1001,7 @@ void RegionSet::assemble_pairs(const std::vector<int> &pairs, double *d_values, 
     d_pairs.upload(pairs, s);
     const int n = (int)pairs.size() / 2;
     if (n) {
-        hipLaunchKernelGGL(near_entry_kernel, dim3(n), dim3(64), 0, s, d_pairs.ptr, n, d_ptr_.ptr,
+        hipLaunchKernelGGL(near_entry_kernel<1>, dim3(n), dim3(64), 0, s, d_pairs.ptr, n, d_ptr_.ptr,
                            d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_values, (size_t)0,
                            0, d_error);
         HIP_CHECK(hipGetLastError());
@@ -1012,7 +1026,7 @@ double region_pair_integral(const std::vector<double> &pa, const std::vector<dou
     d_ptr.upload(ptr, s);
     d_pairs.upload(pairs, s);
     HIP_CHECK(hipMemsetAsync(d_err.ptr, 0, sizeof(int), s));
-    hipLaunchKernelGGL(near_entry_kernel, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr, d_pv.ptr, q, 2,
+    hipLaunchKernelGGL(near_entry_kernel<1>, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr, d_pv.ptr, q, 2,
                        d_aux.ptr, d_dir.ptr, d_val.ptr, (size_t)0, 0, d_err.ptr);
     HIP_CHECK(hipGetLastError());
     double v = NAN;
@@ -1040,7 +1054,7 @@ double patch_pair_integral(const QuadPatch *a, const QuadPatch *b, const Quadrat
     d_ptr.upload(ptr, s);
     d_pairs.upload(pairs, s);
     HIP_CHECK(hipMemsetAsync(d_err.ptr, 0, sizeof(int), s));
-    hipLaunchKernelGGL(near_entry_kernel, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr,
+    hipLaunchKernelGGL(near_entry_kernel<1>, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr,
                        d_pv.ptr, q, 2, d_aux.ptr, d_dir.ptr, d_val.ptr, (size_t)0, 0, d_err.ptr);
     HIP_CHECK(hipGetLastError());
     double v = NAN;
