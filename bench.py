@@ -476,6 +476,7 @@ def main():
         "fp64_fallbacks": int(sum(r.used_fp64_fallback for r in reports)),
         "pivot_order_reused": int(sum(r.pivot_reuse and not r.pivot_refactor for r in reports)),
         "pivot_order_refactors": int(sum(r.pivot_refactor for r in reports)),
+        "ordered_factor_block_widths": sorted({int(r.block_size) for r in reports if r.block_size}),
         "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) for r in reports]))
                           for k in ("assembly", "fill", "factor", "solve", "check")},
         "static_assembly_s": t_static,

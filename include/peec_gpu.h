@@ -47,7 +47,8 @@ typedef struct
     int retarded;               /* 0 quasi-static (reference model), 1 exp(-jkR)/R */
     int smooth_order;           /* GL order of the retarded remainder rule, default 2 */
     int near_smooth_order;      /* same, for entries of nearby regions, default 4 */
-    int block_size;             /* LU panel width, 0 = automatic */
+    int block_size;             /* LU panel width, 0 = automatic (autotuned for the
+                                   reused-row-order factor, see PeecPointReport) */
     int max_refinement_steps;   /* mixed mode cap, default 30 */
     double refinement_target;   /* mixed mode stops at this backward error, default 1e-14 */
     double backward_error_gate; /* a point is accepted at or below this, default 1e-10 */
@@ -74,6 +75,9 @@ typedef struct
     int factor_cached;          /* the engine still held this point's factor (same
                                    frequency and mode as its previous call): no
                                    assembly, fill or factor */
+    int block_size;             /* block width of a reused-row-order factor (0 otherwise);
+                                   with block_size = 0 in the options the engine's first
+                                   two such factors try 128 and 64 and keep the faster */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;

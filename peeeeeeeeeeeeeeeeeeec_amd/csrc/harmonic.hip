@@ -2,6 +2,8 @@
 // frequency-point engine declared in include/peec_gpu.h.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -211,6 +213,11 @@ struct PeecEngine {
     bool cache_ok = false;
     double cached_f = 0.0;
     int cached_mode = -1;
+    // block-width autotuner of the ordered factor (block_size = 0): the
+    // first two ordered factors run at 128 and 64, the faster width is kept
+    int tuned_nb = 0;
+    double trial_s[2] = {0.0, 0.0};
+    int trials = 0;
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
@@ -315,7 +322,19 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
         e->d_Z32.resize(nn);
         if (e->opt.reuse_pivots && e->have_reuse) {
             convert_rows_to_f32(e->d_Z.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, n, s);
-            st = lu_factor_ordered(e->d_Z32.ptr, n, n, e->limit, *e->lu, e->opt.block_size);
+            static constexpr int kTrialWidths[2] = {128, 64};
+            int nb = e->opt.block_size;
+            const bool trial = nb == 0 && e->tuned_nb == 0;
+            if (nb == 0) nb = trial ? kTrialWidths[e->trials] : e->tuned_nb;
+            if (trial) e->stream.sync();
+            const auto t0 = std::chrono::steady_clock::now();
+            st = lu_factor_ordered(e->d_Z32.ptr, n, n, e->limit, *e->lu, nb);
+            report.block_size = nb;
+            if (trial && st.code == 0) {
+                // lu_factor_ordered returns after its stream has drained
+                e->trial_s[e->trials++] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (e->trials == 2) e->tuned_nb = e->trial_s[1] < e->trial_s[0] ? kTrialWidths[1] : kTrialWidths[0];
+            }
             if (st.code == 0) {
                 e->ordered = true;
                 HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice,

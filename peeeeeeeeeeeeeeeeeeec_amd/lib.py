@@ -36,7 +36,8 @@ class PeecPointReport(C.Structure):
                 ("refinement_steps", C.c_int), ("used_fp64_fallback", C.c_int),
                 ("pivot_reuse", C.c_int), ("pivot_refactor", C.c_int), ("reuse_backward_error", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
-                ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int)]
+                ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int),
+                ("block_size", C.c_int)]
 
 
 class PeecServerOptions(C.Structure):

@@ -302,13 +302,14 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
         z64 = [eng.solve(f, [port], plib.Engine.FP64)[1][0] for f in freqs]
         eng.close()
         eng = plib.Engine(m, retarded=True)
-        reports, zm = [], []
+        reports, zm, widths = [], [], []
         for f in freqs:
             st, z, _, _, rep = eng.solve(f, [port], plib.Engine.MIXED)
             assert st == 0
             zm.append(z[0])
             reports.append((rep.pivot_reuse, rep.pivot_refactor, rep.refinement_steps, rep.backward_error,
                             rep.used_fp64_fallback, rep.reuse_backward_error))
+            widths.append(rep.block_size)
         eng.close()
     for f, a, b, r in zip(freqs, zm, z64, reports):
         print(f"f={f:g} mixed {a:.12e} fp64 {b:.12e} rel={abs(a - b) / abs(b):.2e} reuse/refactor/ir/eta/fb/eta_reused={r}",
@@ -319,6 +320,29 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
     assert all(r[0] == 1 for r in reports[1:])
     # the reused order is kept at most points of the sweep
     assert sum(1 for r in reports[1:] if not r[1]) >= len(freqs) // 2
+    # block-width autotuner: the first two ordered factors try 128 and 64,
+    # the later ones keep one of them
+    assert widths[0] == 0 and widths[1:3] == [128, 64]
+    assert len(set(widths[3:])) == 1 and widths[3] in (64, 128)
+
+
+def test_engine_fixed_block_width(require_gpu):
+    """A block width given in the options is used for every ordered factor
+    (no autotuning) and gives the same impedances as the autotuned engine."""
+    xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
+    port = (0, len(xyz) - 1)
+    freqs = [1e5, 2e5, 3e6, 5e7]
+    with plib.Model(xyz, quads) as m:
+        out = {}
+        for nb in (0, 64):
+            eng = plib.Engine(m, retarded=True, block_size=nb)
+            res = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+            eng.close()
+            assert all(r[0] == 0 for r in res)
+            out[nb] = ([r[1][0] for r in res], [r[4].block_size for r in res])
+    assert out[64][1] == [0, 64, 64, 64]
+    assert out[0][1][:3] == [0, 128, 64]
+    np.testing.assert_allclose(out[64][0], out[0][0], rtol=1e-10, atol=0)
 
 
 def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path):
