@@ -415,6 +415,7 @@ def main():
     gemm = eng.kernel_stats("lu_update")
     panel = eng.kernel_stats("panel")
     smooth = eng.kernel_stats("smooth_tile")
+    moments = eng.kernel_stats("smooth_moments")
     eng.set_profiling(False)
     print(f"[bench] mixed: {len(timed) / t_mixed:.3f} points/s", flush=True)
 
@@ -497,7 +498,11 @@ def main():
             "panel_ms_per_point": panel["ms"] / len(timed),
             "smooth_tile_ms_per_point": smooth["ms"] / len(timed),
             "smooth_tile_tflops": smooth["flops"] / (smooth["ms"] * 1e-3) / 1e12 if smooth["ms"] else None,
+            "smooth_moments_launches": moments["launches"],
+            "smooth_moments_ms_per_launch": moments["ms"] / moments["launches"] if moments["launches"] else None,
+            "smooth_moments_gbs": moments["bytes"] / (moments["ms"] * 1e-3) / 1e9 if moments["ms"] else None,
         },
+        "moment_terms_per_point": [int(r.moment_terms) for r in reports],
     }
     if not args.no_cpu:
         try:

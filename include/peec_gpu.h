@@ -56,6 +56,18 @@ typedef struct
                                    point (no pivot search); refactors with partial
                                    pivoting when refinement does not reach 1e-3 x gate.
                                    Default 1 */
+    int smooth_moments;         /* retarded sweeps: terms of the power series of the
+                                   smooth remainder, (exp(-jkR)-1)/R = sum_m (-jk)^m
+                                   R^(m-1)/m!, whose frequency-independent moments are
+                                   assembled once with the engine (8 bytes per term per
+                                   upper-triangle entry of Lp and P). A point whose
+                                   series needs at most that many terms for 1e-17 of
+                                   every entry (k * mesh diameter small enough) is
+                                   assembled from them in one streaming pass, the
+                                   others by quadrature. 0 = off, 1..32 = count,
+                                   -1 (default) = 26 terms when they fit in a quarter
+                                   of the device memory left free by the engine's
+                                   per-point working set, else fewer (off below 8) */
 } PeecEngineOptions;
 
 typedef struct
@@ -75,6 +87,8 @@ typedef struct
     int factor_cached;          /* the engine still held this point's factor (same
                                    frequency and mode as its previous call): no
                                    assembly, fill or factor */
+    int moment_terms;           /* retarded Lp(w) assembled from that many stored moment
+                                   terms (0: by quadrature, or not retarded) */
     int block_size;             /* block width of a reused-row-order factor (0 otherwise);
                                    with block_size = 0 in the options the engine's first
                                    two such factors try 128 and 64 and keep the faster */

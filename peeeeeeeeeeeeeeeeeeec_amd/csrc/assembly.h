@@ -44,6 +44,25 @@ class RegionSet {
     // complex retarded matrix: d_out = d_static + scale * smooth remainder
     void assemble_retarded(const double *d_static, double wave_number, double2 *d_out,
                            hipStream_t s, AssemblyWork *work, KernelTimer *timer = nullptr);
+    // Frequency-independent moments of the smooth remainder for a sweep:
+    // (exp(-jkR)-1)/R = sum_{m>=1} (-jk)^m R^(m-1) / m!, so the remainder of an
+    // entry is sum_m (-jk)^m/m! * U_m with U_m = scale * sum_w w R^(m-1) over
+    // the same quadrature points (smooth rule; near entries with the near
+    // rule). build_moments stores U_1..U_terms (terms <= kMaxMomentTerms) in
+    // 64 x 64 tiles of the upper triangle in original numbering; a point
+    // whose series needs no more terms than stored is then one streaming pass
+    // over the moments instead of a quadrature pass.
+    static constexpr int kMaxMomentTerms = 32;
+    void build_moments(int terms, hipStream_t s);
+    int moment_terms() const { return moment_terms_; }
+    static size_t moment_bytes(int regions, int terms);
+    // terms of the series for wave number k: the tail beyond them is below
+    // 1e-17 of every static entry (k * diameter bounds k R for all points)
+    int series_terms(double wave_number) const;
+    double diameter() const { return diameter_; }
+    // d_out = d_static + remainder from the stored moments (terms <= moment_terms())
+    void assemble_retarded_moments(const double *d_static, double wave_number, int terms, double2 *d_out,
+                                   hipStream_t s, KernelTimer *timer = nullptr);
     // single entries through the near-field kernel (drop-in *_compute_pair)
     void assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,
                         int *d_error);
@@ -72,6 +91,10 @@ class RegionSet {
     DeviceBuffer<int> d_smooth_err_;
     DeviceBuffer<double4> d_far_i_, d_far_j_, d_smooth_i_, d_smooth_j_;
     long long far_evals_ = 0, smooth_evals_ = 0;
+    double diameter_ = 0.0;
+    int moment_terms_ = 0, moment_tiles_ = 0;
+    DeviceBuffer<double> d_moments_;
+    DeviceBuffer<int> d_moment_tiles_;
 
   public:
     long long far_evaluations() const { return far_evals_; }

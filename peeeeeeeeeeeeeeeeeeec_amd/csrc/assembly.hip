@@ -678,6 +678,227 @@ __global__ __launch_bounds__(64) void near_smooth_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// moments of the smooth remainder (frequency sweeps)
+//
+// U_m(i, j) = scale(i, j) * sum_w w R^(m-1), m = 1..terms, stored as
+// [m][upper tile][64 x 64] in original numbering (tile (ta, tb), ta <= tb, of
+// entries a = 64 ta + r, b = 64 tb + c); a diagonal tile holds both halves.
+
+constexpr int kMomentTerms = RegionSet::kMaxMomentTerms;
+
+__device__ inline size_t moment_slot(int a, int b, int nt) {
+    const int ta = a >> 6, tb = b >> 6;
+    const size_t t = (size_t)ta * nt - (size_t)ta * (ta - 1) / 2 + (size_t)(tb - ta);
+    return t * 4096 + (size_t)(a & 63) * 64 + (size_t)(b & 63);
+}
+
+__device__ inline void store_moments(double *mom, size_t mstride, int nt, int oi, int oj, int terms,
+                                     const double (&u)[kMomentTerms], double scale) {
+    const int a = min(oi, oj), b = max(oi, oj);
+    const size_t p = moment_slot(a, b, nt);
+    const bool diag = (a >> 6) == (b >> 6);
+    const size_t q = diag ? p - (size_t)(a & 63) * 64 - (size_t)(b & 63) + (size_t)(b & 63) * 64 + (size_t)(a & 63) : p;
+#pragma unroll
+    for (int m = 0; m < kMomentTerms; ++m)
+        if (m < terms) {
+            mom[(size_t)m * mstride + p] = scale * u[m];
+            if (diag) mom[(size_t)m * mstride + q] = scale * u[m];
+        }
+}
+
+// Same tiling and quadrature points as smooth_tile_kernel; each lane keeps the
+// power sums of its entry in registers.
+template <int KIND>
+__global__ __launch_bounds__(256) void moment_tile_kernel(
+    const double4 *__restrict__ pts_i, const double4 *__restrict__ pts_j,
+    const int *__restrict__ npts, int R, int slots, const double *__restrict__ aux,
+    const double *__restrict__ dir, double dir_tol, const int *__restrict__ perm,
+    const int *__restrict__ tiles, int terms, double *__restrict__ mom, size_t mstride, int nt) {
+    extern __shared__ double4 sj[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int bi = tiles[2 * blockIdx.x], bj = tiles[2 * blockIdx.x + 1];
+    const int i = bi * kTileI + lane;
+    const bool iv = i < R;
+    const double ai = iv ? aux[i] : 1.0;
+    double di[3] = {0.0, 0.0, 0.0};
+    if (KIND == 1 && iv)
+        for (int m = 0; m < 3; ++m) di[m] = dir[3 * i + m];
+    double4 *myj = sj + (size_t)wave * slots;
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = bj * kTileJ + wave * 4 + jj;
+        const bool jv = j < R;
+        const int nj = jv ? npts[j] : 0;
+        for (int q = lane; q < nj; q += 64) myj[q] = pts_j[(size_t)j * slots + q];
+        __syncthreads();
+        bool active = jv && iv && i <= j;
+        double dot = 1.0;
+        if (KIND == 1 && active) {
+            dot = di[0] * dir[3 * j] + di[1] * dir[3 * j + 1] + di[2] * dir[3 * j + 2];
+            active = !(fabs(dot) < dir_tol);
+        }
+        double u[kMomentTerms];
+#pragma unroll
+        for (int m = 0; m < kMomentTerms; ++m) u[m] = 0.0;
+        if (__ballot(active)) {
+            for (int p = 0; p < slots; ++p) {
+                const double4 P = iv ? pts_i[(size_t)p * R + i] : make_double4(0.0, 0.0, 0.0, 0.0);
+                for (int q = 0; q < nj; ++q) {
+                    const double4 Q = myj[q];
+                    const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                    const double rr = r2 == 0.0 ? 0.0 : r2 * rsqrt_nr(r2);
+                    double w = P.w * Q.w;
+#pragma unroll
+                    for (int m = 0; m < kMomentTerms; ++m)
+                        if (m < terms) {
+                            u[m] += w;
+                            w *= rr;
+                        }
+                }
+            }
+        }
+        if (jv && iv && i <= j) {
+            const double sc = !active ? 0.0
+                              : KIND == 0 ? kInv4PiEps0 / (ai * aux[j])
+                                          : kMu0Over4Pi * dot / (ai * aux[j]);
+            store_moments(mom, mstride, nt, perm[i], perm[j], terms, u, sc);
+        }
+        __syncthreads();
+    }
+}
+
+// Near entries with the near rule (overwrites the tile kernel's moments);
+// mirrors near_smooth_kernel.
+__global__ __launch_bounds__(64) void near_moment_kernel(
+    const int *__restrict__ pairs, int n_pairs, const int *__restrict__ ptr,
+    const double *__restrict__ pv, int order, int kind, const double *__restrict__ aux,
+    const double *__restrict__ dir, double dir_tol, int terms, double *__restrict__ mom, size_t mstride,
+    int nt, int *__restrict__ err) {
+    __shared__ double part[kMomentTerms][65];
+    const int e = blockIdx.x;
+    if (e >= n_pairs) return;
+    const int lane = threadIdx.x;
+    const int i = pairs[2 * e], j = pairs[2 * e + 1];
+    const int a0 = ptr[i], na = ptr[i + 1] - a0, b0 = ptr[j], nb = ptr[j + 1] - b0;
+    const int tasks = na * nb * 4;
+    if (tasks > kMaxTasks) {
+        if (lane == 0) atomicOr(err, 2);
+        return;
+    }
+    double u[kMomentTerms];
+#pragma unroll
+    for (int m = 0; m < kMomentTerms; ++m) u[m] = 0.0;
+    for (int t = lane; t < tasks; t += 64) {
+        const int pp = t >> 2, tp = t & 3;
+        const int ia = pp / nb, ib = pp - ia * nb;
+        double ta[3][3], tb[3][3];
+        d_piece_triangle(pv + 12 * (size_t)(a0 + ia), tp >> 1, ta);
+        d_piece_triangle(pv + 12 * (size_t)(b0 + ib), tp & 1, tb);
+        for (int uu = 0; uu < order; ++uu)
+            for (int vv = 0; vv < order; ++vv) {
+                double ra[3], ja;
+                d_tri_map(ta, 0.5 * (c_gl_x[order][uu] + 1.0), 0.5 * (c_gl_x[order][vv] + 1.0), ra, ja);
+                const double wa = 0.25 * c_gl_w[order][uu] * c_gl_w[order][vv] * ja;
+                for (int x = 0; x < order; ++x)
+                    for (int y = 0; y < order; ++y) {
+                        double rb[3], jb;
+                        d_tri_map(tb, 0.5 * (c_gl_x[order][x] + 1.0), 0.5 * (c_gl_x[order][y] + 1.0), rb, jb);
+                        const double wb = 0.25 * c_gl_w[order][x] * c_gl_w[order][y] * jb;
+                        const double R = d_dist3(ra, rb);
+                        double w = wa * wb;
+#pragma unroll
+                        for (int m = 0; m < kMomentTerms; ++m)
+                            if (m < terms) {
+                                u[m] += w;
+                                w *= R;
+                            }
+                    }
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < kMomentTerms; ++m) part[m][lane] = u[m];
+    __syncthreads();
+    double sc;
+    if (kind == (int)RegionKind::Node) {
+        sc = kInv4PiEps0 / (aux[i] * aux[j]);
+    } else {
+        const double dot = d_dot3(dir + 3 * i, dir + 3 * j);
+        if (fabs(dot) < dir_tol) return;
+        sc = kMu0Over4Pi * dot / (aux[i] * aux[j]);
+    }
+    if (lane >= terms) return;
+    double v = 0.0;
+    for (int l = 0; l < 64; ++l) v += part[lane][l];
+    v *= sc;
+    if (!isfinite(v)) atomicOr(err, 1);
+    const int a = min(i, j), b = max(i, j);
+    const size_t p = moment_slot(a, b, nt);
+    mom[(size_t)lane * mstride + p] = v;
+    if ((a >> 6) == (b >> 6)) mom[(size_t)lane * mstride + p - (size_t)(a & 63) * 64 - (b & 63) + (size_t)(b & 63) * 64 + (a & 63)] = v;
+}
+
+struct MomentCoefficients {
+    double c[kMomentTerms];  // c[m]: coefficient of U_(m+1); odd m real, even m imaginary part
+};
+
+// out = static + sum_m c_m U_m over one 64 x 64 upper tile per workgroup:
+// row-major writes of the tile, then its transpose through LDS (the real
+// and imaginary parts in turn) for the lower triangle.
+__global__ __launch_bounds__(256) void moment_eval_kernel(
+    const double *__restrict__ mom, size_t mstride, int terms, MomentCoefficients cf,
+    const int *__restrict__ tile_ids, int R, const double *__restrict__ stat, double2 *__restrict__ out,
+    size_t ld) {
+    __shared__ double tr[64][65];
+    const int t = blockIdx.x;
+    const int ta = tile_ids[2 * t], tb = tile_ids[2 * t + 1];
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const double *base = mom + (size_t)t * 4096 + (size_t)r0 * 64 + c;
+    double re[16], im[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) re[q] = im[q] = 0.0;
+    // smallest terms first
+    for (int m = terms - 1; m >= 0; --m) {
+        const double *src = base + (size_t)m * mstride;
+        const double a = cf.c[m];
+        if (m & 1) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) re[q] = __builtin_fma(a, src[q * 256], re[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) im[q] = __builtin_fma(a, src[q * 256], im[q]);
+        }
+    }
+    const int j = tb * 64 + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = ta * 64 + r0 + 4 * q;
+        if (i < R && j < R) {
+            const size_t o = (size_t)i * ld + j;
+            re[q] += stat[o];
+            out[o] = make_double2(re[q], im[q]);
+        }
+    }
+    if (ta == tb) return;  // the diagonal tile holds both halves
+    // lower triangle: out[j][i] = value(i, j) for (i, j) in this tile
+    double tre[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tr[r0 + 4 * q][c] = re[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tre[q] = tr[c][r0 + 4 * q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tr[r0 + 4 * q][c] = im[q];
+    __syncthreads();
+    const int i = ta * 64 + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int jj = tb * 64 + r0 + 4 * q;
+        if (i < R && jj < R) out[(size_t)jj * ld + i] = make_double2(tre[q], tr[c][r0 + 4 * q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 namespace {
@@ -767,6 +988,20 @@ RegionSet::RegionSet(const Mesh *mesh, const DualMesh *dual, RegionKind kind,
         d_smooth_n_.upload(ns, s);
     }
     build_near_list(mesh);
+    // diameter of the quadrature points (they lie in the pieces' triangles):
+    // the bounding-box diagonal of every piece vertex
+    {
+        double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (size_t v = 0; v < h_pv_.size() / 3; ++v)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], h_pv_[3 * v + d]);
+                hi[d] = std::max(hi[d], h_pv_[3 * v + d]);
+            }
+        double d2 = 0.0;
+        for (int d = 0; d < 3; ++d)
+            if (hi[d] >= lo[d]) d2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+        diameter_ = std::sqrt(d2);
+    }
     d_ptr_.upload(h_ptr_, s);
     d_pv_.upload(h_pv_, s);
     d_aux_.upload(h_aux_, s);
@@ -993,6 +1228,90 @@ void RegionSet::assemble_retarded(const double *d_static, double k, double2 *d_o
         HIP_CHECK(hipGetLastError());
     }
     if (work) work->smooth_evaluations += smooth_evals_;
+}
+
+size_t RegionSet::moment_bytes(int regions, int terms) {
+    const size_t nt = ((size_t)regions + 63) / 64;
+    return nt * (nt + 1) / 2 * 4096 * sizeof(double) * (size_t)std::max(terms, 0);
+}
+
+int RegionSet::series_terms(double k) const {
+    const double x = k * diameter_;
+    double t = x;  // x^m / m!
+    for (int m = 1; m <= kMaxMomentTerms; ++m) {
+        t *= x / (m + 1);
+        if (t <= 1e-17) return m;
+    }
+    return kMaxMomentTerms + 1;
+}
+
+void RegionSet::build_moments(int terms, hipStream_t s) {
+    terms = std::min(terms, (int)kMaxMomentTerms);
+    moment_terms_ = 0;
+    d_moments_.release();
+    if (terms <= 0 || R_ == 0) return;
+    if (smooth_order_ <= 0) throw HipError{"region set built without smooth-rule points"};
+    const int nt = (R_ + 63) / 64;
+    moment_tiles_ = nt * (nt + 1) / 2;
+    std::vector<int> ids;
+    ids.reserve(2 * (size_t)moment_tiles_);
+    for (int a = 0; a < nt; ++a)
+        for (int b = a; b < nt; ++b) {
+            ids.push_back(a);
+            ids.push_back(b);
+        }
+    d_moment_tiles_.upload(ids, s);
+    const size_t mstride = (size_t)moment_tiles_ * 4096;
+    d_moments_.resize(mstride * terms);
+    HIP_CHECK(hipMemsetAsync(d_moments_.ptr, 0, mstride * terms * sizeof(double), s));
+    const int ntiles = (int)h_tiles_.size() / 2;
+    const size_t lds = 4 * (size_t)smooth_slots_ * sizeof(double4);
+    if (kind_ == RegionKind::Node)
+        hipLaunchKernelGGL(moment_tile_kernel<0>, dim3(ntiles), dim3(256), lds, s, d_smooth_i_.ptr,
+                           d_smooth_j_.ptr, d_smooth_n_.ptr, R_, smooth_slots_, d_aux_k_.ptr, d_dir_k_.ptr,
+                           dir_tol_, d_perm_.ptr, d_tiles_.ptr, terms, d_moments_.ptr, mstride, nt);
+    else
+        hipLaunchKernelGGL(moment_tile_kernel<1>, dim3(ntiles), dim3(256), lds, s, d_smooth_i_.ptr,
+                           d_smooth_j_.ptr, d_smooth_n_.ptr, R_, smooth_slots_, d_aux_k_.ptr, d_dir_k_.ptr,
+                           dir_tol_, d_perm_.ptr, d_tiles_.ptr, terms, d_moments_.ptr, mstride, nt);
+    HIP_CHECK(hipGetLastError());
+    const int nn = (int)h_near_.size() / 2;
+    if (nn && near_smooth_order_ > 0) {
+        if (!d_smooth_err_.ptr) d_smooth_err_.resize(1);
+        HIP_CHECK(hipMemsetAsync(d_smooth_err_.ptr, 0, sizeof(int), s));
+        hipLaunchKernelGGL(near_moment_kernel, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr, d_pv_.ptr,
+                           near_smooth_order_, (int)kind_, d_aux_.ptr, d_dir_.ptr, dir_tol_, terms,
+                           d_moments_.ptr, mstride, nt, d_smooth_err_.ptr);
+        HIP_CHECK(hipGetLastError());
+        int err = 0;
+        HIP_CHECK(hipMemcpyAsync(&err, d_smooth_err_.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (err) throw HipError{"near-field moment pass failed (non-finite value or too many tasks)"};
+    }
+    moment_terms_ = terms;
+}
+
+void RegionSet::assemble_retarded_moments(const double *d_static, double k, int terms, double2 *d_out,
+                                          hipStream_t s, KernelTimer *timer) {
+    if (terms < 1 || terms > moment_terms_) throw HipError{"moment terms out of range"};
+    MomentCoefficients cf{};
+    // coefficient of U_m: (-jk)^m / m!; (-j)^m = 1, -j, -1, j for m = 0, 1, 2, 3 (mod 4)
+    double km = 1.0;
+    for (int m = 1; m <= terms; ++m) {
+        km *= k / m;
+        const int r = m & 3;
+        cf.c[m - 1] = (r == 1 || r == 2) ? -km : km;
+    }
+    const int tk = timer ? timer->start(s) : -1;
+    const size_t mstride = (size_t)moment_tiles_ * 4096;
+    hipLaunchKernelGGL(moment_eval_kernel, dim3(moment_tiles_), dim3(256), 0, s, d_moments_.ptr, mstride, terms,
+                       cf, d_moment_tiles_.ptr, R_, d_static, d_out, (size_t)R_);
+    HIP_CHECK(hipGetLastError());
+    // two flops per stored value; bytes: the moments and the static upper
+    // tiles read, the complex matrix written
+    if (timer)
+        timer->stop(tk, s, kKernelSmoothMoments, 2.0 * terms * mstride,
+                    8.0 * (terms + 1) * mstride + 16.0 * R_ * R_);
 }
 
 void RegionSet::assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,

@@ -112,9 +112,27 @@ struct Stream {
     void sync() const { HIP_CHECK(hipStreamSynchronize(s)); }
 };
 
+// Device bytes an engine allocates per frequency point beyond its static
+// Lp/P: the retarded complex Lp(w)/P(w), the FP64 MNA matrix and its
+// unscaled copy, the FP32 factor, and a margin for the vectors, packed LU
+// operands and block inverses.
+inline size_t engine_point_bytes(size_t ne, size_t nv, bool retarded) {
+    const size_t n = ne + nv, sq = ne * ne + nv * nv;
+    size_t b = 32 * n * n + 8 * n * n;
+    if (retarded) b += 16 * sq;
+    return b + n * (size_t)64 * 1024 + ((size_t)64 << 20);
+}
+
 // Per-kernel timing with HIP events recorded on the launching stream; the
 // elapsed times are resolved after the caller's next synchronisation.
-enum KernelKind { kKernelLuUpdate = 0, kKernelPanel = 1, kKernelSmoothTile = 2, kKernelFarTile = 3, kKernelKinds = 4 };
+enum KernelKind {
+    kKernelLuUpdate = 0,
+    kKernelPanel = 1,
+    kKernelSmoothTile = 2,
+    kKernelFarTile = 3,
+    kKernelSmoothMoments = 4,
+    kKernelKinds = 5
+};
 
 struct KernelTotals {
     long long launches = 0;

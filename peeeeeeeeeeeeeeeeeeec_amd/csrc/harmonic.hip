@@ -253,7 +253,7 @@ __global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *
     dst[i] = src[i];
 }
 
-static void assemble_retarded(PeecEngine *e, double k);
+static int assemble_retarded(PeecEngine *e, double k);
 
 static int factor_error(const LUStatus &st) {
     if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
@@ -301,7 +301,7 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
     e->cache_ok = false;
     const bool ret = e->opt.retarded != 0;
     e->retarded_now = ret;
-    if (ret) assemble_retarded(e, omega / kC0);
+    if (ret) report.moment_terms = assemble_retarded(e, omega / kC0);
     HIP_CHECK(hipEventRecord(e->ev[1], s));
     e->d_Z.resize(nn);
     e->d_Z0.resize(nn);
@@ -506,7 +506,12 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->refinement_target = 1e-14;
     o->backward_error_gate = 1e-10;
     o->reuse_pivots = 1;
+    o->smooth_moments = -1;
 }
+
+// Moment terms stored by default: the series then covers k * diameter up to
+// 2.56 (f up to 50 MHz on a body of 2.45 m). Fewer than 8 are not stored.
+static constexpr int kAutoMomentTerms = 26, kMinAutoMomentTerms = 8;
 
 static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngineOptions *o, const double *L_host,
                          const double *P_host, PeecEngine **out);
@@ -570,6 +575,28 @@ static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngin
             PEEC_ERR("static assembly failed (code %d).", err);
             return -1;
         }
+        if (o->retarded && o->smooth_moments != 0) {
+            // moments of the smooth remainder for frequency sweeps: the
+            // requested count, or (-1) kAutoMomentTerms when they fit in a
+            // quarter of the device memory left free by the engine's
+            // per-point working set
+            int terms = std::min(o->smooth_moments, RegionSet::kMaxMomentTerms);
+            if (o->smooth_moments < 0) {
+                size_t free_b = 0, total_b = 0;
+                HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+                const size_t point = engine_point_bytes(e->ne, e->nv, true);
+                const size_t room = free_b > point ? (free_b - point) / 4 : 0;
+                terms = kAutoMomentTerms;
+                while (terms >= kMinAutoMomentTerms &&
+                       RegionSet::moment_bytes(e->ne, terms) + RegionSet::moment_bytes(e->nv, terms) > room)
+                    --terms;
+                if (terms < kMinAutoMomentTerms) terms = 0;
+            }
+            if (terms > 0) {
+                e->edges->build_moments(terms, s);
+                e->nodes->build_moments(terms, s);
+            }
+        }
         e->far_evals = w.far_evaluations;
         e->smooth_evals = e->edges->smooth_evaluations() + e->nodes->smooth_evaluations();
         for (hipEvent_t &ev : e->ev) HIP_CHECK(hipEventCreate(&ev));
@@ -621,12 +648,26 @@ int peec_engine_static_matrices(const PeecEngine *e, double *L, double *P) {
     }
 }
 
-static void assemble_retarded(PeecEngine *e, double k) {
+// Retarded Lp(w) and P(w): from the stored moments when the series at this
+// wave number needs no more terms than stored, else by quadrature. Returns
+// the moment terms used for Lp (0 = quadrature).
+static int assemble_retarded(PeecEngine *e, double k) {
     e->d_Lw.resize((size_t)e->ne * e->ne);
     e->d_Pw.resize((size_t)e->nv * e->nv);
     AssemblyWork w;
-    e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w, &e->timer);
-    e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w, &e->timer);
+    int used = 0;
+    const int tl = e->edges->series_terms(k), tp = e->nodes->series_terms(k);
+    if (tl <= e->edges->moment_terms()) {
+        e->edges->assemble_retarded_moments(e->d_L.ptr, k, tl, e->d_Lw.ptr, e->stream, &e->timer);
+        used = tl;
+    } else {
+        e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w, &e->timer);
+    }
+    if (tp <= e->nodes->moment_terms())
+        e->nodes->assemble_retarded_moments(e->d_P.ptr, k, tp, e->d_Pw.ptr, e->stream, &e->timer);
+    else
+        e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w, &e->timer);
+    return used;
 }
 
 int peec_engine_retarded_matrices(PeecEngine *e, double k, Complex *L, Complex *P) {

@@ -54,15 +54,10 @@ struct Job {
     bool done = false;
 };
 
-// Device bytes of one engine of a mesh: static Lp/P (FP64), the retarded
-// complex Lp(w)/P(w), the FP64 MNA matrix and its unscaled copy, the FP32
-// factor, and a margin for the vectors, packed LU operands and block
-// inverses.
+// Device bytes of one engine of a mesh: static Lp/P (FP64) and the
+// per-point working set.
 size_t engine_bytes(size_t ne, size_t nv, bool retarded) {
-    const size_t n = ne + nv, sq = ne * ne + nv * nv;
-    size_t b = 8 * sq + 32 * n * n + 8 * n * n;
-    if (retarded) b += 16 * sq;
-    return b + n * (size_t)64 * 1024 + ((size_t)64 << 20);
+    return 8 * (ne * ne + nv * nv) + peec::engine_point_bytes(ne, nv, retarded);
 }
 
 }  // namespace
@@ -207,6 +202,9 @@ int peec_server_create(const PeecServerOptions *o, PeecServer **out) {
     }
     auto sv = std::make_unique<PeecServer>();
     sv->opt = *o;
+    // the admission estimate has no room for stored moments of the smooth
+    // remainder: engines assemble every point by quadrature
+    sv->opt.engine.smooth_moments = 0;
     sv->budget = o->hbm_budget_bytes ? o->hbm_budget_bytes : free_b / 10 * 9;
     for (int i = 0; i < o->slots; ++i) sv->workers.emplace_back([s = sv.get()] { s->worker(); });
     *out = sv.release();

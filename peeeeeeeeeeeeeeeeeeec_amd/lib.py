@@ -28,7 +28,7 @@ class PeecEngineOptions(C.Structure):
                 ("retarded", C.c_int), ("smooth_order", C.c_int), ("near_smooth_order", C.c_int),
                 ("block_size", C.c_int),
                 ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
-                ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int)]
+                ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int), ("smooth_moments", C.c_int)]
 
 
 class PeecPointReport(C.Structure):
@@ -37,7 +37,7 @@ class PeecPointReport(C.Structure):
                 ("pivot_reuse", C.c_int), ("pivot_refactor", C.c_int), ("reuse_backward_error", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int),
-                ("block_size", C.c_int)]
+                ("moment_terms", C.c_int), ("block_size", C.c_int)]
 
 
 class PeecServerOptions(C.Structure):
@@ -59,7 +59,7 @@ class PeecKernelStats(C.Structure):
                 ("bytes", C.c_double)]
 
 
-KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3}
+KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4}
 
 
 class RetardationOptions(C.Structure):

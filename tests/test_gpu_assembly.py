@@ -117,3 +117,58 @@ def test_retarded_matches_oracle(require_gpu, k):
     ep = np.abs(P - Po).max() / np.abs(Po).max()
     print(f"retarded k={k}: L {el:.3e}, P {ep:.3e}", flush=True)
     assert el <= 1e-11 and ep <= 1e-11
+
+
+@pytest.mark.parametrize("k", [0.0, 1e-4, 0.05, 1.0, 3.0, 6.0])
+def test_retarded_from_moments_matches_quadrature(require_gpu, k):
+    """Retarded Lp(w)/P(w) from the stored power-series moments of the smooth
+    remainder against the quadrature pass of the same engine options and the
+    oracle. The 0.6 x 0.4 x 0.4 m box has a diameter of 0.825 m, so with 26
+    stored terms k <= 3 takes the moments (k * diameter <= 2.56) and k = 6 the
+    quadrature. Real and imaginary parts are checked separately: the
+    imaginary part (radiation loss) is tiny next to the static values at low k."""
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 6, 4, 4)
+    with plib.Model(xyz, quads) as m:
+        out = {}
+        for terms in (26, 0):
+            eng = plib.Engine(m, retarded=True, smooth_moments=terms)
+            out[terms] = eng.retarded_matrices(k)
+            eng.close()
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Lo, Po = o.retarded(k, 2, o.inductance(), o.potential(), 4)
+    finally:
+        o.close()
+    for name, a, b, c in (("L", out[26][0], out[0][0], Lo), ("P", out[26][1], out[0][1], Po)):
+        assert np.array_equal(a, a.T) and np.array_equal(b, b.T)
+        e_re = np.abs(a.real - b.real).max() / np.abs(b.real).max()
+        im_scale = np.abs(b.imag).max()
+        e_im = np.abs(a.imag - b.imag).max() / im_scale if im_scale > 0 else np.abs(a.imag).max()
+        e_or = np.abs(a - c).max() / np.abs(c).max()
+        print(f"k={k} {name}: moments vs quadrature re {e_re:.2e} im {e_im:.2e}; vs oracle {e_or:.2e}", flush=True)
+        assert e_re <= 1e-14 and e_im <= 1e-12 and e_or <= 1e-11
+        if k == 0.0:
+            assert np.array_equal(a.imag, np.zeros_like(a.imag))
+
+
+def test_engine_points_from_moments(require_gpu):
+    """Sweep points on the moment path (report.moment_terms > 0) and above its
+    range (quadrature) give the impedances of an engine without moments."""
+    xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
+    port = (0, len(xyz) - 1)
+    freqs = [1e4, 3e6, 1e8, 2e8, 6e8]
+    res = {}
+    with plib.Model(xyz, quads) as m:
+        for terms in (-1, 0):
+            eng = plib.Engine(m, retarded=True, smooth_moments=terms)
+            res[terms] = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+            eng.close()
+    used = [r[4].moment_terms for r in res[-1]]
+    print("moment terms per point:", used, flush=True)
+    assert all(r[0] == 0 for r in res[-1] + res[0])
+    assert all(r[4].moment_terms == 0 for r in res[0])
+    # diameter 1.06 m: k * diameter = 2.2 at 100 MHz (moments), 13 at 600 MHz (quadrature)
+    assert used[0] >= 1 and used[1] > used[0] and used[2] > used[1] and used[-1] == 0
+    for a, b in zip(res[-1], res[0]):
+        assert abs(a[1][0] - b[1][0]) <= 1e-10 * abs(b[1][0])
+        assert a[4].backward_error <= 1e-10
