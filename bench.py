@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "fp64_vector": 78.6e12, "hbm": 8.0e12}
+PEAKS = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "fp16_mfma": 2.5e15, "fp64_vector": 78.6e12, "hbm": 8.0e12}
 C0 = 299792458.0
 
 
@@ -111,11 +111,11 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 5}))
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the LU update measured by rocprofv3 --pmc
+def traffic_from_profile(name="lu_update_split16_traffic_c2.md"):
+    """HBM bytes per launch of an LU update kernel measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration), read
     from the committed summary; None when absent."""
-    path = os.path.join(ROOT, "profiles", "lu_update_traffic_c2.md")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -413,6 +413,7 @@ def main():
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t_start
     gemm = eng.kernel_stats("lu_update")
+    g16 = eng.kernel_stats("lu_update_split16")
     panel = eng.kernel_stats("panel")
     smooth = eng.kernel_stats("smooth_tile")
     moments = eng.kernel_stats("smooth_moments")
@@ -443,6 +444,8 @@ def main():
     zerr = np.abs(zm - z64) / np.abs(z64)
     value = len(timed) / t_mixed
     gemm_tf = gemm["flops"] / (gemm["ms"] * 1e-3) if gemm["ms"] > 0 else 0.0
+    g16_gbs = g16["bytes"] / (g16["ms"] * 1e-3) if g16["ms"] > 0 else 0.0
+    g16_tf = g16["flops"] / (g16["ms"] * 1e-3) if g16["ms"] > 0 else 0.0
     out = {
         "metric": "freq-point solves/sec at <=1e-10 rel residual (mixed+IR vs FP64 LU); p99 lat",
         "value": value,
@@ -454,7 +457,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex fp32 MFMA factor + complex fp64 refinement/residual",
+        "dtype": "complex fp32 factor (split-fp16 MFMA trailing update) + complex fp64 refinement/residual",
         "data": "synthetic mesh (closed box), no external inputs",
         "config": {
             "workload": (f"closed box 2x1x1 m, {h*100:.0f} cm quads ({n_cells} cells, Ne={n_e}, "
@@ -482,17 +485,25 @@ def main():
                           for k in ("assembly", "fill", "factor", "solve", "check")},
         "static_assembly_s": t_static,
         "roofline": {
-            "kernel": ("LU trailing update A22 -= L21 U12, complex FP32 on v_mfma_f32_16x16x4f32 "
-                       "(gemm_stream_kernel<float2, 32> + gemm_splitk_kernel<float2> of the look-ahead column)"),
-            "bound": "mfma",
-            "achieved": gemm_tf / 1e12,
-            "peak": PEAKS["fp32_mfma"] / 1e12,
-            "unit": "TFLOP/s",
-            "frac": gemm_tf / PEAKS["fp32_mfma"],
+            "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, complex FP32 storage, "
+                       "split-FP16 operands on v_mfma_f32_16x16x32_f16 (gemm_stream_h3_kernel<8>)"),
+            "bound": "hbm",
+            "achieved": g16_gbs / 1e9,
+            "peak": PEAKS["hbm"] / 1e9,
+            "unit": "GB/s",
+            "frac": g16_gbs / PEAKS["hbm"],
             "traffic": traffic_from_profile(),
+            "launches": g16["launches"],
+            "avg_launch_ms": g16["ms"] / max(1, g16["launches"]),
+            "algorithmic_bytes_per_launch": g16["bytes"] / max(1, g16["launches"]),
+            "algorithmic_tflops": g16_tf / 1e12,
+            "mfma_roof_tflops": PEAKS["fp16_mfma"] / 3 / 1e12,
+        },
+        "fp32_update": {
+            "kernel": "look-ahead column update gemm_splitk_kernel<float2> (v_mfma_f32_16x16x4f32)",
             "launches": gemm["launches"],
-            "avg_launch_ms": gemm["ms"] / max(1, gemm["launches"]),
-            "algorithmic_flops_per_launch": gemm["flops"] / max(1, gemm["launches"]),
+            "tflops": gemm_tf / 1e12,
+            "ms_per_point": gemm["ms"] / len(timed),
         },
         "other_kernels": {
             "panel_ms_per_point": panel["ms"] / len(timed),

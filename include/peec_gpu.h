@@ -68,6 +68,11 @@ typedef struct
                                    -1 (default) = 26 terms when they fit in a quarter
                                    of the device memory left free by the engine's
                                    per-point working set, else fewer (off below 8) */
+    int split_fp16_update;      /* mixed mode: trailing updates of the reused-row-order
+                                   factor on FP16 MFMA with each FP32 operand split into
+                                   two FP16 parts (hi + lo, power-of-two scaled per row
+                                   / column; three products, FP32 accumulation, ~2^-22
+                                   per product). 0 = FP32 MFMA. Default 1 */
 } PeecEngineOptions;
 
 typedef struct

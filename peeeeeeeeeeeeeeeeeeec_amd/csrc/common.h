@@ -131,7 +131,8 @@ enum KernelKind {
     kKernelSmoothTile = 2,
     kKernelFarTile = 3,
     kKernelSmoothMoments = 4,
-    kKernelKinds = 5
+    kKernelLuUpdateSplit16 = 5,
+    kKernelKinds = 6
 };
 
 struct KernelTotals {

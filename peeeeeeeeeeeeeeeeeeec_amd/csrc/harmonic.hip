@@ -507,6 +507,7 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->backward_error_gate = 1e-10;
     o->reuse_pivots = 1;
     o->smooth_moments = -1;
+    o->split_fp16_update = 1;
 }
 
 // Moment terms stored by default: the series then covers k * diameter up to
@@ -551,6 +552,7 @@ static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngin
                                                o->near_smooth_order);
         e->lu = std::make_unique<LUContext>(s);
         e->lu->timer = &e->timer;
+        e->lu->split_fp16_update = o->split_fp16_update != 0;
         if (mesh->edge_centers && mesh->edge_vectors) e->rcs = std::make_unique<RcsGeometry>(mesh, dual, s);
         std::vector<int> edges(mesh->edges, mesh->edges + 2 * e->ne);
         e->d_edges.upload(edges, s);

@@ -31,6 +31,11 @@ struct LUContext {
     DeviceBuffer<double> solve_tmp;
     DeviceBuffer<float2> solve_f32;              // FP32 right-hand sides of the blocked solves
     DeviceBuffer<unsigned char> pack_l, pack_u;  // MFMA-ordered trailing-update operands
+    // split-FP16 operands of the ordered factor's update and their per-row /
+    // per-column inverse scales
+    DeviceBuffer<unsigned char> pack_l16, pack_u16;
+    DeviceBuffer<float> scale_l16, scale_u16;
+    bool split_fp16_update = true;                // ordered factor: FP16 hi/lo update (else FP32 MFMA)
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves
     DeviceBuffer<unsigned char> diag_inv;         // inverses of the 64x64 diagonal blocks (solves)

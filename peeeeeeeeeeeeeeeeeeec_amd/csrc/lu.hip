@@ -1325,6 +1325,278 @@ template LUStatus lu_factor<float2>(float2 *, int, size_t, int *, double, LUCont
 template LUStatus lu_factor<double>(double *, int, size_t, int *, double, LUContext &, int);
 
 // ---------------------------------------------------------------------------
+// split-FP16 trailing update of the ordered factor
+//
+// C -= L21 U12 in complex FP32 storage on v_mfma_f32_16x16x32_f16. Each
+// FP32 operand x is scaled by a power of two 2^e (per row of L21, per column
+// of U12, so that the row's / column's largest part lies in [2^13, 2^14)) and
+// split as x 2^e = hi + lo with hi = fp16(x 2^e), lo = fp16(x 2^e - hi);
+// hi hi' + hi lo' + lo hi' carries the product to ~2^-22 relative (the lo lo'
+// term is below that), accumulated in FP32 like the FP32 MFMA update, and
+// the scales come off exactly at the store. The complex product is two real
+// dot products over interleaved parts k^ = 2k + p:
+//   Re C = sum (Ar, -Ai) . (Br, Bi),   Im C = sum (Ai, Ar) . (Br, Bi),
+// so L21 is packed once as the pairs (Ar, -Ai) and the pairs (Ai, Ar) are
+// formed in registers (swap the halves, flip one sign).
+//
+// Packed layout, in 16-byte lane vectors (8 halves): a chunk is 16 complex k
+// (32 k^); for a 16-row (16-column) block b and chunk c the hi vectors of
+// its 64 lanes, then the lo vectors, at ((c * blocks + b) * 2 + h) * 64 +
+// lane. Lane l holds row (column) l & 15 and complex k = 16 c + 4 (l >> 4)
+// .. + 3. The MFMA pairs A and B lane slots by the same k^ order, so only the
+// row / column of a lane matters.
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline half8_t as_half8(uint4 v) { return __builtin_bit_cast(half8_t, v); }
+// half pair (x, y) -> (-y, x)
+__device__ inline unsigned turn_pair(unsigned d) { return ((d >> 16) ^ 0x8000u) | (d << 16); }
+__device__ inline uint4 turn_pairs(uint4 v) {
+    return make_uint4(turn_pair(v.x), turn_pair(v.y), turn_pair(v.z), turn_pair(v.w));
+}
+__device__ inline unsigned pack_half2(_Float16 a, _Float16 b) {
+    return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+// 2^(13 - e) for the largest magnitude mx (0 or non-finite: 1)
+__device__ inline float split_scale(float mx) {
+    if (!(mx > 0.f) || !isfinite(mx)) return 1.f;
+    return ldexpf(1.f, 13 - ilogbf(mx));
+}
+// x -> hi, lo halves
+__device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+// One wavefront per 16-row block of L21 = A[row0 + r][k0 + k], r < m (rows
+// past m are packed as zeros), k < kw (a multiple of 16, at most 128).
+__global__ __launch_bounds__(256) void pack_l_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
+                                                        int rb_pad, uint4 *out, float *inv, const int *status) {
+    if (status[0] != 0) return;
+    const int rb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (rb >= rb_pad) return;
+    const int r = rb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
+    float2 v[8][4];
+    float mx = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v[c][t] = (c < kc && r < m) ? A[(size_t)(row0 + r) * lda + k0 + 16 * c + 4 * q + t] : make_float2(0.f, 0.f);
+            mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
+        }
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (c >= kc) break;
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            _Float16 xh, xl, yh, yl;
+            split_half(v[c][t].x * sc, xh, xl);
+            split_half(-v[c][t].y * sc, yh, yl);
+            hw[t] = pack_half2(xh, yh);
+            lw[t] = pack_half2(xl, yl);
+        }
+        uint4 *o = out + ((size_t)(c * rb_pad + rb) * 2) * 64 + lane;
+        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+    if (q == 0) inv[r] = 1.f / sc;
+}
+
+// One wavefront per 16-column block of U12 = A[k0 + k][col0 + c], c < ncols
+// (columns past ncols packed as zeros), as pairs (Br, Bi).
+__global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t lda, int k0, int kw, int col0,
+                                                        int ncols, int cb_count, uint4 *out, float *inv,
+                                                        const int *status) {
+    if (status[0] != 0) return;
+    const int cb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (cb >= cb_count) return;
+    const int col = cb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
+    float2 v[8][4];
+    float mx = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v[c][t] = (c < kc && col < ncols) ? A[(size_t)(k0 + 16 * c + 4 * q + t) * lda + col0 + col]
+                                             : make_float2(0.f, 0.f);
+            mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
+        }
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (c >= kc) break;
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            _Float16 xh, xl, yh, yl;
+            split_half(v[c][t].x * sc, xh, xl);
+            split_half(v[c][t].y * sc, yh, yl);
+            hw[t] = pack_half2(xh, yh);
+            lw[t] = pack_half2(xl, yl);
+        }
+        uint4 *o = out + ((size_t)(c * cb_count + cb) * 2) * 64 + lane;
+        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+    if (q == 0) inv[col] = 1.f / sc;
+}
+
+// The persistent streamed update of gemm_stream_kernel<float2> (128 x 128
+// workgroup tiles, 64 x 64 per wave, C read a few elements per chunk under
+// the MFMAs, the next tile's first chunk loaded during the current one) on
+// the split-FP16 operands: per chunk and 16 x 16 block, three MFMAs for the
+// real and three for the imaginary part. KC chunks (kw = 16 KC).
+template <int KC>
+__global__ __launch_bounds__(256, 1) void gemm_stream_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
+                                                                int ncols, const uint4 *__restrict__ Lh,
+                                                                const uint4 *__restrict__ Uh,
+                                                                const float *__restrict__ linv,
+                                                                const float *__restrict__ uinv, int rb_pad,
+                                                                int cb_count, const int *status) {
+    constexpr int MB = 4, NB = 4, BM = 128, BN = 128;
+    constexpr int CL = MB * NB * 4;  // C elements per lane
+    constexpr int NBUF = 2;          // operand register sets (one chunk ahead)
+    static_assert(KC % NBUF == 0, "a tile's chunks must cycle the register sets exactly");
+    if (status[0] != 0) return;
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (ncols + BN - 1) / BN;
+    const int V = 8 * ((tiles_m * tiles_n + 7) / 8);
+    int vt = blockIdx.x, tm = 0, tn = 0;
+    while (vt < V && !gemm_tile_at(vt, tiles_m, tiles_n, tm, tn)) vt += gridDim.x;
+    if (vt >= V) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const size_t lstride = (size_t)rb_pad * 128, ustride = (size_t)cb_count * 128;
+    auto lp_of = [&](int tm_) { return Lh + (size_t)(tm_ * (2 * MB) + (wave >> 1) * MB) * 128 + lane; };
+    auto up_of = [&](int tn_) { return Uh + (size_t)(tn_ * (2 * NB) + (wave & 1) * NB) * 128 + lane; };
+    uint4 ah[NBUF][MB], al[NBUF][MB], bh[NBUF][NB], bl[NBUF][NB];
+    auto load_ops = [&](const uint4 *lp_, const uint4 *up_, int buf, int c) {
+        const uint4 *l_ = lp_ + (size_t)c * lstride, *u_ = up_ + (size_t)c * ustride;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            ah[buf][i] = l_[i * 128];
+            al[buf][i] = l_[i * 128 + 64];
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            bh[buf][j] = u_[j * 128];
+            bl[buf][j] = u_[j * 128 + 64];
+        }
+    };
+    const uint4 *lp = lp_of(tm), *up = up_of(tn);
+    load_ops(lp, up, 0, 0);
+    for (;;) {
+        int vn = vt + gridDim.x, tmn = tm, tnn = tn;
+        while (vn < V && !gemm_tile_at(vn, tiles_m, tiles_n, tmn, tnn)) vn += gridDim.x;
+        const bool more = vn < V;
+        const uint4 *lpn = more ? lp_of(tmn) : lp, *upn = more ? up_of(tnn) : up;
+        const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
+        const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
+        float4_t cre[MB][NB], cim[MB][NB];
+#pragma unroll
+        for (int a = 0; a < MB; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                cre[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+                cim[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+            }
+        // C element e = (i * NB + j) * 4 + v: row (rb0 + i) * 16 + 4 lk + v,
+        // column (cb0 + j) * 16 + li; out-of-range ones read a clamped address
+        float2 cv[CL];
+        auto c_ptr = [&](int e) -> float2 * {
+            const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
+            const int row = min((rb0 + i) * 16 + 4 * lk + v, m - 1);
+            const int col = min((cb0 + j) * 16 + li, ncols - 1);
+            return A + (size_t)(row0 + row) * lda + col0 + col;
+        };
+        static_for<KC>([&](auto step) {
+            constexpr int s = decltype(step)::value;
+            constexpr int e0 = s * CL / KC, e1 = (s + 1) * CL / KC;
+            if constexpr (s + 1 < KC) load_ops(lp, up, (s + 1) % NBUF, s + 1);
+            else load_ops(lpn, upn, 0, 0);
+#pragma unroll
+            for (int e = e0; e < e1; ++e) cv[e] = *c_ptr(e);
+            __builtin_amdgcn_sched_barrier(0);
+            constexpr int b = s % NBUF;
+#pragma unroll
+            for (int i = 0; i < MB; ++i) {
+                const half8_t rh = as_half8(ah[b][i]), rl = as_half8(al[b][i]);
+                const half8_t ih = as_half8(turn_pairs(ah[b][i])), il = as_half8(turn_pairs(al[b][i]));
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const half8_t uh = as_half8(bh[b][j]), ul = as_half8(bl[b][j]);
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, uh, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, uh, cim[i][j], 0, 0, 0);
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, ul, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, ul, cim[i][j], 0, 0, 0);
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rl, uh, cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, uh, cim[i][j], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        float fc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) fc[j] = uinv[(cb0 + j) * 16 + li];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = (rb0 + i) * 16 + 4 * lk + v;
+                const float fr = linv[row];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const int col = (cb0 + j) * 16 + li;
+                    const int e = (i * NB + j) * 4 + v;
+                    const float f = fr * fc[j];
+                    float2 r = cv[e];
+                    r.x -= cre[i][j][v] * f;
+                    r.y -= cim[i][j][v] * f;
+                    if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
+                }
+            }
+        if (!more) break;
+        vt = vn;
+        tm = tmn;
+        tn = tnn;
+        lp = lpn;
+        up = upn;
+    }
+}
+
+// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 16
+// or 8 chunks' worth of columns), persistent like gemm_packed.
+static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
+                    const int *status, LUContext &ctx, hipStream_t s, bool leave_cus) {
+    if (m <= 0 || ncols <= 0) return;
+    const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr);
+    const uint4 *Uh = reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr);
+    const int tiles = ((m + 127) / 128) * ((ncols + 127) / 128);
+    const int grid = (tiles + 7) / 8 * 8;
+    const int pgrid = std::min(grid, std::max(8, (ctx.cus - (leave_cus ? 8 : 0)) / 8 * 8));
+    const int tg = ctx.timer ? ctx.timer->start(s) : -1;
+    if (kw == 128)
+        hipLaunchKernelGGL((gemm_stream_h3_kernel<8>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
+                           Uh, ctx.scale_l16.ptr, ctx.scale_u16.ptr, rb_pad, cb_count, status);
+    else if (kw == 64)
+        hipLaunchKernelGGL((gemm_stream_h3_kernel<4>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
+                           Uh, ctx.scale_l16.ptr, ctx.scale_u16.ptr, rb_pad, cb_count, status);
+    else
+        throw HipError{"split-FP16 update needs a block width of 64 or 128"};
+    HIP_CHECK(hipGetLastError());
+    if (ctx.timer)
+        ctx.timer->stop(tg, s, kKernelLuUpdateSplit16, 8.0 * m * ncols * kw,
+                        (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
+}
+
+// ---------------------------------------------------------------------------
 // factorization with a given row order (no pivot search)
 //
 // The rows have been permuted beforehand (convert_rows_to_f32) into the order
@@ -1584,6 +1856,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_wave_kernel<S>), (size_t)nb * nb * sizeof(S));
     set_dynamic_lds(reinterpret_cast<const void *>(trsm_wave_kernel<S>), trsm_wave_lds(nb) * sizeof(S));
     set_dynamic_lds(reinterpret_cast<const void *>(gemm_splitk_kernel<S>), split_lds);
+    const bool split16 = ctx.split_fp16_update;
     // triangular solves: one wavefront per row of L21 / column of U12, at
     // most one workgroup of 16 per CU
     auto solve_l21 = [&](int k0, int w, int rest) {
@@ -1646,7 +1919,21 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         pending = true;
         // off the critical path: the rest of U12 and of the trailing update
         const int rest2 = rest - w2;
-        if (rest2 > 0) {
+        if (rest2 > 0 && split16 && (w == 128 || w == 64)) {
+            solve_u12(k0, w, r0 + w2, n);
+            const int cb16 = (rest2 + 127) / 128 * 8, kc = w / 16;
+            ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
+            ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
+            ctx.scale_l16.reserve((size_t)rb_pad * 16);
+            ctx.scale_u16.reserve((size_t)cb16 * 16);
+            hipLaunchKernelGGL(pack_l_h3_kernel, dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, dA, lda, r0,
+                               rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr), ctx.scale_l16.ptr,
+                               ctx.status.ptr);
+            hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cb16 + 3) / 4)), dim3(256), 0, s, dA, lda, k0, w,
+                               r0 + w2, rest2, cb16, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr,
+                               ctx.status.ptr);
+            gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16, ctx.status.ptr, ctx, s, true);
+        } else if (rest2 > 0) {
             solve_u12(k0, w, r0 + w2, n);
             const int cb_rest = cb_pad - w2 / 16;
             const size_t tu2 = (size_t)ks_count * cb_rest * 64;

@@ -28,7 +28,8 @@ class PeecEngineOptions(C.Structure):
                 ("retarded", C.c_int), ("smooth_order", C.c_int), ("near_smooth_order", C.c_int),
                 ("block_size", C.c_int),
                 ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
-                ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int), ("smooth_moments", C.c_int)]
+                ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int), ("smooth_moments", C.c_int),
+                ("split_fp16_update", C.c_int)]
 
 
 class PeecPointReport(C.Structure):
@@ -59,7 +60,8 @@ class PeecKernelStats(C.Structure):
                 ("bytes", C.c_double)]
 
 
-KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4}
+KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4,
+           "lu_update_split16": 5}
 
 
 class RetardationOptions(C.Structure):

@@ -345,6 +345,38 @@ def test_engine_fixed_block_width(require_gpu):
     np.testing.assert_allclose(out[64][0], out[0][0], rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("nb", [128, 64])
+def test_engine_split_fp16_update_vs_fp32_update(require_gpu, nb):
+    """Reused-row-order factors with the split-FP16 MFMA trailing update
+    (default) against the FP32 MFMA update and the FP64 LU, over a sweep on a
+    box of N = 1538 (12 block columns at 128): every impedance within 1e-10 of
+    FP64 with the backward error at or below 1e-10, and refinement taking at
+    most two steps more than with the FP32 update."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 12, 8, 8)
+    port = (0, len(xyz) - 1)
+    freqs = [1e4, 2e4, 1e5, 1e6, 1e7, 1e8, 2e8]
+    res = {}
+    with plib.Model(xyz, quads) as m:
+        assert m.ne + m.nv > 10 * nb
+        for split in (1, 0):
+            eng = plib.Engine(m, retarded=True, block_size=nb, split_fp16_update=split)
+            res[split] = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+            eng.close()
+        eng = plib.Engine(m, retarded=True)
+        z64 = [eng.solve(f, [port], plib.Engine.FP64, want_solution=False)[1][0] for f in freqs]
+        eng.close()
+    for f, a, b, zr in zip(freqs, res[1], res[0], z64):
+        ra, rb = a[4], b[4]
+        print(f"f={f:g} split16: steps {ra.refinement_steps} eta {ra.backward_error:.1e} reused {ra.pivot_reuse} "
+              f"refactor {ra.pivot_refactor} | fp32: steps {rb.refinement_steps} eta {rb.backward_error:.1e} "
+              f"| rel {abs(a[1][0] - zr) / abs(zr):.1e}", flush=True)
+        assert a[0] == 0 and b[0] == 0
+        assert ra.backward_error <= 1e-10 and not ra.used_fp64_fallback
+        assert abs(a[1][0] - zr) <= 1e-10 * abs(zr)
+        if ra.pivot_reuse and not ra.pivot_refactor and rb.pivot_reuse and not rb.pivot_refactor:
+            assert ra.refinement_steps <= rb.refinement_steps + 2
+
+
 def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path):
     """The reference's own Lp/P of the 16 x 16 plate (golden), written and read
     back as Matrix Market files, drive the engine without assembly: the port
