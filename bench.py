@@ -486,7 +486,7 @@ def main():
         "static_assembly_s": t_static,
         "roofline": {
             "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, complex FP32 storage, "
-                       "split-FP16 operands on v_mfma_f32_16x16x32_f16 (gemm_stream_h3_kernel<8>)"),
+                       "split-FP16 operands on v_mfma_f32_16x16x32_f16 (gemm_tile_h3_kernel<8, 2>)"),
             "bound": "hbm",
             "achieved": g16_gbs / 1e9,
             "peak": PEAKS["hbm"] / 1e9,

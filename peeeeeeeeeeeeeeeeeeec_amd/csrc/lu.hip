@@ -1349,11 +1349,11 @@ template LUStatus lu_factor<double>(double *, int, size_t, int *, double, LUCont
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline half8_t as_half8(uint4 v) { return __builtin_bit_cast(half8_t, v); }
 // half pair (x, y) -> (-y, x)
 __device__ inline unsigned turn_pair(unsigned d) { return ((d >> 16) ^ 0x8000u) | (d << 16); }
-__device__ inline uint4 turn_pairs(uint4 v) {
-    return make_uint4(turn_pair(v.x), turn_pair(v.y), turn_pair(v.z), turn_pair(v.w));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline u32x4_t turn_pairs(u32x4_t v) {
+    return u32x4_t{turn_pair(v.x), turn_pair(v.y), turn_pair(v.z), turn_pair(v.w)};
 }
 __device__ inline unsigned pack_half2(_Float16 a, _Float16 b) {
     return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
@@ -1449,145 +1449,157 @@ __global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t 
     if (q == 0) inv[col] = 1.f / sc;
 }
 
-// The persistent streamed update of gemm_stream_kernel<float2> (128 x 128
-// workgroup tiles, 64 x 64 per wave, C read a few elements per chunk under
-// the MFMAs, the next tile's first chunk loaded during the current one) on
-// the split-FP16 operands: per chunk and 16 x 16 block, three MFMAs for the
-// real and three for the imaginary part. KC chunks (kw = 16 KC).
-template <int KC>
-__global__ __launch_bounds__(256, 1) void gemm_stream_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
-                                                                int ncols, const uint4 *__restrict__ Lh,
-                                                                const uint4 *__restrict__ Uh,
-                                                                const float *__restrict__ linv,
-                                                                const float *__restrict__ uinv, int rb_pad,
-                                                                int cb_count, const int *status) {
-    constexpr int MB = 4, NB = 4, BM = 128, BN = 128;
-    constexpr int CL = MB * NB * 4;  // C elements per lane
-    constexpr int NBUF = 2;          // operand register sets (one chunk ahead)
-    static_assert(KC % NBUF == 0, "a tile's chunks must cycle the register sets exactly");
+// C -= L21 U12 on the split-FP16 operands, one tile of (32 MB) x (32 NB),
+// NB = 8 / MB, per workgroup of four waves in a 2 x 2 grid (each wave MB x NB
+// blocks of 16 x 16), two workgroups per CU so that one's waits fall under
+// the other's MFMAs. No persistent loop: no loads stay in flight across
+// tiles. A chunk of a tile's operands is 4 MB + 4 NB units of 1 KB (the hi
+// and lo vectors of its 2 MB row blocks and 2 NB column blocks); each wave
+// loads MB L21 and NB U12 units into one of four register sets four chunks
+// ahead and stores them to one of two LDS slots one chunk ahead, so every
+// operand is fetched once per workgroup. C is read after the last chunk.
+// Per chunk and 16 x 16 block: three MFMAs for the real and three for the
+// imaginary part. The update uses MB = 2 (64 x 128 tiles): rows of C are
+// then read and written in 1 KB runs.
+template <int KC, int MB>
+__global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
+                                                              int ncols, const uint4 *__restrict__ Lh,
+                                                              const uint4 *__restrict__ Uh,
+                                                              const float *__restrict__ linv,
+                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
+                                                              const int *status) {
+    constexpr int NB = 8 / MB, BM = 32 * MB, BN = 32 * NB;
+    constexpr int CL = MB * NB * 4;
+    constexpr int UA = 4 * MB, UNITS = UA + 4 * NB;  // L21 units, all units per chunk
+    static_assert(KC % 2 == 0 && KC >= 2 && CL % KC == 0, "chunks alternate between two LDS slots");
+    __shared__ u32x4_t slots_lds[2 * UNITS * 64];
     if (status[0] != 0) return;
-    const int tiles_m = (m + BM - 1) / BM, tiles_n = (ncols + BN - 1) / BN;
-    const int V = 8 * ((tiles_m * tiles_n + 7) / 8);
-    int vt = blockIdx.x, tm = 0, tn = 0;
-    while (vt < V && !gemm_tile_at(vt, tiles_m, tiles_n, tm, tn)) vt += gridDim.x;
-    if (vt >= V) return;
+    int tm, tn;
+    if (!gemm_tile((m + BM - 1) / BM, (ncols + BN - 1) / BN, tm, tn)) return;
+    const u32x4_t *Lv = reinterpret_cast<const u32x4_t *>(Lh), *Uv = reinterpret_cast<const u32x4_t *>(Uh);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const size_t lstride = (size_t)rb_pad * 128, ustride = (size_t)cb_count * 128;
-    auto lp_of = [&](int tm_) { return Lh + (size_t)(tm_ * (2 * MB) + (wave >> 1) * MB) * 128 + lane; };
-    auto up_of = [&](int tn_) { return Uh + (size_t)(tn_ * (2 * NB) + (wave & 1) * NB) * 128 + lane; };
-    uint4 ah[NBUF][MB], al[NBUF][MB], bh[NBUF][NB], bl[NBUF][NB];
-    auto load_ops = [&](const uint4 *lp_, const uint4 *up_, int buf, int c) {
-        const uint4 *l_ = lp_ + (size_t)c * lstride, *u_ = up_ + (size_t)c * ustride;
+    const size_t astride = (size_t)rb_pad * 128, bstride = (size_t)cb_count * 128;
+    // wave w loads L21 units MB w .. MB w + MB - 1 and U12 units NB w ..
+    const u32x4_t *as = Lv + (size_t)(tm * (BM / 8) + wave * MB) * 64 + lane;
+    const u32x4_t *bs = Uv + (size_t)(tn * (BN / 8) + wave * NB) * 64 + lane;
+    constexpr int ND = KC >= 4 ? 4 : 2;  // staging register sets: chunks in flight
+    static_assert(KC % ND == 0, "a tile's chunks must cycle the staging sets exactly");
+    u32x4_t ga[ND][MB], gb[ND][NB];
+    auto load_g = [&](auto set, int c) {
+        constexpr int t = decltype(set)::value;
 #pragma unroll
-        for (int i = 0; i < MB; ++i) {
-            ah[buf][i] = l_[i * 128];
-            al[buf][i] = l_[i * 128 + 64];
+        for (int q = 0; q < MB; ++q) ga[t][q] = as[(size_t)c * astride + q * 64];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) gb[t][q] = bs[(size_t)c * bstride + q * 64];
+    };
+    auto store_g = [&](auto set, int slot) {
+        constexpr int t = decltype(set)::value;
+        u32x4_t *base = slots_lds + (size_t)slot * UNITS * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < MB; ++q) base[(wave * MB + q) * 64] = ga[t][q];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) base[(UA + wave * NB + q) * 64] = gb[t][q];
+    };
+    static_for<ND>([&](auto q) { load_g(std::integral_constant<int, decltype(q)::value>{}, decltype(q)::value); });
+    store_g(std::integral_constant<int, 0>{}, 0);
+    __syncthreads();
+    const int ra = (wave >> 1) * MB, cbl = (wave & 1) * NB;
+    const int rb0 = tm * (2 * MB) + ra, cb0 = tn * (2 * NB) + cbl;
+    float4_t cre[MB][NB], cim[MB][NB];
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            cre[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+            cim[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
         }
+    float2 cv[CL];
+    auto c_ptr = [&](int e) -> float2 * {
+        const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
+        const int row = min((rb0 + i) * 16 + 4 * lk + v, m - 1);
+        const int col = min((cb0 + j) * 16 + li, ncols - 1);
+        return A + (size_t)(row0 + row) * lda + col0 + col;
+    };
+    static_for<KC>([&](auto step) {
+        constexpr int s = decltype(step)::value;
+        using cur = std::integral_constant<int, s % ND>;
+        using nxt = std::integral_constant<int, (s + 1) % ND>;
+        if constexpr (s + ND < KC) load_g(cur{}, s + ND);
+        const u32x4_t *sl = slots_lds + (size_t)(s % 2) * UNITS * 64 + lane;
+        half8_t uh[NB], ul[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            bh[buf][j] = u_[j * 128];
-            bl[buf][j] = u_[j * 128 + 64];
+            uh[j] = __builtin_bit_cast(half8_t, sl[(UA + 2 * (cbl + j)) * 64]);
+            ul[j] = __builtin_bit_cast(half8_t, sl[(UA + 2 * (cbl + j) + 1) * 64]);
         }
-    };
-    const uint4 *lp = lp_of(tm), *up = up_of(tn);
-    load_ops(lp, up, 0, 0);
-    for (;;) {
-        int vn = vt + gridDim.x, tmn = tm, tnn = tn;
-        while (vn < V && !gemm_tile_at(vn, tiles_m, tiles_n, tmn, tnn)) vn += gridDim.x;
-        const bool more = vn < V;
-        const uint4 *lpn = more ? lp_of(tmn) : lp, *upn = more ? up_of(tnn) : up;
-        const int rb0 = tm * (2 * MB) + (wave >> 1) * MB;
-        const int cb0 = tn * (2 * NB) + (wave & 1) * NB;
-        float4_t cre[MB][NB], cim[MB][NB];
 #pragma unroll
-        for (int a = 0; a < MB; ++a)
+        for (int i = 0; i < MB; ++i) {
+            const u32x4_t h = sl[(2 * (ra + i)) * 64], l = sl[(2 * (ra + i) + 1) * 64];
+            const half8_t rh = __builtin_bit_cast(half8_t, h), rl = __builtin_bit_cast(half8_t, l);
+            const half8_t ih = __builtin_bit_cast(half8_t, turn_pairs(h)), il = __builtin_bit_cast(half8_t, turn_pairs(l));
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                cre[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-                cim[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < NB; ++j) {
+                cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, uh[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, uh[j], cim[i][j], 0, 0, 0);
             }
-        // C element e = (i * NB + j) * 4 + v: row (rb0 + i) * 16 + 4 lk + v,
-        // column (cb0 + j) * 16 + li; out-of-range ones read a clamped address
-        float2 cv[CL];
-        auto c_ptr = [&](int e) -> float2 * {
-            const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
-            const int row = min((rb0 + i) * 16 + 4 * lk + v, m - 1);
-            const int col = min((cb0 + j) * 16 + li, ncols - 1);
-            return A + (size_t)(row0 + row) * lda + col0 + col;
-        };
-        static_for<KC>([&](auto step) {
-            constexpr int s = decltype(step)::value;
-            constexpr int e0 = s * CL / KC, e1 = (s + 1) * CL / KC;
-            if constexpr (s + 1 < KC) load_ops(lp, up, (s + 1) % NBUF, s + 1);
-            else load_ops(lpn, upn, 0, 0);
 #pragma unroll
-            for (int e = e0; e < e1; ++e) cv[e] = *c_ptr(e);
-            __builtin_amdgcn_sched_barrier(0);
-            constexpr int b = s % NBUF;
-#pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                const half8_t rh = as_half8(ah[b][i]), rl = as_half8(al[b][i]);
-                const half8_t ih = as_half8(turn_pairs(ah[b][i])), il = as_half8(turn_pairs(al[b][i]));
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    const half8_t uh = as_half8(bh[b][j]), ul = as_half8(bl[b][j]);
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, uh, cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, uh, cim[i][j], 0, 0, 0);
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, ul, cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, ul, cim[i][j], 0, 0, 0);
-                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rl, uh, cre[i][j], 0, 0, 0);
-                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, uh, cim[i][j], 0, 0, 0);
-                }
+            for (int j = 0; j < NB; ++j) {
+                cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, ul[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, ul[j], cim[i][j], 0, 0, 0);
             }
-            __builtin_amdgcn_sched_barrier(0);
-        });
-        float fc[NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) fc[j] = uinv[(cb0 + j) * 16 + li];
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = (rb0 + i) * 16 + 4 * lk + v;
-                const float fr = linv[row];
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    const int col = (cb0 + j) * 16 + li;
-                    const int e = (i * NB + j) * 4 + v;
-                    const float f = fr * fc[j];
-                    float2 r = cv[e];
-                    r.x -= cre[i][j][v] * f;
-                    r.y -= cim[i][j][v] * f;
-                    if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
-                }
+            for (int j = 0; j < NB; ++j) {
+                cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rl, uh[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, uh[j], cim[i][j], 0, 0, 0);
             }
-        if (!more) break;
-        vt = vn;
-        tm = tmn;
-        tn = tnn;
-        lp = lpn;
-        up = upn;
-    }
+        }
+        if constexpr (s + 1 < KC) {
+            store_g(nxt{}, (s + 1) % 2);
+            __syncthreads();
+        }
+    });
+#pragma unroll
+    for (int e = 0; e < CL; ++e) cv[e] = *c_ptr(e);
+    float fc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) fc[j] = uinv[(cb0 + j) * 16 + li];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = (rb0 + i) * 16 + 4 * lk + v;
+            const float fr = linv[row];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int col = (cb0 + j) * 16 + li;
+                const int e = (i * NB + j) * 4 + v;
+                const float f = fr * fc[j];
+                float2 r = cv[e];
+                r.x -= cre[i][j][v] * f;
+                r.y -= cim[i][j][v] * f;
+                if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
+            }
+        }
 }
 
-// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 16
-// or 8 chunks' worth of columns), persistent like gemm_packed.
+// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 128
+// or 64 columns of L21).
 static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
-                    const int *status, LUContext &ctx, hipStream_t s, bool leave_cus) {
+                    const int *status, LUContext &ctx, hipStream_t s) {
     if (m <= 0 || ncols <= 0) return;
     const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr);
     const uint4 *Uh = reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr);
-    const int tiles = ((m + 127) / 128) * ((ncols + 127) / 128);
+    const int tiles = ((m + 63) / 64) * ((ncols + 127) / 128);
     const int grid = (tiles + 7) / 8 * 8;
-    const int pgrid = std::min(grid, std::max(8, (ctx.cus - (leave_cus ? 8 : 0)) / 8 * 8));
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
+    const float *ls = ctx.scale_l16.ptr, *us = ctx.scale_u16.ptr;
     if (kw == 128)
-        hipLaunchKernelGGL((gemm_stream_h3_kernel<8>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
-                           Uh, ctx.scale_l16.ptr, ctx.scale_u16.ptr, rb_pad, cb_count, status);
+        hipLaunchKernelGGL((gemm_tile_h3_kernel<8, 2>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
+                           Uh, ls, us, rb_pad, cb_count, status);
     else if (kw == 64)
-        hipLaunchKernelGGL((gemm_stream_h3_kernel<4>), dim3(pgrid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
-                           Uh, ctx.scale_l16.ptr, ctx.scale_u16.ptr, rb_pad, cb_count, status);
+        hipLaunchKernelGGL((gemm_tile_h3_kernel<4, 2>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
+                           Uh, ls, us, rb_pad, cb_count, status);
     else
         throw HipError{"split-FP16 update needs a block width of 64 or 128"};
     HIP_CHECK(hipGetLastError());
@@ -1932,7 +1944,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cb16 + 3) / 4)), dim3(256), 0, s, dA, lda, k0, w,
                                r0 + w2, rest2, cb16, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr,
                                ctx.status.ptr);
-            gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16, ctx.status.ptr, ctx, s, true);
+            gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16, ctx.status.ptr, ctx, s);
         } else if (rest2 > 0) {
             solve_u12(k0, w, r0 + w2, n);
             const int cb_rest = cb_pad - w2 / 16;
