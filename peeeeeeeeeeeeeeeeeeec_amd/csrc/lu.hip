@@ -1619,6 +1619,29 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
 // exchange. A pivot with |u_kk| <= limit stops it with status -2.
 // Block width w <= 128 (the 32 x 32 tile grid of diag_lu_kernel).
 
+// |u| > limit (compared squared, in FP64) and 1 / u of a diagonal pivot. The
+// FP32 reciprocal is conj(u) / |u|^2 with the hardware reciprocal of |u|^2
+// and one Newton step (about as accurate as the division, a few dependent
+// instructions instead of the division sequence); a pivot whose |u|^2 is not
+// a normal FP32 number is rejected like a small one.
+__device__ inline void pivot_check(float2 u, double limit, int &ok, float2 &rcp) {
+    const double d2 = (double)u.x * u.x + (double)u.y * u.y;
+    const float den = fmaf(u.x, u.x, u.y * u.y);
+    ok = d2 > limit * limit && d2 < HUGE_VAL && den >= 1.17549435e-38f;
+    float r = __builtin_amdgcn_rcpf(den);
+    r = fmaf(r, fmaf(-den, r, 1.f), r);
+    rcp = make_float2(u.x * r, -u.y * r);
+}
+__device__ inline void pivot_check(double2 u, double limit, int &ok, double2 &rcp) {
+    const double m = wmag(u);
+    ok = (m > limit) && isfinite(m);
+    rcp = wdiv(wone<double2>(), u);
+}
+__device__ inline void pivot_check(double u, double limit, int &ok, double &rcp) {
+    ok = fabs(u) > limit && isfinite(u);
+    rcp = 1.0 / u;
+}
+
 // One workgroup of 1024 threads; thread (ti, tc) keeps the 4 x 4 tile at
 // rows 4 ti.., columns 4 tc.. of the w x w block in registers (FP64). Per
 // column j the owners of row j publish it to LDS, the owners of column j
@@ -1665,10 +1688,9 @@ __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0,
                         const T v = a.get(4 * r + c);
                         acol[buf][4 * ti + r] = v;
                         if (4 * ti + r == j) {
-                            // pivot owner: magnitude check and 1 / u
-                            const double m = wmag(v);
-                            s_ok[buf] = (m > limit) && isfinite(m);
-                            s_rcp[buf] = wdiv(wone<T>(), v);
+                            // pivot owner: magnitude check and 1 / u (a few
+                            // instructions: every other wave waits on them)
+                            pivot_check(v, limit, s_ok[buf], s_rcp[buf]);
                         }
                     }
         }

