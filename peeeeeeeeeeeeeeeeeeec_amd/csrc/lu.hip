@@ -1729,6 +1729,141 @@ __global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0,
         }
 }
 
+// The same diagonal-block factor in panels of 4 columns (FP32 complex
+// storage and arithmetic, float2 only): thread (ti, tc) keeps the 4 x 4 tile
+// at rows 4 ti.., columns 4 tc.. (w x w block, padded to the tile grid with
+// the identity). For panel q the owner of tile (q, q) factors it alone, the
+// owners of the tiles left of and above the trailing square form their L21 =
+// A21 U11^-1 and U12 = L11^-1 A12 tiles from the published L11, U11, and
+// every trailing tile takes the rank-4 update A22 -= L21 U12 from LDS: two
+// barriers per 4 columns, no per-element masks (panels align with tiles).
+// Same operations as the column-by-column elimination, grouped by panel.
+__global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t lda, int k0, int w, double limit,
+                                                             int *status) {
+    // double-buffered by panel parity: L11 (unit lower) and U11 of the
+    // diagonal tile, reciprocals of its pivots, L21 tiles by tile row and
+    // U12 tiles by tile column
+    __shared__ float2 s_l11[2][16], s_u11[2][16], s_rd[2][4];
+    __shared__ float2 s_l21[2][32][16], s_u12[2][32][16];
+    __shared__ int s_ok[2];
+    if (status[0] != 0) return;
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+    const int ti = 8 * (wv >> 2) + (ln >> 3), tc = 8 * (wv & 3) + (ln & 7);
+    const int nt = (w + 3) / 4;
+    // the tile as plain floats (arrays of vector types would go to scratch)
+    float ar[4][4], ai[4][4];
+    auto get = [&](int r, int c) { return make_float2(ar[r][c], ai[r][c]); };
+    auto put = [&](int r, int c, float2 v) {
+        ar[r][c] = v.x;
+        ai[r][c] = v.y;
+    };
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * ti + r, cc = 4 * tc + c;
+            put(r, c, (i < w && cc < w) ? A[(size_t)(k0 + i) * lda + k0 + cc] : make_float2(i == cc ? 1.f : 0.f, 0.f));
+        }
+    for (int q = 0; q < nt; ++q) {
+        const int b = q & 1;
+        if (ti == q && tc == q) {
+            // the diagonal tile: unpivoted LU in registers
+            int ok = 1;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                int okc;
+                float2 rd;
+                // padding pivots are exactly 1: any positive magnitude passes
+                pivot_check(get(c, c), 4 * q + c < w ? limit : 0.0, okc, rd);
+                ok &= okc;
+                s_rd[b][c] = rd;
+#pragma unroll
+                for (int r = c + 1; r < 4; ++r) {
+                    const float2 m = wmul(get(r, c), rd);
+                    put(r, c, m);
+#pragma unroll
+                    for (int x = c + 1; x < 4; ++x) put(r, x, wfms(get(r, x), m, get(c, x)));
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    s_l11[b][4 * r + c] = r > c ? get(r, c) : make_float2(0.f, 0.f);
+                    s_u11[b][4 * r + c] = r <= c ? get(r, c) : make_float2(0.f, 0.f);
+                }
+            s_ok[b] = ok;
+        }
+        __syncthreads();
+        if (!s_ok[b]) {
+            if (tid == 0) {
+                status[0] = -2;
+                status[1] = k0 + 4 * q;
+            }
+            return;
+        }
+        if (tc == q && ti > q && ti < nt) {
+            // L21 tile: row by row, x_c = (a_c - sum_{k<c} x_k u_kc) / u_cc
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float2 v = get(r, c);
+#pragma unroll
+                    for (int k = 0; k < c; ++k) v = wfms(v, get(r, k), s_u11[b][4 * k + c]);
+                    v = wmul(v, s_rd[b][c]);
+                    put(r, c, v);
+                    s_l21[b][ti][4 * r + c] = v;
+                }
+        } else if (ti == q && tc > q && tc < nt) {
+            // U12 tile: column by column, y_r = a_r - sum_{k<r} l_rk y_k
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float2 v = get(r, c);
+#pragma unroll
+                    for (int k = 0; k < r; ++k) v = wfms(v, s_l11[b][4 * r + k], get(k, c));
+                    put(r, c, v);
+                    s_u12[b][tc][4 * r + c] = v;
+                }
+        }
+        __syncthreads();
+        if (ti > q && tc > q && ti < nt && tc < nt) {
+            // rank-4 update of the trailing tile, k = 0..3 outermost as in
+            // the column-by-column elimination
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float lr[4], li[4], ur[4], ui[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float2 v = s_l21[b][ti][4 * r + k];
+                    lr[r] = v.x;
+                    li[r] = v.y;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float2 v = s_u12[b][tc][4 * k + c];
+                    ur[c] = v.x;
+                    ui[c] = v.y;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        put(r, c, wfms(get(r, c), make_float2(lr[r], li[r]), make_float2(ur[c], ui[c])));
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * ti + r, cc = 4 * tc + c;
+            if (i < w && cc < w) A[(size_t)(k0 + i) * lda + k0 + cc] = get(r, c);
+        }
+}
+
 // Triangular solves of the ordered factor, one wavefront per row of
 // L21 = A21 U11^{-1} or per column of U12 = L11^{-1} A12 (column-by-column
 // substitution in the order of trsm_kernel: x_j = a_j / u_jj once the
@@ -1906,7 +2041,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     };
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
-        hipLaunchKernelGGL(diag_lu_kernel<S>, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
+        hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
                            ctx.status.ptr);
         if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
     };
