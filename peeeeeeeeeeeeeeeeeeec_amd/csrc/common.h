@@ -153,7 +153,8 @@ class KernelTimer {
         const size_t need = used_ + 2;
         while (pool_.size() < need) {
             hipEvent_t e;
-            HIP_CHECK(hipEventCreate(&e));
+            // timestamps only, resolved after a stream synchronisation
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             pool_.push_back(e);
         }
         HIP_CHECK(hipEventRecord(pool_[used_], s));

@@ -601,7 +601,9 @@ static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngin
         }
         e->far_evals = w.far_evaluations;
         e->smooth_evals = e->edges->smooth_evaluations() + e->nodes->smooth_evaluations();
-        for (hipEvent_t &ev : e->ev) HIP_CHECK(hipEventCreate(&ev));
+        // stage timestamps only (read after the stream has been synchronised):
+        // no system-scope fence
+        for (hipEvent_t &ev : e->ev) HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
         *out = e.release();
         return 0;
     } catch (const HipError &err) {

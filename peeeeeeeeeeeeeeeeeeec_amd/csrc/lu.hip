@@ -261,8 +261,10 @@ LUContext::LUContext(hipStream_t s) : stream(s) {
     HIP_CHECK(hipMemset(tags.ptr, 0, tags.bytes()));
     swap_plan.resize((size_t)1 + 4 * kMaxNB);
     HIP_CHECK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-    HIP_CHECK(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
-    HIP_CHECK(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
+    // ordering between the two streams of one device only: no timing, no
+    // system-scope fence (device-scope release/acquire orders the work)
+    HIP_CHECK(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_CHECK(hipDeviceGetAttribute(&lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 }
 
