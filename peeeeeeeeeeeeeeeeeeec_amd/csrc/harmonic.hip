@@ -65,7 +65,7 @@ __global__ void mna_fill_kernel(double2 *Z, double2 *Z0, size_t ldz, int ne, int
             z = make_double2(0.0, (col - ne) == i ? omega : 0.0);
         }
     }
-    Z[(size_t)row * ldz + col] = z;
+    if (Z) Z[(size_t)row * ldz + col] = z;
     if (Z0) Z0[(size_t)row * ldz + col] = z;
 }
 
@@ -210,6 +210,7 @@ struct PeecEngine {
     bool retarded_now = false;
     // the factor held for (cached_f, cached_mode) may serve further
     // right-hand sides at that point without a new assembly or factor
+    bool z_equilibrated = false;  // d_Z holds this point's equilibrated FP64 matrix
     bool cache_ok = false;
     double cached_f = 0.0;
     int cached_mode = -1;
@@ -269,9 +270,19 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
     const int n = e->n;
     LUStatus st;
     if (fp64) {
+        if (!e->z_equilibrated) {
+            // mixed point falling back to FP64: form the equilibrated matrix
+            e->d_Z.resize((size_t)n * n);
+            copy_equilibrated(e->d_Z0.ptr, e->d_Z.ptr, e->d_row.ptr, e->d_col.ptr, n, s);
+            e->z_equilibrated = true;
+        }
         st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
     } else {
-        convert_to_f32(e->d_Z.ptr, e->d_Z32.ptr, (size_t)n * n, s);
+        double max_scaled = 0.0;
+        if (convert_equilibrated_to_f32(e->d_Z0.ptr, e->d_Z32.ptr, nullptr, e->d_row.ptr, e->d_col.ptr, n,
+                                        &max_scaled, *e->lu) != 0)
+            return -2;
+        e->limit = 1.0e-14 * max_scaled;
         st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
     }
     if (st.code != 0) return factor_error(st);
@@ -303,25 +314,41 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
     e->retarded_now = ret;
     if (ret) report.moment_terms = assemble_retarded(e, omega / kC0);
     HIP_CHECK(hipEventRecord(e->ev[1], s));
-    e->d_Z.resize(nn);
+    const bool mixed = mode == PEEC_SOLVE_MIXED;
     e->d_Z0.resize(nn);
-    launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr, e->d_P.ptr,
-                    ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
     e->d_row.resize(n);
     e->d_col.resize(n);
-    double max_scaled = 0.0;
-    if (equilibrate_complex(e->d_Z.ptr, n, n, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) != 0) return -2;
+    e->z_equilibrated = false;
+    if (mixed) {
+        // the FP32 factor is formed straight from Z0 with the equilibration's
+        // scales (read-only passes); the equilibrated FP64 matrix is only
+        // built if the point falls back to FP64
+        launch_mna_fill(nullptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
+                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
+        if (equilibration_scales(e->d_Z0.ptr, n, n, e->d_row.ptr, e->d_col.ptr, *e->lu) != 0) return -2;
+    } else {
+        e->d_Z.resize(nn);
+        launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
+                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
+        double max_scaled = 0.0;
+        if (equilibrate_complex(e->d_Z.ptr, n, n, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) != 0) return -2;
+        e->z_equilibrated = true;
+        e->limit = 1.0e-14 * max_scaled;
+    }
     HIP_CHECK(hipEventRecord(e->ev[2], s));
-    e->limit = 1.0e-14 * max_scaled;
     e->d_ipiv.resize(n);
     e->d_perm.resize(n);
     e->d_reuse_perm.resize(n);
     LUStatus st;
     e->ordered = false;
-    if (mode == PEEC_SOLVE_MIXED) {
+    if (mixed) {
         e->d_Z32.resize(nn);
         if (e->opt.reuse_pivots && e->have_reuse) {
-            convert_rows_to_f32(e->d_Z.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, n, s);
+            double max_scaled = 0.0;
+            if (convert_equilibrated_to_f32(e->d_Z0.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, e->d_row.ptr,
+                                            e->d_col.ptr, n, &max_scaled, *e->lu) != 0)
+                return -2;
+            e->limit = 1.0e-14 * max_scaled;
             static constexpr int kTrialWidths[2] = {128, 64};
             int nb = e->opt.block_size;
             const bool trial = nb == 0 && e->tuned_nb == 0;

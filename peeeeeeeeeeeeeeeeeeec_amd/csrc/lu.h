@@ -57,6 +57,18 @@ struct LUContext {
 int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d_col,
                         double *max_scaled, LUContext &ctx);
 
+// The same equilibration without writing the matrix (mixed path): row and
+// column scales of an unscaled matrix (0 or -2 as above) ...
+int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx);
+// ... the equilibrated entries (a_ij / r_i) / c_j in FP32, rows perm[i] (or
+// i when d_perm is null), with max |equilibrated entry| in FP64; the values
+// are those equilibrate_complex would produce, rounded once to FP32 ...
+int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_perm, const double *d_row,
+                                const double *d_col, int n, double *max_scaled, LUContext &ctx);
+// ... and the equilibrated FP64 matrix when it is needed after all.
+void copy_equilibrated(const double2 *src, double2 *dst, const double *d_row, const double *d_col, int n,
+                       hipStream_t s);
+
 // Factor in place; d_ipiv[k] = row swapped with k at step k (complex_lu.c:330-510).
 template <class S>
 LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LUContext &ctx,
@@ -68,9 +80,6 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
 // row order of an earlier partial-pivoting factorization.
 LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx,
                            int block_size = 0);
-
-// dst[i][c] = (float2) src[perm[i]][c] for an n x n row-major matrix.
-void convert_rows_to_f32(const double2 *src, float2 *dst, const int *d_perm, int n, hipStream_t s);
 
 // Solve with the factors for `nrhs` right-hand sides stored rhs-major
 // (X[r*n + i]); applies the row scale, pivots, L, U and column scale like
@@ -88,9 +97,6 @@ template <class S, class V>
 void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
                        const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor = false);
 void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s);
-
-// Convert an FP64 complex matrix to FP32 (for the mixed factor).
-void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s);
 
 // y = A x for complex FP64 (rhs-major vectors), and the reference's
 // component-wise backward error terms per row:

@@ -235,6 +235,72 @@ __global__ void scale_cols_kernel(double2 *A, int n, size_t lda, const double *c
     atomic_max_pos(maxv, m);
 }
 
+// column maxima of the row-scaled matrix, |a_ij / r_i| (the arithmetic of
+// scale_rows_colmax_kernel), without writing the matrix
+__global__ void colmax_scaled_kernel(const double2 *A, int n, size_t lda, const double *row, double *col) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int i0 = blockIdx.y * 64, i1 = min(n, i0 + 64);
+    double m = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        double2 v = A[(size_t)i * lda + j];
+        const double r = row[i];
+        v.x = v.x / r;
+        v.y = v.y / r;
+        m = fmax(m, mag(v));
+    }
+    atomic_max_pos(&col[j], m);
+}
+
+// the equilibrated entry (a_ij / r_i) / c_j, divided in the order of
+// scale_rows_colmax_kernel and scale_cols_kernel
+__device__ inline double2 equilibrated(double2 v, double r, double c) {
+    v.x = v.x / r;
+    v.y = v.y / r;
+    v.x /= c;
+    v.y /= c;
+    return v;
+}
+
+// dst row i (FP32) = equilibrated row perm[i] (or i) of src; the largest
+// |equilibrated entry| into maxv[0 .. kMaxSlots). One workgroup per (row,
+// 256 columns).
+constexpr int kMaxSlots = 1024;
+__global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(const double2 *src, float2 *dst, const int *perm,
+                                                                  const double *row, const double *col, int n,
+                                                                  double *maxv) {
+    __shared__ double red[4];
+    const int i = blockIdx.y, c = blockIdx.x * 256 + threadIdx.x;
+    const int si = perm ? perm[i] : i;
+    double m = 0.0;
+    if (c < n) {
+        const double2 v = equilibrated(src[(size_t)si * n + c], row[si], col[c]);
+        dst[(size_t)i * n + c] = make_float2((float)v.x, (float)v.y);
+        m = mag(v);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    // one of kMaxSlots slots per workgroup: no single contended address
+    if (threadIdx.x == 0)
+        atomic_max_pos(&maxv[(blockIdx.x + (size_t)blockIdx.y * gridDim.x) % kMaxSlots],
+                       fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+}
+
+// dst (FP64) = the equilibrated src
+__global__ void equilibrated_copy_kernel(const double2 *src, double2 *dst, const double *row, const double *col,
+                                         int n) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)n * n) return;
+    const int i = (int)(idx / n), c = (int)(idx % n);
+    dst[idx] = equilibrated(src[idx], row[i], col[c]);
+}
+
+__global__ void zero_scale_check_kernel(const double *col, int n, int *bad) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n && col[j] == 0.0) atomicOr(bad, 1);
+}
+
 __global__ void fill_zero_kernel(double *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -296,6 +362,50 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
     HIP_CHECK(hipStreamSynchronize(s));
     if (h_bad) return -2;
     return 0;
+}
+
+int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx) {
+    hipStream_t s = ctx.stream;
+    DeviceBuffer<int> bad(1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, (size_t)n);
+    hipLaunchKernelGGL(row_max_kernel, dim3(n), dim3(256), 0, s, dA, n, lda, d_row, bad.ptr);
+    dim3 grid((n + 255) / 256, (n + 63) / 64);
+    hipLaunchKernelGGL(colmax_scaled_kernel, grid, dim3(256), 0, s, dA, n, lda, d_row, d_col);
+    HIP_CHECK(hipGetLastError());
+    int h_bad = 0;
+    bad.download(&h_bad, 1, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return h_bad ? -2 : 0;
+}
+
+int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_perm, const double *d_row,
+                                const double *d_col, int n, double *max_scaled, LUContext &ctx) {
+    hipStream_t s = ctx.stream;
+    DeviceBuffer<double> mx(kMaxSlots);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(kMaxSlots / 256), dim3(256), 0, s, mx.ptr, (size_t)kMaxSlots);
+    // zero column maxima would divide by zero: reported like the in-place path
+    DeviceBuffer<int> bad(1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
+    hipLaunchKernelGGL(zero_scale_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, n, bad.ptr);
+    hipLaunchKernelGGL(to_f32_equilibrated_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, dA,
+                       dst, d_perm, d_row, d_col, n, mx.ptr);
+    HIP_CHECK(hipGetLastError());
+    int h_bad = 0;
+    std::vector<double> slots(kMaxSlots);
+    bad.download(&h_bad, 1, s);
+    mx.download(slots.data(), kMaxSlots, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    *max_scaled = *std::max_element(slots.begin(), slots.end());
+    return h_bad ? -2 : 0;
+}
+
+void copy_equilibrated(const double2 *src, double2 *dst, const double *d_row, const double *d_col, int n,
+                       hipStream_t s) {
+    const size_t total = (size_t)n * n;
+    hipLaunchKernelGGL(equilibrated_copy_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst,
+                       d_row, d_col, n);
+    HIP_CHECK(hipGetLastError());
 }
 
 // Grid barrier for cooperative kernels (called by one thread per
@@ -1613,13 +1723,13 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
 // ---------------------------------------------------------------------------
 // factorization with a given row order (no pivot search)
 //
-// The rows have been permuted beforehand (convert_rows_to_f32) into the order
-// a previous partial-pivoting factorization chose; the factor is then a
+// The rows have been permuted beforehand (convert_equilibrated_to_f32) into
+// the order a previous partial-pivoting factorization chose; the factor is then a
 // plain blocked LU: the w x w diagonal block is factored by one workgroup in
 // LDS, L21 = A21 U11^{-1} and U12 = L11^{-1} A12 are formed in parallel, and
 // the trailing update is the same MFMA GEMM. There is no cross-workgroup
 // exchange. A pivot with |u_kk| <= limit stops it with status -2.
-// Block width w <= 128 (the 32 x 32 tile grid of diag_lu_kernel).
+// Block width w <= 128 (the 32 x 32 tile grid of the diagonal-block kernels).
 
 // |u| > limit (compared squared, in FP64) and 1 / u of a diagonal pivot. The
 // FP32 reciprocal is conj(u) / |u|^2 with the hardware reciprocal of |u|^2
@@ -1644,95 +1754,9 @@ __device__ inline void pivot_check(double u, double limit, int &ok, double &rcp)
     rcp = 1.0 / u;
 }
 
-// One workgroup of 1024 threads; thread (ti, tc) keeps the 4 x 4 tile at
-// rows 4 ti.., columns 4 tc.. of the w x w block in registers (FP64). Per
-// column j the owners of row j publish it to LDS, the owners of column j
-// form the multipliers, and every thread updates its tile: two barriers per
-// column, double-buffered LDS rows.
-template <class S>
-__global__ __launch_bounds__(1024) void diag_lu_kernel(S *A, size_t lda, int k0, int w, double limit, int *status) {
-    using T = typename Work<S>::T;
-    // row j and the raw column j, double-buffered by the parity of j: one
-    // barrier per column (a thread passing barrier j + 1 knows that every
-    // thread has finished reading buffer j & 1)
-    __shared__ T urow[2][kMaxNB], acol[2][kMaxNB];
-    __shared__ T s_rcp[2];
-    __shared__ int s_ok[2];
-    if (status[0] != 0) return;
-    // wave v covers the 32 x 32 region (v / 4, v % 4); lane l the 4 x 4 tile
-    // (l / 8, l % 8) inside it, so whole waves fall idle as j advances
-    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
-    const int ti = 8 * (wv >> 2) + (ln >> 3), tc = 8 * (wv & 3) + (ln & 7);
-    Regs<T, 16> a;  // element (r, c) at 4 r + c
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int i = 4 * ti + r, cc = 4 * tc + c;
-            a.set(4 * r + c, (i < w && cc < w) ? wld(A + (size_t)(k0 + i) * lda + k0 + cc) : wzero<T>());
-        }
-    for (int j = 0; j < w; ++j) {
-        const int buf = j & 1, jt = j >> 2, jr = j & 3;
-        if (ti == jt) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (r == jr) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) urow[buf][4 * tc + c] = a.get(4 * r + c);
-                }
-        }
-        if (tc == jt) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (c == jr) {
-                        const T v = a.get(4 * r + c);
-                        acol[buf][4 * ti + r] = v;
-                        if (4 * ti + r == j) {
-                            // pivot owner: magnitude check and 1 / u (a few
-                            // instructions: every other wave waits on them)
-                            pivot_check(v, limit, s_ok[buf], s_rcp[buf]);
-                        }
-                    }
-        }
-        __syncthreads();
-        if (!s_ok[buf]) {
-            if (tid == 0) {
-                status[0] = -2;
-                status[1] = k0 + j;
-            }
-            return;
-        }
-        // tiles entirely above row j or left of column j are final
-        if (4 * ti + 3 <= j || 4 * tc + 3 < j) continue;
-        const T rcp = s_rcp[buf];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 4 * ti + r;
-            const T l = wmul(acol[buf][i], rcp);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int cc = 4 * tc + c;
-                const T cur = a.get(4 * r + c);
-                // column j below the diagonal becomes the multiplier, the
-                // columns right of j get the rank-1 update
-                const T upd = cc == j ? l : wfms(cur, l, urow[buf][cc]);
-                a.set(4 * r + c, (i > j && cc >= j) ? upd : cur);
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int i = 4 * ti + r, cc = 4 * tc + c;
-            if (i < w && cc < w) wst(A + (size_t)(k0 + i) * lda + k0 + cc, a.get(4 * r + c));
-        }
-}
-
-// The same diagonal-block factor in panels of 4 columns (FP32 complex
-// storage and arithmetic, float2 only): thread (ti, tc) keeps the 4 x 4 tile
+// The ordered factor's w x w diagonal block (w <= 128), one workgroup of
+// 1024 threads, in panels of 4 columns (FP32 complex storage and
+// arithmetic): thread (ti, tc) keeps the 4 x 4 tile
 // at rows 4 ti.., columns 4 tc.. (w x w block, padded to the tile grid with
 // the identity). For panel q the owner of tile (q, q) factors it alone, the
 // owners of the tiles left of and above the trailing square form their L21 =
@@ -2121,21 +2145,6 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     out.code = st[0];
     out.column = st[1];
     return out;
-}
-
-__global__ void to_f32_rows_kernel(const double2 *src, float2 *dst, const int *perm, int n) {
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (size_t)n * n) return;
-    const int i = (int)(idx / n), c = (int)(idx % n);
-    const double2 v = src[(size_t)perm[i] * n + c];
-    dst[idx] = make_float2((float)v.x, (float)v.y);
-}
-
-void convert_rows_to_f32(const double2 *src, float2 *dst, const int *d_perm, int n, hipStream_t s) {
-    const size_t total = (size_t)n * n;
-    hipLaunchKernelGGL(to_f32_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, d_perm,
-                       n);
-    HIP_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -2656,15 +2665,6 @@ void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t 
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-__global__ void to_f32_kernel(const double2 *src, float2 *dst, size_t count) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < count) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
-}
-
-void convert_to_f32(const double2 *src, float2 *dst, size_t count, hipStream_t s) {
-    hipLaunchKernelGGL(to_f32_kernel, dim3((count + 255) / 256), dim3(256), 0, s, src, dst, count);
-    HIP_CHECK(hipGetLastError());
-}
 
 // residual and backward-error terms (harmonic.c:899-969), one block per row
 __global__ __launch_bounds__(256) void residual_kernel(const double2 *A, size_t lda, int n, const double2 *x,

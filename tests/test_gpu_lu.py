@@ -378,6 +378,29 @@ def test_engine_split_fp16_update_vs_fp32_update(require_gpu, nb):
     assert sum(a[4].refinement_steps for a in res[1]) <= sum(b[4].refinement_steps for b in res[0]) + 4
 
 
+def test_engine_mixed_fp64_fallback_matches_fp64_mode(require_gpu):
+    """A mixed point whose refinement cannot reach the gate (no refinement
+    steps, gate 1e-13) falls back to the FP64 factor; the mixed path builds
+    the equilibrated FP64 matrix only then, from Z0 and the equilibration's
+    scales, and must give exactly the FP64 mode's answer (same matrix, same
+    factor), on the pivoted first point and on a reused-order point."""
+    xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
+    port = (0, len(xyz) - 1)
+    with plib.Model(xyz, quads) as m:
+        mixed = plib.Engine(m, retarded=True, max_refinement_steps=0, backward_error_gate=1e-13)
+        fp64 = plib.Engine(m, retarded=True)
+        for f in (2e5, 3e7):
+            st, z, _, _, rep = mixed.solve(f, [port], plib.Engine.MIXED, want_solution=False)
+            st64, z64, _, _, rep64 = fp64.solve(f, [port], plib.Engine.FP64, want_solution=False)
+            print(f"f={f:g} fallback={rep.used_fp64_fallback} eta={rep.backward_error:.2e} z={z[0]} z64={z64[0]}",
+                  flush=True)
+            assert st == 0 and st64 == 0
+            assert rep.used_fp64_fallback == 1
+            assert z[0] == z64[0]
+        mixed.close()
+        fp64.close()
+
+
 def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path):
     """The reference's own Lp/P of the 16 x 16 plate (golden), written and read
     back as Matrix Market files, drive the engine without assembly: the port
