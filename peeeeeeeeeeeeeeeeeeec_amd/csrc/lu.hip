@@ -1936,17 +1936,31 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
         // four steps per iteration (unrolled by hand: the lane reads are
         // convergent, which keeps the compiler from unrolling), so that their
         // LDS reads, independent of x, are in flight ahead of the dependent
-        // lane-read chain; steps past w - 1 change nothing
-        for (int j0 = 0; j0 < w; j0 += 4) {
+        // lane-read chain; steps past w - 1 change nothing. Columns 0..63
+        // (x0) are final after step 63 and columns 64.. (x1) are never
+        // solved before it, so the steps run in two phases that leave out
+        // the other half's selects and updates (same values as one loop).
+        const int wa = min(w, 64);
+        for (int j0 = 0; j0 < wa; j0 += 4) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = j0 + u, jc = min(j, w - 1);
-                const T xj = wmul(wreadlane(j < 64 ? x0 : x1, j & 63), rcp[jc]);
+                const T xj = wmul(wreadlane(x0, j & 63), rcp[jc]);
                 x0 = (ca == j) ? xj : x0;
-                x1 = (cb == j) ? xj : x1;
                 const T u0 = wld(blk + jc * w + cla), u1 = wld(blk + jc * w + clb);
                 const T n0 = wfms(x0, xj, u0), n1 = wfms(x1, xj, u1);
                 x0 = (ca > j && ca < w) ? n0 : x0;
+                x1 = (cb < w) ? n1 : x1;
+            }
+        }
+        for (int j0 = 64; j0 < w; j0 += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u, jc = min(j, w - 1);
+                const T xj = wmul(wreadlane(x1, j & 63), rcp[jc]);
+                x1 = (cb == j) ? xj : x1;
+                const T u1 = wld(blk + jc * w + clb);
+                const T n1 = wfms(x1, xj, u1);
                 x1 = (cb > j && cb < w) ? n1 : x1;
             }
         }
@@ -1998,16 +2012,27 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
             S *col = tb + wave * wp;
             T x0 = ra < w ? wld(col + ra) : wzero<T>();
             T x1 = rb < w ? wld(col + rb) : wzero<T>();
-#pragma unroll 4
-            // four steps per iteration, as in trsm_right_wave_kernel
-            for (int r0 = 0; r0 + 1 < w; r0 += 4) {
+            // four steps per iteration, in two phases, as in
+            // trsm_right_wave_kernel (rows 0..63 are final after step 63)
+            const int wa = min(w - 1, 64);
+            for (int r0 = 0; r0 < wa; r0 += 4) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int r = r0 + u, rc = min(r, w - 1);
-                    const T xr = wreadlane(r < 64 ? x0 : x1, r & 63);
+                    const T xr = wreadlane(x0, r & 63);
                     const T l0 = wld(lt + rc * wp + rla), l1 = wld(lt + rc * wp + rlb);
                     const T n0 = wfms(x0, l0, xr), n1 = wfms(x1, l1, xr);
                     x0 = (ra > r && ra < w) ? n0 : x0;
+                    x1 = (rb < w) ? n1 : x1;
+                }
+            }
+            for (int r0 = 64; r0 + 1 < w; r0 += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + u, rc = min(r, w - 1);
+                    const T xr = wreadlane(x1, r & 63);
+                    const T l1 = wld(lt + rc * wp + rlb);
+                    const T n1 = wfms(x1, l1, xr);
                     x1 = (rb > r && rb < w) ? n1 : x1;
                 }
             }
