@@ -68,6 +68,18 @@ typedef struct
                                    -1 (default) = 26 terms when they fit in a quarter
                                    of the device memory left free by the engine's
                                    per-point working set, else fewer (off below 8) */
+    int shifted_moments;        /* retarded points above the range of smooth_moments:
+                                   complex moments around a shifted wave number k0,
+                                   (exp(-jkR)-1)/R = (exp(-jk0R)-1)/R + sum_{m>=1}
+                                   (-j(k-k0))^m/m! exp(-jk0R) R^(m-1), built when a point
+                                   needs them (k0 = k + ~3.2/diameter, so a sweep upwards
+                                   reuses them) and kept while they cover the points,
+                                   16 bytes per term per upper-triangle entry. A build
+                                   costs about 6 quadrature passes and covers a window
+                                   of 6.4 / diameter in k: worth it for dense sweeps
+                                   (tens of points per window), not for a few points.
+                                   0 (default) = off, 1..32 = terms, -1 = 30 terms when
+                                   they fit in half the free device memory */
     int split_fp16_update;      /* mixed mode: trailing updates of the reused-row-order
                                    factor on FP16 MFMA with each FP32 operand split into
                                    two FP16 parts (hi + lo, power-of-two scaled per row
@@ -97,6 +109,8 @@ typedef struct
     int block_size;             /* block width of a reused-row-order factor (0 otherwise);
                                    with block_size = 0 in the options the engine's first
                                    two such factors try 128 and 64 and keep the faster */
+    double moment_center;       /* wave number k0 of the shifted-center moments that
+                                   gave this point's retarded matrices (0 otherwise) */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;

@@ -34,7 +34,8 @@ extern "C" {
 typedef struct
 {
     PeecEngineOptions engine;  /* options of every engine (retarded, refinement, ...);
-                                  smooth_moments is set to 0 (no stored moments) */
+                                  smooth_moments and shifted_moments are set to 0
+                                  (no stored moments) */
     int slots;                 /* worker threads = requests running at once, default 4 */
     size_t hbm_budget_bytes;   /* device memory the engines may hold; 0 = 90% of free memory
                                   when the server is created */

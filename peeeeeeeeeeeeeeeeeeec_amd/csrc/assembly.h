@@ -63,6 +63,19 @@ class RegionSet {
     // d_out = d_static + remainder from the stored moments (terms <= moment_terms())
     void assemble_retarded_moments(const double *d_static, double wave_number, int terms, double2 *d_out,
                                    hipStream_t s, KernelTimer *timer = nullptr);
+    // Complex moments around a shifted wave number k0 (points above the
+    // range of the moments at 0): V_0 = scale * sum_w (exp(-jk0R) - 1)/R and
+    // V_m = scale * sum_w exp(-jk0R) R^(m-1), m = 1..terms-1, so that the
+    // remainder at k is sum_m (-j(k - k0))^m / m! V_m.
+    static constexpr int kMaxShiftTerms = 32;
+    void build_shifted_moments(int terms, double k0, hipStream_t s);
+    int shifted_terms() const { return shift_terms_; }
+    size_t shifted_bytes() const { return d_shifted_.bytes(); }
+    double shifted_k0() const { return shift_k0_; }
+    // terms for wave number k around the stored k0 (> stored: not covered)
+    int shifted_terms_for(double wave_number) const;
+    void assemble_retarded_shifted(const double *d_static, double wave_number, int terms, double2 *d_out,
+                                   hipStream_t s, KernelTimer *timer = nullptr);
     // single entries through the near-field kernel (drop-in *_compute_pair)
     void assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,
                         int *d_error);
@@ -94,6 +107,9 @@ class RegionSet {
     double diameter_ = 0.0;
     int moment_terms_ = 0, moment_tiles_ = 0;
     DeviceBuffer<double> d_moments_;
+    int shift_terms_ = 0;
+    double shift_k0_ = 0.0;
+    DeviceBuffer<double> d_shifted_;
     DeviceBuffer<int> d_moment_tiles_;
 
   public:

@@ -899,6 +899,245 @@ __global__ __launch_bounds__(256) void moment_eval_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// moments around a shifted wave number k0 (the points a sweep has above the
+// range of the moments at 0): with d = k - k0,
+//   (exp(-jkR) - 1)/R = (exp(-jk0R) - 1)/R + sum_{m>=1} (-jd)^m/m! exp(-jk0R) R^(m-1),
+// so V_0 = scale * sum_w (exp(-jk0R) - 1)/R (the remainder at k0, formed like
+// smooth_tile_kernel) and V_m = scale * sum_w exp(-jk0R) R^(m-1), complex,
+// stored as [re / im][m][upper tile][64 x 64] like the moments at 0.
+
+constexpr int kShiftTerms = RegionSet::kMaxShiftTerms;
+
+__device__ inline void store_shifted(double *mre, double *mim, size_t mstride, int nt, int oi, int oj, int terms,
+                                     const double (&ur)[kShiftTerms], const double (&ui)[kShiftTerms],
+                                     double scale) {
+    const int a = min(oi, oj), b = max(oi, oj);
+    const size_t p = moment_slot(a, b, nt);
+    const bool diag = (a >> 6) == (b >> 6);
+    const size_t q = p - (size_t)(a & 63) * 64 - (size_t)(b & 63) + (size_t)(b & 63) * 64 + (size_t)(a & 63);
+#pragma unroll
+    for (int m = 0; m < kShiftTerms; ++m)
+        if (m < terms) {
+            mre[(size_t)m * mstride + p] = scale * ur[m];
+            mim[(size_t)m * mstride + p] = scale * ui[m];
+            if (diag) {
+                mre[(size_t)m * mstride + q] = scale * ur[m];
+                mim[(size_t)m * mstride + q] = scale * ui[m];
+            }
+        }
+}
+
+// one point pair's contributions with weight w at distance R (rinv = 1/R, 0
+// at R = 0)
+__device__ inline void shifted_terms(double w, double R, double rinv, bool zero, double half_k0, int terms,
+                                     double (&ur)[kShiftTerms], double (&ui)[kShiftTerms]) {
+    double s2, sc;
+    d_half_angle_terms(half_k0 * R, s2, sc);  // sin^2(k0R/2), sin(k0R/2) cos(k0R/2)
+    ur[0] = __builtin_fma(-2.0 * w * rinv, s2, ur[0]);
+    ui[0] = zero ? __builtin_fma(-2.0 * w, half_k0, ui[0]) : __builtin_fma(-2.0 * w * rinv, sc, ui[0]);
+    // exp(-jk0R) = (1 - 2 sin^2) - 2j sin cos
+    double tr = w * __builtin_fma(-2.0, s2, 1.0), ti = -2.0 * w * sc;
+#pragma unroll
+    for (int m = 1; m < kShiftTerms; ++m)
+        if (m < terms) {
+            ur[m] += tr;
+            ui[m] += ti;
+            tr *= R;
+            ti *= R;
+        }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void shifted_tile_kernel(
+    const double4 *__restrict__ pts_i, const double4 *__restrict__ pts_j,
+    const int *__restrict__ npts, int R, int slots, const double *__restrict__ aux,
+    const double *__restrict__ dir, double dir_tol, const int *__restrict__ perm,
+    const int *__restrict__ tiles, int terms, double k0, double *__restrict__ mre, double *__restrict__ mim,
+    size_t mstride, int nt) {
+    extern __shared__ double4 sj[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int bi = tiles[2 * blockIdx.x], bj = tiles[2 * blockIdx.x + 1];
+    const int i = bi * kTileI + lane;
+    const bool iv = i < R;
+    const double ai = iv ? aux[i] : 1.0;
+    const double half_k0 = 0.5 * k0;
+    double di[3] = {0.0, 0.0, 0.0};
+    if (KIND == 1 && iv)
+        for (int m = 0; m < 3; ++m) di[m] = dir[3 * i + m];
+    double4 *myj = sj + (size_t)wave * slots;
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = bj * kTileJ + wave * 4 + jj;
+        const bool jv = j < R;
+        const int nj = jv ? npts[j] : 0;
+        for (int q = lane; q < nj; q += 64) myj[q] = pts_j[(size_t)j * slots + q];
+        __syncthreads();
+        bool active = jv && iv && i <= j;
+        double dot = 1.0;
+        if (KIND == 1 && active) {
+            dot = di[0] * dir[3 * j] + di[1] * dir[3 * j + 1] + di[2] * dir[3 * j + 2];
+            active = !(fabs(dot) < dir_tol);
+        }
+        double ur[kShiftTerms], ui[kShiftTerms];
+#pragma unroll
+        for (int m = 0; m < kShiftTerms; ++m) ur[m] = ui[m] = 0.0;
+        if (__ballot(active)) {
+            for (int p = 0; p < slots; ++p) {
+                const double4 P = iv ? pts_i[(size_t)p * R + i] : make_double4(0.0, 0.0, 0.0, 0.0);
+                for (int q = 0; q < nj; ++q) {
+                    const double4 Q = myj[q];
+                    const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                    const bool zero = r2 == 0.0;
+                    const double rinv = zero ? 0.0 : rsqrt_nr(r2);
+                    shifted_terms(P.w * Q.w, r2 * rinv, rinv, zero, half_k0, terms, ur, ui);
+                }
+            }
+        }
+        if (jv && iv && i <= j) {
+            const double sc = !active ? 0.0
+                              : KIND == 0 ? kInv4PiEps0 / (ai * aux[j])
+                                          : kMu0Over4Pi * dot / (ai * aux[j]);
+            store_shifted(mre, mim, mstride, nt, perm[i], perm[j], terms, ur, ui, sc);
+        }
+        __syncthreads();
+    }
+}
+
+// near entries with the near rule (overwrites), as near_moment_kernel
+__global__ __launch_bounds__(64) void near_shifted_kernel(
+    const int *__restrict__ pairs, int n_pairs, const int *__restrict__ ptr,
+    const double *__restrict__ pv, int order, int kind, const double *__restrict__ aux,
+    const double *__restrict__ dir, double dir_tol, int terms, double k0, double *__restrict__ mre,
+    double *__restrict__ mim, size_t mstride, int nt, int *__restrict__ err) {
+    __shared__ double part[2][kShiftTerms][65];
+    const int e = blockIdx.x;
+    if (e >= n_pairs) return;
+    const int lane = threadIdx.x;
+    const int i = pairs[2 * e], j = pairs[2 * e + 1];
+    const int a0 = ptr[i], na = ptr[i + 1] - a0, b0 = ptr[j], nb = ptr[j + 1] - b0;
+    const int tasks = na * nb * 4;
+    if (tasks > kMaxTasks) {
+        if (lane == 0) atomicOr(err, 2);
+        return;
+    }
+    const double half_k0 = 0.5 * k0;
+    double ur[kShiftTerms], ui[kShiftTerms];
+#pragma unroll
+    for (int m = 0; m < kShiftTerms; ++m) ur[m] = ui[m] = 0.0;
+    for (int t = lane; t < tasks; t += 64) {
+        const int pp = t >> 2, tp = t & 3;
+        const int ia = pp / nb, ib = pp - ia * nb;
+        double ta[3][3], tb[3][3];
+        d_piece_triangle(pv + 12 * (size_t)(a0 + ia), tp >> 1, ta);
+        d_piece_triangle(pv + 12 * (size_t)(b0 + ib), tp & 1, tb);
+        for (int uu = 0; uu < order; ++uu)
+            for (int vv = 0; vv < order; ++vv) {
+                double ra[3], ja;
+                d_tri_map(ta, 0.5 * (c_gl_x[order][uu] + 1.0), 0.5 * (c_gl_x[order][vv] + 1.0), ra, ja);
+                const double wa = 0.25 * c_gl_w[order][uu] * c_gl_w[order][vv] * ja;
+                for (int x = 0; x < order; ++x)
+                    for (int y = 0; y < order; ++y) {
+                        double rb[3], jb;
+                        d_tri_map(tb, 0.5 * (c_gl_x[order][x] + 1.0), 0.5 * (c_gl_x[order][y] + 1.0), rb, jb);
+                        const double wb = 0.25 * c_gl_w[order][x] * c_gl_w[order][y] * jb;
+                        const double Rd = d_dist3(ra, rb);
+                        const bool zero = !(Rd > 0.0);
+                        shifted_terms(wa * wb, Rd, zero ? 0.0 : 1.0 / Rd, zero, half_k0, terms, ur, ui);
+                    }
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < kShiftTerms; ++m) {
+        part[0][m][lane] = ur[m];
+        part[1][m][lane] = ui[m];
+    }
+    __syncthreads();
+    double sc;
+    if (kind == (int)RegionKind::Node) {
+        sc = kInv4PiEps0 / (aux[i] * aux[j]);
+    } else {
+        const double dot = d_dot3(dir + 3 * i, dir + 3 * j);
+        if (fabs(dot) < dir_tol) return;
+        sc = kMu0Over4Pi * dot / (aux[i] * aux[j]);
+    }
+    if (lane >= 2 * terms) return;
+    const int part_id = lane / terms, m = lane % terms;
+    double v = 0.0;
+    for (int l = 0; l < 64; ++l) v += part[part_id][m][l];
+    v *= sc;
+    if (!isfinite(v)) atomicOr(err, 1);
+    double *dst = part_id ? mim : mre;
+    const int a = min(i, j), b = max(i, j);
+    const size_t p = moment_slot(a, b, nt);
+    dst[(size_t)m * mstride + p] = v;
+    if ((a >> 6) == (b >> 6)) dst[(size_t)m * mstride + p - (size_t)(a & 63) * 64 - (b & 63) + (size_t)(b & 63) * 64 + (a & 63)] = v;
+}
+
+struct ShiftCoefficients {
+    double cr[kShiftTerms], ci[kShiftTerms];  // (-jd)^m / m!, m = 0..terms-1 (c_0 = 1)
+};
+
+// out = static + sum_m c_m V_m (complex), smallest terms first; layout and
+// transposed store as moment_eval_kernel
+__global__ __launch_bounds__(256) void shifted_eval_kernel(
+    const double *__restrict__ mre, const double *__restrict__ mim, size_t mstride, int terms, ShiftCoefficients cf,
+    const int *__restrict__ tile_ids, int R, const double *__restrict__ stat, double2 *__restrict__ out,
+    size_t ld) {
+    __shared__ double tr[64][65];
+    const int t = blockIdx.x;
+    const int ta = tile_ids[2 * t], tb = tile_ids[2 * t + 1];
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const size_t base = (size_t)t * 4096 + (size_t)r0 * 64 + c;
+    double re[16], im[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) re[q] = im[q] = 0.0;
+    for (int m = terms - 1; m >= 0; --m) {
+        const double *sr = mre + (size_t)m * mstride + base, *si = mim + (size_t)m * mstride + base;
+        const double cr = cf.cr[m], ci = cf.ci[m];
+        if (ci == 0.0) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                re[q] = __builtin_fma(cr, sr[q * 256], re[q]);
+                im[q] = __builtin_fma(cr, si[q * 256], im[q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                re[q] = __builtin_fma(-ci, si[q * 256], re[q]);
+                im[q] = __builtin_fma(ci, sr[q * 256], im[q]);
+            }
+        }
+    }
+    const int j = tb * 64 + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = ta * 64 + r0 + 4 * q;
+        if (i < R && j < R) {
+            const size_t o = (size_t)i * ld + j;
+            re[q] += stat[o];
+            out[o] = make_double2(re[q], im[q]);
+        }
+    }
+    if (ta == tb) return;
+    double tre[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tr[r0 + 4 * q][c] = re[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tre[q] = tr[c][r0 + 4 * q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tr[r0 + 4 * q][c] = im[q];
+    __syncthreads();
+    const int i = ta * 64 + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int jj = tb * 64 + r0 + 4 * q;
+        if (i < R && jj < R) out[(size_t)jj * ld + i] = make_double2(tre[q], tr[c][r0 + 4 * q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 namespace {
@@ -1312,6 +1551,92 @@ void RegionSet::assemble_retarded_moments(const double *d_static, double k, int 
     if (timer)
         timer->stop(tk, s, kKernelSmoothMoments, 2.0 * terms * mstride,
                     8.0 * (terms + 1) * mstride + 16.0 * R_ * R_);
+}
+
+int RegionSet::shifted_terms_for(double k) const {
+    if (shift_terms_ <= 0) return kMaxShiftTerms + 1;
+    const double x = std::fabs(k - shift_k0_) * diameter_;
+    double t = 1.0;  // x^m / m!
+    for (int m = 0; m < kMaxShiftTerms; ++m) {
+        t *= x / (m + 1);
+        if (t <= 1e-17) return m + 1;  // terms m = 0..m
+    }
+    return kMaxShiftTerms + 1;
+}
+
+void RegionSet::build_shifted_moments(int terms, double k0, hipStream_t s) {
+    terms = std::min(terms, (int)kMaxShiftTerms);
+    shift_terms_ = 0;
+    d_shifted_.release();
+    if (terms <= 0 || R_ == 0) return;
+    if (smooth_order_ <= 0) throw HipError{"region set built without smooth-rule points"};
+    const int nt = (R_ + 63) / 64;
+    if (!d_moment_tiles_.ptr) {
+        std::vector<int> ids;
+        for (int a = 0; a < nt; ++a)
+            for (int b = a; b < nt; ++b) {
+                ids.push_back(a);
+                ids.push_back(b);
+            }
+        d_moment_tiles_.upload(ids, s);
+        moment_tiles_ = nt * (nt + 1) / 2;
+    }
+    const size_t mstride = (size_t)moment_tiles_ * 4096;
+    d_shifted_.resize(2 * mstride * terms);
+    double *mre = d_shifted_.ptr, *mim = d_shifted_.ptr + mstride * terms;
+    HIP_CHECK(hipMemsetAsync(d_shifted_.ptr, 0, 2 * mstride * terms * sizeof(double), s));
+    const int ntiles = (int)h_tiles_.size() / 2;
+    const size_t lds = 4 * (size_t)smooth_slots_ * sizeof(double4);
+    if (kind_ == RegionKind::Node)
+        hipLaunchKernelGGL(shifted_tile_kernel<0>, dim3(ntiles), dim3(256), lds, s, d_smooth_i_.ptr, d_smooth_j_.ptr,
+                           d_smooth_n_.ptr, R_, smooth_slots_, d_aux_k_.ptr, d_dir_k_.ptr, dir_tol_, d_perm_.ptr,
+                           d_tiles_.ptr, terms, k0, mre, mim, mstride, nt);
+    else
+        hipLaunchKernelGGL(shifted_tile_kernel<1>, dim3(ntiles), dim3(256), lds, s, d_smooth_i_.ptr, d_smooth_j_.ptr,
+                           d_smooth_n_.ptr, R_, smooth_slots_, d_aux_k_.ptr, d_dir_k_.ptr, dir_tol_, d_perm_.ptr,
+                           d_tiles_.ptr, terms, k0, mre, mim, mstride, nt);
+    HIP_CHECK(hipGetLastError());
+    const int nn = (int)h_near_.size() / 2;
+    if (nn && near_smooth_order_ > 0) {
+        if (!d_smooth_err_.ptr) d_smooth_err_.resize(1);
+        HIP_CHECK(hipMemsetAsync(d_smooth_err_.ptr, 0, sizeof(int), s));
+        hipLaunchKernelGGL(near_shifted_kernel, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr, d_pv_.ptr,
+                           near_smooth_order_, (int)kind_, d_aux_.ptr, d_dir_.ptr, dir_tol_, terms, k0, mre, mim,
+                           mstride, nt, d_smooth_err_.ptr);
+        HIP_CHECK(hipGetLastError());
+        int err = 0;
+        HIP_CHECK(hipMemcpyAsync(&err, d_smooth_err_.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (err) throw HipError{"near-field shifted moment pass failed (non-finite value or too many tasks)"};
+    }
+    shift_terms_ = terms;
+    shift_k0_ = k0;
+}
+
+void RegionSet::assemble_retarded_shifted(const double *d_static, double k, int terms, double2 *d_out,
+                                          hipStream_t s, KernelTimer *timer) {
+    if (terms < 1 || terms > shift_terms_) throw HipError{"shifted moment terms out of range"};
+    ShiftCoefficients cf{};
+    const double d = k - shift_k0_;
+    double a = 1.0;  // d^m / m!
+    for (int m = 0; m < terms; ++m) {
+        if (m > 0) a *= d / m;
+        switch (m & 3) {  // (-j)^m = 1, -j, -1, j
+        case 0: cf.cr[m] = a; cf.ci[m] = 0.0; break;
+        case 1: cf.cr[m] = 0.0; cf.ci[m] = -a; break;
+        case 2: cf.cr[m] = -a; cf.ci[m] = 0.0; break;
+        default: cf.cr[m] = 0.0; cf.ci[m] = a; break;
+        }
+    }
+    const int tk = timer ? timer->start(s) : -1;
+    const size_t mstride = (size_t)moment_tiles_ * 4096;
+    const double *mre = d_shifted_.ptr, *mim = d_shifted_.ptr + mstride * shift_terms_;
+    hipLaunchKernelGGL(shifted_eval_kernel, dim3(moment_tiles_), dim3(256), 0, s, mre, mim, mstride, terms, cf,
+                       d_moment_tiles_.ptr, R_, d_static, d_out, (size_t)R_);
+    HIP_CHECK(hipGetLastError());
+    if (timer)
+        timer->stop(tk, s, kKernelSmoothMoments, 4.0 * terms * mstride,
+                    16.0 * terms * mstride + 8.0 * mstride + 16.0 * R_ * R_);
 }
 
 void RegionSet::assemble_pairs(const std::vector<int> &pairs, double *d_values, hipStream_t s,

@@ -211,6 +211,7 @@ struct PeecEngine {
     // the factor held for (cached_f, cached_mode) may serve further
     // right-hand sides at that point without a new assembly or factor
     bool z_equilibrated = false;  // d_Z holds this point's equilibrated FP64 matrix
+    bool shift_ok = true;         // shifted-center moments may still be built
     bool cache_ok = false;
     double cached_f = 0.0;
     int cached_mode = -1;
@@ -254,7 +255,7 @@ __global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *
     dst[i] = src[i];
 }
 
-static int assemble_retarded(PeecEngine *e, double k);
+static int assemble_retarded(PeecEngine *e, double k, double *center = nullptr);
 
 static int factor_error(const LUStatus &st) {
     if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
@@ -312,7 +313,7 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
     e->cache_ok = false;
     const bool ret = e->opt.retarded != 0;
     e->retarded_now = ret;
-    if (ret) report.moment_terms = assemble_retarded(e, omega / kC0);
+    if (ret) report.moment_terms = assemble_retarded(e, omega / kC0, &report.moment_center);
     HIP_CHECK(hipEventRecord(e->ev[1], s));
     const bool mixed = mode == PEEC_SOLVE_MIXED;
     e->d_Z0.resize(nn);
@@ -535,6 +536,7 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->reuse_pivots = 1;
     o->smooth_moments = -1;
     o->split_fp16_update = 1;
+    o->shifted_moments = 0;
 }
 
 // Moment terms stored by default: the series then covers k * diameter up to
@@ -679,26 +681,69 @@ int peec_engine_static_matrices(const PeecEngine *e, double *L, double *P) {
     }
 }
 
-// Retarded Lp(w) and P(w): from the stored moments when the series at this
-// wave number needs no more terms than stored, else by quadrature. Returns
-// the moment terms used for Lp (0 = quadrature).
-static int assemble_retarded(PeecEngine *e, double k) {
+// Shifted-center moments: terms per center (the series then covers
+// |k - k0| * diameter up to ~3.2 each side of the center).
+static constexpr int kAutoShiftTerms = 30;
+static constexpr double kShiftHalfWidth = 3.2;
+
+// Moments around k0 for a point above the range of the moments at 0: kept
+// if they cover k, else (re)built with the window starting at k (a sweep
+// upwards reuses them), when they fit in half the free device memory.
+static bool ensure_shifted(PeecEngine *e, double k) {
+    if (e->opt.shifted_moments == 0 || !e->shift_ok) return false;
+    RegionSet &ed = *e->edges, &nd = *e->nodes;
+    if (ed.shifted_terms_for(k) <= ed.shifted_terms() && nd.shifted_terms_for(k) <= nd.shifted_terms()) return true;
+    const int terms = e->opt.shifted_moments > 0 ? std::min(e->opt.shifted_moments, RegionSet::kMaxShiftTerms)
+                                                 : kAutoShiftTerms;
+    const size_t bytes = 2 * (RegionSet::moment_bytes(e->ne, terms) + RegionSet::moment_bytes(e->nv, terms));
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const size_t held = ed.shifted_bytes() + nd.shifted_bytes();
+    if (bytes > (free_b + held) / 2) {
+        e->shift_ok = false;  // never fits on this device: quadrature from now on
+        return false;
+    }
+    const double diam = std::max(ed.diameter(), nd.diameter());
+    double half = kShiftHalfWidth;
+    // the half width these terms cover: (half)^terms / terms! <= 1e-17
+    for (;;) {
+        double t = 1.0;
+        for (int m = 1; m <= terms; ++m) t *= half / m;
+        if (t <= 1e-17 || half < 0.05) break;
+        half *= 0.9;
+    }
+    const double k0 = k + half / diam;
+    ed.build_shifted_moments(terms, k0, e->stream);
+    nd.build_shifted_moments(terms, k0, e->stream);
+    return ed.shifted_terms_for(k) <= ed.shifted_terms() && nd.shifted_terms_for(k) <= nd.shifted_terms();
+}
+
+// Retarded Lp(w) and P(w): from the stored moments at 0 when the series at
+// this wave number needs no more terms than stored, else from the moments
+// around a shifted center, else by quadrature. Returns the moment terms used
+// for Lp (0 = quadrature); *center = the center's wave number (0 for the
+// series at 0).
+static int assemble_retarded(PeecEngine *e, double k, double *center) {
     e->d_Lw.resize((size_t)e->ne * e->ne);
     e->d_Pw.resize((size_t)e->nv * e->nv);
     AssemblyWork w;
-    int used = 0;
+    if (center) *center = 0.0;
     const int tl = e->edges->series_terms(k), tp = e->nodes->series_terms(k);
-    if (tl <= e->edges->moment_terms()) {
+    if (tl <= e->edges->moment_terms() && tp <= e->nodes->moment_terms()) {
         e->edges->assemble_retarded_moments(e->d_L.ptr, k, tl, e->d_Lw.ptr, e->stream, &e->timer);
-        used = tl;
-    } else {
-        e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w, &e->timer);
-    }
-    if (tp <= e->nodes->moment_terms())
         e->nodes->assemble_retarded_moments(e->d_P.ptr, k, tp, e->d_Pw.ptr, e->stream, &e->timer);
-    else
-        e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w, &e->timer);
-    return used;
+        return tl;
+    }
+    if (ensure_shifted(e, k)) {
+        const int sl = e->edges->shifted_terms_for(k), sp = e->nodes->shifted_terms_for(k);
+        e->edges->assemble_retarded_shifted(e->d_L.ptr, k, sl, e->d_Lw.ptr, e->stream, &e->timer);
+        e->nodes->assemble_retarded_shifted(e->d_P.ptr, k, sp, e->d_Pw.ptr, e->stream, &e->timer);
+        if (center) *center = e->edges->shifted_k0();
+        return sl;
+    }
+    e->edges->assemble_retarded(e->d_L.ptr, k, e->d_Lw.ptr, e->stream, &w, &e->timer);
+    e->nodes->assemble_retarded(e->d_P.ptr, k, e->d_Pw.ptr, e->stream, &w, &e->timer);
+    return 0;
 }
 
 int peec_engine_retarded_matrices(PeecEngine *e, double k, Complex *L, Complex *P) {

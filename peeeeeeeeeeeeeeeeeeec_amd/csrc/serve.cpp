@@ -205,6 +205,7 @@ int peec_server_create(const PeecServerOptions *o, PeecServer **out) {
     // the admission estimate has no room for stored moments of the smooth
     // remainder: engines assemble every point by quadrature
     sv->opt.engine.smooth_moments = 0;
+    sv->opt.engine.shifted_moments = 0;
     sv->budget = o->hbm_budget_bytes ? o->hbm_budget_bytes : free_b / 10 * 9;
     for (int i = 0; i < o->slots; ++i) sv->workers.emplace_back([s = sv.get()] { s->worker(); });
     *out = sv.release();

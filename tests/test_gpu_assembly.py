@@ -131,7 +131,7 @@ def test_retarded_from_moments_matches_quadrature(require_gpu, k):
     with plib.Model(xyz, quads) as m:
         out = {}
         for terms in (26, 0):
-            eng = plib.Engine(m, retarded=True, smooth_moments=terms)
+            eng = plib.Engine(m, retarded=True, smooth_moments=terms, shifted_moments=0)
             out[terms] = eng.retarded_matrices(k)
             eng.close()
     o = ck.OracleModel(xyz, quads)
@@ -160,7 +160,7 @@ def test_engine_points_from_moments(require_gpu):
     res = {}
     with plib.Model(xyz, quads) as m:
         for terms in (-1, 0):
-            eng = plib.Engine(m, retarded=True, smooth_moments=terms)
+            eng = plib.Engine(m, retarded=True, smooth_moments=terms, shifted_moments=0)
             res[terms] = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
             eng.close()
     used = [r[4].moment_terms for r in res[-1]]
@@ -172,3 +172,56 @@ def test_engine_points_from_moments(require_gpu):
     for a, b in zip(res[-1], res[0]):
         assert abs(a[1][0] - b[1][0]) <= 1e-10 * abs(b[1][0])
         assert a[4].backward_error <= 1e-10
+
+
+def test_retarded_from_shifted_moments_matches_quadrature(require_gpu):
+    """Points above the range of the moments at 0 (k * diameter > 2.56 on the
+    0.825 m box) from complex moments around a shifted wave number k0 against
+    the quadrature pass and the oracle: real and imaginary parts to 1e-14 /
+    1e-12 as for the moments at 0. The first point builds the center (k0 above
+    k), the next two reuse it, a point below its window rebuilds it."""
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 6, 4, 4)
+    ks = [6.0, 8.0, 11.0, 4.0]
+    with plib.Model(xyz, quads) as m:
+        sh = plib.Engine(m, retarded=True, smooth_moments=26, shifted_moments=30)
+        qd = plib.Engine(m, retarded=True, smooth_moments=0, shifted_moments=0)
+        res = [(k, sh.retarded_matrices(k), qd.retarded_matrices(k)) for k in ks]
+        sh.close()
+        qd.close()
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Ls, Ps = o.inductance(), o.potential()
+        for k, a, b in res:
+            Lo, Po = o.retarded(k, 2, Ls, Ps, 4)
+            for name, x, y, c in (("L", a[0], b[0], Lo), ("P", a[1], b[1], Po)):
+                e_re = np.abs(x.real - y.real).max() / np.abs(y.real).max()
+                e_im = np.abs(x.imag - y.imag).max() / np.abs(y.imag).max()
+                e_or = np.abs(x - c).max() / np.abs(c).max()
+                print(f"k={k} {name}: shifted vs quadrature re {e_re:.2e} im {e_im:.2e}; vs oracle {e_or:.2e}",
+                      flush=True)
+                assert np.array_equal(x, x.T)
+                assert e_re <= 1e-14 and e_im <= 1e-12 and e_or <= 1e-11
+    finally:
+        o.close()
+
+
+def test_engine_points_from_shifted_moments(require_gpu):
+    """A sweep above the range of the moments at 0: the points report the
+    shifted center they used and give the impedances of an engine without
+    moments."""
+    xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
+    port = (0, len(xyz) - 1)
+    freqs = [2.5e8, 3e8, 4e8]
+    res = {}
+    with plib.Model(xyz, quads) as m:
+        for on in (1, 0):
+            eng = plib.Engine(m, retarded=True, smooth_moments=-on, shifted_moments=-on)
+            res[on] = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+            eng.close()
+    centers = [r[4].moment_center for r in res[1]]
+    print("centers:", centers, "terms:", [r[4].moment_terms for r in res[1]], flush=True)
+    assert all(c > 0 for c in centers) and centers[0] == centers[1]
+    assert all(r[4].moment_center == 0 and r[4].moment_terms == 0 for r in res[0])
+    for a, b in zip(res[1], res[0]):
+        assert a[0] == 0 and b[0] == 0
+        assert abs(a[1][0] - b[1][0]) <= 1e-10 * abs(b[1][0])
