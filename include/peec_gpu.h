@@ -75,7 +75,7 @@ typedef struct
                                    needs them (k0 = k + ~3.2/diameter, so a sweep upwards
                                    reuses them) and kept while they cover the points,
                                    16 bytes per term per upper-triangle entry. A build
-                                   costs about 6 quadrature passes and covers a window
+                                   costs about 8 quadrature passes and covers a window
                                    of 6.4 / diameter in k: worth it for dense sweeps
                                    (tens of points per window), not for a few points.
                                    0 (default) = off, 1..32 = terms, -1 = 30 terms when
