@@ -43,7 +43,7 @@ struct LUContext {
     int dinv_n = 0;
     int lds_per_block = 64 * 1024;
     hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
-    hipEvent_t ev_next = nullptr, ev_diag = nullptr;
+    hipEvent_t ev_next = nullptr, ev_diag = nullptr, ev_prev = nullptr, ev_u12 = nullptr;
     explicit LUContext(hipStream_t s);
     ~LUContext();
     LUContext(const LUContext &) = delete;

@@ -331,12 +331,16 @@ LUContext::LUContext(hipStream_t s) : stream(s) {
     // system-scope fence (device-scope release/acquire orders the work)
     HIP_CHECK(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_CHECK(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_prev, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_u12, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_CHECK(hipDeviceGetAttribute(&lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 }
 
 LUContext::~LUContext() {
     if (ev_next) (void)hipEventDestroy(ev_next);
     if (ev_diag) (void)hipEventDestroy(ev_diag);
+    if (ev_prev) (void)hipEventDestroy(ev_prev);
+    if (ev_u12) (void)hipEventDestroy(ev_u12);
     if (aux) (void)hipStreamDestroy(aux);
 }
 
@@ -2084,11 +2088,11 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
                            dA, lda, n, k0, w, ctx.status.ptr);
     };
-    auto solve_u12 = [&](int k0, int w, int c0, int c1) {
+    auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st) {
         const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
         const int grid = std::max(1, std::min(tiles, ctx.cus));
-        hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), s, dA,
-                           lda, c0, c1, k0, w, ctx.status.ptr);
+        hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
+                           dA, lda, c0, c1, k0, w, ctx.status.ptr);
     };
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
@@ -2100,12 +2104,24 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     bool pending = false;  // a diagonal factor is in flight on aux
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int w = std::min(nb, n - k0), rest = n - k0 - w;
+        if (rest <= 0) {
+            if (pending) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
+            pending = false;
+            break;
+        }
+        const int w2 = std::min(nb, rest), r0 = k0 + w;
+        // U12 of the next block column needs this block's diagonal factor
+        // and the previous trailing update; with a look-ahead factor in
+        // flight it runs on the second stream behind both, beside L21
+        const bool u12_aux = pending;
         if (pending) {
+            HIP_CHECK(hipEventRecord(ctx.ev_prev, s));
+            HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_prev, 0));
+            solve_u12(k0, w, r0, r0 + w2, aux);
+            HIP_CHECK(hipEventRecord(ctx.ev_u12, aux));
             HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
             pending = false;
         }
-        if (rest <= 0) break;
-        const int w2 = std::min(nb, rest), r0 = k0 + w;
         const int ks_count = (w + 3) / 4;
         int rb_pad, cb_pad;
         gemm_pads<S>(rest, rest, rb_pad, cb_pad);
@@ -2115,10 +2131,11 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         S *Up = reinterpret_cast<S *>(ctx.pack_u.ptr);
         // critical path: L21, U12 of the next block column, its update
         solve_l21(k0, w, rest);
-        solve_u12(k0, w, r0, r0 + w2);
+        if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
         const size_t tl = (size_t)ks_count * rb_pad * 64;
         hipLaunchKernelGGL(pack_l_kernel<S>, dim3((unsigned)((tl + 255) / 256)), dim3(256), 0, s, dA, lda, r0, rest,
                            k0, w, rb_pad, Lp, ctx.status.ptr);
+        if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
         const int cb_next = 4 * ((w2 + 63) / 64);
         const size_t tu1 = (size_t)ks_count * cb_next * 64;
         hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu1 + 255) / 256)), dim3(256), 0, s, dA, lda, k0, w,
@@ -2140,7 +2157,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         // off the critical path: the rest of U12 and of the trailing update
         const int rest2 = rest - w2;
         if (rest2 > 0 && split16 && (w == 128 || w == 64)) {
-            solve_u12(k0, w, r0 + w2, n);
+            solve_u12(k0, w, r0 + w2, n, s);
             const int cb16 = (rest2 + 127) / 128 * 8, kc = w / 16;
             ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
             ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
@@ -2154,7 +2171,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                                ctx.status.ptr);
             gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16, ctx.status.ptr, ctx, s);
         } else if (rest2 > 0) {
-            solve_u12(k0, w, r0 + w2, n);
+            solve_u12(k0, w, r0 + w2, n, s);
             const int cb_rest = cb_pad - w2 / 16;
             const size_t tu2 = (size_t)ks_count * cb_rest * 64;
             hipLaunchKernelGGL(pack_u_kernel<S>, dim3((unsigned)((tu2 + 255) / 256)), dim3(256), 0, s, dA, lda, k0,
