@@ -2283,7 +2283,25 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const S *M, size_t lda, V
 }
 
 constexpr int kChainTB = 64;
-constexpr int kChainPad = kChainTB + 1;  // LDS row stride of the inverse block
+// LDS layout of a 64-wide row in the chain kernel: column c at
+// c + 4 (c / 16) in rows of 77, so that the 16 lanes of one LDS access group
+// (4 rows x the 4 column quarters of a thread quad) fall in distinct banks
+constexpr int kChainRow = 77;
+__host__ __device__ constexpr int chain_col(int c) { return c + 4 * (c >> 4); }
+
+// sum over the 4 lanes of a quad (lanes 4j .. 4j + 3), every lane ends
+// with it: (v0 + v1) + (v2 + v3) by DPP quad permutations, no LDS traffic
+template <int CTRL>
+__device__ inline double dpp_quad(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double quad_sum(double v) {
+    v = v + dpp_quad<0xB1>(v);  // lanes 1, 0, 3, 2
+    return v + dpp_quad<0x4E>(v);  // lanes 2, 3, 0, 1
+}
+__device__ inline double2 quad_sum(double2 v) { return make_double2(quad_sum(v.x), quad_sum(v.y)); }
 
 // Inverses of the 64 x 64 diagonal blocks of a factor, in FP64: the
 // unit-lower L part to linv, then the upper U part to uinv, both
@@ -2378,32 +2396,38 @@ __device__ inline bool chain_empty(double v) { return (unsigned long long)__doub
 // Thread t works on row t / 4 and the 16 columns 16 (t % 4) .. + 16 of each
 // block, so the four threads of a row read 64 consecutive elements; the
 // factor entries of a dependency are loaded before its values are polled.
-// Up to 8 right-hand sides per launch; all arithmetic in FP64 (FP32 factors
-// are widened on load).
-template <class S, class V, bool UPPER>
+// Every workgroup waits at the front of the chain, so the time per block is
+// the poll round trip plus what follows it: the right-hand sides are a
+// template parameter (NQ = 1, or up to 8) so that those steps are straight
+// code whose LDS reads are issued together, the row sums use DPP, and the
+// LDS rows are skewed against bank conflicts. All arithmetic in FP64 (FP32
+// factors are widened on load).
+template <class S, class V, bool UPPER, int NQ>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, const V *B, V *Y, int n, int nrhs,
                                                          const typename VecT<V>::A *dinv, int *bad) {
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
-    __shared__ T dblk[TB * kChainPad];
-    __shared__ T xs[8][TB];
-    __shared__ T bs[8][TB];
+    __shared__ T dblk[TB * kChainRow];
+    __shared__ T xs[NQ][kChainRow];
+    __shared__ T bs[NQ][TB];
     __shared__ int s_abort;
     const int nblk = (n + TB - 1) / TB;
+    const int nq = NQ == 1 ? 1 : nrhs;
     const int tid = threadIdx.x;
     const int i = tid >> 2, qt = tid & 3, c0 = qt * 16;
+    const int cs = chain_col(c0);  // this thread's 16 columns start here in an LDS row
     if (tid == 0) s_abort = 0;
     for (int w = blockIdx.x; w < nblk; w += gridDim.x) {
         const int b = UPPER ? nblk - 1 - w : w;
         const int r0 = b * TB, rows = min(TB, n - r0);
         const int ri = min(i, rows - 1);  // clamped row for loads
-        Regs<T, 8> acc;
+        Regs<T, NQ> acc;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc.set(q, ezero<T>());
+        for (int q = 0; q < NQ; ++q) acc.set(q, ezero<T>());
         __syncthreads();  // the previous block is done with dblk, xs, bs
         for (int idx = tid; idx < TB * TB; idx += 256)
-            dblk[(idx / TB) * kChainPad + idx % TB] = dinv[(size_t)b * TB * TB + idx];
-        for (int idx = tid; idx < nrhs * TB; idx += 256) {
+            dblk[(idx / TB) * kChainRow + chain_col(idx % TB)] = dinv[(size_t)b * TB * TB + idx];
+        for (int idx = tid; idx < nq * TB; idx += 256) {
             const int q = idx / TB, c = idx % TB;
             bs[q][c] = c < rows ? B[(size_t)q * n + r0 + c] : ezero<T>();
         }
@@ -2416,7 +2440,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
             Regs<T, 16> m;
 #pragma unroll
             for (int c = 0; c < 16; ++c) m.set(c, ld(mrow + cb + min(c0 + c, cols - 1)));
-            for (int idx = tid; idx < nrhs * TB; idx += 256) {
+            for (int idx = tid; idx < nq * TB; idx += 256) {
                 const int q = idx / TB, c = idx % TB;
                 T xv = ezero<T>();
                 if (c < cols) {
@@ -2434,39 +2458,46 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
                         __builtin_amdgcn_s_sleep(1);
                     }
                 }
-                xs[q][c] = xv;
+                xs[q][chain_col(c)] = xv;
             }
             __syncthreads();
             if (s_abort) return;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const T mv = m.get(c);
+            for (int q = 0; q < NQ; ++q)
+                if (NQ == 1 || q < nq) {
+                    Regs<T, 16> xr;
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (q < nrhs) acc.set(q, eadd(acc.get(q), emul(mv, xs[q][c0 + c])));
-            }
+                    for (int c = 0; c < 16; ++c) xr.set(c, xs[q][cs + c]);
+                    T a = acc.get(q);
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) a = eadd(a, emul(m.get(c), xr.get(c)));
+                    acc.set(q, a);
+                }
             __syncthreads();
         }
         __syncthreads();  // bs and dblk of a block without dependencies
         // row sums over the four threads of a row; z = b - sum into xs
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q < nrhs) {
-                T v = acc.get(q);
-                v = eadd(v, eshfl_xor(v, 1));
-                v = eadd(v, eshfl_xor(v, 2));
-                if (qt == 0) xs[q][i] = esub(bs[q][i], v);
+        for (int q = 0; q < NQ; ++q)
+            if (NQ == 1 || q < nq) {
+                const T v = quad_sum(acc.get(q));
+                if (qt == 0) xs[q][chain_col(i)] = esub(bs[q][i], v);
             }
         __syncthreads();
         // x_b = D^{-1} z, published with cache-bypassing stores
+        Regs<T, 16> drow;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q < nrhs) {
+        for (int k = 0; k < 16; ++k) drow.set(k, dblk[i * kChainRow + cs + k]);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            if (NQ == 1 || q < nq) {
+                Regs<T, 16> zr;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) zr.set(k, xs[q][cs + k]);
                 T v = ezero<T>();
 #pragma unroll
-                for (int k = 0; k < 16; ++k) v = eadd(v, emul(dblk[i * kChainPad + c0 + k], xs[q][c0 + k]));
-                v = eadd(v, eshfl_xor(v, 1));
-                v = eadd(v, eshfl_xor(v, 2));
+                for (int k = 0; k < 16; ++k) v = eadd(v, emul(drow.get(k), zr.get(k)));
+                v = quad_sum(v);
                 if (qt == (q & 3) && i < rows) pub_elem(reinterpret_cast<double *>(Y + (size_t)q * n + r0 + i), v);
             }
     }
@@ -2477,11 +2508,12 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
 template <class S, class V>
 int chain_grid(int n, const LUContext &ctx) {
     const int nblk = (n + kChainTB - 1) / kChainTB;
+    // the eight-right-hand-side variants use the most LDS
     int f = 0, bw = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &f, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, false>), 256, 0));
+        &f, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, false, 8>), 256, 0));
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &bw, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, true>), 256, 0));
+        &bw, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, true, 8>), 256, 0));
     long long g = std::min<long long>(nblk, (long long)std::min(f, bw) * ctx.cus);
     // PEEC_CHAIN_WORKGROUPS caps the grid (tests use it to run several
     // blocks per workgroup on small systems)
@@ -2506,8 +2538,9 @@ void chain_solve(const S *dLU, int n, size_t lda, const V *B, V *Y, int nrhs, co
         int nq = std::min(8, nrhs - q0), nn = n;
         size_t ld = lda;
         void *args[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, &nq, (void *)&dinv, &bad};
-        HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER>), dim3(grid),
-                                             dim3(256), args, 0, ctx.stream));
+        void *fn = nq == 1 ? reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 1>)
+                           : reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 8>);
+        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, ctx.stream));
     }
 }
 
