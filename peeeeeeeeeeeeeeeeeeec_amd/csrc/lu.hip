@@ -1920,14 +1920,17 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
             const int idx = (int)threadIdx.x + u * 1024;
             v.set(u, idx < w * w ? A[(size_t)(k0 + idx / w) * lda + k0 + idx % w] : S{});
         }
+        // only the strict upper triangle: the zeros on and below the
+        // diagonal make every step's update a no-op for the entries already
+        // solved, so the steps need no per-lane selects for them
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int idx = (int)threadIdx.x + u * 1024;
-            if (idx < w * w) blk[idx] = v.get(u);
+            if (idx < w * w) blk[idx] = idx % w > idx / w ? v.get(u) : S{};
         }
     }
-    __syncthreads();
-    if ((int)threadIdx.x < w) rcp[threadIdx.x] = wdiv(wone<T>(), wld(blk + threadIdx.x * w + threadIdx.x));
+    if ((int)threadIdx.x < w)
+        rcp[threadIdx.x] = wdiv(wone<T>(), wld(A + (size_t)(k0 + threadIdx.x) * lda + k0 + threadIdx.x));
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ca = lane, cb = lane + 64;
@@ -1943,7 +1946,8 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
         // lane-read chain; steps past w - 1 change nothing. Columns 0..63
         // (x0) are final after step 63 and columns 64.. (x1) are never
         // solved before it, so the steps run in two phases that leave out
-        // the other half's selects and updates (same values as one loop).
+        // the other half's updates (same values as one loop). Lanes past w
+        // compute values that are never read or stored.
         const int wa = min(w, 64);
         for (int j0 = 0; j0 < wa; j0 += 4) {
 #pragma unroll
@@ -1952,9 +1956,8 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
                 const T xj = wmul(wreadlane(x0, j & 63), rcp[jc]);
                 x0 = (ca == j) ? xj : x0;
                 const T u0 = wld(blk + jc * w + cla), u1 = wld(blk + jc * w + clb);
-                const T n0 = wfms(x0, xj, u0), n1 = wfms(x1, xj, u1);
-                x0 = (ca > j && ca < w) ? n0 : x0;
-                x1 = (cb < w) ? n1 : x1;
+                x0 = wfms(x0, xj, u0);
+                x1 = wfms(x1, xj, u1);
             }
         }
         for (int j0 = 64; j0 < w; j0 += 4) {
@@ -1964,8 +1967,7 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
                 const T xj = wmul(wreadlane(x1, j & 63), rcp[jc]);
                 x1 = (cb == j) ? xj : x1;
                 const T u1 = wld(blk + jc * w + clb);
-                const T n1 = wfms(x1, xj, u1);
-                x1 = (cb > j && cb < w) ? n1 : x1;
+                x1 = wfms(x1, xj, u1);
             }
         }
         if (ca < w) wst(row + ca, x0);
@@ -1995,7 +1997,9 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int idx = (int)threadIdx.x + u * 1024;
-            if (idx < w * w) lt[(idx % w) * wp + idx / w] = v.get(u);
+            // the strict lower triangle only (zeros elsewhere make every
+            // step a no-op for the rows already solved: no per-lane selects)
+            if (idx < w * w) lt[(idx % w) * wp + idx / w] = idx / w > idx % w ? v.get(u) : S{};
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2025,9 +2029,8 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
                     const int r = r0 + u, rc = min(r, w - 1);
                     const T xr = wreadlane(x0, r & 63);
                     const T l0 = wld(lt + rc * wp + rla), l1 = wld(lt + rc * wp + rlb);
-                    const T n0 = wfms(x0, l0, xr), n1 = wfms(x1, l1, xr);
-                    x0 = (ra > r && ra < w) ? n0 : x0;
-                    x1 = (rb < w) ? n1 : x1;
+                    x0 = wfms(x0, l0, xr);
+                    x1 = wfms(x1, l1, xr);
                 }
             }
             for (int r0 = 64; r0 + 1 < w; r0 += 4) {
@@ -2036,8 +2039,7 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
                     const int r = r0 + u, rc = min(r, w - 1);
                     const T xr = wreadlane(x1, r & 63);
                     const T l1 = wld(lt + rc * wp + rlb);
-                    const T n1 = wfms(x1, l1, xr);
-                    x1 = (rb > r && rb < w) ? n1 : x1;
+                    x1 = wfms(x1, l1, xr);
                 }
             }
             if (ra < w) wst(col + ra, x0);
