@@ -180,6 +180,19 @@ def test_real_lu_random(require_gpu, n):
     assert np.abs(X[0] - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
+def test_real_lu_few_rhs_chain(require_gpu):
+    """Five right-hand sides through the real chained solve (the variant for
+    up to eight per launch), with a partial last 64-row block."""
+    rng = np.random.default_rng(15)
+    n = 515
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((5, n))
+    st, X, _ = plib.real_lu_factor_solve(A, B)
+    assert st == 0
+    ref = np.linalg.solve(A, B.T).T
+    assert np.abs(X - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
 def _dropin_harmonic(model, frequency, a, b):
     """The reference driver's call sequence (main.c:1349-1465) on the GPU library."""
     lib = plib.load()
