@@ -2308,9 +2308,11 @@ __device__ inline double2 quad_sum(double2 v) { return make_double2(quad_sum(v.x
 // Inverses of the 64 x 64 diagonal blocks of a factor, in FP64: the
 // unit-lower L part to linv, then the upper U part to uinv, both
 // identity-padded past row n. One workgroup per block; thread t owns row
-// t / 4 and columns 16 (t % 4) .. + 16 of X, and the block is inverted by
-// substitution against the identity (L X = I forward, U X = I backward) in
-// LDS. A zero or non-finite diagonal of U sets bad |= 1.
+// t / 4 and columns 16 (t % 4) .. + 16 of X in registers, and the block is
+// inverted by substitution against the identity (L X = I forward, U X = I
+// backward): at each step the finished row goes through LDS to the rows
+// still to update (one barrier per step, the row buffer double-buffered).
+// A zero or non-finite diagonal of U sets bad |= 1.
 template <class S, class V>
 __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t lda, int n,
                                                            typename VecT<V>::A *linv, typename VecT<V>::A *uinv,
@@ -2318,7 +2320,7 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t ld
     using T = typename VecT<V>::A;
     constexpr int TB = kChainTB;
     __shared__ T blk[TB][TB];
-    __shared__ T X[TB][TB];
+    __shared__ T xrow[2][TB];  // the finished row of X at each step (double-buffered)
     const int b = blockIdx.x, r0 = b * TB, rows = min(TB, n - r0);
     T one = ezero<T>();
     if constexpr (Elem<V>::cplx) one.x = 1.0; else one = 1.0;
@@ -2327,19 +2329,29 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t ld
         const int r = idx / TB, c = idx % TB;
         blk[r][c] = (r < rows && c < rows) ? ld(M + (size_t)(r0 + r) * lda + r0 + c) : (r == c ? one : ezero<T>());
     }
-    for (int c = c0; c < c0 + 16; ++c) X[i][c] = (i == c) ? one : ezero<T>();
+    // this thread's 16 entries of row i of X, in registers
+    Regs<T, 16> x;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x.set(c, (i == c0 + c) ? one : ezero<T>());
     __syncthreads();
     // L X = I: row k is final at step k; rows below subtract L[i][k] X[k][:]
     for (int k = 0; k < TB - 1; ++k) {
-        if (i > k) {
-            const T l = blk[i][k];
-            for (int c = c0; c < c0 + 16; ++c) X[i][c] = esub(X[i][c], emul(l, X[k][c]));
+        T *xk = xrow[k & 1];
+        if (i == k) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) xk[c0 + c] = x.get(c);
         }
         __syncthreads();
+        if (i > k) {
+            const T l = blk[i][k];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x.set(c, esub(x.get(c), emul(l, xk[c0 + c])));
+        }
     }
-    for (int c = c0; c < c0 + 16; ++c) linv[((size_t)b * TB + i) * TB + c] = X[i][c];
-    __syncthreads();
-    for (int c = c0; c < c0 + 16; ++c) X[i][c] = (i == c) ? one : ezero<T>();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) linv[((size_t)b * TB + i) * TB + c0 + c] = x.get(c);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x.set(c, (i == c0 + c) ? one : ezero<T>());
     if (tid < TB) {
         const T d = blk[tid][tid];
         if constexpr (Elem<V>::cplx) {
@@ -2349,22 +2361,27 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t ld
             if (d == 0.0 || !isfinite(d)) atomicOr(bad, 1);
         }
     }
-    __syncthreads();
-    // U X = I backward: scale row k by 1 / U[k][k], then rows above subtract
-    // U[i][k] X[k][:]
+    // U X = I backward: row k is scaled by 1 / U[k][k] and is then final;
+    // rows above subtract U[i][k] X[k][:]
     for (int k = TB - 1; k >= 0; --k) {
+        T *xk = xrow[k & 1];
         if (i == k) {
             const T r = ediv(one, blk[k][k]);
-            for (int c = c0; c < c0 + 16; ++c) X[k][c] = emul(r, X[k][c]);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                x.set(c, emul(r, x.get(c)));
+                xk[c0 + c] = x.get(c);
+            }
         }
         __syncthreads();
         if (i < k) {
             const T u = blk[i][k];
-            for (int c = c0; c < c0 + 16; ++c) X[i][c] = esub(X[i][c], emul(u, X[k][c]));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x.set(c, esub(x.get(c), emul(u, xk[c0 + c])));
         }
-        __syncthreads();
     }
-    for (int c = c0; c < c0 + 16; ++c) uinv[((size_t)b * TB + i) * TB + c] = X[i][c];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) uinv[((size_t)b * TB + i) * TB + c0 + c] = x.get(c);
 }
 
 // Marker of a solved value not yet written: a signalling NaN, which no
