@@ -499,8 +499,8 @@ def main():
             "algorithmic_tflops": g16_tf / 1e12,
             "mfma_roof_tflops": PEAKS["fp16_mfma"] / 3 / 1e12,
         },
-        "fp32_update": {
-            "kernel": "look-ahead column update gemm_splitk_kernel<float2> (v_mfma_f32_16x16x4f32)",
+        "next_column_update": {
+            "kernel": "look-ahead update of the next block column, split-FP16 (gemm_next_h3_kernel<8, 2>)",
             "launches": gemm["launches"],
             "tflops": gemm_tf / 1e12,
             "ms_per_point": gemm["ms"] / len(timed),

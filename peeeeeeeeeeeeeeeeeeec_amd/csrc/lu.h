@@ -35,6 +35,8 @@ struct LUContext {
     // per-column inverse scales
     DeviceBuffer<unsigned char> pack_l16, pack_u16;
     DeviceBuffer<float> scale_l16, scale_u16;
+    DeviceBuffer<unsigned char> pack_u16n;  // the next block column's U12 (look-ahead update)
+    DeviceBuffer<float> scale_u16n;
     bool split_fp16_update = true;                // ordered factor: FP16 hi/lo update (else FP32 MFMA)
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves

@@ -1578,12 +1578,10 @@ __global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t 
 // imaginary part. The update uses MB = 2 (64 x 128 tiles): rows of C are
 // then read and written in 1 KB runs.
 template <int KC, int MB>
-__global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
-                                                              int ncols, const uint4 *__restrict__ Lh,
-                                                              const uint4 *__restrict__ Uh,
-                                                              const float *__restrict__ linv,
-                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
-                                                              const int *status) {
+__device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
+                                                  const uint4 *__restrict__ Lh, const uint4 *__restrict__ Uh,
+                                                  const float *__restrict__ linv, const float *__restrict__ uinv,
+                                                  int rb_pad, int cb_count, const int *status) {
     constexpr int NB = 8 / MB, BM = 32 * MB, BN = 32 * NB;
     constexpr int CL = MB * NB * 4;
     constexpr int UA = 4 * MB, UNITS = UA + 4 * NB;  // L21 units, all units per chunk
@@ -1699,28 +1697,50 @@ __global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t 
         }
 }
 
+// The trailing update, and the same code under its own name for the next
+// block column's update (a separate entry in kernel traces).
+template <int KC, int MB>
+__global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
+                                                              int ncols, const uint4 *__restrict__ Lh,
+                                                              const uint4 *__restrict__ Uh,
+                                                              const float *__restrict__ linv,
+                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
+                                                              const int *status) {
+    gemm_tile_h3_body<KC, MB>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+}
+template <int KC, int MB>
+__global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
+                                                              int ncols, const uint4 *__restrict__ Lh,
+                                                              const uint4 *__restrict__ Uh,
+                                                              const float *__restrict__ linv,
+                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
+                                                              const int *status) {
+    gemm_tile_h3_body<KC, MB>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+}
+
 // A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 128
-// or 64 columns of L21).
+// or 64 columns of L21); `next`: the next block column's update.
 static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
-                    const int *status, LUContext &ctx, hipStream_t s) {
+                    const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
+                    hipStream_t s) {
     if (m <= 0 || ncols <= 0) return;
     const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr);
-    const uint4 *Uh = reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr);
     const int tiles = ((m + 63) / 64) * ((ncols + 127) / 128);
     const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-    const float *ls = ctx.scale_l16.ptr, *us = ctx.scale_u16.ptr;
+    const float *ls = ctx.scale_l16.ptr;
+    const bool next = kind == kKernelLuUpdate;
     if (kw == 128)
-        hipLaunchKernelGGL((gemm_tile_h3_kernel<8, 2>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
-                           Uh, ls, us, rb_pad, cb_count, status);
+        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<8, 2> : gemm_tile_h3_kernel<8, 2>), dim3(grid), dim3(256), 0,
+                           s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
     else if (kw == 64)
-        hipLaunchKernelGGL((gemm_tile_h3_kernel<4, 2>), dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh,
-                           Uh, ls, us, rb_pad, cb_count, status);
+        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<4, 2> : gemm_tile_h3_kernel<4, 2>), dim3(grid), dim3(256), 0,
+                           s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
     else
         throw HipError{"split-FP16 update needs a block width of 64 or 128"};
     HIP_CHECK(hipGetLastError());
     if (ctx.timer)
-        ctx.timer->stop(tg, s, kKernelLuUpdateSplit16, 8.0 * m * ncols * kw,
+        ctx.timer->stop(tg, s, kind, 8.0 * m * ncols * kw,
                         (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
 }
 
@@ -2127,6 +2147,46 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         const int ks_count = (w + 3) / 4;
         int rb_pad, cb_pad;
         gemm_pads<S>(rest, rest, rb_pad, cb_pad);
+        const int rest2 = rest - w2;
+        if (split16 && (w == 128 || w == 64)) {
+            // split-FP16 operands for both updates: L21 packed once, U12 of
+            // the next block column into its own small buffer (critical
+            // path), then the rest of U12 off the critical path
+            const int kc = w / 16, cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
+            ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
+            ctx.scale_l16.reserve((size_t)rb_pad * 16);
+            ctx.pack_u16n.reserve((size_t)kc * cbn * 128 * sizeof(uint4));
+            ctx.scale_u16n.reserve((size_t)cbn * 16);
+            solve_l21(k0, w, rest);
+            if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
+            hipLaunchKernelGGL(pack_l_h3_kernel, dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, dA, lda, r0,
+                               rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr), ctx.scale_l16.ptr,
+                               ctx.status.ptr);
+            if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
+            hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cbn + 3) / 4)), dim3(256), 0, s, dA, lda, k0, w, r0,
+                               w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr), ctx.scale_u16n.ptr,
+                               ctx.status.ptr);
+            gemm_h3(dA, lda, r0, r0, rest, w2, w, rb_pad, cbn, reinterpret_cast<const uint4 *>(ctx.pack_u16n.ptr),
+                    ctx.scale_u16n.ptr, kKernelLuUpdate, ctx.status.ptr, ctx, s);
+            HIP_CHECK(hipEventRecord(ctx.ev_next, s));
+            HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
+            diag(r0, w2, aux);
+            HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
+            pending = true;
+            if (rest2 > 0) {
+                solve_u12(k0, w, r0 + w2, n, s);
+                ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
+                ctx.scale_u16.reserve((size_t)cb16 * 16);
+                hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cb16 + 3) / 4)), dim3(256), 0, s, dA, lda, k0,
+                                   w, r0 + w2, rest2, cb16, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr),
+                                   ctx.scale_u16.ptr, ctx.status.ptr);
+                gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16,
+                        reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr, kKernelLuUpdateSplit16,
+                        ctx.status.ptr, ctx, s);
+            }
+            HIP_CHECK(hipGetLastError());
+            continue;
+        }
         ctx.pack_l.reserve((size_t)(ks_count + 2) * rb_pad * 64 * sizeof(S));
         ctx.pack_u.reserve((size_t)(ks_count + 2) * cb_pad * 64 * sizeof(S));
         S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
@@ -2157,22 +2217,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
         pending = true;
         // off the critical path: the rest of U12 and of the trailing update
-        const int rest2 = rest - w2;
-        if (rest2 > 0 && split16 && (w == 128 || w == 64)) {
-            solve_u12(k0, w, r0 + w2, n, s);
-            const int cb16 = (rest2 + 127) / 128 * 8, kc = w / 16;
-            ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
-            ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
-            ctx.scale_l16.reserve((size_t)rb_pad * 16);
-            ctx.scale_u16.reserve((size_t)cb16 * 16);
-            hipLaunchKernelGGL(pack_l_h3_kernel, dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, dA, lda, r0,
-                               rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr), ctx.scale_l16.ptr,
-                               ctx.status.ptr);
-            hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cb16 + 3) / 4)), dim3(256), 0, s, dA, lda, k0, w,
-                               r0 + w2, rest2, cb16, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr,
-                               ctx.status.ptr);
-            gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16, ctx.status.ptr, ctx, s);
-        } else if (rest2 > 0) {
+        if (rest2 > 0) {
             solve_u12(k0, w, r0 + w2, n, s);
             const int cb_rest = cb_pad - w2 / 16;
             const size_t tu2 = (size_t)ks_count * cb_rest * 64;
