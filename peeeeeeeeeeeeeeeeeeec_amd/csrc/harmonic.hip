@@ -21,16 +21,10 @@ namespace peec {
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kC0 = 299792458.0;  // main.c:47-54
 
-// Z = [[R + jwL, A], [-P A^T, jw I]] written row-major with leading
-// dimension ldz; A is implicit in the edge end points (incidence.c:34-115).
-// When Lw/Pw are given (retarded form) they replace L/P.
-__global__ void mna_fill_kernel(double2 *Z, double2 *Z0, size_t ldz, int ne, int nv, const double *L,
-                                const double2 *Lw, const double *R, const double *P, const double2 *Pw,
-                                const double *PAT, const int *edges, double omega, int chunks) {
-    const int n = ne + nv;
-    const int row = blockIdx.x / chunks;
-    const int col = (blockIdx.x % chunks) * 256 + threadIdx.x;
-    if (row >= n || col >= n) return;
+// One entry of Z = [[R + jwL, A], [-P A^T, jw I]].
+__device__ inline double2 mna_entry(int row, int col, int ne, int nv, const double *L, const double2 *Lw,
+                                    const double *R, const double *P, const double2 *Pw, const double *PAT,
+                                    const int *edges, double omega) {
     double2 z;
     if (row < ne) {
         if (col < ne) {
@@ -65,17 +59,51 @@ __global__ void mna_fill_kernel(double2 *Z, double2 *Z0, size_t ldz, int ne, int
             z = make_double2(0.0, (col - ne) == i ? omega : 0.0);
         }
     }
-    if (Z) Z[(size_t)row * ldz + col] = z;
-    if (Z0) Z0[(size_t)row * ldz + col] = z;
+    return z;
+}
+
+// Z = [[R + jwL, A], [-P A^T, jw I]] written row-major with leading
+// dimension ldz; A is implicit in the edge end points (incidence.c:34-115).
+// When Lw/Pw are given (retarded form) they replace L/P.
+// With rowmax, each workgroup also folds max |z| of its part of the row into
+// rowmax[row] (zero beforehand; a non-finite entry counts as infinity), the
+// row maxima of the equilibration without another pass over Z0.
+__global__ __launch_bounds__(256) void mna_fill_kernel(double2 *Z, double2 *Z0, size_t ldz, int ne, int nv,
+                                                       const double *L, const double2 *Lw, const double *R,
+                                                       const double *P, const double2 *Pw, const double *PAT,
+                                                       const int *edges, double omega, int chunks, double *rowmax) {
+    const int n = ne + nv;
+    const int row = blockIdx.x / chunks;
+    const int col = (blockIdx.x % chunks) * 256 + threadIdx.x;
+    const double2 z = col < n ? mna_entry(row, col, ne, nv, L, Lw, R, P, Pw, PAT, edges, omega) : double2{};
+    if (col < n) {
+        if (Z) Z[(size_t)row * ldz + col] = z;
+        if (Z0) Z0[(size_t)row * ldz + col] = z;
+    }
+    if (!rowmax) return;
+    __shared__ double red[4];
+    double m = 0.0;
+    if (col < n) {
+        const double v = hypot(z.x, z.y);
+        m = isfinite(v) ? v : INFINITY;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        // non-negative doubles order like their bit patterns
+        atomicMax(reinterpret_cast<unsigned long long *>(rowmax + row), (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 void launch_mna_fill(double2 *Z, double2 *Z0, size_t ldz, int ne, int nv, const double *L, const double2 *Lw,
                      const double *R, const double *P, const double2 *Pw, const double *PAT, const int *edges,
-                     double omega, hipStream_t s) {
+                     double omega, hipStream_t s, double *rowmax) {
     const int n = ne + nv;
     const int chunks = (n + 255) / 256;
     hipLaunchKernelGGL(mna_fill_kernel, dim3((unsigned)((size_t)n * chunks)), dim3(256), 0, s, Z, Z0, ldz, ne, nv,
-                       L, Lw, R, P, Pw, PAT, edges, omega, chunks);
+                       L, Lw, R, P, Pw, PAT, edges, omega, chunks, rowmax);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -324,9 +352,10 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
         // the FP32 factor is formed straight from Z0 with the equilibration's
         // scales (read-only passes); the equilibrated FP64 matrix is only
         // built if the point falls back to FP64
+        zero_doubles(e->d_row.ptr, n, s);
         launch_mna_fill(nullptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
-                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
-        if (equilibration_scales(e->d_Z0.ptr, n, n, e->d_row.ptr, e->d_col.ptr, *e->lu) != 0) return -2;
+                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s, e->d_row.ptr);
+        if (equilibration_scales(e->d_Z0.ptr, n, n, e->d_row.ptr, e->d_col.ptr, *e->lu, true) != 0) return -2;
     } else {
         e->d_Z.resize(nn);
         launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,

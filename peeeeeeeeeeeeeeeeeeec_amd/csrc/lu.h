@@ -60,8 +60,12 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
                         double *max_scaled, LUContext &ctx);
 
 // The same equilibration without writing the matrix (mixed path): row and
-// column scales of an unscaled matrix (0 or -2 as above) ...
-int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx);
+// column scales of an unscaled matrix (0 or -2 as above; rows_ready: d_row
+// already holds the row maxima) ...
+int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx,
+                         bool rows_ready = false);
+// p[0..n) = 0 on stream s
+void zero_doubles(double *p, size_t n, hipStream_t s);
 // ... the equilibrated entries (a_ij / r_i) / c_j in FP32, rows perm[i] (or
 // i when d_perm is null), with max |equilibrated entry| in FP64; the values
 // are those equilibrate_complex would produce, rounded once to FP32 ...

@@ -368,12 +368,25 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
     return 0;
 }
 
-int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx) {
+__global__ void row_check_kernel(const double *row, int n, int *bad) {
+    const int i = (int)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && (!isfinite(row[i]) || row[i] == 0.0)) atomicOr(bad, 1);
+}
+
+void zero_doubles(double *p, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx,
+                         bool rows_ready) {
     hipStream_t s = ctx.stream;
     DeviceBuffer<int> bad(1);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
     hipLaunchKernelGGL(fill_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, (size_t)n);
-    hipLaunchKernelGGL(row_max_kernel, dim3(n), dim3(256), 0, s, dA, n, lda, d_row, bad.ptr);
+    if (rows_ready)
+        hipLaunchKernelGGL(row_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_row, n, bad.ptr);
+    else
+        hipLaunchKernelGGL(row_max_kernel, dim3(n), dim3(256), 0, s, dA, n, lda, d_row, bad.ptr);
     dim3 grid((n + 255) / 256, (n + 63) / 64);
     hipLaunchKernelGGL(colmax_scaled_kernel, grid, dim3(256), 0, s, dA, n, lda, d_row, d_col);
     HIP_CHECK(hipGetLastError());
