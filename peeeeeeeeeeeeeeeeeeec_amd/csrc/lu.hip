@@ -270,13 +270,23 @@ __global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(const double2 
                                                                   const double *row, const double *col, int n,
                                                                   double *maxv) {
     __shared__ double red[4];
-    const int i = blockIdx.y, c = blockIdx.x * 256 + threadIdx.x;
+    // four entries per thread (1024 per workgroup), loads issued first
+    const int i = blockIdx.y, c0 = blockIdx.x * 1024 + threadIdx.x;
     const int si = perm ? perm[i] : i;
+    const double ri = row[si];
+    const double2 *srow = src + (size_t)si * n;
+    double2 a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = c0 + 256 * u < n ? srow[c0 + 256 * u] : double2{};
     double m = 0.0;
-    if (c < n) {
-        const double2 v = equilibrated(src[(size_t)si * n + c], row[si], col[c]);
-        dst[(size_t)i * n + c] = make_float2((float)v.x, (float)v.y);
-        m = mag(v);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = c0 + 256 * u;
+        if (c < n) {
+            const double2 v = equilibrated(a[u], ri, col[c]);
+            dst[(size_t)i * n + c] = make_float2((float)v.x, (float)v.y);
+            m = fmax(m, mag(v));
+        }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -405,8 +415,8 @@ int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_per
     DeviceBuffer<int> bad(1);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
     hipLaunchKernelGGL(zero_scale_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, n, bad.ptr);
-    hipLaunchKernelGGL(to_f32_equilibrated_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, dA,
-                       dst, d_perm, d_row, d_col, n, mx.ptr);
+    hipLaunchKernelGGL(to_f32_equilibrated_kernel, dim3((unsigned)((n + 1023) / 1024), (unsigned)n), dim3(256), 0, s,
+                       dA, dst, d_perm, d_row, d_col, n, mx.ptr);
     HIP_CHECK(hipGetLastError());
     int h_bad = 0;
     std::vector<double> slots(kMaxSlots);
