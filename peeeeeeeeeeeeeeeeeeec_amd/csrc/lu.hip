@@ -2880,7 +2880,7 @@ void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, c
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-// Residuals of up to kResidualRhs right-hand sides per pass, one wavefront
+// Residuals of up to kResidualRhs right-hand sides per pass, one workgroup
 // per row: R[q] = A X[q] - B[q] and the component-wise backward error
 // eta[q] = max_i |r_qi| / (sum_j |a_ij| |x_qj| + |b_qi|) (harmonic.c:932-969),
 // reduced with an integer max on the bit patterns of the non-negative
@@ -2897,22 +2897,27 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// One workgroup per row (a wavefront per row left most of the device idle
+// in the last of the ~1.5 rounds of rows); NQ = 1, or up to 8 right-hand
+// sides, so the one-right-hand-side case keeps one set of sums.
+template <int NQ>
 __global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, size_t lda, int n, const double2 *X,
                                                              const double *AX, const double2 *B, double2 *R,
                                                              int nq, unsigned long long *eta_bits) {
+    __shared__ double red[4][3 * NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = (int)blockIdx.x * 4 + wave;
-    if (i >= n) return;
-    double yr[kResidualRhs], yi[kResidualRhs], den[kResidualRhs];
+    const int i = (int)blockIdx.x;
+    const int nqq = NQ == 1 ? 1 : nq;
+    double yr[NQ], yi[NQ], den[NQ];
 #pragma unroll
-    for (int q = 0; q < kResidualRhs; ++q) yr[q] = yi[q] = den[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) yr[q] = yi[q] = den[q] = 0.0;
     const double2 *row = A + (size_t)i * lda;
-    for (int j = lane; j < n; j += 64) {
+    for (int j = threadIdx.x; j < n; j += 256) {
         const double2 z = row[j];
         const double za = hypot(z.x, z.y);
 #pragma unroll
-        for (int q = 0; q < kResidualRhs; ++q)
-            if (q < nq) {
+        for (int q = 0; q < NQ; ++q)
+            if (NQ == 1 || q < nqq) {
                 const double2 v = X[(size_t)q * n + j];
                 yr[q] += z.x * v.x - z.y * v.y;
                 yi[q] += z.x * v.y + z.y * v.x;
@@ -2920,22 +2925,31 @@ __global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, s
             }
     }
 #pragma unroll
-    for (int q = 0; q < kResidualRhs; ++q)
-        if (q < nq) {
+    for (int q = 0; q < NQ; ++q)
+        if (NQ == 1 || q < nqq) {
             for (int off = 32; off > 0; off >>= 1) {
                 yr[q] += __shfl_xor(yr[q], off);
                 yi[q] += __shfl_xor(yi[q], off);
                 den[q] += __shfl_xor(den[q], off);
             }
+            if (lane == 0) {
+                red[wave][3 * q] = yr[q];
+                red[wave][3 * q + 1] = yi[q];
+                red[wave][3 * q + 2] = den[q];
+            }
         }
-    if (lane == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < kResidualRhs; ++q)
-            if (q < nq) {
+        for (int q = 0; q < NQ; ++q)
+            if (NQ == 1 || q < nqq) {
+                const double sr = (red[0][3 * q] + red[1][3 * q]) + (red[2][3 * q] + red[3][3 * q]);
+                const double si = (red[0][3 * q + 1] + red[1][3 * q + 1]) + (red[2][3 * q + 1] + red[3][3 * q + 1]);
+                const double sd = (red[0][3 * q + 2] + red[1][3 * q + 2]) + (red[2][3 * q + 2] + red[3][3 * q + 2]);
                 const double2 bi = B[(size_t)q * n + i];
-                const double rr = yr[q] - bi.x, ri = yi[q] - bi.y;
+                const double rr = sr - bi.x, ri = si - bi.y;
                 R[(size_t)q * n + i] = make_double2(rr, ri);
-                const double ra = hypot(rr, ri), d = den[q] + hypot(bi.x, bi.y);
+                const double ra = hypot(rr, ri), d = sd + hypot(bi.x, bi.y);
                 const double eta = d > 0.0 ? ra / d : ra;
                 atomicMax(&eta_bits[q], (unsigned long long)__double_as_longlong(eta));
             }
@@ -3048,8 +3062,12 @@ void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, con
     for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
         const int nq = std::min(kResidualRhs, nrhs - q0);
         const size_t off = (size_t)q0 * n;
-        hipLaunchKernelGGL(residual_multi_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, dA, lda, n,
-                           dX + off, d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
+        if (nq == 1)
+            hipLaunchKernelGGL(residual_multi_kernel<1>, dim3((unsigned)n), dim3(256), 0, s, dA, lda, n, dX + off,
+                               d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
+        else
+            hipLaunchKernelGGL(residual_multi_kernel<kResidualRhs>, dim3((unsigned)n), dim3(256), 0, s, dA, lda, n,
+                               dX + off, d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
     }
     HIP_CHECK(hipGetLastError());
 }
