@@ -1510,11 +1510,8 @@ __device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
 
 // One wavefront per 16-row block of L21 = A[row0 + r][k0 + k], r < m (rows
 // past m are packed as zeros), k < kw (a multiple of 16, at most 128).
-__global__ __launch_bounds__(256) void pack_l_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
-                                                        int rb_pad, uint4 *out, float *inv, const int *status) {
-    if (status[0] != 0) return;
-    const int rb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (rb >= rb_pad) return;
+__device__ __forceinline__ void pack_l_h3_wave(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
+                                               int rb_pad, uint4 *out, float *inv, int rb, int lane) {
     const int r = rb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
     float2 v[8][4];
     float mx = 0.f;
@@ -1549,12 +1546,8 @@ __global__ __launch_bounds__(256) void pack_l_h3_kernel(const float2 *A, size_t 
 
 // One wavefront per 16-column block of U12 = A[k0 + k][col0 + c], c < ncols
 // (columns past ncols packed as zeros), as pairs (Br, Bi).
-__global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t lda, int k0, int kw, int col0,
-                                                        int ncols, int cb_count, uint4 *out, float *inv,
-                                                        const int *status) {
-    if (status[0] != 0) return;
-    const int cb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (cb >= cb_count) return;
+__device__ __forceinline__ void pack_u_h3_wave(const float2 *A, size_t lda, int k0, int kw, int col0, int ncols,
+                                               int cb_count, uint4 *out, float *inv, int cb, int lane) {
     const int col = cb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
     float2 v[8][4];
     float mx = 0.f;
@@ -1586,6 +1579,32 @@ __global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t 
         o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
     }
     if (q == 0) inv[col] = 1.f / sc;
+}
+__global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t lda, int k0, int kw, int col0,
+                                                        int ncols, int cb_count, uint4 *out, float *inv,
+                                                        const int *status) {
+    if (status[0] != 0) return;
+    const int cb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (cb >= cb_count) return;
+    pack_u_h3_wave(A, lda, k0, kw, col0, ncols, cb_count, out, inv, cb, lane);
+}
+
+// Both packs of the next block column's update in one launch: the first
+// lgroups workgroups pack L21 (one wavefront per 16-row block), the others
+// U12 of the next block column (as pack_u_h3_kernel).
+__global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
+                                                         int rb_pad, uint4 *lout, float *linv, int col0, int ncols,
+                                                         int cb_count, uint4 *uout, float *uinv, int lgroups,
+                                                         const int *status) {
+    if (status[0] != 0) return;
+    const int lane = threadIdx.x & 63;
+    if ((int)blockIdx.x < lgroups) {
+        const int rb = blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (rb < rb_pad) pack_l_h3_wave(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
+    } else {
+        const int cb = (blockIdx.x - lgroups) * 4 + (threadIdx.x >> 6);
+        if (cb < cb_count) pack_u_h3_wave(A, lda, k0, kw, col0, ncols, cb_count, uout, uinv, cb, lane);
+    }
 }
 
 // C -= L21 U12 on the split-FP16 operands, one tile of (32 MB) x (32 NB),
@@ -2182,13 +2201,14 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             ctx.scale_u16n.reserve((size_t)cbn * 16);
             solve_l21(k0, w, rest);
             if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
-            hipLaunchKernelGGL(pack_l_h3_kernel, dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, dA, lda, r0,
-                               rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr), ctx.scale_l16.ptr,
-                               ctx.status.ptr);
             if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
-            hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cbn + 3) / 4)), dim3(256), 0, s, dA, lda, k0, w, r0,
-                               w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr), ctx.scale_u16n.ptr,
-                               ctx.status.ptr);
+            {
+                const int lgroups = (rb_pad + 3) / 4, ugroups = (cbn + 3) / 4;
+                hipLaunchKernelGGL(pack_lu_h3_kernel, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA, lda,
+                                   r0, rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
+                                   ctx.scale_l16.ptr, r0, w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr),
+                                   ctx.scale_u16n.ptr, lgroups, ctx.status.ptr);
+            }
             gemm_h3(dA, lda, r0, r0, rest, w2, w, rb_pad, cbn, reinterpret_cast<const uint4 *>(ctx.pack_u16n.ptr),
                     ctx.scale_u16n.ptr, kKernelLuUpdate, ctx.status.ptr, ctx, s);
             HIP_CHECK(hipEventRecord(ctx.ev_next, s));
