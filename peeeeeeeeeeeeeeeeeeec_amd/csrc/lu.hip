@@ -2041,7 +2041,7 @@ constexpr int kTrsmWaveCols = 16;  // columns of A12 per workgroup pass
 
 template <class S>
 __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c0, int c1, int k0, int w,
-                                                         const int *status) {
+                                                         const int *status, uint4 *pk, float *pinv, int pcb) {
     using T = typename Work<S>::T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int wp = w + 1;                      // padded rows: transposed writes spread over the banks
@@ -2113,6 +2113,44 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
             const int idx = (int)threadIdx.x + u * 1024, i = idx / kTrsmWaveCols, c = idx % kTrsmWaveCols;
             if (i < w && cbase + c < c1) A[(size_t)(k0 + i) * lda + cbase + c] = tb[c * wp + i];
         }
+        if constexpr (std::is_same<S, float2>::value) {
+            // with pk: the solved tile is also the split-FP16 operand block
+            // `tile` of the trailing update (as pack_u_h3_kernel would write
+            // it), packed by the first wavefront straight from LDS
+            if (pk && wave == 0) {
+                const int col = lane & 15, q = lane >> 4, kc = w / 16;
+                const bool cv = cbase + col < c1;
+                float2 v[8][4];
+                float mx = 0.f;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        v[c][t] = (c < kc && cv) ? tb[col * wp + 16 * c + 4 * q + t] : make_float2(0.f, 0.f);
+                        mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
+                    }
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                const float sc = split_scale(mx);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    if (c >= kc) break;
+                    unsigned hw[4], lw[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        _Float16 xh, xl, yh, yl;
+                        split_half(v[c][t].x * sc, xh, xl);
+                        split_half(v[c][t].y * sc, yh, yl);
+                        hw[t] = pack_half2(xh, yh);
+                        lw[t] = pack_half2(xl, yl);
+                    }
+                    uint4 *o = pk + ((size_t)(c * pcb + tile) * 2) * 64 + lane;
+                    o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+                    o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+                }
+                if (q == 0) pinv[tile * 16 + col] = 1.f / sc;
+            }
+        }
     }
 }
 
@@ -2152,11 +2190,13 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
                            dA, lda, n, k0, w, ctx.status.ptr);
     };
-    auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st) {
+    // with pk, U12 is also packed as the split-FP16 operand of pcb blocks
+    auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st, uint4 *pk = nullptr, float *pinv = nullptr,
+                         int pcb = 0) {
         const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
         const int grid = std::max(1, std::min(tiles, ctx.cus));
         hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
-                           dA, lda, c0, c1, k0, w, ctx.status.ptr);
+                           dA, lda, c0, c1, k0, w, ctx.status.ptr, pk, pinv, pcb);
     };
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
@@ -2217,12 +2257,11 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
             pending = true;
             if (rest2 > 0) {
-                solve_u12(k0, w, r0 + w2, n, s);
+                // the rest of U12, packed by the solve itself (blocks past
+                // the last column are never read into a stored result)
                 ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
                 ctx.scale_u16.reserve((size_t)cb16 * 16);
-                hipLaunchKernelGGL(pack_u_h3_kernel, dim3((unsigned)((cb16 + 3) / 4)), dim3(256), 0, s, dA, lda, k0,
-                                   w, r0 + w2, rest2, cb16, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr),
-                                   ctx.scale_u16.ptr, ctx.status.ptr);
+                solve_u12(k0, w, r0 + w2, n, s, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr, cb16);
                 gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16,
                         reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr, kKernelLuUpdateSplit16,
                         ctx.status.ptr, ctx, s);
