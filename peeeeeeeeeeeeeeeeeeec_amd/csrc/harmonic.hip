@@ -74,19 +74,31 @@ __global__ __launch_bounds__(256) void mna_fill_kernel(double2 *Z, double2 *Z0, 
                                                        const int *edges, double omega, int chunks, double *rowmax) {
     const int n = ne + nv;
     const int row = blockIdx.x / chunks;
-    const int col = (blockIdx.x % chunks) * 256 + threadIdx.x;
-    const double2 z = col < n ? mna_entry(row, col, ne, nv, L, Lw, R, P, Pw, PAT, edges, omega) : double2{};
-    if (col < n) {
-        if (Z) Z[(size_t)row * ldz + col] = z;
-        if (Z0) Z0[(size_t)row * ldz + col] = z;
+    // four entries per thread, 1024 columns per workgroup
+    const int col0 = (blockIdx.x % chunks) * 1024 + threadIdx.x;
+    double2 z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int col = col0 + 256 * u;
+        z[u] = col < n ? mna_entry(row, col, ne, nv, L, Lw, R, P, Pw, PAT, edges, omega) : double2{};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int col = col0 + 256 * u;
+        if (col < n) {
+            if (Z) Z[(size_t)row * ldz + col] = z[u];
+            if (Z0) Z0[(size_t)row * ldz + col] = z[u];
+        }
     }
     if (!rowmax) return;
     __shared__ double red[4];
     double m = 0.0;
-    if (col < n) {
-        const double v = hypot(z.x, z.y);
-        m = isfinite(v) ? v : INFINITY;
-    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (col0 + 256 * u < n) {
+            const double v = hypot(z[u].x, z[u].y);
+            m = fmax(m, isfinite(v) ? v : INFINITY);
+        }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -101,7 +113,7 @@ void launch_mna_fill(double2 *Z, double2 *Z0, size_t ldz, int ne, int nv, const 
                      const double *R, const double *P, const double2 *Pw, const double *PAT, const int *edges,
                      double omega, hipStream_t s, double *rowmax) {
     const int n = ne + nv;
-    const int chunks = (n + 255) / 256;
+    const int chunks = (n + 1023) / 1024;
     hipLaunchKernelGGL(mna_fill_kernel, dim3((unsigned)((size_t)n * chunks)), dim3(256), 0, s, Z, Z0, ldz, ne, nv,
                        L, Lw, R, P, Pw, PAT, edges, omega, chunks, rowmax);
     HIP_CHECK(hipGetLastError());
