@@ -282,16 +282,22 @@ static void fill_zero(double2 *p, size_t n, hipStream_t s) {
 }
 
 // X[q] -= D[q] (or dst[q] = src[q] for the copy) for the right-hand sides
-// whose flag is set
-__global__ void masked_axpy_kernel(double2 *X, const double2 *D, const int *flags, int n, int nrhs) {
+// whose flag is set: flags[q] in device memory, or with flags null bit q of
+// mask (up to 64 right-hand sides, no upload)
+__device__ inline bool rhs_on(const int *flags, unsigned long long mask, size_t q) {
+    return flags ? flags[q] != 0 : ((mask >> q) & 1ull) != 0;
+}
+__global__ void masked_axpy_kernel(double2 *X, const double2 *D, const int *flags, unsigned long long mask, int n,
+                                   int nrhs) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)n * nrhs || !flags[i / n]) return;
+    if (i >= (size_t)n * nrhs || !rhs_on(flags, mask, i / n)) return;
     X[i].x -= D[i].x;
     X[i].y -= D[i].y;
 }
-__global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *flags, int n, int nrhs) {
+__global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *flags, unsigned long long mask, int n,
+                                   int nrhs) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)n * nrhs || !flags[i / n]) return;
+    if (i >= (size_t)n * nrhs || !rhs_on(flags, mask, i / n)) return;
     dst[i] = src[i];
 }
 
@@ -456,6 +462,19 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
         }
     };
     const unsigned grid = (unsigned)((tot + 255) / 256);
+    // the per-right-hand-side flags as a kernel argument when they fit,
+    // else uploaded
+    auto flag_args = [&](const std::vector<int> &fl, const int *&dflags, unsigned long long &mask) {
+        dflags = nullptr;
+        mask = 0ull;
+        if (nrhs <= 64) {
+            for (int q = 0; q < nrhs; ++q)
+                if (fl[q]) mask |= 1ull << q;
+        } else {
+            e->d_flags.upload(fl.data(), nrhs, s);
+            dflags = e->d_flags.ptr;
+        }
+    };
     auto refine = [&]() {
         HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
         lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr,
@@ -471,9 +490,11 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             if (!any) break;
             lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                                e->d_R.ptr, nrhs, *e->lu, true);
-            e->d_flags.upload(active.data(), nrhs, s);
-            hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr,
-                               e->d_flags.ptr, n, nrhs);
+            const int *dfl;
+            unsigned long long msk;
+            flag_args(active, dfl, msk);
+            hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr, dfl, msk, n,
+                               nrhs);
             ++report.refinement_steps;
             measure();
             bool improved = false;
@@ -491,9 +512,9 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
                 if (best[q] <= e->opt.refinement_target) active[q] = 0;
             }
             if (improved) {
-                e->d_flags.upload(flags.data(), nrhs, s);
-                hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_Xbest.ptr, e->d_X.ptr,
-                                   e->d_flags.ptr, n, nrhs);
+                flag_args(flags, dfl, msk);
+                hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_Xbest.ptr, e->d_X.ptr, dfl,
+                                   msk, n, nrhs);
             }
         }
         bool restore = false;
@@ -504,9 +525,11 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             w = std::fmax(w, best[q]);
         }
         if (restore) {
-            e->d_flags.upload(flags.data(), nrhs, s);
-            hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_Xbest.ptr,
-                               e->d_flags.ptr, n, nrhs);
+            const int *dfl;
+            unsigned long long msk;
+            flag_args(flags, dfl, msk);
+            hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_Xbest.ptr, dfl, msk,
+                               n, nrhs);
         }
         return w;
     };
