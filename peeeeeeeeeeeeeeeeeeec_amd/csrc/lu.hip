@@ -1580,18 +1580,10 @@ __device__ __forceinline__ void pack_u_h3_wave(const float2 *A, size_t lda, int 
     }
     if (q == 0) inv[col] = 1.f / sc;
 }
-__global__ __launch_bounds__(256) void pack_u_h3_kernel(const float2 *A, size_t lda, int k0, int kw, int col0,
-                                                        int ncols, int cb_count, uint4 *out, float *inv,
-                                                        const int *status) {
-    if (status[0] != 0) return;
-    const int cb = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (cb >= cb_count) return;
-    pack_u_h3_wave(A, lda, k0, kw, col0, ncols, cb_count, out, inv, cb, lane);
-}
 
 // Both packs of the next block column's update in one launch: the first
 // lgroups workgroups pack L21 (one wavefront per 16-row block), the others
-// U12 of the next block column (as pack_u_h3_kernel).
+// U12 of the next block column (one wavefront per 16-column block).
 __global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
                                                          int rb_pad, uint4 *lout, float *linv, int col0, int ncols,
                                                          int cb_count, uint4 *uout, float *uinv, int lgroups,
@@ -2115,7 +2107,7 @@ __global__ __launch_bounds__(1024) void trsm_wave_kernel(S *A, size_t lda, int c
         }
         if constexpr (std::is_same<S, float2>::value) {
             // with pk: the solved tile is also the split-FP16 operand block
-            // `tile` of the trailing update (as pack_u_h3_kernel would write
+            // `tile` of the trailing update (as pack_u_h3_wave would write
             // it), packed by the first wavefront straight from LDS
             if (pk && wave == 0) {
                 const int col = lane & 15, q = lane >> 4, kc = w / 16;
