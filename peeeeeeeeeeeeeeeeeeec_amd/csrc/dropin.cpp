@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -402,22 +403,28 @@ void complex_matrix_free(ComplexMatrix *M) {
     complex_matrix_init(M);
 }
 
+// complex_matrix.c:542: y = A x for any rows x cols
 int complex_matrix_vector_multiply(const ComplexMatrix *A, const Complex *x, Complex *y, int threads) {
     if (!A || !A->data || !x || !y || threads < 1) return -1;
-    if (A->rows != A->cols) {
-        PEEC_ERR("complex_matrix_vector_multiply: GPU path needs a square matrix.");
+    if (A->cols == 0) {
+        for (size_t i = 0; i < A->rows; ++i) y[i] = Complex{0.0, 0.0};
+        return 0;
+    }
+    if (A->rows == 0) return 0;
+    if (A->rows > (size_t)INT32_MAX || A->cols > (size_t)INT32_MAX) {
+        PEEC_ERR("complex_matrix_vector_multiply: dimensions above 2^31 - 1.");
         return -1;
     }
     if (!require_device()) return -1;
     try {
         Stream s;
-        const int n = (int)A->rows;
-        DeviceBuffer<double2> dA, dx, dy(n);
-        DeviceBuffer<double> terms(3 * (size_t)n);
-        dA.upload(reinterpret_cast<const double2 *>(A->data), (size_t)n * n, s);
-        dx.upload(reinterpret_cast<const double2 *>(x), n, s);
-        complex_residual(dA.ptr, n, n, dx.ptr, nullptr, dy.ptr, terms.ptr, s);
-        dy.download(reinterpret_cast<double2 *>(y), n, s);
+        const int rows = (int)A->rows, cols = (int)A->cols;
+        DeviceBuffer<double2> dA, dx, dy(rows);
+        DeviceBuffer<double> terms(3 * (size_t)rows);
+        dA.upload(reinterpret_cast<const double2 *>(A->data), (size_t)rows * cols, s);
+        dx.upload(reinterpret_cast<const double2 *>(x), cols, s);
+        complex_residual(dA.ptr, rows, cols, (size_t)cols, dx.ptr, nullptr, dy.ptr, terms.ptr, s);
+        dy.download(reinterpret_cast<double2 *>(y), rows, s);
         s.sync();
         return 0;
     } catch (const HipError &e) {
@@ -1016,7 +1023,7 @@ int harmonic_compute_residual(const HarmonicSystem *sys, const PotentialMatrix *
         dZ.upload(reinterpret_cast<const double2 *>(sys->matrix.data), n * n, s);
         dx.upload(reinterpret_cast<const double2 *>(x.data()), n, s);
         db.upload(reinterpret_cast<const double2 *>(rhs.data()), n, s);
-        complex_residual(dZ.ptr, (int)n, n, dx.ptr, db.ptr, nullptr, terms.ptr, s);
+        complex_residual(dZ.ptr, (int)n, (int)n, n, dx.ptr, db.ptr, nullptr, terms.ptr, s);
         std::vector<double> h(3 * n);
         terms.download(h.data(), 3 * n, s);
         s.sync();

@@ -316,6 +316,9 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
     hipStream_t s = e->stream;
     const int n = e->n;
     LUStatus st;
+    // the held factor (if any) is overwritten here: a caller that keeps this
+    // one marks it cached again afterwards
+    e->cache_ok = false;
     if (fp64) {
         if (!e->z_equilibrated) {
             // mixed point falling back to FP64: form the equilibrated matrix
@@ -458,7 +461,10 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
         for (int q = 0; q < nrhs; ++q) {
             double v;
             std::memcpy(&v, &bits[q], sizeof v);
-            eta[q] = v;
+            // a NaN or infinite residual (overflowing or NaN iterate or
+            // right-hand side) is a failure: +inf keeps every later
+            // comparison and maximum treating it as one
+            eta[q] = std::isfinite(v) ? v : INFINITY;
         }
     };
     const unsigned grid = (unsigned)((tot + 255) / 256);
@@ -569,6 +575,16 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
     HIP_CHECK(hipEventRecord(e->ev[5], s));
     report.backward_error = worst;
     return worst <= e->opt.backward_error_gate ? 0 : -3;
+}
+
+// After the solves of a point: the factor now held serves later calls at
+// (f, mode) only if the point succeeded on the factor of that mode (a mixed
+// point that fell back to FP64 holds the FP64 factor under the FP64 row
+// order, and a failed point holds nothing reusable).
+static void note_cached_factor(PeecEngine *e, double f, int mode, int code, const PeecPointReport &report) {
+    e->cache_ok = code == 0 && !report.used_fp64_fallback;
+    e->cached_f = f;
+    e->cached_mode = mode;
 }
 
 static void engine_timings(PeecEngine *e, PeecPointReport &report) {
@@ -846,15 +862,11 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
         hipLaunchKernelGGL(port_rhs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, e->d_B.ptr, ne, nv,
                            e->d_P.ptr, e->retarded_now ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
         code = engine_solve(e, mode, nr, report);
+        note_cached_factor(e, f, mode, code, report);
         if (code == -1 || code == -2) {
-            e->cache_ok = false;
             if (rep) *rep = report;
             return code;
         }
-        // a point that needed the FP64 fallback keeps no mixed factor
-        e->cache_ok = code == 0 && !report.used_fp64_fallback;
-        e->cached_f = f;
-        e->cached_mode = mode;
         std::vector<double2> x(tot);
         e->d_X.download(x.data(), tot, s);
         e->stream.sync();
@@ -946,6 +958,7 @@ int peec_engine_monostatic_rcs(PeecEngine *e, double f, int mode, const PeecRcsO
         dw.upload(waves.data(), (size_t)na, s);
         launch_excitation(*e->rcs, dw.ptr, na, o->excitation, -1.0, e->d_B.ptr, (size_t)n, s);
         code = engine_solve(e, mode, na, report);
+        note_cached_factor(e, f, mode, code, report);
         if (code == -1 || code == -2) {
             if (rep) *rep = report;
             return code;

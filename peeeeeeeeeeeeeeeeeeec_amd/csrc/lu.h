@@ -104,10 +104,10 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
                        const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor = false);
 void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s);
 
-// y = A x for complex FP64 (rhs-major vectors), and the reference's
-// component-wise backward error terms per row:
+// y = A x for a complex FP64 rows x cols matrix (leading dimension lda),
+// and the reference's component-wise backward error terms per row:
 //   out[i] = (|r_i|, sum_j |A_ij||x_j| + |b_i|, |b_i|) with r = A x - b.
-void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, const double2 *db,
+void complex_residual(const double2 *dA, int rows, int cols, size_t lda, const double2 *dx, const double2 *db,
                       double2 *dr, double *dterms, hipStream_t s);
 
 // R = A X - B for nrhs right-hand sides (rhs-major) and each one's

@@ -2880,14 +2880,15 @@ void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t 
 
 
 // residual and backward-error terms (harmonic.c:899-969), one block per row
-__global__ __launch_bounds__(256) void residual_kernel(const double2 *A, size_t lda, int n, const double2 *x,
+// of a rows x cols matrix (cols summed per row)
+__global__ __launch_bounds__(256) void residual_kernel(const double2 *A, size_t lda, int cols, const double2 *x,
                                                       const double *ax, const double2 *b, double2 *r,
                                                       double *terms) {
     __shared__ double sre[256], sim[256], sden[256];
     const int i = blockIdx.x;
     double yr = 0.0, yi = 0.0, den = 0.0;
     const double2 *row = A + (size_t)i * lda;
-    for (int j = threadIdx.x; j < n; j += 256) {
+    for (int j = threadIdx.x; j < cols; j += 256) {
         const double2 z = row[j];
         const double2 v = x[j];
         yr += z.x * v.x - z.y * v.y;
@@ -2922,11 +2923,11 @@ __global__ void abs_kernel(const double2 *x, double *ax, int n) {
     if (i < n) ax[i] = hypot(x[i].x, x[i].y);
 }
 
-void complex_residual(const double2 *dA, int n, size_t lda, const double2 *dx, const double2 *db, double2 *dr,
-                      double *dterms, hipStream_t s) {
-    DeviceBuffer<double> ax(n);
-    hipLaunchKernelGGL(abs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dx, ax.ptr, n);
-    hipLaunchKernelGGL(residual_kernel, dim3(n), dim3(256), 0, s, dA, lda, n, dx, ax.ptr, db, dr, dterms);
+void complex_residual(const double2 *dA, int rows, int cols, size_t lda, const double2 *dx, const double2 *db,
+                      double2 *dr, double *dterms, hipStream_t s) {
+    DeviceBuffer<double> ax(cols);
+    hipLaunchKernelGGL(abs_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, dx, ax.ptr, cols);
+    hipLaunchKernelGGL(residual_kernel, dim3(rows), dim3(256), 0, s, dA, lda, cols, dx, ax.ptr, db, dr, dterms);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
 }

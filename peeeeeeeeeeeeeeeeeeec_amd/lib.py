@@ -466,6 +466,7 @@ def complex_lu_factor_solve(Z, B):
     st = lib.complex_lu_solve_multiple(C.byref(lu), Bm.ctypes.data_as(_cp), X.ctypes.data_as(_cp),
                                        Bm.shape[0], 1)
     info = {
+        "data": api.complex_array(lu.data, n * n).reshape(n, n),
         "pivots": api.as_array(lu.pivots, n, np.uint64).astype(np.int64),
         "row_scale": api.as_array(lu.row_scale, n),
         "column_scale": api.as_array(lu.column_scale, n),
@@ -476,7 +477,9 @@ def complex_lu_factor_solve(Z, B):
     return st, X, info
 
 
-def real_lu_factor_solve(A, B):
+def real_lu_factor_solve(A, B, want_factor=False):
+    """Factor A with lu_factorize and solve the columns of B; the third result is
+    the determinant, or (determinant, factor, pivots) with want_factor."""
     lib = load()
     A = np.ascontiguousarray(A, dtype=np.float64)
     n = A.shape[0]
@@ -491,8 +494,26 @@ def real_lu_factor_solve(A, B):
     X = np.zeros_like(Bm)
     st = lib.lu_solve_multiple(C.byref(lu), Bm.ctypes.data_as(_dp), X.ctypes.data_as(_dp), Bm.shape[0], 1)
     det = lib.lu_determinant(C.byref(lu))
+    if want_factor:
+        det = (det, api.as_array(lu.data, n * n).reshape(n, n).copy(),
+               api.as_array(lu.pivots, n, np.uint64).astype(np.int64))
     lib.lu_free(C.byref(lu))
     return st, X, det
+
+
+def complex_matvec(A, x):
+    """y = A x through complex_matrix_vector_multiply (any rows x cols)."""
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    rows, cols = A.shape
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    M = api.ComplexMatrix()
+    M.rows, M.cols = rows, cols
+    M.data = A.ctypes.data_as(_cp)
+    y = np.zeros(rows, np.complex128)
+    if lib.complex_matrix_vector_multiply(C.byref(M), x.ctypes.data_as(_cp), y.ctypes.data_as(_cp), 1) != 0:
+        raise RuntimeError("complex_matrix_vector_multiply failed")
+    return y
 
 
 class Engine:

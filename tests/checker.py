@@ -322,6 +322,18 @@ def oracle_complex_lu(Z, tol=1e-14):
     return st, (LU, piv, rs, cs, ms.value)
 
 
+def oracle_real_lu(A, tol=1e-14):
+    """Real LU of the oracle: (status, factor, pivots, matrix scale)."""
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    LU = np.zeros_like(A)
+    piv = np.zeros(n, dtype=np.uintp)
+    ms = C.c_double()
+    st = oracle().oracle_real_lu(A.ctypes.data_as(_dp), n, tol, LU.ctypes.data_as(_dp), piv.ctypes.data_as(_sp),
+                                 C.byref(ms))
+    return st, LU, piv.astype(np.int64), ms.value
+
+
 def oracle_complex_solve(fac, b):
     LU, piv, rs, cs, _ = fac
     n = LU.shape[0]

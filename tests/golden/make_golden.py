@@ -13,6 +13,14 @@ Outputs (numpy .npz, float64/complex128, compressed) next to this script:
                   backward error) from the reference's harmonic_* path
   pairs.npz       quad_patch_kernel_1_over_r on self/touching/near/far piece
                   pairs with extreme aspect ratios (adaptive-rule values)
+  residual_lu.npz harmonic_compute_residual's seven figures on plate.msh
+                  port solutions (1 kHz, 1 MHz), as solved and with every
+                  unknown perturbed by ~1e-6 relative (seeded), so that the
+                  residual stands far above rounding; complex_lu_factorize
+                  and lu_factorize outputs (factor, pivots, scales) of seeded
+                  random matrices
+
+`python tests/golden/make_golden.py residual` writes residual_lu.npz only.
 """
 from __future__ import annotations
 
@@ -91,6 +99,127 @@ def ref_port_solve(m, d, L, P, frequency, a, b, threads=8):
     return V[a] - V[b], I, V, res.backward_error
 
 
+RESIDUAL_FIELDS = ("absolute_residual", "rhs_relative_residual", "backward_error", "edge_absolute_residual",
+                   "node_absolute_residual", "edge_backward_error", "node_backward_error")
+
+
+def perturbed(x, seed, eps=1e-6):
+    """x with every entry multiplied by 1 + eps (g1 + j g2), g ~ N(0, 1) seeded."""
+    rng = np.random.default_rng(seed)
+    n = x.shape[0]
+    return x * (1.0 + eps * (rng.standard_normal(n) + 1j * rng.standard_normal(n)))
+
+
+def ref_residual_cases(msh, frequencies, threads=4):
+    """The reference's harmonic_* path on a mesh: for each frequency the port
+    solution (node 0 -> last node) and harmonic_compute_residual of it as
+    solved and perturbed (perturbed(x, seed = index of the case))."""
+    ref = ck.ref()
+    m, d = ref_model(msh=msh)
+    L, P = ref_LP(m, d, threads)
+    A = api.IncidenceMatrix()
+    assert ref.incidence_build(C.byref(m), C.byref(A)) == 0
+    PAT = api.Matrix()
+    assert ref.incidence_build_PAT(C.byref(m), C.byref(P), C.byref(PAT), threads) == 0
+    ro = api.ResistanceOptions()
+    ref.resistance_options_default(C.byref(ro), threads)
+    R = api.ResistanceMatrix()
+    assert ref.resistance_compute_matrix(C.byref(m), C.byref(d), C.byref(ro), C.byref(R)) == 0
+    ne, nv = m.n_edges, m.n_nodes
+    a, b = 0, nv - 1
+    Is = ck.port_current(nv, a, b)
+    Ib = api.complex_buffer(Is)
+    ex = api.HarmonicExcitation()
+    ex.n_edges, ex.n_nodes = ne, nv
+    ex.node_current = C.cast(Ib, C.POINTER(api.Complex))
+    out = []
+    for f in frequencies:
+        sysm = api.HarmonicSystem()
+        ref.harmonic_system_init(C.byref(sysm))
+        assert ref.harmonic_build_system(C.byref(L), C.byref(R), C.byref(A), C.byref(PAT), f, threads,
+                                         C.byref(sysm)) == 0
+        assert ref.harmonic_factorize(C.byref(sysm), threads) == 0
+        sol = api.HarmonicSolution()
+        ref.harmonic_solution_init(C.byref(sol))
+        assert ref.harmonic_solve(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), threads) == 0
+        x = np.concatenate([api.complex_array(sol.edge_current, ne), api.complex_array(sol.node_voltage, nv)])
+        for eps in (0.0, 1e-6):
+            xp = perturbed(x, len(out), eps) if eps else x
+            for i in range(ne):
+                sol.edge_current[i] = api.Complex(xp[i].real, xp[i].imag)
+            for i in range(nv):
+                sol.node_voltage[i] = api.Complex(xp[ne + i].real, xp[ne + i].imag)
+            res = api.HarmonicResidual()
+            assert ref.harmonic_compute_residual(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), threads,
+                                                 C.byref(res)) == 0
+            out.append((f, eps, xp, np.array([getattr(res, k) for k in RESIDUAL_FIELDS])))
+        ref.harmonic_solution_free(C.byref(sol))
+        ref.harmonic_system_free(C.byref(sysm))
+    return out
+
+
+def ref_complex_lu(A, threads=4):
+    """complex_lu_factorize of the reference: (status, factor, pivots, row
+    scales, column scales, matrix scale, swap count)."""
+    ref = ck.ref()
+    n = A.shape[0]
+    o = api.ComplexLUOptions()
+    ref.complex_lu_options_default(C.byref(o), threads)
+    lu = api.ComplexLUFactorization()
+    ref.complex_lu_init(C.byref(lu))
+    buf = api.complex_buffer(np.ascontiguousarray(A, np.complex128).ravel())
+    st = ref.complex_lu_factorize(C.cast(buf, C.POINTER(api.Complex)), n, C.byref(o), C.byref(lu))
+    if st != 0:
+        ref.complex_lu_free(C.byref(lu))
+        return (st,)
+    out = (st, api.complex_array(lu.data, n * n).reshape(n, n),
+           np.array([lu.pivots[i] for i in range(n)], np.int64), api.as_array(lu.row_scale, n).copy(),
+           api.as_array(lu.column_scale, n).copy(), lu.matrix_scale, lu.swap_count)
+    ref.complex_lu_free(C.byref(lu))
+    return out
+
+
+def ref_real_lu(A, threads=4):
+    """lu_factorize of the reference: (status, factor, pivots, matrix scale, swap count)."""
+    ref = ck.ref()
+    n = A.shape[0]
+    o = api.LUOptions()
+    ref.lu_options_default(C.byref(o), threads)
+    lu = api.LUFactorization()
+    ref.lu_init(C.byref(lu))
+    Ac = np.ascontiguousarray(A, np.float64)
+    st = ref.lu_factorize(Ac.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(o), C.byref(lu))
+    if st != 0:
+        ref.lu_free(C.byref(lu))
+        return (st,)
+    out = (st, api.as_array(lu.data, n * n).reshape(n, n).copy(), np.array([lu.pivots[i] for i in range(n)], np.int64),
+           lu.matrix_scale, lu.swap_count)
+    ref.lu_free(C.byref(lu))
+    return out
+
+
+def lu_test_matrix(n, seed, complex_=True):
+    """Seeded random matrix with row magnitudes spread over 1e-3..1e3."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if complex_ else 0.0)
+    return A * rng.uniform(1e-3, 1e3, size=(n, 1))
+
+
+def write_residual_lu():
+    cases = ref_residual_cases(os.path.join(HERE, "plate.msh"), (1.0e3, 1.0e6))
+    Ac = lu_test_matrix(24, 2024)
+    Ar = lu_test_matrix(24, 2025, complex_=False)
+    c = ref_complex_lu(Ac)
+    r = ref_real_lu(Ar)
+    assert c[0] == 0 and r[0] == 0
+    np.savez_compressed(os.path.join(HERE, "residual_lu.npz"),
+                        frequency=np.array([k[0] for k in cases]), eps=np.array([k[1] for k in cases]),
+                        x=np.array([k[2] for k in cases]), residual=np.array([k[3] for k in cases]),
+                        clu_A=Ac, clu_factor=c[1], clu_pivots=c[2], clu_row_scale=c[3], clu_column_scale=c[4],
+                        clu_matrix_scale=c[5], clu_swap_count=c[6],
+                        rlu_A=Ar, rlu_factor=r[1], rlu_pivots=r[2], rlu_matrix_scale=r[3], rlu_swap_count=r[4])
+
+
 def piece(points):
     q = api.QuadPatch()
     for i, v in enumerate(np.asarray(points, dtype=float).ravel()):
@@ -122,6 +251,11 @@ def pair_cases():
 def main():
     if not ck.ref_available():
         raise SystemExit("oracle/_ref/libpeecref.so missing: run `make -C oracle ref` first")
+    if len(sys.argv) > 1 and sys.argv[1] == "residual":
+        write_residual_lu()
+        print("residual_lu.npz written")
+        return
+    write_residual_lu()
     ref = ck.ref()
     # plate.msh
     m, d = ref_model(msh=os.path.join(HERE, "plate.msh"))

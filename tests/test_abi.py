@@ -52,6 +52,62 @@ def test_struct_sizes_match_reference_layout():
         assert C.sizeof(getattr(api, name)) == size, name
 
 
+SHARED_STRUCTS = ["Mesh", "QuadPatch", "PatchList", "DualMesh", "Complex", "ComplexMatrix", "Matrix",
+                  "InductanceOptions", "InductanceMatrix", "PotentialOptions", "PotentialMatrix", "IncidenceMatrix",
+                  "ResistanceOptions", "ResistanceMatrix", "LUOptions", "LUFactorization", "ComplexLUOptions",
+                  "ComplexLUFactorization", "HarmonicSystem", "HarmonicExcitation", "HarmonicSolution",
+                  "HarmonicResidual", "IncidentField", "ScatteringOptions", "RcsOptions", "RcsResult",
+                  "LightningSource"]
+PROBE_HEADERS = ["mesh.h", "dual_mesh.h", "quadrature.h", "complex_matrix.h", "matrix.h", "inductance.h",
+                 "potential.h", "incidence.h", "resistance.h", "lu.h", "complex_lu.h", "harmonic.h",
+                 "incident_field.h", "scattering.h", "rcs.h", "lightning.h"]
+
+
+def layout_probe(tmp_path, include_dir):
+    """sizeof and the offset of every field (field names from the ctypes
+    mirrors) of the shared structs, as gcc lays them out from the headers in
+    include_dir."""
+    import subprocess
+    lines = ["#include <stdio.h>", "#include <stddef.h>"] + [f'#include "{h}"' for h in PROBE_HEADERS]
+    lines.append("int main(void){")
+    for name in SHARED_STRUCTS:
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for field, *_ in getattr(api, name)._fields_:
+            lines.append(f'printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines.append("return 0;}")
+    tag = "ref" if include_dir != os.path.join(ROOT, "include") else "ours"
+    src = tmp_path / f"layout_{tag}.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / f"layout_{tag}"
+    subprocess.run(["gcc", "-std=c17", "-w", "-I", include_dir, str(src), "-o", str(exe), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    return dict(line.split() for line in out if line)
+
+
+def test_headers_layout_matches_ctypes_mirrors(tmp_path):
+    """include/*.h and the ctypes mirrors agree on every size and field offset."""
+    got = layout_probe(tmp_path, os.path.join(ROOT, "include"))
+    for name in SHARED_STRUCTS:
+        cls = getattr(api, name)
+        assert int(got[name]) == C.sizeof(cls), name
+        for field, *_ in cls._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(cls, field).offset, (name, field)
+
+
+REF_INCLUDE = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INCLUDE), reason="reference headers not present")
+def test_headers_layout_matches_reference_headers(tmp_path):
+    """The same probe compiled against the reference's own headers
+    (include/*.h of the reference) gives the same size and field offsets for
+    every shared struct: a drop-in caller compiled against either header set
+    sees one layout."""
+    ours = layout_probe(tmp_path, os.path.join(ROOT, "include"))
+    ref = layout_probe(tmp_path, REF_INCLUDE)
+    assert ours == ref
+
+
 def test_headers_compile_to_reference_layout(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text(

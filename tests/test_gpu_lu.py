@@ -436,3 +436,124 @@ def test_engine_from_reference_matrices_via_matrix_market(require_gpu, tmp_path)
     assert st == 0
     assert abs(z[0] - g["z_in"][0]) <= 1e-10 * abs(g["z_in"][0])
     assert np.abs(I[0] - g["I"]).max() <= 1e-10 * np.abs(g["I"]).max()
+
+
+@pytest.mark.parametrize("n", [24, 130, 515])
+def test_complex_lu_factor_and_pivots_match_oracle(require_gpu, n):
+    """The factor, pivots and scales complex_lu_factorize returns to the caller
+    (lu->data, lu->pivots, ...) against the oracle, itself bit-identical to the
+    reference's complex_lu_factorize: pivots and scales exactly (n = 24 also
+    against the reference's own output), the factor entries to 1e-12 of the
+    largest (the blocked MFMA update rounds the Schur complements in another
+    order than the unblocked elimination)."""
+    import tests.golden.make_golden as mg
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    A = g["clu_A"] if n == 24 else mg.lu_test_matrix(n, 3 * n)
+    st, _, info = plib.complex_lu_factor_solve(A, np.ones(n))
+    assert st == 0
+    st_o, (LU, piv, rs, cs, ms) = ck.oracle_complex_lu(A)
+    assert st_o == 0
+    assert np.array_equal(info["pivots"], piv.astype(np.int64))
+    assert info["swap_count"] == int(np.sum(piv.astype(np.int64) != np.arange(n)))
+    np.testing.assert_allclose(info["row_scale"], rs, rtol=5e-16, atol=0)
+    np.testing.assert_allclose(info["column_scale"], cs, rtol=1e-15, atol=0)
+    assert abs(info["matrix_scale"] - ms) <= 1e-15 * ms
+    err = np.abs(info["data"] - LU).max() / np.abs(LU).max()
+    print(f"n={n}: factor max rel deviation {err:.2e}", flush=True)
+    assert err <= 1e-12
+    if n == 24:
+        assert np.array_equal(info["pivots"], g["clu_pivots"])
+        assert np.abs(info["data"] - g["clu_factor"]).max() <= 1e-12 * np.abs(g["clu_factor"]).max()
+
+
+@pytest.mark.parametrize("n", [24, 300])
+def test_real_lu_factor_and_pivots_match_oracle(require_gpu, n):
+    """lu_factorize's returned factor and pivots against the oracle (and, at
+    n = 24, the reference's own output)."""
+    import tests.golden.make_golden as mg
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    A = g["rlu_A"] if n == 24 else mg.lu_test_matrix(n, 5 * n, complex_=False)
+    st, _, (det, LUg, pivg) = plib.real_lu_factor_solve(A, np.ones(n), want_factor=True)
+    assert st == 0
+    st_o, LU, piv, ms = ck.oracle_real_lu(A)
+    assert st_o == 0
+    assert np.array_equal(pivg, piv)
+    assert np.abs(LUg - LU).max() <= 1e-12 * np.abs(LU).max()
+    if n == 24:
+        assert np.array_equal(pivg, g["rlu_pivots"])
+
+
+def test_complex_matrix_vector_multiply_rectangular(require_gpu):
+    """complex_matrix.c:542 takes any rows x cols: a 37 x 80 and an 80 x 37
+    product against numpy."""
+    rng = np.random.default_rng(11)
+    for rows, cols in ((37, 80), (80, 37), (1, 5)):
+        A = rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))
+        x = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+        y = plib.complex_matvec(A, x)
+        ref = A @ x
+        assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max(), (rows, cols)
+
+
+@pytest.mark.parametrize("mode", [plib.Engine.FP64, plib.Engine.MIXED])
+def test_engine_nonfinite_solution_fails_the_gate(require_gpu, mode):
+    """A point whose solution overflows (plane wave of 1e308 V/m at 1 MHz on a
+    small box: the oracle's FP64 solution is NaN there) must not pass: the
+    NaN residual counts as an infinite backward error and the call returns -3
+    (the reference's formula would report 0, NaN dropping out of its maxima)."""
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 6, 4, 4)
+    f, amp = 1.0e6, 1.0e308
+    o = ck.OracleModel(xyz, quads)
+    try:
+        L, P = o.inductance(), o.potential()
+        U = o.excitation(ck.oracle_plane_wave(f, 90.0, 0.0, E_amp=amp))
+        Z = o.mna(L, o.pat(P), f)
+        st, fac = ck.oracle_complex_lu(Z)
+        x = ck.oracle_complex_solve(fac, ck.oracle_rhs(o.ne, o.nv, P=P, U=U))
+    finally:
+        o.close()
+    assert not np.isfinite(x).all()  # precondition: the FP64 solution overflows
+    with plib.Model(xyz, quads) as m:
+        eng = plib.Engine(m)
+        st, res, _, rep = eng.monostatic_rcs(f, [90.0], [0.0], mode=mode, E_amp=amp)
+        # the same engine still solves a normal point afterwards
+        st2, _, _, _, rep2 = eng.solve(f, [(0, m.nv - 1)], mode, want_solution=False)
+        eng.close()
+    print(f"mode {mode}: status {st} eta {rep.backward_error} fallback {rep.used_fp64_fallback}; then {st2}",
+          flush=True)
+    assert st == -3 and rep.backward_error == math.inf
+    assert st2 == 0 and rep2.backward_error <= 1e-10
+
+
+def test_engine_rcs_fallback_then_solve_at_same_point(require_gpu):
+    """With no refinement steps every mixed point falls back to the FP64
+    factor. solve(f) -> monostatic RCS(f) -> solve(f) -> RCS(f) on one engine:
+    every call succeeds and gives the FP64 engine's answers (the fallback
+    factors the same matrix the same way), so no call reuses a factor that a
+    fallback has overwritten."""
+    xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
+    port = (0, len(xyz) - 1)
+    f = 2.0e7
+    th, ph = [90.0, 60.0, 90.0], [0.0, 45.0, 200.0]
+    with plib.Model(xyz, quads) as m:
+        fp64 = plib.Engine(m)
+        z64 = fp64.solve(f, [port], plib.Engine.FP64, want_solution=False)[1][0]
+        s64 = [r.sigma for r in fp64.monostatic_rcs(f, th, ph, mode=plib.Engine.FP64)[1]]
+        fp64.close()
+        mixed = plib.Engine(m, max_refinement_steps=0)
+        out = []
+        for call in ("solve", "rcs", "solve", "rcs"):
+            if call == "solve":
+                st, z, _, _, rep = mixed.solve(f, [port], plib.Engine.MIXED, want_solution=False)
+                out.append((call, st, rep.used_fp64_fallback, rep.factor_cached, z[0]))
+            else:
+                st, res, _, rep = mixed.monostatic_rcs(f, th, ph, mode=plib.Engine.MIXED)
+                out.append((call, st, rep.used_fp64_fallback, rep.factor_cached, [r.sigma for r in res]))
+        mixed.close()
+    for call, st, fb, cached, val in out:
+        print(call, st, fb, cached, flush=True)
+        assert st == 0 and fb == 1 and cached == 0
+        if call == "solve":
+            assert abs(val - z64) <= 1e-12 * abs(z64)
+        else:
+            np.testing.assert_allclose(val, s64, rtol=1e-12, atol=0)

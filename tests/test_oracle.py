@@ -329,6 +329,164 @@ def test_plate16_port_solution_matches_golden():
         m.close()
 
 
+# ------------------------------------------- backward error and LU factor pins
+
+RESIDUAL_FIELDS = ("absolute_residual", "rhs_relative_residual", "backward_error", "edge_absolute_residual",
+                   "node_absolute_residual", "edge_backward_error", "node_backward_error")
+
+
+def _residual_inputs(model, P, frequency):
+    """Z and b of the plate.msh port case the residual fixture was written on
+    (node 0 -> last node), from the oracle's matrices."""
+    Z = model.mna(model.inductance(), model.pat(P), frequency)
+    rhs = ck.oracle_rhs(model.ne, model.nv, P=P, Is=ck.port_current(model.nv, 0, model.nv - 1))
+    return Z, rhs
+
+
+def test_backward_error_matches_reference_golden(plate_msh):
+    """harmonic_compute_residual (harmonic.c:720-1080) of the reference, all
+    seven figures, on plate.msh port solutions at 1 kHz and 1 MHz as solved and
+    with every unknown perturbed by ~1e-6 relative (residual far above
+    rounding): the oracle reproduces them bit for bit."""
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    P = plate_msh.potential()
+    for f, eps, x, want in zip(g["frequency"], g["eps"], g["x"], g["residual"]):
+        Z, rhs = _residual_inputs(plate_msh, P, float(f))
+        got = ck.oracle_backward_error(Z, x, rhs, plate_msh.ne)
+        assert np.array_equal(got, want), (f, eps, got, want)
+        if eps > 0:
+            assert want[2] > 1e-7  # the perturbed cases sit far above rounding
+
+
+def test_backward_error_fixture_rejects_formula_mistakes(plate_msh):
+    """The perturbed fixture tells the reference's formula max_i |r_i| /
+    (sum_j |Z_ij||x_j| + |b_i|) apart from plausible mistakes: a dropped |b_i|,
+    the ratio of the maxima instead of the maximum of the ratios, |Zx| in
+    place of |Z||x|, and the edge/node split at the wrong row (each differs
+    from the reference's value by far more than rounding)."""
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    P = plate_msh.potential()
+    ne = plate_msh.ne
+    for f, eps, x, want in zip(g["frequency"], g["eps"], g["x"], g["residual"]):
+        if eps == 0:
+            continue
+        Z, b = _residual_inputs(plate_msh, P, float(f))
+        r = np.abs(Z @ x - b)
+        den = np.abs(Z) @ np.abs(x) + np.abs(b)
+        right = np.max(r / den)
+        assert abs(right - want[2]) <= 1e-9 * want[2]
+        assert abs(np.max((r / den)[ne:]) - want[6]) <= 1e-9 * want[6]
+        variants = {
+            # edge rows have b_i = 0: the dropped term shows in the node block
+            "no |b_i|": (np.max((r / (np.abs(Z) @ np.abs(x)))[ne:]), want[6]),
+            "ratio of maxima": (r.max() / den.max(), want[2]),
+            "|Zx| in place of |Z||x|": (np.max(r / (np.abs(Z @ x) + np.abs(b))), want[2]),
+        }
+        for name, (v, w) in variants.items():
+            assert abs(v - w) > 1e-3 * w, name
+        split = [np.max((r / den)[:ne + 1]), np.max((r / den)[ne + 1:])]
+        assert (split[0], split[1]) != (want[5], want[6])
+
+
+@pytest.mark.skipif(not ck.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("frequency", [1.0, 1.0e6])
+def test_backward_error_bit_exact_against_reference_build(frequency):
+    """The same comparison live against the compiled reference on a closed box
+    (every node row carries P I_s), as solved and perturbed."""
+    import tests.golden.make_golden as mg
+    xyz, quads = meshgen.box(1.0, 0.5, 0.5, 4, 2, 2)
+    m, d = mg.ref_model(xyz=xyz, quads=quads)
+    ne, nv = m.n_edges, m.n_nodes
+    L, P = mg.ref_LP(m, d)
+    Pn = api.as_array(P.data, nv * nv).reshape(nv, nv)
+    Ln = api.as_array(L.data, ne * ne).reshape(ne, ne)
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Z = o.mna(Ln, o.pat(Pn), frequency)
+    finally:
+        o.close()
+    b = ck.oracle_rhs(ne, nv, P=Pn, Is=ck.port_current(nv, 0, nv - 1))
+    st, fac = ck.oracle_complex_lu(Z)
+    assert st == 0
+    x = ck.oracle_complex_solve(fac, b)
+    ref = ck.ref()
+    sysm = api.HarmonicSystem()
+    ref.harmonic_system_init(C.byref(sysm))
+    sysm.n_edges, sysm.n_nodes, sysm.system_size = ne, nv, ne + nv
+    sysm.matrix.rows = sysm.matrix.cols = ne + nv
+    zbuf = api.complex_buffer(Z.ravel())
+    sysm.matrix.data = C.cast(zbuf, C.POINTER(api.Complex))
+    sysm.matrix_built = 1
+    Ib = api.complex_buffer(ck.port_current(nv, 0, nv - 1))
+    ex = api.HarmonicExcitation()
+    ex.n_edges, ex.n_nodes = ne, nv
+    ex.node_current = C.cast(Ib, C.POINTER(api.Complex))
+    for seed, eps in ((0, 0.0), (1, 1e-6), (2, 1e-3)):
+        xp = mg.perturbed(x, seed, eps) if eps else x
+        ib, vb = api.complex_buffer(xp[:ne]), api.complex_buffer(xp[ne:])
+        sol = api.HarmonicSolution()
+        sol.n_edges, sol.n_nodes = ne, nv
+        sol.edge_current = C.cast(ib, C.POINTER(api.Complex))
+        sol.node_voltage = C.cast(vb, C.POINTER(api.Complex))
+        res = api.HarmonicResidual()
+        assert ref.harmonic_compute_residual(C.byref(sysm), C.byref(P), C.byref(ex), C.byref(sol), 4,
+                                             C.byref(res)) == 0
+        want = np.array([getattr(res, k) for k in RESIDUAL_FIELDS])
+        got = ck.oracle_backward_error(Z, xp, b, ne)
+        assert np.array_equal(got, want), (eps, got, want)
+    ref.inductance_free(C.byref(L))
+    ref.potential_free(C.byref(P))
+
+
+def test_complex_lu_factor_matches_reference_golden():
+    """complex_lu_factorize (complex_lu.c:208-550) of the reference on a seeded
+    random matrix: the oracle's factor, pivots, row and column scales and
+    matrix scale are bit-identical."""
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    st, (LU, piv, rs, cs, ms) = ck.oracle_complex_lu(g["clu_A"])
+    assert st == 0
+    assert np.array_equal(LU, g["clu_factor"])
+    assert np.array_equal(piv.astype(np.int64), g["clu_pivots"])
+    assert np.array_equal(rs, g["clu_row_scale"]) and np.array_equal(cs, g["clu_column_scale"])
+    assert ms == float(g["clu_matrix_scale"])
+    assert int(np.sum(piv.astype(np.int64) != np.arange(len(piv)))) == int(g["clu_swap_count"])
+
+
+def test_real_lu_factor_matches_reference_golden():
+    """lu_factorize (lu.c:194-479) of the reference on a seeded random matrix."""
+    g = np.load(os.path.join(GOLD, "residual_lu.npz"))
+    st, LU, piv, ms = ck.oracle_real_lu(g["rlu_A"])
+    assert st == 0
+    assert np.array_equal(LU, g["rlu_factor"])
+    assert np.array_equal(piv, g["rlu_pivots"])
+    assert ms == float(g["rlu_matrix_scale"])
+
+
+@pytest.mark.skipif(not ck.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("n", [7, 65, 200])
+def test_lu_factors_bit_exact_against_reference_build(n, plate_msh):
+    """Live: complex and real LU of the compiled reference against the oracle
+    on seeded random matrices (n = 200 crosses the reference's OpenMP
+    threshold of 64 remaining rows) and, for the complex one, the plate.msh
+    MNA matrix at 1 kHz (badly scaled: jwL ~ 1e-6 against P ~ 1e10)."""
+    import tests.golden.make_golden as mg
+    mats = [mg.lu_test_matrix(n, n)]
+    if n == 200:
+        P = plate_msh.potential()
+        mats.append(plate_msh.mna(plate_msh.inductance(), plate_msh.pat(P), 1.0e3))
+    for A in mats:
+        want = mg.ref_complex_lu(A)
+        st, (LU, piv, rs, cs, ms) = ck.oracle_complex_lu(A)
+        assert want[0] == 0 and st == 0
+        assert np.array_equal(LU, want[1]) and np.array_equal(piv.astype(np.int64), want[2])
+        assert np.array_equal(rs, want[3]) and np.array_equal(cs, want[4]) and ms == want[5]
+    Ar = mg.lu_test_matrix(n, n + 1, complex_=False)
+    want = mg.ref_real_lu(Ar)
+    st, LU, piv, ms = ck.oracle_real_lu(Ar)
+    assert want[0] == 0 and st == 0
+    assert np.array_equal(LU, want[1]) and np.array_equal(piv, want[2]) and ms == want[3]
+
+
 # -------------------------------------------------------- compiled reference
 
 @pytest.mark.skipif(not ck.ref_available(), reason="oracle/_ref not built")
