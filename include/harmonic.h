@@ -89,6 +89,19 @@ int harmonic_compute_residual(const HarmonicSystem *system,
                               const HarmonicExcitation *excitation,
                               const HarmonicSolution *solution,
                               int parallel_threads, HarmonicResidual *residual);
+/* No counterpart in the reference. harmonic_compute_residual with the same
+ * arguments and results, for callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it makes no
+ * device or host allocation. Its inputs and partial results are staged in a
+ * fixed 32 MiB device array per GPU, so N = Ne + Nv is limited to about 1400
+ * (-1 with an ERROR line above that). Calls that overlap on one GPU share
+ * that array and race. */
+int harmonic_compute_residual_noalloc(const HarmonicSystem *system,
+                                      const PotentialMatrix *P,
+                                      const HarmonicExcitation *excitation,
+                                      const HarmonicSolution *solution,
+                                      int parallel_threads,
+                                      HarmonicResidual *residual);
 /* harmonic.h:373 */
 void harmonic_print_info(const HarmonicSystem *system);
 

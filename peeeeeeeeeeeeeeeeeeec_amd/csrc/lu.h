@@ -107,8 +107,9 @@ void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t 
 // y = A x for a complex FP64 rows x cols matrix (leading dimension lda),
 // and the reference's component-wise backward error terms per row:
 //   out[i] = (|r_i|, sum_j |A_ij||x_j| + |b_i|, |b_i|) with r = A x - b.
+// d_ax: cols doubles of workspace (|x_j|), or null to allocate them.
 void complex_residual(const double2 *dA, int rows, int cols, size_t lda, const double2 *dx, const double2 *db,
-                      double2 *dr, double *dterms, hipStream_t s);
+                      double2 *dr, double *dterms, hipStream_t s, double *d_ax = nullptr);
 
 // R = A X - B for nrhs right-hand sides (rhs-major) and each one's
 // component-wise backward error (harmonic.c:932-969) as the bit pattern of

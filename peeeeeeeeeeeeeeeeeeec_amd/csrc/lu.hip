@@ -2924,10 +2924,14 @@ __global__ void abs_kernel(const double2 *x, double *ax, int n) {
 }
 
 void complex_residual(const double2 *dA, int rows, int cols, size_t lda, const double2 *dx, const double2 *db,
-                      double2 *dr, double *dterms, hipStream_t s) {
-    DeviceBuffer<double> ax(cols);
-    hipLaunchKernelGGL(abs_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, dx, ax.ptr, cols);
-    hipLaunchKernelGGL(residual_kernel, dim3(rows), dim3(256), 0, s, dA, lda, cols, dx, ax.ptr, db, dr, dterms);
+                      double2 *dr, double *dterms, hipStream_t s, double *d_ax) {
+    DeviceBuffer<double> own;
+    if (!d_ax) {
+        own.resize(cols);
+        d_ax = own.ptr;
+    }
+    hipLaunchKernelGGL(abs_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, dx, d_ax, cols);
+    hipLaunchKernelGGL(residual_kernel, dim3(rows), dim3(256), 0, s, dA, lda, cols, dx, d_ax, db, dr, dterms);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
 }

@@ -100,7 +100,7 @@ EXPORTED = [
     # harmonic.h
     "harmonic_system_init", "harmonic_system_free", "harmonic_solution_init",
     "harmonic_solution_free", "harmonic_build_system", "harmonic_factorize", "harmonic_build_rhs",
-    "harmonic_solve", "harmonic_compute_residual", "harmonic_print_info",
+    "harmonic_solve", "harmonic_compute_residual", "harmonic_compute_residual_noalloc", "harmonic_print_info",
     # peec_gpu.h
     "peec_engine_options_default", "peec_engine_create", "peec_engine_create_with_matrices", "peec_engine_solve",
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
@@ -216,6 +216,9 @@ def load():
     _sig(lib, "harmonic_solve", C.c_int, C.POINTER(api.HarmonicSystem), C.POINTER(api.PotentialMatrix),
          C.POINTER(api.HarmonicExcitation), C.POINTER(api.HarmonicSolution), C.c_int)
     _sig(lib, "harmonic_compute_residual", C.c_int, C.POINTER(api.HarmonicSystem),
+         C.POINTER(api.PotentialMatrix), C.POINTER(api.HarmonicExcitation),
+         C.POINTER(api.HarmonicSolution), C.c_int, C.POINTER(api.HarmonicResidual))
+    _sig(lib, "harmonic_compute_residual_noalloc", C.c_int, C.POINTER(api.HarmonicSystem),
          C.POINTER(api.PotentialMatrix), C.POINTER(api.HarmonicExcitation),
          C.POINTER(api.HarmonicSolution), C.c_int, C.POINTER(api.HarmonicResidual))
     _sig(lib, "peec_engine_options_default", None, C.POINTER(PeecEngineOptions))
