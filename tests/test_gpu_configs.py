@@ -1,0 +1,256 @@
+"""The workload configurations at their own shapes on the GPU:
+
+* closed box 2 x 1 x 1 m with 5 cm quads (N = 12002), retarded: one point
+  from the stored moments and one by quadrature, mixed against FP64, exact
+  symmetry of Lp(w)/P(w), moments against quadrature at one wave number;
+* ragged batch (seeded plates/boxes of 300-3000 cells, 8 frequencies each)
+  through the request server, mixed and FP64, the smallest mesh against
+  the oracle;
+* serving: meshes of N ~ 500 / 3000 / 10500 with requests of 16-64 right-hand
+  sides (the blocked-solve path), each answer alone and in the mix, one
+  request against the oracle;
+* cylinder r = 1.5 m, length 20 m: a coarse mesh's monostatic RCS against
+  the oracle, and the full ~19000-quad body (N ~ 57000) with 360 incidences
+  on one factor, mixed against FP64.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from peeeeeeeeeeeeeeeeeeec_amd import lib as plib
+from peeeeeeeeeeeeeeeeeeec_amd import meshgen
+from tests import checker as ck
+
+pytestmark = pytest.mark.gpu
+C0 = 299792458.0
+
+
+def corner_port(xyz, lo, hi):
+    a = int(np.argmin(np.linalg.norm(xyz - np.asarray(lo), axis=1)))
+    b = int(np.argmin(np.linalg.norm(xyz - np.asarray(hi), axis=1)))
+    return a, b
+
+
+def extreme_corner_port(xyz):
+    s = xyz.sum(axis=1)
+    return int(np.argmin(s)), int(np.argmax(s))
+
+
+def oracle_port_z(o, f, ports, static=None):
+    """Retarded port solution through the oracle: Z_in of each (a, b)."""
+    Ls, Ps = static if static is not None else (o.inductance(), o.potential())
+    k = 2 * math.pi * f / C0
+    Lc, Pc = o.retarded(k, 2, Ls, Ps, 4)
+    Z = o.mna(None, None, f, Lc=Lc, Pc=Pc)
+    st, fac = ck.oracle_complex_lu(Z)
+    assert st == 0
+    out = []
+    for a, b in ports:
+        rhs = ck.oracle_rhs(o.ne, o.nv, Pc=Pc, Is=ck.port_current(o.nv, a, b))
+        x = ck.oracle_complex_solve(fac, rhs)
+        out.append(x[o.ne + a] - x[o.ne + b])
+    return np.array(out)
+
+
+# ----------------------------------------------------------------- box, N = 12002
+
+def test_box_full_size_points(require_gpu):
+    xyz, quads = meshgen.box(2.0, 1.0, 1.0, 40, 20, 20)
+    port = corner_port(xyz, (0, 0, 0), (2, 1, 1))
+    with plib.Model(xyz, quads) as m:
+        assert m.ne + m.nv == 12002
+        eng = plib.Engine(m, retarded=True)
+        for f, from_moments in ((1.0e6, True), (1.5e8, False)):
+            st64, z64, _, _, r64 = eng.solve(f, [port], plib.Engine.FP64, want_solution=False)
+            stm, zm, _, _, rm = eng.solve(f, [port], plib.Engine.MIXED, want_solution=False)
+            rel = abs(zm[0] - z64[0]) / abs(z64[0])
+            print(f"f={f:g}: Z64={z64[0]:.10e} rel(mixed)={rel:.2e} eta64={r64.backward_error:.1e} "
+                  f"etam={rm.backward_error:.1e} ir={rm.refinement_steps} moments={r64.moment_terms}/"
+                  f"{rm.moment_terms}", flush=True)
+            assert st64 == 0 and stm == 0
+            assert rel <= 1e-10
+            assert r64.backward_error <= 1e-10 and rm.backward_error <= 1e-10
+            assert (r64.moment_terms > 0) == from_moments and (rm.moment_terms > 0) == from_moments
+        k = 2 * math.pi * 3.0e7 / C0
+        Lm, Pm = eng.retarded_matrices(k)
+        Ls, Ps = eng.static_matrices()
+        eng.close()
+        assert np.array_equal(Lm, Lm.T) and np.array_equal(Pm, Pm.T)
+        quad = plib.Engine(m, retarded=True, static_matrices=(Ls, Ps), smooth_moments=0)
+        Lq, Pq = quad.retarded_matrices(k)
+        quad.close()
+    assert np.array_equal(Lq, Lq.T) and np.array_equal(Pq, Pq.T)
+    for a, q in ((Lm, Lq), (Pm, Pq)):
+        dr = np.abs(a.real - q.real).max() / np.abs(q.real).max()
+        di = np.abs(a.imag - q.imag).max() / np.abs(q.imag).max()
+        print(f"moments vs quadrature at k={k:.3f}: real {dr:.1e}, imag {di:.1e}", flush=True)
+        assert dr <= 1e-14 and di <= 1e-12
+
+
+# ----------------------------------------------------------------- ragged batch
+
+def test_ragged_batch_through_server(require_gpu):
+    """Four meshes of the seeded batch spanning its cell counts (smallest,
+    one third, two thirds, largest), all 8 frequencies each, queued at once
+    in both modes: every point passes the gate, mixed matches FP64 to 1e-10,
+    and the smallest mesh matches the oracle at two of its frequencies."""
+    batch = meshgen.ragged_batch(count=64, seed=20261203)
+    order = sorted(range(len(batch)), key=lambda i: len(batch[i]["quads"]))
+    pick = [order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]]
+    models = [plib.Model(batch[i]["xyz"], batch[i]["quads"]) for i in pick]
+    sv = plib.Server(slots=4, retarded=True)
+    ids = [sv.add_mesh(m) for m in models]
+    work = [(ids[j], f, [extreme_corner_port(batch[i]["xyz"])]) for j, i in enumerate(pick)
+            for f in batch[i]["frequencies"]]
+    res = {}
+    for mode in (plib.Server.MIXED, plib.Server.FP64):
+        tickets = [sv.submit(m, f, p, mode) for m, f, p in work]
+        res[mode] = [sv.wait(t) for t in tickets]
+    sv.close()
+    print("cells", [len(batch[i]["quads"]) for i in pick], "N", [m.ne + m.nv for m in models], flush=True)
+    for (m, f, p), a, b in zip(work, res[plib.Server.MIXED], res[plib.Server.FP64]):
+        assert a[0] == 0 and b[0] == 0, (m, f)
+        assert a[2].report.backward_error <= 1e-10 and b[2].report.backward_error <= 1e-10
+        assert abs(a[1][0] - b[1][0]) <= 1e-10 * abs(b[1][0]), (m, f)
+    small = batch[pick[0]]
+    o = ck.OracleModel(small["xyz"], small["quads"])
+    try:
+        static = (o.inductance(), o.potential())
+        for q in (0, 5):
+            f = small["frequencies"][q]
+            z_o = oracle_port_z(o, f, [extreme_corner_port(small["xyz"])], static)[0]
+            z64 = res[plib.Server.FP64][q][1][0]
+            zm = res[plib.Server.MIXED][q][1][0]
+            print(f"smallest mesh f={f:g}: oracle {z_o:.10e} fp64 rel {abs(z64 - z_o) / abs(z_o):.1e} "
+                  f"mixed rel {abs(zm - z_o) / abs(z_o):.1e}", flush=True)
+            assert abs(z64 - z_o) <= 1e-9 * abs(z_o) and abs(zm - z_o) <= 1e-9 * abs(z_o)
+    finally:
+        o.close()
+    for m in models:
+        m.close()
+
+
+# ----------------------------------------------------------------- serving
+
+def test_serving_many_rhs_alone_and_in_the_mix(require_gpu):
+    """Meshes of N = 519, ~3000 and 10562; ten requests of 16-64 random port
+    pairs at log-uniform frequencies in 10 kHz-200 MHz, mixed and FP64. Each
+    answer is the same alone (one slot) and in the mix (four slots under a
+    budget of 1.5 times the largest engine, so engines are evicted), within
+    1e-10; mixed against FP64 within 1e-9; one request on the smallest mesh
+    against the oracle."""
+    meshes = [meshgen.plate(13, 12, 0.5, 0.45), meshgen.box_by_size(1.2, 0.6, 0.5, 0.057),
+              meshgen.box(2.0, 1.0, 0.8, 40, 20, 16)]
+    models = [plib.Model(*x) for x in meshes]
+    sizes = [m.ne + m.nv for m in models]
+    assert sizes[0] == 519 and 2500 < sizes[1] < 3500 and sizes[2] == 10562
+    rng = np.random.default_rng(20261204)
+    reqs = []
+    for k in range(10):
+        mi = k % 3
+        nv = models[mi].nv
+        f = float(10 ** rng.uniform(4.0, math.log10(2e8)))
+        nrhs = int(rng.integers(16, 65))
+        ports = [tuple(int(x) for x in rng.choice(nv, 2, replace=False)) for _ in range(nrhs)]
+        reqs.append((mi, f, ports, plib.Server.MIXED if k % 2 == 0 else plib.Server.FP64))
+
+    def run(slots, budget, together):
+        sv = plib.Server(slots=slots, hbm_budget_bytes=budget, retarded=True)
+        ids = [sv.add_mesh(m) for m in models]
+        if together:
+            out = [sv.wait(t) for t in [sv.submit(ids[m], f, p, md) for m, f, p, md in reqs]]
+        else:
+            out = [sv.wait(sv.submit(ids[m], f, p, md)) for m, f, p, md in reqs]
+        mem, _ = sv.memory()
+        sv.close()
+        return out, mem
+
+    alone, _ = run(1, 0, False)
+    budget = int(1.5 * max(plib.Server.engine_bytes(m.ne, m.nv, True) for m in models))
+    mixed, mem = run(4, budget, True)
+    assert mem <= budget
+    for (st_a, za, ra), (st_b, zb, rb), (mi, f, ports, mode) in zip(alone, mixed, reqs):
+        assert st_a == 0 and st_b == 0, (mi, f, mode)
+        assert ra.report.backward_error <= 1e-10 and rb.report.backward_error <= 1e-10
+        np.testing.assert_allclose(zb, za, rtol=1e-10, atol=0)
+    # mixed requests against FP64 re-solves of the same points
+    sv = plib.Server(slots=2, retarded=True)
+    ids = [sv.add_mesh(m) for m in models]
+    for (st, z, _), (mi, f, ports, mode) in zip(alone, reqs):
+        if mode == plib.Server.MIXED:
+            st64, z64, _ = sv.wait(sv.submit(ids[mi], f, ports, plib.Server.FP64))
+            assert st64 == 0 and np.max(np.abs(z - z64) / np.abs(z64)) <= 1e-9
+    sv.close()
+    mi, f, ports, _ = reqs[0]
+    assert mi == 0
+    o = ck.OracleModel(*meshes[0])
+    try:
+        z_o = oracle_port_z(o, f, ports)
+    finally:
+        o.close()
+    err = np.max(np.abs(alone[0][1] - z_o) / np.abs(z_o))
+    print(f"serving: N {sizes}, request 0 ({len(ports)} RHS at {f:g} Hz) vs oracle {err:.1e}", flush=True)
+    assert err <= 1e-9
+    for m in models:
+        m.close()
+
+
+# ----------------------------------------------------------------- cylinder
+
+def test_cylinder_coarse_rcs_against_oracle(require_gpu):
+    """The 20 m cylinder with a coarse O-grid (m = 3, 16 rings: 210 quads),
+    monostatic RCS at 100 MHz for four azimuths (theta 90, e_theta) on one
+    factor: FP64 against the oracle pipeline to 1e-9 in sigma, mixed against
+    FP64 to 1e-9 dB."""
+    from tests.test_gpu_rcs import _oracle_monostatic
+    xyz, quads = meshgen.cylinder(1.5, 20.0, 3, 16)
+    phis = np.array([0.0, 90.0, 180.0, 270.0])
+    thetas = np.full_like(phis, 90.0)
+    o = ck.OracleModel(xyz, quads)
+    try:
+        ro = _oracle_monostatic(o, 1.0e8, thetas, phis, 1)
+    finally:
+        o.close()
+    with plib.Model(xyz, quads) as m:
+        eng = plib.Engine(m, retarded=True, smooth_moments=0)
+        st64, r64, _, rep64 = eng.monostatic_rcs(1.0e8, thetas, phis, plib.Engine.FP64)
+        stm, rm, _, repm = eng.monostatic_rcs(1.0e8, thetas, phis, plib.Engine.MIXED)
+        eng.close()
+    assert st64 == 0 and stm == 0
+    s_o = np.array([r.sigma for r in ro])
+    s64 = np.array([r.sigma for r in r64])
+    sm = np.array([r.sigma for r in rm])
+    print(f"coarse cylinder: sigma dBsm {np.round(10 * np.log10(s64), 3)}, fp64 vs oracle "
+          f"{np.max(np.abs(s64 - s_o) / s_o):.1e}, mixed vs fp64 {np.max(np.abs(10 * np.log10(sm / s64))):.1e} dB",
+          flush=True)
+    assert np.max(np.abs(s64 - s_o) / s_o) <= 1e-9
+    assert np.max(np.abs(10 * np.log10(sm / s64))) <= 1e-9
+    assert rep64.backward_error <= 1e-10 and repm.backward_error <= 1e-10
+
+
+def test_cylinder_full_size_360_incidences(require_gpu):
+    """The full body (cap lattice 23 x 23, 195 rings: 18998 quads, N = 56996,
+    ~140 GB of device memory), 360 azimuths at 100 MHz on one factor, mixed
+    and FP64: both pass the gate and sigma agrees to 1e-6 dB."""
+    xyz, quads = meshgen.cylinder(1.5, 20.0, 23, 195)
+    phis = np.arange(360.0)
+    thetas = np.full_like(phis, 90.0)
+    with plib.Model(xyz, quads) as m:
+        assert len(quads) == 18998 and m.ne + m.nv == 56996
+        eng = plib.Engine(m, retarded=True, smooth_moments=0)
+        stm, rm, _, repm = eng.monostatic_rcs(1.0e8, thetas, phis, plib.Engine.MIXED)
+        print(f"mixed: status {stm} eta {repm.backward_error:.1e} steps {repm.refinement_steps} factor "
+              f"{repm.ms_factor / 1e3:.1f} s solve {repm.ms_solve / 1e3:.1f} s", flush=True)
+        st64, r64, _, rep64 = eng.monostatic_rcs(1.0e8, thetas, phis, plib.Engine.FP64)
+        print(f"fp64: status {st64} eta {rep64.backward_error:.1e} factor {rep64.ms_factor / 1e3:.1f} s", flush=True)
+        eng.close()
+    assert stm == 0 and st64 == 0
+    assert repm.backward_error <= 1e-10 and rep64.backward_error <= 1e-10
+    sm = np.array([r.sigma for r in rm])
+    s64 = np.array([r.sigma for r in r64])
+    db = np.abs(10 * np.log10(sm / s64))
+    print(f"sigma mixed vs fp64: max {db.max():.1e} dB", flush=True)
+    assert db.max() <= 1e-6
