@@ -44,14 +44,15 @@ def corner_nodes(xyz, lo, hi):
 
 def cpu_baseline():
     """The reference's own per-point pipeline (compiled from its sources into
-    oracle/_ref) on a reduced box (h = 10 cm: 1000 cells, N = 3002), timed on
-    all host cores: harmonic_build_system, harmonic_factorize (complex LU),
-    harmonic_solve and harmonic_compute_residual for one port at 1 MHz. Its
-    static Lp/P come from the GPU engine (equal to the reference's to 1e-12)
-    because assembling them on the CPU takes longer than the sample budget and
-    is once per mesh. The per-point time is scaled by (12002/3002)^3 (the LU
-    dominates) to the benchmark size. Without the reference build, the same
-    sequence runs through the oracle port."""
+    oracle/_ref) at the benchmark size, timed on all host cores:
+    harmonic_build_system, harmonic_factorize (equilibrated complex LU),
+    harmonic_solve and harmonic_compute_residual for the benchmark port at
+    1 MHz on the quasi-static 2 x 1 x 1 m box with 5 cm quads (N = 12002).
+    Its static Lp/P come from the GPU engine (equal to the reference's to
+    1e-12): assembling them on the CPU is once per mesh and takes far longer
+    than the sample budget; the reference has no retarded kernel, so per-point
+    assembly is absent from the figure. Without the reference build the same
+    sequence runs through the oracle."""
     import ctypes as C
 
     from peeeeeeeeeeeeeeeeeeec_amd import ctypes_api as api
@@ -59,40 +60,40 @@ def cpu_baseline():
     from tests import checker as ck
 
     cores = len(os.sched_getaffinity(0))
-    xyz, quads = meshgen.box(2.0, 1.0, 1.0, 20, 10, 10)
+    xyz, quads = meshgen.box(2.0, 1.0, 1.0, 40, 20, 20)
     a, b = corner_nodes(xyz, (0, 0, 0), (2, 1, 1))
     f = 1.0e6
     with lib.Model(xyz, quads) as gm:
-        eng = lib.Engine(gm)
+        eng = lib.Engine(gm, smooth_moments=0)
         Ls, Ps = eng.static_matrices()
         eng.close()
-    n_small = Ls.shape[0] + Ps.shape[0]
+    n = Ls.shape[0] + Ps.shape[0]
     if ck.ref_available():
         import tests.golden.make_golden as mg
         kind = "reference"
         m, d = mg.ref_model(xyz=xyz, quads=quads)
         L = api.InductanceMatrix(n=Ls.shape[0], data=api.c_malloc_copy(Ls, C.c_double))
         P = api.PotentialMatrix(n=Ps.shape[0], data=api.c_malloc_copy(Ps, C.c_double))
+        del Ls, Ps
         t0 = time.perf_counter()
-        mg.ref_port_solve(m, d, L, P, f, a, b, threads=cores)
+        _, _, _, eta = mg.ref_port_solve(m, d, L, P, f, a, b, threads=cores)
         t_point = time.perf_counter() - t0
     else:
         kind = "port"
         o = ck.OracleModel(xyz, quads)
         t0 = time.perf_counter()
-        ck.oracle_port_solve(o, Ls, Ps, f, a, b)
+        _, _, _, etas = ck.oracle_port_solve(o, Ls, Ps, f, a, b)
         t_point = time.perf_counter() - t0
+        eta = etas[2]
         o.close()
-    scale = (12002 / n_small) ** 3
     return {
-        "value": 1.0 / (t_point * scale),
+        "value": 1.0 / t_point,
         "unit": "freq-point solves/s",
         "cores": cores,
         "kind": kind,
-        "sample": (f"quasi-static box 2x1x1 m at h=10 cm (1000 cells, N={n_small}), one 1 MHz port point: "
-                   f"MNA build + complex LU + solve + backward error measured {t_point:.2f} s on {cores} cores, "
-                   f"scaled by (12002/{n_small})^3 to N=12002; the reference has no retarded kernel, so its "
-                   f"per-point assembly is absent from this figure"),
+        "sample": (f"one full-size point: quasi-static box 2x1x1 m at h=5 cm (N={n}), 1 MHz port, MNA build + "
+                   f"equilibrated complex LU + solve + backward error ({eta:.1e}) in {t_point:.1f} s on {cores} host "
+                   f"threads; static Lp/P supplied (once per mesh), no retarded kernel in the reference"),
     }
 
 
@@ -352,6 +353,8 @@ def main():
     ap.add_argument("--requests", type=int, default=24, help="serving: requests per rate")
     ap.add_argument("--fp64-sample", type=int, default=8, help="serving: requests re-solved in FP64")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--refinement-target", type=float, default=None,
+                    help="sweep: mixed-mode refinement stops at this backward error (engine default if omitted)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -383,7 +386,8 @@ def main():
     model = lib.Model(xyz, quads)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    eng = lib.Engine(model, retarded=True)
+    eng_opts = {} if args.refinement_target is None else {"refinement_target": args.refinement_target}
+    eng = lib.Engine(model, retarded=True, **eng_opts)
     torch.cuda.synchronize()
     t_static = time.perf_counter() - t0
     n = model.ne + model.nv
@@ -477,6 +481,7 @@ def main():
         "impedance_rel_error_vs_fp64": {"max": float(zerr.max()), "median": float(np.median(zerr))},
         "backward_error_max": float(max(r.backward_error for r in reports)),
         "refinement_steps_mean": float(np.mean([r.refinement_steps for r in reports])),
+        "refinement_steps": [int(r.refinement_steps) for r in reports],
         "fp64_fallbacks": int(sum(r.used_fp64_fallback for r in reports)),
         "pivot_order_reused": int(sum(r.pivot_reuse and not r.pivot_refactor for r in reports)),
         "pivot_order_refactors": int(sum(r.pivot_refactor for r in reports)),

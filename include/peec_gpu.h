@@ -50,11 +50,14 @@ typedef struct
     int block_size;             /* LU panel width, 0 = automatic (autotuned for the
                                    reused-row-order factor, see PeecPointReport) */
     int max_refinement_steps;   /* mixed mode cap, default 30 */
-    double refinement_target;   /* mixed mode stops at this backward error, default 1e-14 */
+    double refinement_target;   /* mixed mode stops at this backward error, default 1e-12
+                                   (a hundredth of the gate; the FP64 LU itself reaches
+                                   ~1e-12 on systems of N ~ 10^4) */
     double backward_error_gate; /* a point is accepted at or below this, default 1e-10 */
     int reuse_pivots;           /* mixed mode: factor in the row order of the previous
                                    point (no pivot search); refactors with partial
-                                   pivoting when refinement does not reach 1e-3 x gate.
+                                   pivoting when refinement does not reach
+                                   max(1e-3 x gate, 10 x refinement_target).
                                    Default 1 */
     int smooth_moments;         /* retarded sweeps: terms of the power series of the
                                    smooth remainder, (exp(-jkR)-1)/R = sum_m (-jk)^m

@@ -132,7 +132,8 @@ enum KernelKind {
     kKernelFarTile = 3,
     kKernelSmoothMoments = 4,
     kKernelLuUpdateSplit16 = 5,
-    kKernelKinds = 6
+    kKernelLuUpdateThin = 6,  // first panel of a pair: update of the next panel's rows only
+    kKernelKinds = 7
 };
 
 struct KernelTotals {

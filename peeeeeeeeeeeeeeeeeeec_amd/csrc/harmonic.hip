@@ -542,7 +542,10 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
     double worst = 0.0;
     if (mode == PEEC_SOLVE_MIXED) {
         worst = refine();
-        if (e->ordered && !(worst <= 1e-3 * e->opt.backward_error_gate)) {
+        // a reused row order is kept when refinement reaches ten times its
+        // target (and at least 1e-3 of the gate): beyond that it stalled
+        const double keep = std::fmax(1e-3 * e->opt.backward_error_gate, 10.0 * e->opt.refinement_target);
+        if (e->ordered && !(worst <= keep)) {
             // the reused row order is not good enough at this point:
             // refactor with partial pivoting and refine again
             report.pivot_refactor = 1;
@@ -611,7 +614,7 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->near_smooth_order = 4;
     o->block_size = 0;
     o->max_refinement_steps = 30;
-    o->refinement_target = 1e-14;
+    o->refinement_target = 1e-12;
     o->backward_error_gate = 1e-10;
     o->reuse_pivots = 1;
     o->smooth_moments = -1;

@@ -1508,82 +1508,93 @@ __device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
     lo = (_Float16)(x - (float)hi);
 }
 
-// One wavefront per 16-row block of L21 = A[row0 + r][k0 + k], r < m (rows
-// past m are packed as zeros), k < kw (a multiple of 16, at most 128).
-__device__ __forceinline__ void pack_l_h3_wave(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
-                                               int rb_pad, uint4 *out, float *inv, int rb, int lane) {
-    const int r = rb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
+// Split-FP16 packing of one wavefront's 16 rows of L21 (16 columns of U12):
+// lane l holds row (column) l & 15 and complex k = 16 c + 4 (l >> 4) .. + 3 of
+// chunk c. `load(c, t)` fetches element t of the lane's quarter of chunk c;
+// NEG flips the imaginary part (L21 is packed as the pairs (Ar, -Ai)). The
+// scale covers all kc chunks; at most 8 chunks are held in registers (a
+// wider pack takes the maximum over all chunks while holding the upper
+// eight, packs those, then re-loads and packs the lower ones).
+template <bool NEG>
+__device__ __forceinline__ void split_store_chunk(const float2 (&v)[4], float sc, uint4 *o) {
+    unsigned hw[4], lw[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        _Float16 xh, xl, yh, yl;
+        split_half(v[t].x * sc, xh, xl);
+        split_half((NEG ? -v[t].y : v[t].y) * sc, yh, yl);
+        hw[t] = pack_half2(xh, yh);
+        lw[t] = pack_half2(xl, yl);
+    }
+    o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+template <int KCM, bool NEG, class Load, class Out>
+__device__ __forceinline__ float pack_h3_lanes(int kc, Load load, Out out) {
+    static_assert(KCM == 8 || KCM == 16, "8 or 16 chunks");
     float2 v[8][4];
     float mx = 0.f;
+    auto fetch = [&](int c0) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v[c][t] = (c < kc && r < m) ? A[(size_t)(row0 + r) * lda + k0 + 16 * c + 4 * q + t] : make_float2(0.f, 0.f);
-            mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
-        }
+            for (int t = 0; t < 4; ++t) {
+                v[c][t] = c0 + c < kc ? load(c0 + c, t) : make_float2(0.f, 0.f);
+                mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
+            }
+    };
+    auto emit = [&](int c0, float sc) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c0 + c < kc) split_store_chunk<NEG>(v[c], sc, out(c0 + c));
+    };
+    if constexpr (KCM == 16) fetch(0);  // maximum only: re-loaded below
+    fetch(KCM - 8);
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
     const float sc = split_scale(mx);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        if (c >= kc) break;
-        unsigned hw[4], lw[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            _Float16 xh, xl, yh, yl;
-            split_half(v[c][t].x * sc, xh, xl);
-            split_half(-v[c][t].y * sc, yh, yl);
-            hw[t] = pack_half2(xh, yh);
-            lw[t] = pack_half2(xl, yl);
-        }
-        uint4 *o = out + ((size_t)(c * rb_pad + rb) * 2) * 64 + lane;
-        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    emit(KCM - 8, sc);
+    if constexpr (KCM == 16) {
+        fetch(0);
+        emit(0, sc);
     }
+    return sc;
+}
+
+// One wavefront per 16-row block of L21 = A[row0 + r][k0 + k], r < m (rows
+// past m are packed as zeros), k < kw (a multiple of 16, at most 16 KCM).
+template <int KCM>
+__device__ __forceinline__ void pack_l_h3_wave(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
+                                               int rb_pad, uint4 *out, float *inv, int rb, int lane) {
+    const int r = rb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
+    const bool in = r < m;
+    const float2 *src = A + (size_t)(row0 + (in ? r : 0)) * lda + k0 + 4 * q;
+    const float sc = pack_h3_lanes<KCM, true>(
+        kc, [&](int c, int t) { return in ? src[16 * c + t] : make_float2(0.f, 0.f); },
+        [&](int c) { return out + ((size_t)(c * rb_pad + rb) * 2) * 64 + lane; });
     if (q == 0) inv[r] = 1.f / sc;
 }
 
 // One wavefront per 16-column block of U12 = A[k0 + k][col0 + c], c < ncols
 // (columns past ncols packed as zeros), as pairs (Br, Bi).
+template <int KCM>
 __device__ __forceinline__ void pack_u_h3_wave(const float2 *A, size_t lda, int k0, int kw, int col0, int ncols,
                                                int cb_count, uint4 *out, float *inv, int cb, int lane) {
     const int col = cb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
-    float2 v[8][4];
-    float mx = 0.f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v[c][t] = (c < kc && col < ncols) ? A[(size_t)(k0 + 16 * c + 4 * q + t) * lda + col0 + col]
-                                             : make_float2(0.f, 0.f);
-            mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
-        }
-    mx = fmaxf(mx, __shfl_xor(mx, 16));
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    const float sc = split_scale(mx);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        if (c >= kc) break;
-        unsigned hw[4], lw[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            _Float16 xh, xl, yh, yl;
-            split_half(v[c][t].x * sc, xh, xl);
-            split_half(v[c][t].y * sc, yh, yl);
-            hw[t] = pack_half2(xh, yh);
-            lw[t] = pack_half2(xl, yl);
-        }
-        uint4 *o = out + ((size_t)(c * cb_count + cb) * 2) * 64 + lane;
-        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-    }
+    const bool in = col < ncols;
+    const float2 *src = A + (size_t)(k0 + 4 * q) * lda + col0 + (in ? col : 0);
+    const float sc = pack_h3_lanes<KCM, false>(
+        kc, [&](int c, int t) { return in ? src[(size_t)(16 * c + t) * lda] : make_float2(0.f, 0.f); },
+        [&](int c) { return out + ((size_t)(c * cb_count + cb) * 2) * 64 + lane; });
     if (q == 0) inv[col] = 1.f / sc;
 }
 
-// Both packs of the next block column's update in one launch: the first
-// lgroups workgroups pack L21 (one wavefront per 16-row block), the others
-// U12 of the next block column (one wavefront per 16-column block).
+// Both packs of an update in one launch: the first lgroups workgroups pack
+// L21 (one wavefront per 16-row block), the others U12 (one wavefront per
+// 16-column block); k runs over kw <= 16 KCM columns of L21 / rows of U12
+// from k0 (the per-row / per-column scales cover all of them).
+template <int KCM>
 __global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
                                                          int rb_pad, uint4 *lout, float *linv, int col0, int ncols,
                                                          int cb_count, uint4 *uout, float *uinv, int lgroups,
@@ -1592,10 +1603,10 @@ __global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t
     const int lane = threadIdx.x & 63;
     if ((int)blockIdx.x < lgroups) {
         const int rb = blockIdx.x * 4 + (threadIdx.x >> 6);
-        if (rb < rb_pad) pack_l_h3_wave(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
+        if (rb < rb_pad) pack_l_h3_wave<KCM>(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
     } else {
         const int cb = (blockIdx.x - lgroups) * 4 + (threadIdx.x >> 6);
-        if (cb < cb_count) pack_u_h3_wave(A, lda, k0, kw, col0, ncols, cb_count, uout, uinv, cb, lane);
+        if (cb < cb_count) pack_u_h3_wave<KCM>(A, lda, k0, kw, col0, ncols, cb_count, uout, uinv, cb, lane);
     }
 }
 
@@ -1752,8 +1763,9 @@ __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t 
     gemm_tile_h3_body<KC, MB>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
 }
 
-// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 128
-// or 64 columns of L21); `next`: the next block column's update.
+// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 64,
+// 128 or 256 columns of L21); kinds other than the trailing update run under
+// the next-column kernel's name.
 static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
                     const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
                     hipStream_t s) {
@@ -1763,15 +1775,18 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
     const float *ls = ctx.scale_l16.ptr;
-    const bool next = kind == kKernelLuUpdate;
-    if (kw == 128)
+    const bool next = kind != kKernelLuUpdateSplit16;
+    if (kw == 256)
+        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<16, 2> : gemm_tile_h3_kernel<16, 2>), dim3(grid), dim3(256),
+                           0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
+    else if (kw == 128)
         hipLaunchKernelGGL((next ? gemm_next_h3_kernel<8, 2> : gemm_tile_h3_kernel<8, 2>), dim3(grid), dim3(256), 0,
                            s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
     else if (kw == 64)
         hipLaunchKernelGGL((next ? gemm_next_h3_kernel<4, 2> : gemm_tile_h3_kernel<4, 2>), dim3(grid), dim3(256), 0,
                            s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
     else
-        throw HipError{"split-FP16 update needs a block width of 64 or 128"};
+        throw HipError{"split-FP16 update needs a k width of 64, 128 or 256"};
     HIP_CHECK(hipGetLastError());
     if (ctx.timer)
         ctx.timer->stop(tg, s, kind, 8.0 * m * ncols * kw,
@@ -2197,7 +2212,9 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
     };
     diag(0, std::min(nb, n), s);
-    bool pending = false;  // a diagonal factor is in flight on aux
+    bool pending = false;    // a diagonal factor is in flight on aux
+    bool pair_open = false;  // the previous panel began a pair of split-FP16 panels
+    int wprev = 0;
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int w = std::min(nb, n - k0), rest = n - k0 - w;
         if (rest <= 0) {
@@ -2223,10 +2240,18 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         gemm_pads<S>(rest, rest, rb_pad, cb_pad);
         const int rest2 = rest - w2;
         if (split16 && (w == 128 || w == 64)) {
-            // split-FP16 operands for both updates: L21 packed once, U12 of
-            // the next block column into its own small buffer (critical
-            // path), then the rest of U12 off the critical path
-            const int kc = w / 16, cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
+            // Split-FP16 updates, in pairs of block columns (the first panel of
+            // the matrix starts a pair). The first panel of a pair updates the
+            // next block column (look-ahead) and, of the rest of the matrix,
+            // only the next panel's w2 rows (the thin update: they become that
+            // panel's U12); the second applies both panels at once, rank 2w,
+            // to its next block column and to the trailing matrix, so the
+            // trailing matrix is read and written once per pair. L21 and U12
+            // stay in A between the two panels and are packed together with
+            // one scale per row / column over all 2w of k.
+            const bool second = pair_open && wprev == w;
+            const int kp = second ? k0 - wprev : k0, kw = second ? 2 * w : w, kc = kw / 16;
+            const int cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
             ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
             ctx.scale_l16.reserve((size_t)rb_pad * 16);
             ctx.pack_u16n.reserve((size_t)kc * cbn * 128 * sizeof(uint4));
@@ -2234,14 +2259,20 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             solve_l21(k0, w, rest);
             if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
             if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
-            {
-                const int lgroups = (rb_pad + 3) / 4, ugroups = (cbn + 3) / 4;
-                hipLaunchKernelGGL(pack_lu_h3_kernel, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA, lda,
-                                   r0, rest, k0, w, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
-                                   ctx.scale_l16.ptr, r0, w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr),
-                                   ctx.scale_u16n.ptr, lgroups, ctx.status.ptr);
-            }
-            gemm_h3(dA, lda, r0, r0, rest, w2, w, rb_pad, cbn, reinterpret_cast<const uint4 *>(ctx.pack_u16n.ptr),
+            auto pack = [&](int m_rows, int lgroups, int col0, int ncols, int cbc, uint4 *uo, float *ui) {
+                const int ugroups = (cbc + 3) / 4;
+                if (kw > 128)
+                    hipLaunchKernelGGL(pack_lu_h3_kernel<16>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
+                                       lda, r0, m_rows, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
+                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, ctx.status.ptr);
+                else
+                    hipLaunchKernelGGL(pack_lu_h3_kernel<8>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
+                                       lda, r0, m_rows, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
+                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, ctx.status.ptr);
+            };
+            // L21 (rows r0..) and U12 of the next block column: critical path
+            pack(rest, (rb_pad + 3) / 4, r0, w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr), ctx.scale_u16n.ptr);
+            gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, reinterpret_cast<const uint4 *>(ctx.pack_u16n.ptr),
                     ctx.scale_u16n.ptr, kKernelLuUpdate, ctx.status.ptr, ctx, s);
             HIP_CHECK(hipEventRecord(ctx.ev_next, s));
             HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
@@ -2249,18 +2280,35 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
             pending = true;
             if (rest2 > 0) {
-                // the rest of U12, packed by the solve itself (blocks past
-                // the last column are never read into a stored result)
                 ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
                 ctx.scale_u16.reserve((size_t)cb16 * 16);
-                solve_u12(k0, w, r0 + w2, n, s, reinterpret_cast<uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr, cb16);
-                gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, w, rb_pad, cb16,
-                        reinterpret_cast<const uint4 *>(ctx.pack_u16.ptr), ctx.scale_u16.ptr, kKernelLuUpdateSplit16,
-                        ctx.status.ptr, ctx, s);
+                uint4 *pu = reinterpret_cast<uint4 *>(ctx.pack_u16.ptr);
+                if (!second) {
+                    // the rest of U12, packed by the solve itself (blocks past
+                    // the last column are never read into a stored result);
+                    // then only the next panel's rows
+                    solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16);
+                    gemm_h3(dA, lda, r0, r0 + w2, w2, rest2, w, rb_pad, cb16, pu, ctx.scale_u16.ptr,
+                            kKernelLuUpdateThin, ctx.status.ptr, ctx, s);
+                } else {
+                    // the rest of this panel's U12, then both panels' U12 rows
+                    // packed together, and the rank-2w trailing update
+                    solve_u12(k0, w, r0 + w2, n, s);
+                    pack(rest, 0, r0 + w2, rest2, cb16, pu, ctx.scale_u16.ptr);
+                    gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, ctx.scale_u16.ptr,
+                            kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s);
+                }
             }
+            pair_open = !second;
+            wprev = w;
             HIP_CHECK(hipGetLastError());
             continue;
         }
+        // a pair's second panel always takes the split-FP16 branch (every
+        // panel but the last has the full width, and the last has no rows
+        // below it): the first panel's thin update left the rows below the
+        // pair without its contribution
+        if (pair_open) throw HipError{"ordered factor: a split-FP16 panel pair was left unfinished"};
         ctx.pack_l.reserve((size_t)(ks_count + 2) * rb_pad * 64 * sizeof(S));
         ctx.pack_u.reserve((size_t)(ks_count + 2) * cb_pad * 64 * sizeof(S));
         S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);

@@ -61,7 +61,7 @@ class PeecKernelStats(C.Structure):
 
 
 KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4,
-           "lu_update_split16": 5}
+           "lu_update_split16": 5, "lu_update_thin": 6}
 
 
 class RetardationOptions(C.Structure):
