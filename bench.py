@@ -372,23 +372,23 @@ def main():
                                    "(independent frequency points per GPU)", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import torch
-
     from peeeeeeeeeeeeeeeeeeec_amd import lib, meshgen
 
-    assert torch.cuda.is_available(), "bench needs a GPU"
-    torch.cuda.set_device(0)
+    # the library's own HIP runtime only (no second GPU runtime in the
+    # process): device 0, synchronised with hipDeviceSynchronize
+    assert lib.device_count() > 0, "bench needs a GPU"
+    sync = lib.device_synchronize
     h = args.cells
     nx, ny, nz = int(round(2 / h)), int(round(1 / h)), int(round(1 / h))
     xyz, quads = meshgen.box(2.0, 1.0, 1.0, nx, ny, nz)
     port = corner_nodes(xyz, (0, 0, 0), (2, 1, 1))
     freqs = sweep_frequencies()
     model = lib.Model(xyz, quads)
-    torch.cuda.synchronize()
+    sync()
     t0 = time.perf_counter()
     eng_opts = {} if args.refinement_target is None else {"refinement_target": args.refinement_target}
     eng = lib.Engine(model, retarded=True, **eng_opts)
-    torch.cuda.synchronize()
+    sync()
     t_static = time.perf_counter() - t0
     n = model.ne + model.nv
     n_cells, n_e, n_v = int(model.mesh.n_quads), model.ne, model.nv
@@ -403,7 +403,7 @@ def main():
     print(f"[bench] mesh N={n} static assembly {t_static:.2f} s; timing {len(timed)} mixed points", flush=True)
 
     eng.set_profiling(True)
-    torch.cuda.synchronize()
+    sync()
     lat, zm, reports = [], [], []
     t_start = time.perf_counter()
     for f in timed:
@@ -414,7 +414,7 @@ def main():
             raise SystemExit(f"mixed point f={f} failed with status {st}")
         zm.append(z[0])
         reports.append(rep)
-    torch.cuda.synchronize()
+    sync()
     t_mixed = time.perf_counter() - t_start
     gemm = eng.kernel_stats("lu_update")
     g16 = eng.kernel_stats("lu_update_split16")
@@ -427,7 +427,7 @@ def main():
     # plain FP64 LU on the same points
     eng.set_profiling(True)
     z64, lat64 = [], []
-    torch.cuda.synchronize()
+    sync()
     t_start = time.perf_counter()
     for f in timed:
         t1 = time.perf_counter()
@@ -436,7 +436,7 @@ def main():
         if st != 0:
             raise SystemExit(f"fp64 point f={f} failed with status {st}")
         z64.append(z[0])
-    torch.cuda.synchronize()
+    sync()
     t_fp64 = time.perf_counter() - t_start
     gemm64 = eng.kernel_stats("lu_update")
     eng.set_profiling(False)

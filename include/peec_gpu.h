@@ -67,10 +67,11 @@ typedef struct
                                    series needs at most that many terms for 1e-17 of
                                    every entry (k * mesh diameter small enough) is
                                    assembled from them in one streaming pass, the
-                                   others by quadrature. 0 = off, 1..32 = count,
-                                   -1 (default) = 26 terms when they fit in a quarter
-                                   of the device memory left free by the engine's
-                                   per-point working set, else fewer (off below 8) */
+                                   others by quadrature. 0 = off, 1..56 = count,
+                                   -1 (default) = 56 terms (k * diameter up to ~10.5)
+                                   when they fit in a quarter of the device memory
+                                   left free by the engine's per-point working set,
+                                   else fewer (off below 8) */
     int shifted_moments;        /* retarded points above the range of smooth_moments:
                                    complex moments around a shifted wave number k0,
                                    (exp(-jkR)-1)/R = (exp(-jk0R)-1)/R + sum_{m>=1}
@@ -236,6 +237,13 @@ int peec_engine_set_profiling(PeecEngine *engine, int enable);
 int peec_engine_kernel_stats(const PeecEngine *engine, int kernel, PeecKernelStats *stats);
 
 void peec_engine_destroy(PeecEngine *engine);
+
+/* No counterpart in the reference. Visible HIP devices (0 without a GPU),
+ * and a device-wide synchronisation of the current device (the bracket a
+ * benchmark puts around its timed region), so that callers need no second
+ * GPU runtime in the process. 0 on success, -1 on a HIP error. */
+int peec_device_count(void);
+int peec_device_synchronize(void);
 
 /* Library self-test: runs the 16x16x4 FP64 MFMA on known operands and
  * returns every lane's accumulators (FP64 then FP32, 1024 doubles) so the operand layout the

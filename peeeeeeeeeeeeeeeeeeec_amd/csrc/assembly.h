@@ -52,7 +52,7 @@ class RegionSet {
     // 64 x 64 tiles of the upper triangle in original numbering; a point
     // whose series needs no more terms than stored is then one streaming pass
     // over the moments instead of a quadrature pass.
-    static constexpr int kMaxMomentTerms = 32;
+    static constexpr int kMaxMomentTerms = 56;
     void build_moments(int terms, hipStream_t s);
     int moment_terms() const { return moment_terms_; }
     static size_t moment_bytes(int regions, int terms);

@@ -1594,16 +1594,18 @@ __device__ __forceinline__ void pack_u_h3_wave(const float2 *A, size_t lda, int 
 // L21 (one wavefront per 16-row block), the others U12 (one wavefront per
 // 16-column block); k runs over kw <= 16 KCM columns of L21 / rows of U12
 // from k0 (the per-row / per-column scales cover all of them).
+// The L21 wavefronts take row blocks rb0, rb0 + 1, ... (of the rows from
+// row0) up to rb_end.
 template <int KCM>
 __global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
                                                          int rb_pad, uint4 *lout, float *linv, int col0, int ncols,
-                                                         int cb_count, uint4 *uout, float *uinv, int lgroups,
-                                                         const int *status) {
+                                                         int cb_count, uint4 *uout, float *uinv, int lgroups, int rb0,
+                                                         int rb_end, const int *status) {
     if (status[0] != 0) return;
     const int lane = threadIdx.x & 63;
     if ((int)blockIdx.x < lgroups) {
-        const int rb = blockIdx.x * 4 + (threadIdx.x >> 6);
-        if (rb < rb_pad) pack_l_h3_wave<KCM>(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
+        const int rb = rb0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (rb < rb_end) pack_l_h3_wave<KCM>(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
     } else {
         const int cb = (blockIdx.x - lgroups) * 4 + (threadIdx.x >> 6);
         if (cb < cb_count) pack_u_h3_wave<KCM>(A, lda, k0, kw, col0, ncols, cb_count, uout, uinv, cb, lane);
@@ -1768,13 +1770,14 @@ __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t 
 // the next-column kernel's name.
 static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
                     const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
-                    hipStream_t s) {
+                    hipStream_t s, int rb_first = 0) {
     if (m <= 0 || ncols <= 0) return;
-    const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr);
+    // rows from row0 are L21 row blocks rb_first.. of the pack
+    const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr) + (size_t)rb_first * 128;
     const int tiles = ((m + 63) / 64) * ((ncols + 127) / 128);
     const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-    const float *ls = ctx.scale_l16.ptr;
+    const float *ls = ctx.scale_l16.ptr + (size_t)rb_first * 16;
     const bool next = kind != kKernelLuUpdateSplit16;
     if (kw == 256)
         hipLaunchKernelGGL((next ? gemm_next_h3_kernel<16, 2> : gemm_tile_h3_kernel<16, 2>), dim3(grid), dim3(256),
@@ -1974,8 +1977,8 @@ __global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t l
 // 16 wavefronts stage the triangle once in LDS and walk their share of the
 // rows (columns).
 template <class S>
-__global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda, int n, int k0, int w,
-                                                               const int *status) {
+__global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda, int k0, int w, int r_begin,
+                                                               int r_end, const int *status) {
     using T = typename Work<S>::T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     S *blk = reinterpret_cast<S *>(smem_raw);  // U11 row-major, w x w
@@ -2004,8 +2007,8 @@ __global__ __launch_bounds__(1024) void trsm_right_wave_kernel(S *A, size_t lda,
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ca = lane, cb = lane + 64;
     const int cla = min(ca, w - 1), clb = min(cb, w - 1);
-    const int nrows = n - k0 - w;
-    for (int rr = (int)blockIdx.x * 16 + wave; rr < nrows; rr += (int)gridDim.x * 16) {
+    // rows k0 + w + r_begin .. k0 + w + r_end - 1 of L21
+    for (int rr = r_begin + (int)blockIdx.x * 16 + wave; rr < r_end; rr += (int)gridDim.x * 16) {
         S *row = A + (size_t)(k0 + w + rr) * lda + k0;
         T x0 = ca < w ? wld(row + ca) : wzero<T>();
         T x1 = cb < w ? wld(row + cb) : wzero<T>();
@@ -2192,10 +2195,12 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     const bool split16 = ctx.split_fp16_update;
     // triangular solves: one wavefront per row of L21 / column of U12, at
     // most one workgroup of 16 per CU
-    auto solve_l21 = [&](int k0, int w, int rest) {
-        const int grid = std::max(1, std::min((rest + 15) / 16, ctx.cus));
+    // rows r_begin .. r_end - 1 below the diagonal block (all of them by default)
+    auto solve_l21 = [&](int k0, int w, int rest, int r_begin = 0, int r_end = -1) {
+        if (r_end < 0) r_end = rest;
+        const int grid = std::max(1, std::min((r_end - r_begin + 15) / 16, ctx.cus));
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
-                           dA, lda, n, k0, w, ctx.status.ptr);
+                           dA, lda, k0, w, r_begin, r_end, ctx.status.ptr);
     };
     // with pk, U12 is also packed as the split-FP16 operand of pcb blocks
     auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st, uint4 *pk = nullptr, float *pinv = nullptr,
@@ -2256,29 +2261,63 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             ctx.scale_l16.reserve((size_t)rb_pad * 16);
             ctx.pack_u16n.reserve((size_t)kc * cbn * 128 * sizeof(uint4));
             ctx.scale_u16n.reserve((size_t)cbn * 16);
-            solve_l21(k0, w, rest);
-            if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
-            if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
-            auto pack = [&](int m_rows, int lgroups, int col0, int ncols, int cbc, uint4 *uo, float *ui) {
-                const int ugroups = (cbc + 3) / 4;
+            // L21 row blocks [rb_a, rb_b) (of the rows from r0) and, with cbc,
+            // U12 of columns col0.. in one launch
+            auto pack = [&](int rb_a, int rb_b, int col0, int ncols, int cbc, uint4 *uo, float *ui) {
+                const int lgroups = (rb_b - rb_a + 3) / 4, ugroups = (cbc + 3) / 4;
+                if (lgroups + ugroups == 0) return;
                 if (kw > 128)
                     hipLaunchKernelGGL(pack_lu_h3_kernel<16>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
-                                       lda, r0, m_rows, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
-                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, ctx.status.ptr);
+                                       lda, r0, rest, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
+                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, rb_a, rb_b,
+                                       ctx.status.ptr);
                 else
                     hipLaunchKernelGGL(pack_lu_h3_kernel<8>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
-                                       lda, r0, m_rows, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
-                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, ctx.status.ptr);
+                                       lda, r0, rest, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
+                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, rb_a, rb_b,
+                                       ctx.status.ptr);
             };
-            // L21 (rows r0..) and U12 of the next block column: critical path
-            pack(rest, (rb_pad + 3) / 4, r0, w2, cbn, reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr), ctx.scale_u16n.ptr);
-            gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, reinterpret_cast<const uint4 *>(ctx.pack_u16n.ptr),
-                    ctx.scale_u16n.ptr, kKernelLuUpdate, ctx.status.ptr, ctx, s);
-            HIP_CHECK(hipEventRecord(ctx.ev_next, s));
-            HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
-            diag(r0, w2, aux);
-            HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
-            pending = true;
+            uint4 *pun = reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr);
+            auto start_next_diag = [&]() {
+                HIP_CHECK(hipEventRecord(ctx.ev_next, s));
+                HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
+                diag(r0, w2, aux);
+                HIP_CHECK(hipEventRecord(ctx.ev_diag, aux));
+                pending = true;
+            };
+            if (!second && w2 % 16 == 0) {
+                // first panel of a pair (nothing long follows it on this
+                // stream): the next diagonal block first (its rows of L21,
+                // U12 of its columns, the w2 x w2 update), so that its factor
+                // on the second stream overlaps the rest of L21 and of the
+                // next block column's update. The split falls on a 16-row
+                // block of the L21 pack (a last, narrower next panel takes
+                // the path below).
+                const int rbd = w2 / 16;
+                solve_l21(k0, w, rest, 0, w2);
+                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
+                if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
+                pack(0, rbd, r0, w2, cbn, pun, ctx.scale_u16n.ptr);
+                gemm_h3(dA, lda, r0, r0, w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
+                        ctx.status.ptr, ctx, s);
+                start_next_diag();
+                if (rest > w2) {
+                    solve_l21(k0, w, rest, w2, rest);
+                    pack(rbd, rb_pad, 0, 0, 0, nullptr, nullptr);
+                    gemm_h3(dA, lda, r0 + w2, r0, rest - w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr,
+                            kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd);
+                }
+            } else {
+                // L21 (rows r0..) and U12 of the next block column, then its
+                // update (rank 2w for a pair's second panel; critical path)
+                solve_l21(k0, w, rest);
+                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
+                if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
+                pack(0, rb_pad, r0, w2, cbn, pun, ctx.scale_u16n.ptr);
+                gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
+                        ctx.status.ptr, ctx, s);
+                start_next_diag();
+            }
             if (rest2 > 0) {
                 ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
                 ctx.scale_u16.reserve((size_t)cb16 * 16);
@@ -2294,7 +2333,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                     // the rest of this panel's U12, then both panels' U12 rows
                     // packed together, and the rank-2w trailing update
                     solve_u12(k0, w, r0 + w2, n, s);
-                    pack(rest, 0, r0 + w2, rest2, cb16, pu, ctx.scale_u16.ptr);
+                    pack(0, 0, r0 + w2, rest2, cb16, pu, ctx.scale_u16.ptr);
                     gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, ctx.scale_u16.ptr,
                             kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s);
                 }

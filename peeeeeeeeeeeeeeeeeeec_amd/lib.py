@@ -106,7 +106,7 @@ EXPORTED = [
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
     "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats",
-    "peec_selftest_mfma_layout", "peec_rcs_options_default", "peec_engine_monostatic_rcs",
+    "peec_device_count", "peec_device_synchronize", "peec_selftest_mfma_layout", "peec_rcs_options_default", "peec_engine_monostatic_rcs",
     # incident_field.h / scattering.h / rcs.h
     "incident_field_init", "incident_field_configure_harmonic", "incident_field_configure_pulse",
     "incident_field_evaluate_time", "incident_field_evaluate_phasor", "incident_field_print_info",
@@ -245,6 +245,8 @@ def load():
     _sig(lib, "peec_server_engine_bytes", C.c_size_t, C.c_size_t, C.c_size_t, C.c_int)
     _sig(lib, "peec_engine_set_profiling", C.c_int, C.c_void_p, C.c_int)
     _sig(lib, "peec_selftest_mfma_layout", C.c_int, _dp)
+    _sig(lib, "peec_device_count", C.c_int)
+    _sig(lib, "peec_device_synchronize", C.c_int)
     _sig(lib, "peec_engine_kernel_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(PeecKernelStats))
     _F = C.POINTER(api.IncidentField)
     _sig(lib, "incident_field_configure_harmonic", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -502,6 +504,17 @@ def real_lu_factor_solve(A, B, want_factor=False):
                api.as_array(lu.pivots, n, np.uint64).astype(np.int64))
     lib.lu_free(C.byref(lu))
     return st, X, det
+
+
+def device_count():
+    """Visible HIP devices, through the library's own runtime."""
+    return load().peec_device_count()
+
+
+def device_synchronize():
+    """hipDeviceSynchronize on the current device."""
+    if load().peec_device_synchronize() != 0:
+        raise RuntimeError("peec_device_synchronize failed")
 
 
 def complex_matvec(A, x):

@@ -205,6 +205,38 @@ def test_retarded_from_shifted_moments_matches_quadrature(require_gpu):
         o.close()
 
 
+def test_retarded_from_many_moments_matches_quadrature(require_gpu):
+    """56 stored terms on the 0.825 m box cover k * diameter up to ~10.5: at
+    k = 11.5 (k * diameter = 9.5) the alternating series loses up to
+    e^9.5 ~ 1.3e4 to cancellation, so the moment path matches the quadrature
+    pass to ~1e-12 (5e-12 allowed, against the largest real / imaginary part)
+    and the oracle to 1e-11; at k = 3 as closely as with 26 terms."""
+    xyz, quads = meshgen.box(0.6, 0.4, 0.4, 6, 4, 4)
+    ks = [3.0, 11.5]
+    with plib.Model(xyz, quads) as m:
+        mo = plib.Engine(m, retarded=True, smooth_moments=56, shifted_moments=0)
+        qd = plib.Engine(m, retarded=True, smooth_moments=0, shifted_moments=0)
+        res = [(k, mo.retarded_matrices(k), qd.retarded_matrices(k)) for k in ks]
+        mo.close()
+        qd.close()
+    o = ck.OracleModel(xyz, quads)
+    try:
+        Ls, Ps = o.inductance(), o.potential()
+        for k, a, b in res:
+            Lo, Po = o.retarded(k, 2, Ls, Ps, 4)
+            tol = 1e-14 if k * 0.825 <= 2.56 else 5e-12
+            for name, x, y, c in (("L", a[0], b[0], Lo), ("P", a[1], b[1], Po)):
+                e_re = np.abs(x.real - y.real).max() / np.abs(y.real).max()
+                e_im = np.abs(x.imag - y.imag).max() / np.abs(y.imag).max()
+                e_or = np.abs(x - c).max() / np.abs(c).max()
+                print(f"k={k} {name}: 56 moments vs quadrature re {e_re:.2e} im {e_im:.2e}; vs oracle {e_or:.2e}",
+                      flush=True)
+                assert np.array_equal(x, x.T)
+                assert e_re <= tol and e_im <= max(tol, 1e-12) and e_or <= 1e-11
+    finally:
+        o.close()
+
+
 def test_engine_points_from_shifted_moments(require_gpu):
     """A sweep above the range of the moments at 0: the points report the
     shifted center they used and give the impedances of an engine without
@@ -215,7 +247,7 @@ def test_engine_points_from_shifted_moments(require_gpu):
     res = {}
     with plib.Model(xyz, quads) as m:
         for on in (1, 0):
-            eng = plib.Engine(m, retarded=True, smooth_moments=-on, shifted_moments=-on)
+            eng = plib.Engine(m, retarded=True, smooth_moments=26 if on else 0, shifted_moments=-on)
             res[on] = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
             eng.close()
     centers = [r[4].moment_center for r in res[1]]
