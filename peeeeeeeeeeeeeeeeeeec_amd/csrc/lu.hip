@@ -1624,15 +1624,19 @@ __global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t
 // Per chunk and 16 x 16 block: three MFMAs for the real and three for the
 // imaginary part. The update uses MB = 2 (64 x 128 tiles): rows of C are
 // then read and written in 1 KB runs.
-template <int KC, int MB>
+// MB x NB blocks per wave (2 x 2 waves: tiles of 32 MB x 32 NB); ND staging
+// register sets (chunks loaded ahead).
+template <int KC, int MB, int NB, int ND>
 __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
                                                   const uint4 *__restrict__ Lh, const uint4 *__restrict__ Uh,
                                                   const float *__restrict__ linv, const float *__restrict__ uinv,
                                                   int rb_pad, int cb_count, const int *status) {
-    constexpr int NB = 8 / MB, BM = 32 * MB, BN = 32 * NB;
-    constexpr int CL = MB * NB * 4;
+    constexpr int BM = 32 * MB, BN = 32 * NB;
     constexpr int UA = 4 * MB, UNITS = UA + 4 * NB;  // L21 units, all units per chunk
-    static_assert(KC % 2 == 0 && KC >= 2 && CL % KC == 0, "chunks alternate between two LDS slots");
+    // C is read and written EG block rows of the wave at a time (at most 32
+    // values in registers beside the accumulators)
+    constexpr int EG = MB * NB <= 8 ? MB : 8 / NB, CL = EG * NB * 4;
+    static_assert(KC % 2 == 0 && KC >= 2 && MB % EG == 0, "chunks alternate between two LDS slots");
     __shared__ u32x4_t slots_lds[2 * UNITS * 64];
     if (status[0] != 0) return;
     int tm, tn;
@@ -1644,7 +1648,6 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
     // wave w loads L21 units MB w .. MB w + MB - 1 and U12 units NB w ..
     const u32x4_t *as = Lv + (size_t)(tm * (BM / 8) + wave * MB) * 64 + lane;
     const u32x4_t *bs = Uv + (size_t)(tn * (BN / 8) + wave * NB) * 64 + lane;
-    constexpr int ND = KC >= 4 ? 4 : 2;  // staging register sets: chunks in flight
     static_assert(KC % ND == 0, "a tile's chunks must cycle the staging sets exactly");
     u32x4_t ga[ND][MB], gb[ND][NB];
     auto load_g = [&](auto set, int c) {
@@ -1676,8 +1679,7 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
             cim[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
         }
     float2 cv[CL];
-    auto c_ptr = [&](int e) -> float2 * {
-        const int i = e / (NB * 4), j = (e / 4) % NB, v = e % 4;
+    auto c_ptr = [&](int i, int j, int v) -> float2 * {
         const int row = min((rb0 + i) * 16 + 4 * lk + v, m - 1);
         const int col = min((cb0 + j) * 16 + li, ncols - 1);
         return A + (size_t)(row0 + row) * lda + col0 + col;
@@ -1720,49 +1722,55 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
             __syncthreads();
         }
     });
-#pragma unroll
-    for (int e = 0; e < CL; ++e) cv[e] = *c_ptr(e);
     float fc[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) fc[j] = uinv[(cb0 + j) * 16 + li];
 #pragma unroll
-    for (int i = 0; i < MB; ++i)
+    for (int g = 0; g < MB; g += EG) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = (rb0 + i) * 16 + 4 * lk + v;
-            const float fr = linv[row];
+        for (int i = 0; i < EG; ++i)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const int col = (cb0 + j) * 16 + li;
-                const int e = (i * NB + j) * 4 + v;
-                const float f = fr * fc[j];
-                float2 r = cv[e];
-                r.x -= cre[i][j][v] * f;
-                r.y -= cim[i][j][v] * f;
-                if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cv[(i * NB + j) * 4 + v] = *c_ptr(g + i, j, v);
+#pragma unroll
+        for (int i = 0; i < EG; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = (rb0 + g + i) * 16 + 4 * lk + v;
+                const float fr = linv[row];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const int col = (cb0 + j) * 16 + li;
+                    const float f = fr * fc[j];
+                    float2 r = cv[(i * NB + j) * 4 + v];
+                    r.x -= cre[g + i][j][v] * f;
+                    r.y -= cim[g + i][j][v] * f;
+                    if (row < m && col < ncols) A[(size_t)(row0 + row) * lda + col0 + col] = r;
+                }
             }
-        }
+    }
 }
 
 // The trailing update, and the same code under its own name for the next
 // block column's update (a separate entry in kernel traces).
-template <int KC, int MB>
+template <int KC, int MB, int NB, int ND>
 __global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
                                                               int ncols, const uint4 *__restrict__ Lh,
                                                               const uint4 *__restrict__ Uh,
                                                               const float *__restrict__ linv,
                                                               const float *__restrict__ uinv, int rb_pad, int cb_count,
                                                               const int *status) {
-    gemm_tile_h3_body<KC, MB>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+    gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
 }
-template <int KC, int MB>
+template <int KC, int MB, int NB, int ND>
 __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
                                                               int ncols, const uint4 *__restrict__ Lh,
                                                               const uint4 *__restrict__ Uh,
                                                               const float *__restrict__ linv,
                                                               const float *__restrict__ uinv, int rb_pad, int cb_count,
                                                               const int *status) {
-    gemm_tile_h3_body<KC, MB>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+    gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
 }
 
 // A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 64,
@@ -1774,22 +1782,28 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     if (m <= 0 || ncols <= 0) return;
     // rows from row0 are L21 row blocks rb_first.. of the pack
     const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr) + (size_t)rb_first * 128;
-    const int tiles = ((m + 63) / 64) * ((ncols + 127) / 128);
-    const int grid = (tiles + 7) / 8 * 8;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
     const float *ls = ctx.scale_l16.ptr + (size_t)rb_first * 16;
     const bool next = kind != kKernelLuUpdateSplit16;
+    // 64 x 128 tiles (128 x 128 tiles, with a third less operand traffic per
+    // output, measured no faster: the update is bound by latency, not by
+    // operand bandwidth)
+    const int BM = 64, BN = 128;
+    const int tiles = ((m + BM - 1) / BM) * ((ncols + BN - 1) / BN);
+    const int grid = (tiles + 7) / 8 * 8;
+#define PEEC_H3(KC, MB, NB, ND)                                                                                     \
+    hipLaunchKernelGGL((next ? gemm_next_h3_kernel<KC, MB, NB, ND> : gemm_tile_h3_kernel<KC, MB, NB, ND>),          \
+                       dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, \
+                       status)
     if (kw == 256)
-        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<16, 2> : gemm_tile_h3_kernel<16, 2>), dim3(grid), dim3(256),
-                           0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
+        PEEC_H3(16, 2, 4, 4);
     else if (kw == 128)
-        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<8, 2> : gemm_tile_h3_kernel<8, 2>), dim3(grid), dim3(256), 0,
-                           s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
+        PEEC_H3(8, 2, 4, 4);
     else if (kw == 64)
-        hipLaunchKernelGGL((next ? gemm_next_h3_kernel<4, 2> : gemm_tile_h3_kernel<4, 2>), dim3(grid), dim3(256), 0,
-                           s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, status);
+        PEEC_H3(4, 2, 4, 4);
     else
         throw HipError{"split-FP16 update needs a k width of 64, 128 or 256"};
+#undef PEEC_H3
     HIP_CHECK(hipGetLastError());
     if (ctx.timer)
         ctx.timer->stop(tg, s, kind, 8.0 * m * ncols * kw,
@@ -1933,8 +1947,10 @@ __global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t l
         __syncthreads();
         if (ti > q && tc > q && ti < nt && tc < nt) {
             // rank-4 update of the trailing tile, k = 0..3 outermost as in
-            // the column-by-column elimination
-#pragma unroll
+            // the column-by-column elimination; one k's operands live at a
+            // time (all four hoisted exceed the 128 registers a thread of a
+            // 1024-thread workgroup has, and spill)
+#pragma unroll 1
             for (int k = 0; k < 4; ++k) {
                 float lr[4], li[4], ur[4], ui[4];
 #pragma unroll
