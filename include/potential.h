@@ -1,5 +1,5 @@
 /* Potential-coefficient matrix P over mesh nodes, drop-in for the
- * reference's include/potential.h:15-241 (assembly part). */
+ * reference's include/potential.h:15-241. */
 #ifndef PEEC_POTENTIAL_H
 #define PEEC_POTENTIAL_H
 
@@ -43,6 +43,16 @@ int potential_compute_matrix(const Mesh *mesh, const DualMesh *dual,
                              PotentialMatrix *P);
 /* potential.h:162 */
 void potential_print_info(const PotentialMatrix *P);
+/* potential.h:185-188 (potential.c): F[j] = 1 / P_jj, the self capacitances
+ * (-1 for a non-positive or non-finite diagonal) */
+int potential_build_self_capacitance(const PotentialMatrix *P, double *F);
+/* potential.h:204-208: charge[j] = V_c[j] / P_jj */
+int potential_self_voltage_to_charge(const PotentialMatrix *P, const double *V_c,
+                                     double *charge);
+/* potential.h:234-238: phi = P diag(1/P_aa) V_c, the full node potentials of
+ * the self voltages (Nv x Nv product on the GPU) */
+int potential_self_voltage_to_full_potential(const PotentialMatrix *P,
+                                             const double *V_c, double *phi);
 
 #ifdef __cplusplus
 }

@@ -1443,3 +1443,116 @@ int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentF
     out->sigma_dbsm = out->sigma > DBL_MIN ? 10.0 * log10(out->sigma) : -INFINITY;
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* theta-scheme transient (transient.c:334-1749, stages 1 and 2)        */
+/* ------------------------------------------------------------------ */
+
+static double oracle_theta(int scheme)
+{
+    return scheme == 1 ? 1.0 : scheme == 2 ? 0.5 : NAN;
+}
+
+int oracle_transient_matrix(const double *L, const double *R, const double *A, const double *P, size_t ne,
+                            size_t nv, int scheme, double dt, long shunt_a, long shunt_b, double r_shunt,
+                            double *M, double *F, double *AS, double *bR, double *c)
+{
+    if (!L || !R || !A || !P || !M || ne == 0 || nv == 0 || !isfinite(dt) || dt <= 0.0) return -1;
+    if (scheme != 1 && scheme != 2) return -1;
+    const int st2 = shunt_b >= 0;
+    const double theta = oracle_theta(scheme);
+    const size_t nt = ne + nv + (st2 ? 1 : 0);
+    double *f = malloc(sizeof(double) * nv), *as = malloc(sizeof(double) * ne * nv);
+    double *b = calloc(nv, sizeof(double)), *cc = calloc(nv, sizeof(double));
+    int rc = -1;
+    if (!f || !as || !b || !cc) goto done;
+    /* F_j = 1 / P_jj (potential_build_self_capacitance) */
+    for (size_t j = 0; j < nv; ++j) {
+        const double d = P[j * nv + j];
+        if (!isfinite(d) || d <= 0.0) goto done;
+        f[j] = 1.0 / d;
+    }
+    /* AS[e,a] = (P[plus,a] - P[minus,a]) / P_aa, the last -1 / +1 of a row */
+    for (size_t e = 0; e < ne; ++e) {
+        size_t mi = (size_t)-1, pl = (size_t)-1;
+        for (size_t j = 0; j < nv; ++j) {
+            const double a = A[e * nv + j];
+            if (a == -1.0) mi = j;
+            else if (a == 1.0) pl = j;
+            else if (a != 0.0) goto done;
+        }
+        if (mi == (size_t)-1 || pl == (size_t)-1) goto done;
+        for (size_t a = 0; a < nv; ++a) as[e * nv + a] = (P[pl * nv + a] - P[mi * nv + a]) / P[a * nv + a];
+    }
+    if (st2) {
+        if (shunt_a < 0 || (size_t)shunt_a >= nv || (size_t)shunt_b >= nv || shunt_a == shunt_b) goto done;
+        b[shunt_a] = -1.0;
+        b[shunt_b] = +1.0;
+        for (size_t a = 0; a < nv; ++a)
+            cc[a] = (P[(size_t)shunt_b * nv + a] - P[(size_t)shunt_a * nv + a]) / P[a * nv + a];
+    }
+    memset(M, 0, sizeof(double) * nt * nt);
+    for (size_t i = 0; i < ne; ++i)
+        for (size_t j = 0; j < ne; ++j) {
+            double v = L[i * ne + j] / dt;
+            if (i == j) v += theta * R[i];
+            M[i * nt + j] = v;
+        }
+    for (size_t e = 0; e < ne; ++e)
+        for (size_t j = 0; j < nv; ++j) M[e * nt + ne + j] = theta * as[e * nv + j];
+    for (size_t j = 0; j < nv; ++j)
+        for (size_t e = 0; e < ne; ++e) M[(ne + j) * nt + e] = -theta * A[e * nv + j];
+    for (size_t r = 0; r < nv; ++r)
+        for (size_t col = 0; col < nv; ++col) {
+            double v = 0.0;
+            if (r == col) v += f[r];
+            M[(ne + r) * nt + ne + col] = v / dt;
+        }
+    if (st2) {
+        const size_t s = ne + nv;
+        for (size_t j = 0; j < nv; ++j) M[(ne + j) * nt + s] = -theta * b[j];
+        for (size_t j = 0; j < nv; ++j) M[s * nt + ne + j] = cc[j];
+        M[s * nt + s] = r_shunt;
+    }
+    if (F) memcpy(F, f, sizeof(double) * nv);
+    if (AS) memcpy(AS, as, sizeof(double) * ne * nv);
+    if (bR) memcpy(bR, b, sizeof(double) * nv);
+    if (c) memcpy(c, cc, sizeof(double) * nv);
+    rc = 0;
+done:
+    free(f);
+    free(as);
+    free(b);
+    free(cc);
+    return rc;
+}
+
+int oracle_transient_rhs(const double *L, const double *R, const double *A, const double *F, const double *AS,
+                         size_t ne, size_t nv, int scheme, double dt, const double *I, const double *V,
+                         const double *sn, const double *snp1, const double *un, const double *unp1,
+                         const double *bR, double ishunt, double *rhs)
+{
+    if (!L || !R || !A || !F || !AS || !I || !V || !rhs || (scheme != 1 && scheme != 2)) return -1;
+    const double theta = oracle_theta(scheme), omt = 1.0 - theta;
+    for (size_t i = 0; i < ne; ++i) {
+        double v = 0.0;
+        for (size_t j = 0; j < ne; ++j) v += L[i * ne + j] * I[j] / dt;
+        v -= omt * R[i] * I[i];
+        for (size_t j = 0; j < nv; ++j) v -= omt * AS[i * nv + j] * V[j];
+        if (un) v -= omt * un[i];
+        if (unp1) v -= theta * unp1[i];
+        rhs[i] = v;
+    }
+    for (size_t j = 0; j < nv; ++j) {
+        double v = F[j] * V[j] / dt;
+        for (size_t e = 0; e < ne; ++e) v += omt * A[e * nv + j] * I[e];
+        if (sn) v += omt * sn[j];
+        if (snp1) v += theta * snp1[j];
+        rhs[ne + j] = v;
+    }
+    if (bR) {
+        for (size_t j = 0; j < nv; ++j) rhs[ne + j] += omt * bR[j] * ishunt;
+        rhs[ne + nv] = 0.0;
+    }
+    return 0;
+}

@@ -161,4 +161,22 @@ int oracle_radiation_vector(const Mesh *mesh, const DualMesh *dual, const Comple
 int oracle_rcs_direction(const Mesh *mesh, const DualMesh *dual, const IncidentField *field, const Complex *I,
                          int integration, int order, double theta_deg, double phi_deg, RcsResult *result);
 
+/* ---- theta-scheme transient (solver/transient.c stages 1 and 2) ---- */
+/* transient.c:334-383, incidence.c:593-720, potential.c (self capacitance),
+ * transient.c:940-1018 (shunt operators, shunt_b < 0: stage 1) and
+ * transient.c:1020-1419: the nt x nt system matrix (nt = ne + nv, + 1 with a
+ * shunt) of the scheme (1 backward Euler, 2 trapezoidal); F (nv), AS
+ * (ne x nv), b_R and c (nv, stage 2) are returned when non-null. A is the
+ * dense ne x nv incidence matrix. -1 on invalid input. */
+int oracle_transient_matrix(const double *L, const double *R, const double *A, const double *P, size_t ne,
+                            size_t nv, int scheme, double dt, long shunt_a, long shunt_b, double r_shunt,
+                            double *M, double *F, double *AS, double *bR, double *c);
+/* transient.c:1421-1749: the right-hand side of the step n -> n+1 (optional
+ * sources NULL); with bR non-null the stage-2 shunt terms are added and
+ * rhs[ne + nv] = 0. */
+int oracle_transient_rhs(const double *L, const double *R, const double *A, const double *F, const double *AS,
+                         size_t ne, size_t nv, int scheme, double dt, const double *I, const double *V,
+                         const double *sn, const double *snp1, const double *un, const double *unp1,
+                         const double *bR, double ishunt, double *rhs);
+
 #endif

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpeec_mi355x.so")
 SOURCES = ["geometry.cpp", "assembly.hip", "lu.hip", "harmonic.hip", "rcs.hip", "lightning.cpp", "mtx.cpp", "dropin.cpp",
-           "serve.cpp", "noalloc.hip"]
+           "serve.cpp", "noalloc.hip", "transient.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-x", "hip", "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

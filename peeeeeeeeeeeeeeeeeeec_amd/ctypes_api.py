@@ -119,6 +119,25 @@ class ComplexLUFactorization(C.Structure):
                 ("row_scale", _dp), ("column_scale", _dp), ("factorized", C.c_int)]
 
 
+class InternalCircuitShunt(C.Structure):
+    """internal_circuit.h"""
+    _fields_ = [("type", C.c_int), ("node_a", C.c_size_t), ("node_b", C.c_size_t), ("resistance", C.c_double),
+                ("requested_a", C.c_double * 3), ("requested_b", C.c_double * 3), ("mapped_a", C.c_double * 3),
+                ("mapped_b", C.c_double * 3), ("mapping_error_a", C.c_double), ("mapping_error_b", C.c_double),
+                ("nodes_mapped", C.c_int), ("initialized", C.c_int)]
+
+
+class TransientSystem(C.Structure):
+    """transient.h"""
+    _fields_ = [("stage", C.c_int), ("scheme", C.c_int), ("n_edges", C.c_size_t), ("n_nodes", C.c_size_t),
+                ("n_total", C.c_size_t), ("dt", C.c_double), ("L", C.c_void_p), ("R", C.c_void_p),
+                ("A", C.c_void_p), ("P", C.c_void_p), ("F", _dp), ("AS", Matrix), ("shunt", C.c_void_p),
+                ("b_R", _dp), ("shunt_voltage_operator", _dp), ("slot_circuit", C.c_void_p),
+                ("n_slot_cells", C.c_size_t), ("slot_mass_matrix", _dp), ("slot_voltage_operators", _dp),
+                ("system_matrix", _dp), ("rhs", _dp), ("solution", _dp), ("lu", LUFactorization),
+                ("factorized", C.c_int), ("initialized", C.c_int)]
+
+
 class HarmonicSystem(C.Structure):
     _fields_ = [("n_edges", C.c_size_t), ("n_nodes", C.c_size_t),
                 ("system_size", C.c_size_t), ("frequency", C.c_double),

@@ -82,11 +82,25 @@ EXPORTED = [
     "inductance_options_default", "inductance_init", "inductance_free", "inductance_compute_pair",
     "inductance_compute_matrix", "inductance_print_info",
     "potential_options_default", "potential_init", "potential_free", "potential_compute_pair",
-    "potential_compute_matrix", "potential_print_info",
+    "potential_compute_matrix", "potential_print_info", "potential_build_self_capacitance",
+    "potential_self_voltage_to_charge", "potential_self_voltage_to_full_potential",
     # retardation.h
     "retardation_options_default", "retardation_compute_matrices",
     # incidence.h / resistance.h
-    "incidence_init", "incidence_free", "incidence_build", "incidence_build_PAT",
+    "incidence_init", "incidence_free", "incidence_build", "incidence_build_PAT", "incidence_build_AS",
+    "incidence_build_APAT", "incidence_check_row_sum", "incidence_print_info",
+    # internal_circuit.h / transient.h
+    "internal_circuit_shunt_default", "internal_circuit_shunt_set_nodes", "internal_circuit_shunt_map_coordinates",
+    "internal_circuit_find_nearest_node", "internal_circuit_shunt_validate",
+    "internal_circuit_build_incidence_vector", "internal_circuit_shunt_voltage",
+    "internal_circuit_shunt_current_from_voltage", "internal_circuit_shunt_residual",
+    "internal_circuit_shunt_print_info",
+    "transient_system_init", "transient_system_free", "transient_system_create_stage1",
+    "transient_system_create_stage2", "transient_build_shunt_operators", "transient_build_system_matrix",
+    "transient_slot_current_offset", "transient_factorize", "transient_build_rhs_stage1",
+    "transient_build_rhs_stage2", "transient_solve_step", "transient_current_offset", "transient_voltage_offset",
+    "transient_shunt_index", "transient_extract_stage1_solution", "transient_extract_stage2_solution",
+    "transient_recover_node_state", "transient_print_info",
     "resistance_options_default", "resistance_init", "resistance_free", "resistance_compute_matrix",
     "resistance_get",
     # matrix.h / complex_matrix.h
@@ -193,6 +207,38 @@ def load():
     _sig(lib, "lu_factorize", C.c_int, _dp, C.c_size_t, C.POINTER(api.LUOptions),
          C.POINTER(api.LUFactorization))
     _sig(lib, "lu_solve", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp)
+    _PM, _IM, _MX = C.POINTER(api.PotentialMatrix), C.POINTER(api.IncidenceMatrix), C.POINTER(api.Matrix)
+    _SH, _TS = C.POINTER(api.InternalCircuitShunt), C.POINTER(api.TransientSystem)
+    _sig(lib, "potential_build_self_capacitance", C.c_int, _PM, _dp)
+    _sig(lib, "potential_self_voltage_to_charge", C.c_int, _PM, _dp, _dp)
+    _sig(lib, "potential_self_voltage_to_full_potential", C.c_int, _PM, _dp, _dp)
+    _sig(lib, "incidence_build_AS", C.c_int, _IM, _PM, _MX, C.c_int)
+    _sig(lib, "incidence_build_APAT", C.c_int, _M, _PM, _MX, C.c_int)
+    _sig(lib, "incidence_check_row_sum", C.c_double, _IM)
+    _sig(lib, "internal_circuit_shunt_default", None, _SH)
+    _sig(lib, "internal_circuit_shunt_set_nodes", C.c_int, _SH, _M, C.c_size_t, C.c_size_t)
+    _sig(lib, "internal_circuit_shunt_map_coordinates", C.c_int, _SH, _M, _dp, _dp)
+    _sig(lib, "internal_circuit_find_nearest_node", C.c_int, _M, _dp, C.POINTER(C.c_size_t), _dp)
+    _sig(lib, "internal_circuit_shunt_validate", C.c_int, _SH, C.c_size_t)
+    _sig(lib, "internal_circuit_build_incidence_vector", C.c_int, _SH, C.c_size_t, _dp)
+    _sig(lib, "internal_circuit_shunt_voltage", C.c_double, _SH, _dp, C.c_size_t)
+    _sig(lib, "internal_circuit_shunt_current_from_voltage", C.c_double, _SH, _dp, C.c_size_t)
+    _sig(lib, "internal_circuit_shunt_residual", C.c_double, _SH, _dp, C.c_size_t, C.c_double)
+    _sig(lib, "transient_system_init", None, _TS)
+    _sig(lib, "transient_system_free", None, _TS)
+    _sig(lib, "transient_system_create_stage1", C.c_int, _TS, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.ResistanceMatrix), _IM, _PM, C.c_int, C.c_double)
+    _sig(lib, "transient_system_create_stage2", C.c_int, _TS, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.ResistanceMatrix), _IM, _PM, _SH, C.c_int, C.c_double)
+    _sig(lib, "transient_build_system_matrix", C.c_int, _TS)
+    _sig(lib, "transient_factorize", C.c_int, _TS, C.c_int)
+    _sig(lib, "transient_build_rhs_stage1", C.c_int, _TS, _dp, _dp, _dp, _dp, _dp, _dp)
+    _sig(lib, "transient_build_rhs_stage2", C.c_int, _TS, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp)
+    _sig(lib, "transient_solve_step", C.c_int, _TS)
+    _sig(lib, "transient_shunt_index", C.c_size_t, _TS)
+    _sig(lib, "transient_extract_stage1_solution", C.c_int, _TS, _dp, _dp)
+    _sig(lib, "transient_extract_stage2_solution", C.c_int, _TS, _dp, _dp, _dp)
+    _sig(lib, "transient_recover_node_state", C.c_int, _TS, _dp, _dp, _dp)
     _sig(lib, "lu_solve_multiple", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp, C.c_size_t, C.c_int)
     _sig(lib, "lu_determinant", C.c_double, C.POINTER(api.LUFactorization))
     _sig(lib, "complex_lu_options_default", None, C.POINTER(api.ComplexLUOptions), C.c_int)

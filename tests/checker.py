@@ -84,8 +84,163 @@ def oracle():
         _sig(lib, "oracle_radiation_vector", C.c_int, _M, _D, _cp, C.c_double, _dp, C.c_int, C.c_int, _cp)
         _sig(lib, "oracle_rcs_direction", C.c_int, _M, _D, _F, _cp, C.c_int, C.c_int, C.c_double, C.c_double,
              C.POINTER(api.RcsResult))
+        _sig(lib, "oracle_transient_matrix", C.c_int, _dp, _dp, _dp, _dp, C.c_size_t, C.c_size_t, C.c_int,
+             C.c_double, C.c_long, C.c_long, C.c_double, _dp, _dp, _dp, _dp, _dp)
+        _sig(lib, "oracle_transient_rhs", C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_size_t, C.c_size_t, C.c_int,
+             C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp)
         _oracle = lib
     return _oracle
+
+
+def transient_signatures(lib):
+    """transient.h / internal_circuit.h signatures (the same in the reference
+    and in the product library)."""
+    _TS, _SH = C.POINTER(api.TransientSystem), C.POINTER(api.InternalCircuitShunt)
+    _IM, _PM = C.POINTER(api.IncidenceMatrix), C.POINTER(api.PotentialMatrix)
+    _sig(lib, "internal_circuit_shunt_default", None, _SH)
+    _sig(lib, "internal_circuit_shunt_set_nodes", C.c_int, _SH, _M, C.c_size_t, C.c_size_t)
+    _sig(lib, "transient_system_init", None, _TS)
+    _sig(lib, "transient_system_free", None, _TS)
+    _sig(lib, "transient_system_create_stage1", C.c_int, _TS, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.ResistanceMatrix), _IM, _PM, C.c_int, C.c_double)
+    _sig(lib, "transient_system_create_stage2", C.c_int, _TS, C.POINTER(api.InductanceMatrix),
+         C.POINTER(api.ResistanceMatrix), _IM, _PM, _SH, C.c_int, C.c_double)
+    _sig(lib, "transient_factorize", C.c_int, _TS, C.c_int)
+    _sig(lib, "transient_build_rhs_stage1", C.c_int, _TS, _dp, _dp, _dp, _dp, _dp, _dp)
+    _sig(lib, "transient_build_rhs_stage2", C.c_int, _TS, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp)
+    _sig(lib, "transient_solve_step", C.c_int, _TS)
+    _sig(lib, "transient_extract_stage1_solution", C.c_int, _TS, _dp, _dp)
+    _sig(lib, "transient_extract_stage2_solution", C.c_int, _TS, _dp, _dp, _dp)
+    _sig(lib, "transient_recover_node_state", C.c_int, _TS, _dp, _dp, _dp)
+    _sig(lib, "incidence_build_AS", C.c_int, _IM, _PM, C.POINTER(api.Matrix), C.c_int)
+
+
+class TransientRun:
+    """The reference driver's time loop (transient_system_create_stage1/2,
+    transient_factorize, then per step build_rhs -> solve_step -> extract) on
+    any library with the transient.h ABI (the compiled reference or the
+    product). Matrices are numpy arrays kept alive by the object; mesh is
+    only needed for a shunt (internal_circuit_shunt_set_nodes)."""
+
+    def __init__(self, lib, L, R, A, P, scheme, dt, shunt=None, mesh=None):
+        self.lib = lib
+        self.arrays = [np.ascontiguousarray(x, np.float64) for x in (L, R, A, P)]
+        Lh, Rh, Ah, Ph = self.arrays
+        self.ne, self.nv = Ah.shape
+        self.L = api.InductanceMatrix(n=self.ne, data=Lh.ctypes.data_as(_dp))
+        self.R = api.ResistanceMatrix(n=self.ne, diagonal=Rh.ctypes.data_as(_dp))
+        self.A = api.IncidenceMatrix(rows=self.ne, cols=self.nv, data=Ah.ctypes.data_as(_dp))
+        self.P = api.PotentialMatrix(n=self.nv, data=Ph.ctypes.data_as(_dp))
+        self.sys = api.TransientSystem()
+        lib.transient_system_init(C.byref(self.sys))
+        self.stage2 = shunt is not None
+        if self.stage2:
+            a, b, res = shunt
+            self.shunt = api.InternalCircuitShunt()
+            lib.internal_circuit_shunt_default(C.byref(self.shunt))
+            self.shunt.resistance = res
+            if mesh is None:
+                mesh = api.Mesh()
+                self._xyz = np.zeros(3 * self.nv)
+                mesh.n_nodes = self.nv
+                mesh.xyz = self._xyz.ctypes.data_as(_dp)
+            self._mesh = mesh
+            assert lib.internal_circuit_shunt_set_nodes(C.byref(self.shunt), C.byref(mesh), a, b) == 0
+            st = lib.transient_system_create_stage2(C.byref(self.sys), C.byref(self.L), C.byref(self.R),
+                                                    C.byref(self.A), C.byref(self.P), C.byref(self.shunt), scheme,
+                                                    dt)
+        else:
+            st = lib.transient_system_create_stage1(C.byref(self.sys), C.byref(self.L), C.byref(self.R),
+                                                    C.byref(self.A), C.byref(self.P), scheme, dt)
+        if st != 0:
+            raise RuntimeError(f"transient create failed ({st})")
+        self.nt = self.sys.n_total
+        if lib.transient_factorize(C.byref(self.sys), 1) != 0:
+            raise RuntimeError("transient_factorize failed")
+
+    def matrix(self):
+        return api.as_array(self.sys.system_matrix, self.nt * self.nt).reshape(self.nt, self.nt).copy()
+
+    def step(self, I, V, ishunt=0.0, sn=None, snp1=None, un=None, unp1=None):
+        """One step from (I, V[, ishunt]); returns (I, V[, ishunt], rhs)."""
+        p = lambda x: None if x is None else np.ascontiguousarray(x, np.float64).ctypes.data_as(_dp)
+        keep = [np.ascontiguousarray(x, np.float64) if x is not None else None for x in (I, V, sn, snp1, un, unp1)]
+        Ic, Vc, s0, s1, u0, u1 = keep
+        if self.stage2:
+            st = self.lib.transient_build_rhs_stage2(C.byref(self.sys), p(Ic), p(Vc), float(ishunt), p(s0), p(s1),
+                                                     p(u0), p(u1))
+        else:
+            st = self.lib.transient_build_rhs_stage1(C.byref(self.sys), p(Ic), p(Vc), p(s0), p(s1), p(u0), p(u1))
+        if st != 0 or self.lib.transient_solve_step(C.byref(self.sys)) != 0:
+            raise RuntimeError("transient step failed")
+        rhs = api.as_array(self.sys.rhs, self.nt).copy()
+        In, Vn = np.zeros(self.ne), np.zeros(self.nv)
+        if self.stage2:
+            sh = C.c_double()
+            assert self.lib.transient_extract_stage2_solution(C.byref(self.sys), In.ctypes.data_as(_dp),
+                                                              Vn.ctypes.data_as(_dp), C.byref(sh)) == 0
+            return In, Vn, sh.value, rhs
+        assert self.lib.transient_extract_stage1_solution(C.byref(self.sys), In.ctypes.data_as(_dp),
+                                                          Vn.ctypes.data_as(_dp)) == 0
+        return In, Vn, 0.0, rhs
+
+    def close(self):
+        self.lib.transient_system_free(C.byref(self.sys))
+
+
+def oracle_transient(L, R, A, P, scheme, dt, shunt=None):
+    """The oracle's system matrix and operators: (M, F, AS, b_R, c)."""
+    ne, nv = A.shape
+    nt = ne + nv + (1 if shunt else 0)
+    M = np.zeros((nt, nt)); F = np.zeros(nv); AS = np.zeros((ne, nv)); bR = np.zeros(nv); c = np.zeros(nv)
+    a, b, res = shunt if shunt else (-1, -1, 0.0)
+    arr = [np.ascontiguousarray(x, np.float64) for x in (L, R, A, P)]
+    st = oracle().oracle_transient_matrix(*(x.ctypes.data_as(_dp) for x in arr), ne, nv, scheme, dt, a, b, res,
+                                          M.ctypes.data_as(_dp), F.ctypes.data_as(_dp), AS.ctypes.data_as(_dp),
+                                          bR.ctypes.data_as(_dp), c.ctypes.data_as(_dp))
+    if st != 0:
+        raise RuntimeError("oracle transient matrix failed")
+    return M, F, AS, bR, c
+
+
+def oracle_transient_rhs(L, R, A, F, AS, scheme, dt, I, V, sn=None, snp1=None, un=None, unp1=None, bR=None,
+                         ishunt=0.0):
+    ne, nv = A.shape
+    rhs = np.zeros(ne + nv + (1 if bR is not None else 0))
+    arr = [np.ascontiguousarray(x, np.float64) for x in (L, R, A, F, AS, I, V)]
+    p = lambda x: None if x is None else np.ascontiguousarray(x, np.float64).ctypes.data_as(_dp)
+    opt = [np.ascontiguousarray(x, np.float64) if x is not None else None for x in (sn, snp1, un, unp1, bR)]
+    st = oracle().oracle_transient_rhs(*(x.ctypes.data_as(_dp) for x in arr[:5]), ne, nv, scheme, dt,
+                                       arr[5].ctypes.data_as(_dp), arr[6].ctypes.data_as(_dp),
+                                       *(p(x) for x in opt[:4]), p(opt[4]), float(ishunt), rhs.ctypes.data_as(_dp))
+    if st != 0:
+        raise RuntimeError("oracle transient rhs failed")
+    return rhs
+
+
+class OracleTransient:
+    """The same time loop through the oracle (matrix, rhs, real LU, solve)."""
+
+    def __init__(self, L, R, A, P, scheme, dt, shunt=None):
+        self.L, self.R, self.A = (np.ascontiguousarray(x, np.float64) for x in (L, R, A))
+        self.scheme, self.dt, self.stage2 = scheme, dt, shunt is not None
+        self.M, self.F, self.AS, self.bR, self.c = oracle_transient(L, R, A, P, scheme, dt, shunt)
+        self.ne, self.nv = self.A.shape
+        st, self.LU, self.piv, _ = oracle_real_lu(self.M)
+        if st != 0:
+            raise RuntimeError("oracle transient LU failed")
+
+    def step(self, I, V, ishunt=0.0, sn=None, snp1=None, un=None, unp1=None):
+        rhs = oracle_transient_rhs(self.L, self.R, self.A, self.F, self.AS, self.scheme, self.dt, I, V, sn, snp1, un,
+                                   unp1, self.bR if self.stage2 else None, ishunt)
+        x = np.zeros_like(rhs)
+        n = rhs.shape[0]
+        piv = np.ascontiguousarray(self.piv.astype(np.uintp))
+        if oracle().oracle_real_solve(self.LU.ctypes.data_as(_dp), piv.ctypes.data_as(_sp), n,
+                                      rhs.ctypes.data_as(_dp), x.ctypes.data_as(_dp)) != 0:
+            raise RuntimeError("oracle solve failed")
+        ne, nv = self.ne, self.nv
+        return x[:ne], x[ne:ne + nv], (x[ne + nv] if self.stage2 else 0.0), rhs
 
 
 def ref_available() -> bool:
@@ -149,6 +304,7 @@ def ref():
         _sig(lib, "lu_factorize", C.c_int, _dp, C.c_size_t, C.POINTER(api.LUOptions),
              C.POINTER(api.LUFactorization))
         _sig(lib, "lu_solve", C.c_int, C.POINTER(api.LUFactorization), _dp, _dp)
+        transient_signatures(lib)
         _F = C.POINTER(api.IncidentField)
         _sig(lib, "incident_field_configure_harmonic", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double,
              C.c_double, C.c_double, C.c_double)

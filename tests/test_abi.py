@@ -57,10 +57,10 @@ SHARED_STRUCTS = ["Mesh", "QuadPatch", "PatchList", "DualMesh", "Complex", "Comp
                   "ResistanceOptions", "ResistanceMatrix", "LUOptions", "LUFactorization", "ComplexLUOptions",
                   "ComplexLUFactorization", "HarmonicSystem", "HarmonicExcitation", "HarmonicSolution",
                   "HarmonicResidual", "IncidentField", "ScatteringOptions", "RcsOptions", "RcsResult",
-                  "LightningSource"]
+                  "LightningSource", "InternalCircuitShunt", "TransientSystem"]
 PROBE_HEADERS = ["mesh.h", "dual_mesh.h", "quadrature.h", "complex_matrix.h", "matrix.h", "inductance.h",
                  "potential.h", "incidence.h", "resistance.h", "lu.h", "complex_lu.h", "harmonic.h",
-                 "incident_field.h", "scattering.h", "rcs.h", "lightning.h"]
+                 "incident_field.h", "scattering.h", "rcs.h", "lightning.h", "internal_circuit.h", "transient.h"]
 
 
 def layout_probe(tmp_path, include_dir):
