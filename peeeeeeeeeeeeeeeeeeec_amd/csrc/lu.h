@@ -37,6 +37,10 @@ struct LUContext {
     DeviceBuffer<float> scale_l16, scale_u16;
     DeviceBuffer<unsigned char> pack_u16n;  // the next block column's U12 (look-ahead update)
     DeviceBuffer<float> scale_u16n;
+    // inverses of the ordered factor's 16 x 16 diagonal blocks, two panels'
+    // worth (by panel parity): the look-ahead factor of the next panel writes
+    // its set while the current panel's solves read theirs
+    DeviceBuffer<float2> dblk_inv;
     bool split_fp16_update = true;                // ordered factor: FP16 hi/lo update (else FP32 MFMA)
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves

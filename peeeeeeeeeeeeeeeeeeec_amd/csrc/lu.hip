@@ -1854,8 +1854,12 @@ __device__ inline void pivot_check(double u, double limit, int &ok, double &rcp)
 // every trailing tile takes the rank-4 update A22 -= L21 U12 from LDS: two
 // barriers per 4 columns, no per-element masks (panels align with tiles).
 // Same operations as the column-by-column elimination, grouped by panel.
+// With dinv (w a multiple of 16), the inverses of the factor's 16 x 16
+// diagonal blocks follow, for the blocked triangular solves: U_b^-1 row-major
+// at dinv[256 b], and (L_b^-1)^T (L_b unit lower) row-major at
+// dinv[2048 + 256 b], each column by substitution in FP32.
 __global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t lda, int k0, int w, double limit,
-                                                             int *status) {
+                                                             int *status, float2 *dinv) {
     // double-buffered by panel parity: L11 (unit lower) and U11 of the
     // diagonal tile, reciprocals of its pivots, L21 tiles by tile row and
     // U12 tiles by tile column
@@ -1980,6 +1984,204 @@ __global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t l
             const int i = 4 * ti + r, cc = 4 * tc + c;
             if (i < w && cc < w) A[(size_t)(k0 + i) * lda + k0 + cc] = get(r, c);
         }
+    if (!dinv) return;
+    // the 16 x 16 diagonal blocks, staged from the tiles that hold them
+    __shared__ float2 s_db[8][16][17];
+    const int nbk = w / 16;
+    if ((ti >> 2) == (tc >> 2) && (ti >> 2) < nbk) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_db[ti >> 2][4 * (ti & 3) + r][4 * (tc & 3) + c] = get(r, c);
+    }
+    __syncthreads();
+    if (tid >= 256) return;
+    const int lower = tid >> 7, b = (tid >> 4) & 7, col = tid & 15;
+    if (b >= nbk) return;
+    float2 x[16];
+    if (!lower) {
+        // column col of U_b^-1: back substitution from row 15 (rows past
+        // col come out exactly zero)
+#pragma unroll
+        for (int r = 15; r >= 0; --r) {
+            float2 acc = make_float2(r == col ? 1.f : 0.f, 0.f);
+#pragma unroll
+            for (int k = r + 1; k < 16; ++k) acc = wfms(acc, s_db[b][r][k], x[k]);
+            x[r] = wmul(acc, wdiv(wone<float2>(), s_db[b][r][r]));
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) dinv[256 * b + 16 * r + col] = x[r];
+    } else {
+        // column col of L_b^-1 (unit diagonal), forward substitution
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float2 acc = make_float2(r == col ? 1.f : 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < r; ++k) acc = wfms(acc, s_db[b][r][k], x[k]);
+            x[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) dinv[2048 + 256 * b + 16 * col + r] = x[r];
+    }
+}
+
+// Blocked triangular solves of the ordered factor on FP32 MFMA
+// (v_mfma_f32_16x16x4f32), for a block width w = 16 NBK: a wavefront holds
+// 16 rows of L21 (16 columns of U12) as NBK accumulator tiles and walks the
+// 16-wide diagonal blocks: X_j = (A_j - sum_{i<j} X_i U_ij) U_jj^-1 for
+// L21 = A21 U11^-1, X_j = L_jj^-1 (A_j - sum_{i<j} L_ji X_i) for U12 =
+// L11^-1 A12, with the diagonal-block inverses of diag_lu_tiled_kernel.
+//
+// Register layout: lane (li, lk) = (lane % 16, lane / 16) holds value v of
+// tile j at L21 row li, column 16 j + 4 lk + v (U12 row 16 j + 4 lk + v,
+// column li). That is the MFMA accumulator layout of the transposed product
+// (row 4 lk + v, column li), and at the same time a B operand whose k index
+// runs over {4 lk + t} in the t-th MFMA of a 16-deep product: the solved
+// tile feeds the next products with no data movement. The A operands (U11
+// blocks transposed, L11 blocks, the inverses) are read from global memory
+// in the matching order (k = 4 lk + t at lane lk of the t-th MFMA).
+//
+// Per complex product C += A B: re += ar br - ai bi, im += ar bi + ai br,
+// four real 16 x 16 x 4 MFMAs per t.
+__device__ __forceinline__ void cmfma4(float ar, float ai, float br, float bi, v4f &cr, v4f &ci) {
+    cr = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, ci, 0, 0, 0);
+}
+
+// L21 rows r_begin .. r_end - 1 (below the diagonal block at k0), in place;
+// workgroups of four wavefronts, 16 rows each.
+template <int NBK>
+__global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t lda, int k0, int r_begin, int r_end,
+                                                              const float2 *__restrict__ dinv, const int *status) {
+    constexpr int W = 16 * NBK;
+    if (status[0] != 0) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int rb = r_begin + ((int)blockIdx.x * 4 + wave) * 16;
+    if (rb >= r_end) return;
+    const bool in = rb + li < r_end;
+    // lanes past r_end work on a copy of row rb and store nothing
+    float2 *row = A + (size_t)(k0 + W + (in ? rb + li : rb)) * lda + k0;
+    const float2 *U = A + (size_t)k0 * lda + k0;
+    v4f xr[NBK], xi[NBK];
+#pragma unroll
+    for (int j = 0; j < NBK; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float2 a = row[16 * j + 4 * lk + v];
+            xr[j][v] = a.x;
+            xi[j][v] = a.y;
+        }
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) {
+        // X_j^T = (U_jj^-1)^T (A_j - ...)^T
+        v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
+        const float2 *ui = dinv + 256 * j + li;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float2 a = ui[16 * (4 * lk + t)];
+            cmfma4(a.x, a.y, xr[j][t], xi[j][t], nr, ni);
+        }
+        xr[j] = nr;
+        xi[j] = ni;
+        // A_c -= X_j U_jc for the tiles right of it: (U_jc)^T X_j^T
+#pragma unroll
+        for (int c = j + 1; c < NBK; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 a = U[(size_t)(16 * j + 4 * lk + t) * lda + 16 * c + li];
+                cmfma4(-a.x, -a.y, xr[j][t], xi[j][t], xr[c], xi[c]);
+            }
+    }
+    if (in) {
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) row[16 * j + 4 * lk + v] = make_float2(xr[j][v], xi[j][v]);
+    }
+}
+
+// U12 columns c0 .. c1 - 1 (right of the diagonal block at k0), in place;
+// one wavefront per 16 columns, four per workgroup. With pk, the solved
+// columns are also written as the split-FP16 operand of the trailing update
+// (block `tile` = the wavefront's 16-column block from c0, pcb blocks per
+// chunk), as pack_u_h3_wave would write them.
+template <int NBK>
+__global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t lda, int k0, int c0, int c1,
+                                                             const float2 *__restrict__ dinv, const int *status,
+                                                             uint4 *pk, float *pinv, int pcb) {
+    if (status[0] != 0) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int tile = (int)blockIdx.x * 4 + wave, cb = c0 + 16 * tile;
+    if (cb >= c1) return;
+    const bool in = cb + li < c1;
+    float2 *colp = A + (size_t)k0 * lda + (in ? cb + li : cb);
+    const float2 *L = A + (size_t)k0 * lda + k0;
+    v4f xr[NBK], xi[NBK];
+#pragma unroll
+    for (int j = 0; j < NBK; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float2 a = colp[(size_t)(16 * j + 4 * lk + v) * lda];
+            xr[j][v] = a.x;
+            xi[j][v] = a.y;
+        }
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) {
+        v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
+        const float2 *lt = dinv + 2048 + 256 * j + li;  // (L_jj^-1)^T
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float2 a = lt[16 * (4 * lk + t)];
+            cmfma4(a.x, a.y, xr[j][t], xi[j][t], nr, ni);
+        }
+        xr[j] = nr;
+        xi[j] = ni;
+        // A_c -= L_cj X_j for the tiles below it
+#pragma unroll
+        for (int c = j + 1; c < NBK; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 a = L[(size_t)(16 * c + li) * lda + 16 * j + 4 * lk + t];
+                cmfma4(-a.x, -a.y, xr[j][t], xi[j][t], xr[c], xi[c]);
+            }
+    }
+    if (in) {
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) colp[(size_t)(16 * j + 4 * lk + v) * lda] = make_float2(xr[j][v], xi[j][v]);
+    }
+    if (pk) {
+        // lane (li, lk) holds rows 16 j + 4 lk + v of column li: the lane
+        // order of the operand pack (chunk j, quarter lk, entry v)
+        float mx = 0.f;
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(xr[j][v]), fabsf(xi[j][v])));
+        if (!in) mx = 0.f;
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        const float sc = split_scale(mx);
+#pragma unroll
+        for (int j = 0; j < NBK; ++j) {
+            unsigned hw[4], lw[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                _Float16 xh, xl, yh, yl;
+                split_half(in ? xr[j][v] * sc : 0.f, xh, xl);
+                split_half(in ? xi[j][v] * sc : 0.f, yh, yl);
+                hw[v] = pack_half2(xh, yh);
+                lw[v] = pack_half2(xl, yl);
+            }
+            uint4 *o = pk + ((size_t)(j * pcb + tile) * 2) * 64 + lane;
+            o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+            o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+        }
+        if (lk == 0) pinv[tile * 16 + li] = 1.f / sc;
+    }
 }
 
 // Triangular solves of the ordered factor, one wavefront per row of
@@ -2212,8 +2414,28 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     // triangular solves: one wavefront per row of L21 / column of U12, at
     // most one workgroup of 16 per CU
     // rows r_begin .. r_end - 1 below the diagonal block (all of them by default)
+    // blocked MFMA solves for block widths of 32, 64 and 128 (the diagonal
+    // factor then also leaves its 16 x 16 block inverses, in the set of the
+    // panel's parity); substitution kernels otherwise
+    const bool blocked_solves = nb % 16 == 0 && (nb / 16 == 2 || nb / 16 == 4 || nb / 16 == 8);
+    ctx.dblk_inv.reserve(2 * 4096);
+    auto dinv_of = [&](int k0) { return ctx.dblk_inv.ptr + (size_t)((k0 / nb) & 1) * 4096; };
     auto solve_l21 = [&](int k0, int w, int rest, int r_begin = 0, int r_end = -1) {
         if (r_end < 0) r_end = rest;
+        if (blocked_solves && w == nb) {
+            const unsigned grid = (unsigned)((r_end - r_begin + 63) / 64);
+#define PEEC_TRSM_R(K)                                                                                           \
+    hipLaunchKernelGGL(trsm_right_mfma_kernel<K>, dim3(grid), dim3(256), 0, s, dA, lda, k0, r_begin, r_end, \
+                       dinv_of(k0), ctx.status.ptr)
+            if (w == 128)
+                PEEC_TRSM_R(8);
+            else if (w == 64)
+                PEEC_TRSM_R(4);
+            else
+                PEEC_TRSM_R(2);
+#undef PEEC_TRSM_R
+            return;
+        }
         const int grid = std::max(1, std::min((r_end - r_begin + 15) / 16, ctx.cus));
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
                            dA, lda, k0, w, r_begin, r_end, ctx.status.ptr);
@@ -2221,6 +2443,20 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     // with pk, U12 is also packed as the split-FP16 operand of pcb blocks
     auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st, uint4 *pk = nullptr, float *pinv = nullptr,
                          int pcb = 0) {
+        if (blocked_solves && w == nb) {
+            const unsigned grid = (unsigned)((c1 - c0 + 63) / 64);
+#define PEEC_TRSM_L(K)                                                                                             \
+    hipLaunchKernelGGL(trsm_left_mfma_kernel<K>, dim3(grid), dim3(256), 0, st, dA, lda, k0, c0, c1, dinv_of(k0), \
+                       ctx.status.ptr, pk, pinv, pcb)
+            if (w == 128)
+                PEEC_TRSM_L(8);
+            else if (w == 64)
+                PEEC_TRSM_L(4);
+            else
+                PEEC_TRSM_L(2);
+#undef PEEC_TRSM_L
+            return;
+        }
         const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
         const int grid = std::max(1, std::min(tiles, ctx.cus));
         hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
@@ -2229,7 +2465,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
         hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
-                           ctx.status.ptr);
+                           ctx.status.ptr, blocked_solves && w == nb ? dinv_of(k0) : nullptr);
         if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
     };
     diag(0, std::min(nb, n), s);
