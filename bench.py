@@ -112,7 +112,7 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 5}))
 
 
-def traffic_from_profile(name="lu_update_split16_traffic_c2.md"):
+def traffic_from_profile(name="lu_update_rank256_traffic_c2.md"):
     """HBM bytes per launch of an LU update kernel measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration), read
     from the committed summary; None when absent."""
@@ -123,6 +123,32 @@ def traffic_from_profile(name="lu_update_split16_traffic_c2.md"):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def assembly_roofline(static, t_static):
+    """Static Lp/P assembly against the FP64 vector peak: work from the exact
+    evaluation counts (12 flops per 1/R point, 140 per analytic triangle
+    potential), times from HIP events around the near-field and far-tile
+    kernels during the engine's construction."""
+    t_k = (static["near_ms"] + static["far_ms"]) * 1e-3
+    rate = static["flops"] / t_k if t_k > 0 else 0.0
+    near = static["near_flops"] / (static["near_ms"] * 1e-3) if static["near_ms"] > 0 else 0.0
+    far = static["far_flops"] / (static["far_ms"] * 1e-3) if static["far_ms"] > 0 else 0.0
+    return {
+        "kernel": "static assembly: near_entry_kernel (adaptive analytic near field) + far_tile_kernel (1/R tiles)",
+        "bound": "fp64 valu",
+        "achieved": rate / 1e12,
+        "peak": PEAKS["fp64_vector"] / 1e12,
+        "unit": "TFLOP/s",
+        "frac": rate / PEAKS["fp64_vector"],
+        "near_entry_tflops": near / 1e12,
+        "far_tile_tflops": far / 1e12,
+        "near_ms": static["near_ms"],
+        "far_ms": static["far_ms"],
+        "static_assembly_s": t_static,
+        "evaluations": {k: static[k] for k in ("far_evaluations", "near_potential_evaluations",
+                                                "near_pair_evaluations")},
+    }
 
 
 def _extreme_corner_port(xyz):
@@ -336,8 +362,9 @@ def bench_c5(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=200, help="sweep: points timed (200 = the whole sweep)")
+    ap.add_argument("--warmup", type=int, default=3,
+                    help="sweep: untimed points (the first factor with pivoting and the two block-width trials)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -390,6 +417,7 @@ def main():
     eng = lib.Engine(model, retarded=True, **eng_opts)
     sync()
     t_static = time.perf_counter() - t0
+    static = eng.static_stats()
     n = model.ne + model.nv
     n_cells, n_e, n_v = int(model.mesh.n_quads), model.ne, model.nv
     K, W = args.steps, args.warmup
@@ -489,23 +517,37 @@ def main():
         "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) for r in reports]))
                           for k in ("assembly", "fill", "factor", "solve", "check")},
         "static_assembly_s": t_static,
+        "sweep_amortized": {
+            "points": len(freqs),
+            "mixed_points_per_s": len(freqs) / (t_static + len(freqs) * t_mixed / len(timed)),
+            "fp64_points_per_s": len(freqs) / (t_static + len(freqs) * t_fp64 / len(timed)),
+            "note": ("static Lp/P assembly once plus every point of the sweep"
+                     + ("" if len(timed) == len(freqs) else f", points extrapolated from {len(timed)} timed")),
+        },
         "roofline": {
-            "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, complex FP32 storage, "
-                       "split-FP16 operands on v_mfma_f32_16x16x32_f16 (gemm_tile_h3_kernel<8, 2>)"),
-            "bound": "hbm",
-            "achieved": g16_gbs / 1e9,
-            "peak": PEAKS["hbm"] / 1e9,
-            "unit": "GB/s",
-            "frac": g16_gbs / PEAKS["hbm"],
+            "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, rank 256 (two "
+                       "128-wide panels per pass), complex FP32 storage, split-FP16 operands on "
+                       "v_mfma_f32_16x16x32_f16 (gemm_tile_h3_kernel<16, 2>)"),
+            "bound": "mfma",
+            "achieved": g16_tf / 1e12,
+            "peak": PEAKS["fp16_mfma"] / 1e12,
+            "unit": "TFLOP/s",
+            "frac": g16_tf / PEAKS["fp16_mfma"],
             "traffic": traffic_from_profile(),
             "launches": g16["launches"],
             "avg_launch_ms": g16["ms"] / max(1, g16["launches"]),
+            "algorithmic_flops_per_launch": g16["flops"] / max(1, g16["launches"]),
             "algorithmic_bytes_per_launch": g16["bytes"] / max(1, g16["launches"]),
-            "algorithmic_tflops": g16_tf / 1e12,
-            "mfma_roof_tflops": PEAKS["fp16_mfma"] / 3 / 1e12,
+            "mfma_issued_tflops": 3 * g16_tf / 1e12,
+            "split_fp16_roof_tflops": PEAKS["fp16_mfma"] / 3 / 1e12,
+            "frac_of_split_fp16_roof": 3 * g16_tf / PEAKS["fp16_mfma"],
+            "hbm_gbs": g16_gbs / 1e9,
+            "hbm_frac": g16_gbs / PEAKS["hbm"],
         },
+        "roofline_assembly": assembly_roofline(static, t_static),
         "next_column_update": {
-            "kernel": "look-ahead update of the next block column, split-FP16 (gemm_next_h3_kernel<8, 2>)",
+            "kernel": ("look-ahead update of the next block column, split-FP16 (gemm_next_h3_kernel<8, 2> after "
+                       "the first panel of a pair, <16, 2> after the second)"),
             "launches": gemm["launches"],
             "tflops": gemm_tf / 1e12,
             "ms_per_point": gemm["ms"] / len(timed),

@@ -213,15 +213,35 @@ size_t peec_engine_nodes(const PeecEngine *engine);
 long long peec_engine_far_evaluations(const PeecEngine *engine);
 long long peec_engine_smooth_evaluations(const PeecEngine *engine);
 
+/* The static Lp/P assembly of the engine's construction: evaluation counts
+ * and the kernels' times (HIP events on the engine stream). Work in flop
+ * terms: 12 flops per 1/R evaluation (far tiles, and near pairs far enough
+ * for the plain Gauss rule), 140 per analytic triangle-potential
+ * evaluation. All zero when the static matrices were supplied. */
+typedef struct
+{
+    long long far_evaluations;            /* far_tile_kernel 1/R points */
+    long long near_potential_evaluations; /* near_entry_kernel analytic potentials */
+    long long near_pair_evaluations;      /* near_entry_kernel 1/R points (Gauss pairs) */
+    double far_ms;                        /* far_tile_kernel, Lp and P */
+    double near_ms;                       /* near_entry_kernel, Lp and P */
+} PeecStaticStats;
+int peec_engine_static_stats(const PeecEngine *engine, PeecStaticStats *out);
+
 /* Per-kernel timing: HIP events recorded on the engine's stream around
  * every launch of the kernel family, summed over launches. flops/bytes are
  * the algorithmic counts of those launches. */
 typedef enum
 {
-    PEEC_KERNEL_LU_UPDATE = 0,  /* trailing-update GEMM on MFMA: 8mnk (complex) */
-    PEEC_KERNEL_PANEL = 1,      /* cooperative panel factorization */
-    PEEC_KERNEL_SMOOTH_TILE = 2,/* retarded remainder tile kernel: 42 flops/eval */
-    PEEC_KERNEL_FAR_TILE = 3    /* static 1/R tile kernel: 12 flops/eval */
+    PEEC_KERNEL_LU_UPDATE = 0,         /* update GEMMs on MFMA (FP64, FP32, and the split-FP16
+                                          next-column update): 8mnk (complex) */
+    PEEC_KERNEL_PANEL = 1,             /* panel / diagonal-block factorization */
+    PEEC_KERNEL_SMOOTH_TILE = 2,       /* retarded remainder tile kernel: 38 flops/eval */
+    PEEC_KERNEL_FAR_TILE = 3,          /* static 1/R tile kernel: 12 flops/eval */
+    PEEC_KERNEL_SMOOTH_MOMENTS = 4,    /* retarded remainder from stored moments */
+    PEEC_KERNEL_LU_UPDATE_SPLIT16 = 5, /* split-FP16 trailing update of the ordered factor */
+    PEEC_KERNEL_LU_UPDATE_THIN = 6,    /* split-FP16 update of the next panel's rows */
+    PEEC_KERNEL_NEAR_ENTRY = 7         /* static near-field entries (timed at construction) */
 } PeecKernel;
 
 typedef struct

@@ -39,8 +39,10 @@ class RegionSet {
     RegionKind kind() const { return kind_; }
 
     // static matrix (row-major R x R, original numbering) into d_out
+    // With d_counts, the near-field kernel adds its analytic potential
+    // evaluations to d_counts[0] and its 1/R evaluations to d_counts[1].
     void assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work,
-                         KernelTimer *timer = nullptr);
+                         KernelTimer *timer = nullptr, unsigned long long *d_counts = nullptr);
     // complex retarded matrix: d_out = d_static + scale * smooth remainder
     void assemble_retarded(const double *d_static, double wave_number, double2 *d_out,
                            hipStream_t s, AssemblyWork *work, KernelTimer *timer = nullptr);

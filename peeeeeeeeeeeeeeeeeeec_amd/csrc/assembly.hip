@@ -204,7 +204,8 @@ struct Frame {
     int state;  // 0 fresh, 1 left pending, 2 right pending
 };
 
-__device__ double d_outer_adaptive(const double (&a)[3][3], const TriPot &T, int order, double tol) {
+// n counts the analytic potential evaluations (quadrature points).
+__device__ double d_outer_adaptive(const double (&a)[3][3], const TriPot &T, int order, double tol, unsigned &n) {
     Frame st[kMaxDepth + 1];
     int sp = 0;
     for (int r = 0; r < 3; ++r)
@@ -218,6 +219,7 @@ __device__ double d_outer_adaptive(const double (&a)[3][3], const TriPot &T, int
         Frame &f = st[sp];
         bool done = false;
         if (f.state == 0) {
+            n += low * low + order * order;
             const double lo = d_outer_rule(f.t, T, low);
             const double hi = d_outer_rule(f.t, T, order);
             if (!isfinite(lo) || !isfinite(hi)) return NAN;
@@ -256,7 +258,7 @@ __device__ double d_outer_adaptive(const double (&a)[3][3], const TriPot &T, int
 
 // near-field triangle pair (quadrature.c:940-973); cls 0 self, 1 touching, 2 near
 __device__ double d_tri_pair_near(const double (&a)[3][3], const double (&b)[3][3], int cls,
-                                  const QuadratureOrders &q) {
+                                  const QuadratureOrders &q, unsigned &n) {
     TriPot Ta, Tb;
     d_tripot_setup(a, Ta);
     d_tripot_setup(b, Tb);
@@ -271,13 +273,14 @@ __device__ double d_tri_pair_near(const double (&a)[3][3], const double (&b)[3][
     if (same) {
         double s = 0.0;
         for (int i = 0; i < 3; ++i) s += d_tripot_eval(Tb, a[i]);
+        n += 3;
         return Ta.area2 * s / 3.0;
     }
     int order = cls == 0 ? q.self_order : cls == 1 ? q.touching_order : q.near_order;
     if (order < 6) order = 6;
     const double scale = Ta.area2 * Tb.area2 / sqrt(fmax(Ta.area2, Tb.area2));
-    const double ab = d_outer_adaptive(a, Tb, order, 2e-5 * scale);
-    const double ba = d_outer_adaptive(b, Ta, order, 2e-5 * scale);
+    const double ab = d_outer_adaptive(a, Tb, order, 2e-5 * scale, n);
+    const double ba = d_outer_adaptive(b, Ta, order, 2e-5 * scale, n);
     return 0.5 * (ab + ba);
 }
 
@@ -337,7 +340,7 @@ __global__ __launch_bounds__(64) void near_entry_kernel(
     const int *__restrict__ pairs, int n_pairs, const int *__restrict__ ptr,
     const double *__restrict__ pv, QuadratureOrders q, int kind, const double *__restrict__ aux,
     const double *__restrict__ dir, double *__restrict__ out, size_t ld, int mirror,
-    int *__restrict__ err) {
+    int *__restrict__ err, unsigned long long *__restrict__ counts = nullptr) {
     constexpr int G = 64 / PER_WAVE, CAP = kMaxTasks / PER_WAVE;
     __shared__ double vals[kMaxTasks];
     const int slot = threadIdx.x / G, sub = threadIdx.x % G;
@@ -351,6 +354,7 @@ __global__ __launch_bounds__(64) void near_entry_kernel(
         if (sub == 0) atomicOr(err, 2);
         return;
     }
+    unsigned n_pot = 0, n_gl = 0;  // analytic potential evaluations, 1/R evaluations
     for (int t = sub; t < tasks; t += G) {
         const int pp = t >> 2, tp = t & 3;
         const int ia = pp / nb, ib = pp - ia * nb;
@@ -360,7 +364,25 @@ __global__ __launch_bounds__(64) void near_entry_kernel(
         double ta[3][3], tb[3][3];
         d_piece_triangle(pa, tp >> 1, ta);
         d_piece_triangle(pb, tp & 1, tb);
-        v[t] = cls == 3 ? d_tri_pair_gl(ta, tb, q.far_order) : d_tri_pair_near(ta, tb, cls, q);
+        if (cls == 3) {
+            v[t] = d_tri_pair_gl(ta, tb, q.far_order);
+            n_gl += q.far_order * q.far_order * q.far_order * q.far_order;
+        } else {
+            v[t] = d_tri_pair_near(ta, tb, cls, q, n_pot);
+        }
+    }
+    if (counts) {
+        // the lanes of one entry are active together: sum them, one atomic
+        // per entry
+        unsigned long long cp = n_pot, cg = n_gl;
+        for (int m = 1; m < G; m <<= 1) {
+            cp += __shfl_xor(cp, m);
+            cg += __shfl_xor(cg, m);
+        }
+        if (sub == 0) {
+            atomicAdd(&counts[0], cp);
+            atomicAdd(&counts[1], cg);
+        }
     }
     // the workgroup is one wavefront: lanes that returned above are simply
     // inactive at the barrier
@@ -1399,7 +1421,7 @@ void RegionSet::build_near_list(const Mesh *mesh) {
 }
 
 void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, AssemblyWork *work,
-                                KernelTimer *timer) {
+                                KernelTimer *timer, unsigned long long *d_counts) {
     const int ntiles = (int)h_tiles_.size() / 2;
     const size_t lds = 4 * (size_t)far_slots_ * sizeof(double4);
     if (ntiles) {
@@ -1418,16 +1440,19 @@ void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, Asse
     }
     const int nn = (int)h_near_.size() / 2;
     if (nn) {
+        const int tk = timer ? timer->start(s) : -1;
         // edge regions (at most 2 x 2 pieces): four entries per wavefront
         if (kind_ == RegionKind::Edge && maxp_ <= 2)
             hipLaunchKernelGGL(near_entry_kernel<4>, dim3((nn + 3) / 4), dim3(64), 0, s, d_near_.ptr, nn,
                                d_ptr_.ptr, d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
-                               d_error);
+                               d_error, d_counts);
         else
             hipLaunchKernelGGL(near_entry_kernel<1>, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr,
                                d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
-                               d_error);
+                               d_error, d_counts);
         HIP_CHECK(hipGetLastError());
+        // flops follow from the evaluation counts once the kernel is done
+        if (timer) timer->stop(tk, s, kKernelNearEntry, 0.0, 16.0 * nn);
     }
     if (work) {
         work->far_evaluations += far_evals_;

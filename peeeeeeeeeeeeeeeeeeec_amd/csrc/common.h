@@ -133,7 +133,8 @@ enum KernelKind {
     kKernelSmoothMoments = 4,
     kKernelLuUpdateSplit16 = 5,
     kKernelLuUpdateThin = 6,  // first panel of a pair: update of the next panel's rows only
-    kKernelKinds = 7
+    kKernelNearEntry = 7,     // static near-field entries
+    kKernelKinds = 8
 };
 
 struct KernelTotals {

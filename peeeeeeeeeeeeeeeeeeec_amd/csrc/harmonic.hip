@@ -263,6 +263,7 @@ struct PeecEngine {
     DeviceBuffer<float2> d_Z32;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
+    PeecStaticStats static_stats{};
     KernelTimer timer;
     ~PeecEngine() {
         for (hipEvent_t &e : ev)
@@ -676,18 +677,37 @@ static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngin
         e->d_err.resize(1);
         HIP_CHECK(hipMemsetAsync(e->d_err.ptr, 0, sizeof(int), s));
         AssemblyWork w;
+        DeviceBuffer<unsigned long long> counts;
         if (L_host && P_host) {
             // static matrices supplied by the caller (e.g. read back from
             // Matrix Market files): no assembly
             HIP_CHECK(hipMemcpyAsync(e->d_L.ptr, L_host, sizeof(double) * e->ne * e->ne, hipMemcpyHostToDevice, s));
             HIP_CHECK(hipMemcpyAsync(e->d_P.ptr, P_host, sizeof(double) * e->nv * e->nv, hipMemcpyHostToDevice, s));
         } else {
-            e->edges->assemble_static(e->d_L.ptr, s, e->d_err.ptr, &w);
-            e->nodes->assemble_static(e->d_P.ptr, s, e->d_err.ptr, &w);
+            // timed, with the near-field evaluation counts (the work of the
+            // static assembly in flop-model terms)
+            counts.resize(2);
+            HIP_CHECK(hipMemsetAsync(counts.ptr, 0, 2 * sizeof(unsigned long long), s));
+            e->timer.enabled = true;
+            e->edges->assemble_static(e->d_L.ptr, s, e->d_err.ptr, &w, &e->timer, counts.ptr);
+            e->nodes->assemble_static(e->d_P.ptr, s, e->d_err.ptr, &w, &e->timer, counts.ptr);
         }
         int err = 0;
         e->d_err.download(&err, 1, s);
+        unsigned long long hc[2] = {0, 0};
+        if (counts.ptr) counts.download(hc, 2, s);
         e->stream.sync();
+        if (e->timer.enabled) {
+            e->timer.resolve();
+            PeecStaticStats &st = e->static_stats;
+            st.far_evaluations = w.far_evaluations;
+            st.near_potential_evaluations = (long long)hc[0];
+            st.near_pair_evaluations = (long long)hc[1];
+            st.far_ms = e->timer.totals[kKernelFarTile].ms;
+            st.near_ms = e->timer.totals[kKernelNearEntry].ms;
+            e->timer.reset();
+            e->timer.enabled = false;
+        }
         if (err) {
             PEEC_ERR("static assembly failed (code %d).", err);
             return -1;
@@ -766,6 +786,11 @@ int peec_engine_kernel_stats(const PeecEngine *e, int kernel, PeecKernelStats *o
 size_t peec_engine_edges(const PeecEngine *e) { return e ? (size_t)e->ne : 0; }
 size_t peec_engine_nodes(const PeecEngine *e) { return e ? (size_t)e->nv : 0; }
 long long peec_engine_far_evaluations(const PeecEngine *e) { return e ? e->far_evals : 0; }
+int peec_engine_static_stats(const PeecEngine *e, PeecStaticStats *out) {
+    if (!e || !out) return -1;
+    *out = e->static_stats;
+    return 0;
+}
 long long peec_engine_smooth_evaluations(const PeecEngine *e) { return e ? e->smooth_evals : 0; }
 
 int peec_engine_static_matrices(const PeecEngine *e, double *L, double *P) {

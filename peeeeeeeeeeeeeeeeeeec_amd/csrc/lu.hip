@@ -2050,20 +2050,42 @@ __device__ __forceinline__ void cmfma4(float ar, float ai, float br, float bi, v
     ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, ci, 0, 0, 0);
 }
 
+// 16 x 16 operand blocks in LDS: element (k, m) at ((k % 4) 4 + k / 4) 16 + m,
+// so that the t-th MFMA's operand read (k = 4 lk + t at lane (m, lk)) is one
+// contiguous 512-byte run of the wavefront.
+__device__ __forceinline__ int blk_slot(int k, int m) { return (((k & 3) << 2) + (k >> 2)) * 16 + m; }
+template <int NBK> __host__ __device__ constexpr size_t trsm_mfma_lds() {
+    return (size_t)(NBK + NBK * (NBK - 1) / 2) * 256 * sizeof(float2);
+}
+
 // L21 rows r_begin .. r_end - 1 (below the diagonal block at k0), in place;
-// workgroups of four wavefronts, 16 rows each.
+// workgroups of four wavefronts, 16 rows each. The workgroup stages the
+// inverses U_jj^-1 and the blocks U_jc (j < c) of U11 in LDS first, all loads
+// in flight together.
 template <int NBK>
 __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t lda, int k0, int r_begin, int r_end,
                                                               const float2 *__restrict__ dinv, const int *status) {
-    constexpr int W = 16 * NBK;
+    constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_u = s_inv + NBK * 256;
     if (status[0] != 0) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int rb = r_begin + ((int)blockIdx.x * 4 + wave) * 16;
-    if (rb >= r_end) return;
-    const bool in = rb + li < r_end;
+    const bool live = rb < r_end, in = rb + li < r_end;
     // lanes past r_end work on a copy of row rb and store nothing
-    float2 *row = A + (size_t)(k0 + W + (in ? rb + li : rb)) * lda + k0;
+    float2 *row = A + (size_t)(k0 + W + (in ? rb + li : live ? rb : r_begin)) * lda + k0;
     const float2 *U = A + (size_t)k0 * lda + k0;
+    const int er = tid >> 4, ec = tid & 15;  // the element of each block this thread stages
+    float2 g[NBK + NU];
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) g[j] = dinv[256 * j + tid];  // U_jj^-1 [er][ec]
+    {
+        int q = NBK;
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int c = j + 1; c < NBK; ++c) g[q++] = U[(size_t)(16 * j + er) * lda + 16 * c + ec];
+    }
     v4f xr[NBK], xi[NBK];
 #pragma unroll
     for (int j = 0; j < NBK; ++j)
@@ -2074,25 +2096,31 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
             xi[j][v] = a.y;
         }
 #pragma unroll
+    for (int q = 0; q < NBK + NU; ++q) s_inv[q * 256 + blk_slot(er, ec)] = g[q];
+    __syncthreads();
+    if (!live) return;
+    const int lo = lk * 16 + li;  // + 64 t: operand of the t-th MFMA
+#pragma unroll
     for (int j = 0; j < NBK; ++j) {
         // X_j^T = (U_jj^-1)^T (A_j - ...)^T
         v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
-        const float2 *ui = dinv + 256 * j + li;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const float2 a = ui[16 * (4 * lk + t)];
+            const float2 a = s_inv[256 * j + 64 * t + lo];
             cmfma4(a.x, a.y, xr[j][t], xi[j][t], nr, ni);
         }
         xr[j] = nr;
         xi[j] = ni;
         // A_c -= X_j U_jc for the tiles right of it: (U_jc)^T X_j^T
 #pragma unroll
-        for (int c = j + 1; c < NBK; ++c)
+        for (int c = j + 1; c < NBK; ++c) {
+            const int p = j * (NBK - 1) - j * (j - 1) / 2 + (c - j - 1);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const float2 a = U[(size_t)(16 * j + 4 * lk + t) * lda + 16 * c + li];
+                const float2 a = s_u[256 * p + 64 * t + lo];
                 cmfma4(-a.x, -a.y, xr[j][t], xi[j][t], xr[c], xi[c]);
             }
+        }
     }
     if (in) {
 #pragma unroll
@@ -2103,7 +2131,8 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
 }
 
 // U12 columns c0 .. c1 - 1 (right of the diagonal block at k0), in place;
-// one wavefront per 16 columns, four per workgroup. With pk, the solved
+// one wavefront per 16 columns, four per workgroup, with (L_jj^-1)^T and the
+// blocks L_cj (c > j) of L11, transposed, staged in LDS. With pk, the solved
 // columns are also written as the split-FP16 operand of the trailing update
 // (block `tile` = the wavefront's 16-column block from c0, pcb blocks per
 // chunk), as pack_u_h3_wave would write them.
@@ -2111,13 +2140,26 @@ template <int NBK>
 __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t lda, int k0, int c0, int c1,
                                                              const float2 *__restrict__ dinv, const int *status,
                                                              uint4 *pk, float *pinv, int pcb) {
+    constexpr int NU = NBK * (NBK - 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_l = s_inv + NBK * 256;
     if (status[0] != 0) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int tile = (int)blockIdx.x * 4 + wave, cb = c0 + 16 * tile;
-    if (cb >= c1) return;
-    const bool in = cb + li < c1;
-    float2 *colp = A + (size_t)k0 * lda + (in ? cb + li : cb);
+    const bool live = cb < c1, in = cb + li < c1;
+    float2 *colp = A + (size_t)k0 * lda + (in ? cb + li : live ? cb : c0);
     const float2 *L = A + (size_t)k0 * lda + k0;
+    const int er = tid >> 4, ec = tid & 15;
+    float2 g[NBK + NU];
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) g[j] = dinv[2048 + 256 * j + tid];  // (L_jj^-1)^T [er][ec]
+    {
+        int q = NBK;
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int c = j + 1; c < NBK; ++c) g[q++] = L[(size_t)(16 * c + er) * lda + 16 * j + ec];  // L_cj[er][ec]
+    }
     v4f xr[NBK], xi[NBK];
 #pragma unroll
     for (int j = 0; j < NBK; ++j)
@@ -2128,24 +2170,33 @@ __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t l
             xi[j][v] = a.y;
         }
 #pragma unroll
+    for (int j = 0; j < NBK; ++j) s_inv[j * 256 + blk_slot(er, ec)] = g[j];
+    // L_cj stored transposed: the A operand's (k, m) = L_cj[m][k]
+#pragma unroll
+    for (int q = 0; q < NU; ++q) s_l[q * 256 + blk_slot(ec, er)] = g[NBK + q];
+    __syncthreads();
+    if (!live) return;
+    const int lo = lk * 16 + li;
+#pragma unroll
     for (int j = 0; j < NBK; ++j) {
         v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
-        const float2 *lt = dinv + 2048 + 256 * j + li;  // (L_jj^-1)^T
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const float2 a = lt[16 * (4 * lk + t)];
+            const float2 a = s_inv[256 * j + 64 * t + lo];
             cmfma4(a.x, a.y, xr[j][t], xi[j][t], nr, ni);
         }
         xr[j] = nr;
         xi[j] = ni;
         // A_c -= L_cj X_j for the tiles below it
 #pragma unroll
-        for (int c = j + 1; c < NBK; ++c)
+        for (int c = j + 1; c < NBK; ++c) {
+            const int p = j * (NBK - 1) - j * (j - 1) / 2 + (c - j - 1);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const float2 a = L[(size_t)(16 * c + li) * lda + 16 * j + 4 * lk + t];
+                const float2 a = s_l[256 * p + 64 * t + lo];
                 cmfma4(-a.x, -a.y, xr[j][t], xi[j][t], xr[c], xi[c]);
             }
+        }
     }
     if (in) {
 #pragma unroll
@@ -2419,14 +2470,18 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     // panel's parity); substitution kernels otherwise
     const bool blocked_solves = nb % 16 == 0 && (nb / 16 == 2 || nb / 16 == 4 || nb / 16 == 8);
     ctx.dblk_inv.reserve(2 * 4096);
+    if (blocked_solves && nb == 128) {
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8>), trsm_mfma_lds<8>());
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_left_mfma_kernel<8>), trsm_mfma_lds<8>());
+    }
     auto dinv_of = [&](int k0) { return ctx.dblk_inv.ptr + (size_t)((k0 / nb) & 1) * 4096; };
     auto solve_l21 = [&](int k0, int w, int rest, int r_begin = 0, int r_end = -1) {
         if (r_end < 0) r_end = rest;
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((r_end - r_begin + 63) / 64);
 #define PEEC_TRSM_R(K)                                                                                           \
-    hipLaunchKernelGGL(trsm_right_mfma_kernel<K>, dim3(grid), dim3(256), 0, s, dA, lda, k0, r_begin, r_end, \
-                       dinv_of(k0), ctx.status.ptr)
+    hipLaunchKernelGGL(trsm_right_mfma_kernel<K>, dim3(grid), dim3(256), trsm_mfma_lds<K>(), s, dA, lda, k0, \
+                       r_begin, r_end, dinv_of(k0), ctx.status.ptr)
             if (w == 128)
                 PEEC_TRSM_R(8);
             else if (w == 64)
@@ -2446,8 +2501,8 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((c1 - c0 + 63) / 64);
 #define PEEC_TRSM_L(K)                                                                                             \
-    hipLaunchKernelGGL(trsm_left_mfma_kernel<K>, dim3(grid), dim3(256), 0, st, dA, lda, k0, c0, c1, dinv_of(k0), \
-                       ctx.status.ptr, pk, pinv, pcb)
+    hipLaunchKernelGGL(trsm_left_mfma_kernel<K>, dim3(grid), dim3(256), trsm_mfma_lds<K>(), st, dA, lda, k0, c0, \
+                       c1, dinv_of(k0), ctx.status.ptr, pk, pinv, pcb)
             if (w == 128)
                 PEEC_TRSM_L(8);
             else if (w == 64)

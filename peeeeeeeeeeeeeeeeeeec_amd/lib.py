@@ -61,7 +61,16 @@ class PeecKernelStats(C.Structure):
 
 
 KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4,
-           "lu_update_split16": 5, "lu_update_thin": 6}
+           "lu_update_split16": 5, "lu_update_thin": 6, "near_entry": 7}
+
+# flop model of the static assembly: 1/R evaluation, analytic triangle potential
+FLOPS_PER_INV_R = 12.0
+FLOPS_PER_TRIANGLE_POTENTIAL = 140.0
+
+
+class PeecStaticStats(C.Structure):
+    _fields_ = [("far_evaluations", C.c_longlong), ("near_potential_evaluations", C.c_longlong),
+                ("near_pair_evaluations", C.c_longlong), ("far_ms", C.c_double), ("near_ms", C.c_double)]
 
 
 class RetardationOptions(C.Structure):
@@ -119,7 +128,7 @@ EXPORTED = [
     "peec_engine_options_default", "peec_engine_create", "peec_engine_create_with_matrices", "peec_engine_solve",
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
-    "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats",
+    "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats", "peec_engine_static_stats",
     "peec_device_count", "peec_device_synchronize", "peec_selftest_mfma_layout", "peec_rcs_options_default", "peec_engine_monostatic_rcs",
     # incident_field.h / scattering.h / rcs.h
     "incident_field_init", "incident_field_configure_harmonic", "incident_field_configure_pulse",
@@ -294,6 +303,7 @@ def load():
     _sig(lib, "peec_device_count", C.c_int)
     _sig(lib, "peec_device_synchronize", C.c_int)
     _sig(lib, "peec_engine_kernel_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(PeecKernelStats))
+    _sig(lib, "peec_engine_static_stats", C.c_int, C.c_void_p, C.POINTER(PeecStaticStats))
     _F = C.POINTER(api.IncidentField)
     _sig(lib, "incident_field_configure_harmonic", C.c_int, _F, C.c_int, C.c_int, C.c_double, C.c_double,
          C.c_double, C.c_double, C.c_double)
@@ -704,6 +714,20 @@ class Engine:
     @property
     def far_evaluations(self):
         return load().peec_engine_far_evaluations(self.handle)
+
+    def static_stats(self):
+        """Evaluation counts and kernel times of the static assembly, with its
+        work in flop-model terms."""
+        st = PeecStaticStats()
+        if load().peec_engine_static_stats(self.handle, C.byref(st)) != 0:
+            raise RuntimeError("peec_engine_static_stats failed")
+        flops = (FLOPS_PER_INV_R * (st.far_evaluations + st.near_pair_evaluations)
+                 + FLOPS_PER_TRIANGLE_POTENTIAL * st.near_potential_evaluations)
+        return {"far_evaluations": st.far_evaluations, "near_potential_evaluations": st.near_potential_evaluations,
+                "near_pair_evaluations": st.near_pair_evaluations, "far_ms": st.far_ms, "near_ms": st.near_ms,
+                "far_flops": FLOPS_PER_INV_R * st.far_evaluations,
+                "near_flops": FLOPS_PER_INV_R * st.near_pair_evaluations
+                + FLOPS_PER_TRIANGLE_POTENTIAL * st.near_potential_evaluations, "flops": flops}
 
     @property
     def smooth_evaluations(self):
