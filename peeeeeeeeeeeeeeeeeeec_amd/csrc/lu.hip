@@ -1508,109 +1508,13 @@ __device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
     lo = (_Float16)(x - (float)hi);
 }
 
-// Split-FP16 packing of one wavefront's 16 rows of L21 (16 columns of U12):
-// lane l holds row (column) l & 15 and complex k = 16 c + 4 (l >> 4) .. + 3 of
-// chunk c. `load(c, t)` fetches element t of the lane's quarter of chunk c;
-// NEG flips the imaginary part (L21 is packed as the pairs (Ar, -Ai)). The
-// scale covers all kc chunks; at most 8 chunks are held in registers (a
-// wider pack takes the maximum over all chunks while holding the upper
-// eight, packs those, then re-loads and packs the lower ones).
-template <bool NEG>
-__device__ __forceinline__ void split_store_chunk(const float2 (&v)[4], float sc, uint4 *o) {
-    unsigned hw[4], lw[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        _Float16 xh, xl, yh, yl;
-        split_half(v[t].x * sc, xh, xl);
-        split_half((NEG ? -v[t].y : v[t].y) * sc, yh, yl);
-        hw[t] = pack_half2(xh, yh);
-        lw[t] = pack_half2(xl, yl);
-    }
-    o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-    o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-}
-
-template <int KCM, bool NEG, class Load, class Out>
-__device__ __forceinline__ float pack_h3_lanes(int kc, Load load, Out out) {
-    static_assert(KCM == 8 || KCM == 16, "8 or 16 chunks");
-    float2 v[8][4];
-    float mx = 0.f;
-    auto fetch = [&](int c0) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                v[c][t] = c0 + c < kc ? load(c0 + c, t) : make_float2(0.f, 0.f);
-                mx = fmaxf(mx, fmaxf(fabsf(v[c][t].x), fabsf(v[c][t].y)));
-            }
-    };
-    auto emit = [&](int c0, float sc) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (c0 + c < kc) split_store_chunk<NEG>(v[c], sc, out(c0 + c));
-    };
-    if constexpr (KCM == 16) fetch(0);  // maximum only: re-loaded below
-    fetch(KCM - 8);
-    mx = fmaxf(mx, __shfl_xor(mx, 16));
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    const float sc = split_scale(mx);
-    emit(KCM - 8, sc);
-    if constexpr (KCM == 16) {
-        fetch(0);
-        emit(0, sc);
-    }
-    return sc;
-}
-
-// One wavefront per 16-row block of L21 = A[row0 + r][k0 + k], r < m (rows
-// past m are packed as zeros), k < kw (a multiple of 16, at most 16 KCM).
-template <int KCM>
-__device__ __forceinline__ void pack_l_h3_wave(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
-                                               int rb_pad, uint4 *out, float *inv, int rb, int lane) {
-    const int r = rb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
-    const bool in = r < m;
-    const float2 *src = A + (size_t)(row0 + (in ? r : 0)) * lda + k0 + 4 * q;
-    const float sc = pack_h3_lanes<KCM, true>(
-        kc, [&](int c, int t) { return in ? src[16 * c + t] : make_float2(0.f, 0.f); },
-        [&](int c) { return out + ((size_t)(c * rb_pad + rb) * 2) * 64 + lane; });
-    if (q == 0) inv[r] = 1.f / sc;
-}
-
-// One wavefront per 16-column block of U12 = A[k0 + k][col0 + c], c < ncols
-// (columns past ncols packed as zeros), as pairs (Br, Bi).
-template <int KCM>
-__device__ __forceinline__ void pack_u_h3_wave(const float2 *A, size_t lda, int k0, int kw, int col0, int ncols,
-                                               int cb_count, uint4 *out, float *inv, int cb, int lane) {
-    const int col = cb * 16 + (lane & 15), q = lane >> 4, kc = kw / 16;
-    const bool in = col < ncols;
-    const float2 *src = A + (size_t)(k0 + 4 * q) * lda + col0 + (in ? col : 0);
-    const float sc = pack_h3_lanes<KCM, false>(
-        kc, [&](int c, int t) { return in ? src[(size_t)(16 * c + t) * lda] : make_float2(0.f, 0.f); },
-        [&](int c) { return out + ((size_t)(c * cb_count + cb) * 2) * 64 + lane; });
-    if (q == 0) inv[col] = 1.f / sc;
-}
-
-// Both packs of an update in one launch: the first lgroups workgroups pack
-// L21 (one wavefront per 16-row block), the others U12 (one wavefront per
-// 16-column block); k runs over kw <= 16 KCM columns of L21 / rows of U12
-// from k0 (the per-row / per-column scales cover all of them).
-// The L21 wavefronts take row blocks rb0, rb0 + 1, ... (of the rows from
-// row0) up to rb_end.
-template <int KCM>
-__global__ __launch_bounds__(256) void pack_lu_h3_kernel(const float2 *A, size_t lda, int row0, int m, int k0, int kw,
-                                                         int rb_pad, uint4 *lout, float *linv, int col0, int ncols,
-                                                         int cb_count, uint4 *uout, float *uinv, int lgroups, int rb0,
-                                                         int rb_end, const int *status) {
-    if (status[0] != 0) return;
-    const int lane = threadIdx.x & 63;
-    if ((int)blockIdx.x < lgroups) {
-        const int rb = rb0 + blockIdx.x * 4 + (threadIdx.x >> 6);
-        if (rb < rb_end) pack_l_h3_wave<KCM>(A, lda, row0, m, k0, kw, rb_pad, lout, linv, rb, lane);
-    } else {
-        const int cb = (blockIdx.x - lgroups) * 4 + (threadIdx.x >> 6);
-        if (cb < cb_count) pack_u_h3_wave<KCM>(A, lda, k0, kw, col0, ncols, cb_count, uout, uinv, cb, lane);
-    }
-}
+// Split-FP16 operand packs (written by the blocked triangular solves,
+// pack_h3_regs): per chunk of 16 complex k and per 16-line block (rows of
+// L21, columns of U12), 64 lanes of uint4 hi halves then 64 of lo halves;
+// lane l holds line l & 15 and k = 16 c + 4 (l >> 4) .. + 3 as (re, im)
+// half pairs, L21 as (re, -im). Each line is scaled by a power of two
+// (split_scale of its largest magnitude over all k of the update), whose
+// inverse is stored beside the pack.
 
 // C -= L21 U12 on the split-FP16 operands, one tile of (32 MB) x (32 NB),
 // NB = 8 / MB, per workgroup of four waves in a 2 x 2 grid (each wave MB x NB
@@ -2058,14 +1962,75 @@ template <int NBK> __host__ __device__ constexpr size_t trsm_mfma_lds() {
     return (size_t)(NBK + NBK * (NBK - 1) / 2) * 256 * sizeof(float2);
 }
 
+// The split-FP16 operand block of a solved 16-row (16-column) block straight
+// from the solve's registers: lane (li, lk) holds entries k = 16 j + 4 lk + v
+// of line li (L21 row / U12 column), the lane order of the operand pack
+// (chunk j, quarter lk, entry v). With NP > 0, the NP chunks of the previous
+// panel of a pair (pre, already solved and stored) come first, and the scale
+// of a line covers both panels. Lines past the matrix (in false) are packed
+// as zeros; NEG packs the pairs (re, -im) (the L21 operand). Chunk c goes to
+// pk[((c stride + blk) 2) 64 + lane] (hi) and + 64 (lo); pinv[16 blk + li] =
+// 1 / scale.
+template <int NBK, int NP, bool NEG>
+__device__ __forceinline__ void pack_h3_regs(const v4f (&xr)[NBK], const v4f (&xi)[NBK],
+                                             const float2 (&pre)[NP > 0 ? NP : 1][4], bool in, uint4 *pk,
+                                             float *pinv, int stride, int blk, int lane) {
+    const int li = lane & 15, lk = lane >> 4;
+    float mx = 0.f;
+#pragma unroll
+    for (int j = 0; j < NBK; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(xr[j][v]), fabsf(xi[j][v])));
+    if constexpr (NP > 0) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(pre[j][v].x), fabsf(pre[j][v].y)));
+    }
+    if (!in) mx = 0.f;
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+    auto emit = [&](int c, const float2 (&val)[4]) {
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            _Float16 xh, xl, yh, yl;
+            split_half(in ? val[v].x * sc : 0.f, xh, xl);
+            split_half(in ? (NEG ? -val[v].y : val[v].y) * sc : 0.f, yh, yl);
+            hw[v] = pack_half2(xh, yh);
+            lw[v] = pack_half2(xl, yl);
+        }
+        uint4 *o = pk + ((size_t)(c * stride + blk) * 2) * 64 + lane;
+        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    };
+    if constexpr (NP > 0) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) emit(j, pre[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) {
+        const float2 val[4] = {make_float2(xr[j][0], xi[j][0]), make_float2(xr[j][1], xi[j][1]),
+                               make_float2(xr[j][2], xi[j][2]), make_float2(xr[j][3], xi[j][3])};
+        emit(NP + j, val);
+    }
+    if (lk == 0) pinv[16 * blk + li] = 1.f / sc;
+}
+
 // L21 rows r_begin .. r_end - 1 (below the diagonal block at k0), in place;
 // workgroups of four wavefronts, 16 rows each. The workgroup stages the
 // inverses U_jj^-1 and the blocks U_jc (j < c) of U11 in LDS first, all loads
-// in flight together.
-template <int NBK>
+// in flight together. With pk, the solved rows are also written as the
+// split-FP16 L21 operand of the update (row block (r_begin + 16 t) / 16 for
+// the t-th wavefront, rb_pad blocks per chunk); with PRE, the k range of the
+// operand starts at k0 - 16 NBK (the first panel of a pair, whose L21 is in
+// place) and covers both panels.
+template <int NBK, bool PRE>
 __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t lda, int k0, int r_begin, int r_end,
-                                                              const float2 *__restrict__ dinv, const int *status) {
-    constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2;
+                                                              const float2 *__restrict__ dinv, const int *status,
+                                                              uint4 *pk, float *pinv, int rb_pad) {
+    constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2, NP = PRE ? NBK : 0;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_u = s_inv + NBK * 256;
     if (status[0] != 0) return;
@@ -2095,6 +2060,15 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
             xr[j][v] = a.x;
             xi[j][v] = a.y;
         }
+    float2 pre[NP > 0 ? NP : 1][4];
+    if constexpr (NP > 0) {
+        if (pk) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pre[j][v] = row[16 * j + 4 * lk + v - W];
+        }
+    }
 #pragma unroll
     for (int q = 0; q < NBK + NU; ++q) s_inv[q * 256 + blk_slot(er, ec)] = g[q];
     __syncthreads();
@@ -2128,26 +2102,28 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
 #pragma unroll
             for (int v = 0; v < 4; ++v) row[16 * j + 4 * lk + v] = make_float2(xr[j][v], xi[j][v]);
     }
+    if (pk) pack_h3_regs<NBK, NP, true>(xr, xi, pre, in, pk, pinv, rb_pad, rb / 16, lane);
 }
 
 // U12 columns c0 .. c1 - 1 (right of the diagonal block at k0), in place;
 // one wavefront per 16 columns, four per workgroup, with (L_jj^-1)^T and the
 // blocks L_cj (c > j) of L11, transposed, staged in LDS. With pk, the solved
-// columns are also written as the split-FP16 operand of the trailing update
+// columns are also written as the split-FP16 U12 operand of the update
 // (block `tile` = the wavefront's 16-column block from c0, pcb blocks per
-// chunk), as pack_u_h3_wave would write them.
-template <int NBK>
+// chunk); with PRE its k range starts at k0 - 16 NBK (both panels of a pair).
+template <int NBK, bool PRE>
 __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t lda, int k0, int c0, int c1,
                                                              const float2 *__restrict__ dinv, const int *status,
                                                              uint4 *pk, float *pinv, int pcb) {
-    constexpr int NU = NBK * (NBK - 1) / 2;
+    constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2, NP = PRE ? NBK : 0;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_l = s_inv + NBK * 256;
     if (status[0] != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int tile = (int)blockIdx.x * 4 + wave, cb = c0 + 16 * tile;
     const bool live = cb < c1, in = cb + li < c1;
-    float2 *colp = A + (size_t)k0 * lda + (in ? cb + li : live ? cb : c0);
+    const int col = in ? cb + li : live ? cb : c0;
+    float2 *colp = A + (size_t)k0 * lda + col;
     const float2 *L = A + (size_t)k0 * lda + k0;
     const int er = tid >> 4, ec = tid & 15;
     float2 g[NBK + NU];
@@ -2169,6 +2145,16 @@ __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t l
             xr[j][v] = a.x;
             xi[j][v] = a.y;
         }
+    float2 pre[NP > 0 ? NP : 1][4];
+    if constexpr (NP > 0) {
+        if (pk) {
+            const float2 *pp = A + (size_t)(k0 - W) * lda + col;
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pre[j][v] = pp[(size_t)(16 * j + 4 * lk + v) * lda];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < NBK; ++j) s_inv[j * 256 + blk_slot(er, ec)] = g[j];
     // L_cj stored transposed: the A operand's (k, m) = L_cj[m][k]
@@ -2204,35 +2190,7 @@ __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t l
 #pragma unroll
             for (int v = 0; v < 4; ++v) colp[(size_t)(16 * j + 4 * lk + v) * lda] = make_float2(xr[j][v], xi[j][v]);
     }
-    if (pk) {
-        // lane (li, lk) holds rows 16 j + 4 lk + v of column li: the lane
-        // order of the operand pack (chunk j, quarter lk, entry v)
-        float mx = 0.f;
-#pragma unroll
-        for (int j = 0; j < NBK; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(xr[j][v]), fabsf(xi[j][v])));
-        if (!in) mx = 0.f;
-        mx = fmaxf(mx, __shfl_xor(mx, 16));
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        const float sc = split_scale(mx);
-#pragma unroll
-        for (int j = 0; j < NBK; ++j) {
-            unsigned hw[4], lw[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                _Float16 xh, xl, yh, yl;
-                split_half(in ? xr[j][v] * sc : 0.f, xh, xl);
-                split_half(in ? xi[j][v] * sc : 0.f, yh, yl);
-                hw[v] = pack_half2(xh, yh);
-                lw[v] = pack_half2(xl, yl);
-            }
-            uint4 *o = pk + ((size_t)(j * pcb + tile) * 2) * 64 + lane;
-            o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-            o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-        }
-        if (lk == 0) pinv[tile * 16 + li] = 1.f / sc;
-    }
+    if (pk) pack_h3_regs<NBK, NP, false>(xr, xi, pre, in, pk, pinv, pcb, tile, lane);
 }
 
 // Triangular solves of the ordered factor, one wavefront per row of
@@ -2471,47 +2429,68 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     const bool blocked_solves = nb % 16 == 0 && (nb / 16 == 2 || nb / 16 == 4 || nb / 16 == 8);
     ctx.dblk_inv.reserve(2 * 4096);
     if (blocked_solves && nb == 128) {
-        set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8>), trsm_mfma_lds<8>());
-        set_dynamic_lds(reinterpret_cast<const void *>(trsm_left_mfma_kernel<8>), trsm_mfma_lds<8>());
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8, false>), trsm_mfma_lds<8>());
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8, true>), trsm_mfma_lds<8>());
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_left_mfma_kernel<8, false>), trsm_mfma_lds<8>());
+        set_dynamic_lds(reinterpret_cast<const void *>(trsm_left_mfma_kernel<8, true>), trsm_mfma_lds<8>());
     }
     auto dinv_of = [&](int k0) { return ctx.dblk_inv.ptr + (size_t)((k0 / nb) & 1) * 4096; };
-    auto solve_l21 = [&](int k0, int w, int rest, int r_begin = 0, int r_end = -1) {
+    // With pk (block widths 64 and 128), the solved rows of L21 are also
+    // written as the split-FP16 operand (stride row blocks per chunk), with
+    // pre over the previous panel's k range as well.
+    auto solve_l21 = [&](int k0, int w, int rest, int r_begin = 0, int r_end = -1, uint4 *pk = nullptr,
+                         float *pinv = nullptr, int stride = 0, bool pre = false) {
         if (r_end < 0) r_end = rest;
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((r_end - r_begin + 63) / 64);
-#define PEEC_TRSM_R(K)                                                                                           \
-    hipLaunchKernelGGL(trsm_right_mfma_kernel<K>, dim3(grid), dim3(256), trsm_mfma_lds<K>(), s, dA, lda, k0, \
-                       r_begin, r_end, dinv_of(k0), ctx.status.ptr)
-            if (w == 128)
-                PEEC_TRSM_R(8);
+#define PEEC_TRSM_R(K, P)                                                                                          \
+    hipLaunchKernelGGL((trsm_right_mfma_kernel<K, P>), dim3(grid), dim3(256), trsm_mfma_lds<K>(), s, dA, lda, k0, \
+                       r_begin, r_end, dinv_of(k0), ctx.status.ptr, pk, pinv, stride)
+            if (w == 128 && pre)
+                PEEC_TRSM_R(8, true);
+            else if (w == 128)
+                PEEC_TRSM_R(8, false);
+            else if (w == 64 && pre)
+                PEEC_TRSM_R(4, true);
             else if (w == 64)
-                PEEC_TRSM_R(4);
+                PEEC_TRSM_R(4, false);
+            else if (!pk)
+                PEEC_TRSM_R(2, false);
             else
-                PEEC_TRSM_R(2);
+                throw HipError{"ordered factor: packed L21 solves need a block width of 64 or 128"};
 #undef PEEC_TRSM_R
             return;
         }
+        if (pk) throw HipError{"ordered factor: packed L21 solves need the blocked solve kernels"};
         const int grid = std::max(1, std::min((r_end - r_begin + 15) / 16, ctx.cus));
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
                            dA, lda, k0, w, r_begin, r_end, ctx.status.ptr);
     };
     // with pk, U12 is also packed as the split-FP16 operand of pcb blocks
+    // (and with pre, over the previous panel's k range as well)
     auto solve_u12 = [&](int k0, int w, int c0, int c1, hipStream_t st, uint4 *pk = nullptr, float *pinv = nullptr,
-                         int pcb = 0) {
+                         int pcb = 0, bool pre = false) {
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((c1 - c0 + 63) / 64);
-#define PEEC_TRSM_L(K)                                                                                             \
-    hipLaunchKernelGGL(trsm_left_mfma_kernel<K>, dim3(grid), dim3(256), trsm_mfma_lds<K>(), st, dA, lda, k0, c0, \
-                       c1, dinv_of(k0), ctx.status.ptr, pk, pinv, pcb)
-            if (w == 128)
-                PEEC_TRSM_L(8);
+#define PEEC_TRSM_L(K, P)                                                                                          \
+    hipLaunchKernelGGL((trsm_left_mfma_kernel<K, P>), dim3(grid), dim3(256), trsm_mfma_lds<K>(), st, dA, lda, k0, \
+                       c0, c1, dinv_of(k0), ctx.status.ptr, pk, pinv, pcb)
+            if (w == 128 && pre)
+                PEEC_TRSM_L(8, true);
+            else if (w == 128)
+                PEEC_TRSM_L(8, false);
+            else if (w == 64 && pre)
+                PEEC_TRSM_L(4, true);
             else if (w == 64)
-                PEEC_TRSM_L(4);
+                PEEC_TRSM_L(4, false);
+            else if (!pk)
+                PEEC_TRSM_L(2, false);
             else
-                PEEC_TRSM_L(2);
+                throw HipError{"ordered factor: packed U12 solves need a block width of 64 or 128"};
 #undef PEEC_TRSM_L
             return;
         }
+        if (pre) throw HipError{"ordered factor: two-panel U12 packs need the blocked solve kernels"};
         const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
         const int grid = std::max(1, std::min(tiles, ctx.cus));
         hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
@@ -2527,6 +2506,20 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     bool pending = false;    // a diagonal factor is in flight on aux
     bool pair_open = false;  // the previous panel began a pair of split-FP16 panels
     int wprev = 0;
+    // split-FP16 operand buffers at their largest (the first panels), so
+    // that no buffer moves while the other stream may use it
+    const bool split_panels = split16 && blocked_solves && (nb == 128 || nb == 64) && n > nb;
+    if (split_panels) {
+        int rbp, cbp;
+        gemm_pads<S>(n, n, rbp, cbp);
+        const int kcm = 2 * nb / 16, cbm = (n + 127) / 128 * 8;
+        ctx.pack_l16.reserve((size_t)kcm * rbp * 128 * sizeof(uint4));
+        ctx.scale_l16.reserve((size_t)rbp * 16);
+        ctx.pack_u16n.reserve((size_t)kcm * 8 * 128 * sizeof(uint4));
+        ctx.scale_u16n.reserve((size_t)8 * 16);
+        ctx.pack_u16.reserve((size_t)kcm * cbm * 128 * sizeof(uint4));
+        ctx.scale_u16.reserve((size_t)cbm * 16);
+    }
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int w = std::min(nb, n - k0), rest = n - k0 - w;
         if (rest <= 0) {
@@ -2535,6 +2528,17 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             break;
         }
         const int w2 = std::min(nb, rest), r0 = k0 + w;
+        int rb_pad, cb_pad;
+        gemm_pads<S>(rest, rest, rb_pad, cb_pad);
+        const int rest2 = rest - w2;
+        // split-FP16 pair state of this panel (the second panel of a pair
+        // packs both panels' L21 / U12 with one scale per row / column)
+        const bool h3 = split_panels && w == nb;
+        const bool second = h3 && pair_open && wprev == w;
+        const int cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
+        uint4 *pln = reinterpret_cast<uint4 *>(ctx.pack_l16.ptr);
+        uint4 *pun = reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr);
+        uint4 *pu = reinterpret_cast<uint4 *>(ctx.pack_u16.ptr);
         // U12 of the next block column needs this block's diagonal factor
         // and the previous trailing update; with a look-ahead factor in
         // flight it runs on the second stream behind both, beside L21
@@ -2542,16 +2546,16 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         if (pending) {
             HIP_CHECK(hipEventRecord(ctx.ev_prev, s));
             HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_prev, 0));
-            solve_u12(k0, w, r0, r0 + w2, aux);
+            if (h3)
+                solve_u12(k0, w, r0, r0 + w2, aux, pun, ctx.scale_u16n.ptr, cbn, second);
+            else
+                solve_u12(k0, w, r0, r0 + w2, aux);
             HIP_CHECK(hipEventRecord(ctx.ev_u12, aux));
             HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
             pending = false;
         }
         const int ks_count = (w + 3) / 4;
-        int rb_pad, cb_pad;
-        gemm_pads<S>(rest, rest, rb_pad, cb_pad);
-        const int rest2 = rest - w2;
-        if (split16 && (w == 128 || w == 64)) {
+        if (h3) {
             // Split-FP16 updates, in pairs of block columns (the first panel of
             // the matrix starts a pair). The first panel of a pair updates the
             // next block column (look-ahead) and, of the rest of the matrix,
@@ -2561,30 +2565,10 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             // trailing matrix is read and written once per pair. L21 and U12
             // stay in A between the two panels and are packed together with
             // one scale per row / column over all 2w of k.
-            const bool second = pair_open && wprev == w;
-            const int kp = second ? k0 - wprev : k0, kw = second ? 2 * w : w, kc = kw / 16;
-            const int cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
-            ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
-            ctx.scale_l16.reserve((size_t)rb_pad * 16);
-            ctx.pack_u16n.reserve((size_t)kc * cbn * 128 * sizeof(uint4));
-            ctx.scale_u16n.reserve((size_t)cbn * 16);
-            // L21 row blocks [rb_a, rb_b) (of the rows from r0) and, with cbc,
-            // U12 of columns col0.. in one launch
-            auto pack = [&](int rb_a, int rb_b, int col0, int ncols, int cbc, uint4 *uo, float *ui) {
-                const int lgroups = (rb_b - rb_a + 3) / 4, ugroups = (cbc + 3) / 4;
-                if (lgroups + ugroups == 0) return;
-                if (kw > 128)
-                    hipLaunchKernelGGL(pack_lu_h3_kernel<16>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
-                                       lda, r0, rest, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
-                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, rb_a, rb_b,
-                                       ctx.status.ptr);
-                else
-                    hipLaunchKernelGGL(pack_lu_h3_kernel<8>, dim3((unsigned)(lgroups + ugroups)), dim3(256), 0, s, dA,
-                                       lda, r0, rest, kp, kw, rb_pad, reinterpret_cast<uint4 *>(ctx.pack_l16.ptr),
-                                       ctx.scale_l16.ptr, col0, ncols, cbc, uo, ui, lgroups, rb_a, rb_b,
-                                       ctx.status.ptr);
-            };
-            uint4 *pun = reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr);
+            // The solves write the operands of the updates as they go (the
+            // split-FP16 packs of L21 rows and U12 columns; pad rows and
+            // columns of the packs only feed results that are never stored).
+            const int kw = second ? 2 * w : w;
             auto start_next_diag = [&]() {
                 HIP_CHECK(hipEventRecord(ctx.ev_next, s));
                 HIP_CHECK(hipStreamWaitEvent(aux, ctx.ev_next, 0));
@@ -2601,34 +2585,28 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                 // block of the L21 pack (a last, narrower next panel takes
                 // the path below).
                 const int rbd = w2 / 16;
-                solve_l21(k0, w, rest, 0, w2);
-                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
+                solve_l21(k0, w, rest, 0, w2, pln, ctx.scale_l16.ptr, rb_pad, false);
+                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, false);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
-                pack(0, rbd, r0, w2, cbn, pun, ctx.scale_u16n.ptr);
                 gemm_h3(dA, lda, r0, r0, w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
                         ctx.status.ptr, ctx, s);
                 start_next_diag();
                 if (rest > w2) {
-                    solve_l21(k0, w, rest, w2, rest);
-                    pack(rbd, rb_pad, 0, 0, 0, nullptr, nullptr);
+                    solve_l21(k0, w, rest, w2, rest, pln, ctx.scale_l16.ptr, rb_pad, false);
                     gemm_h3(dA, lda, r0 + w2, r0, rest - w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr,
                             kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd);
                 }
             } else {
                 // L21 (rows r0..) and U12 of the next block column, then its
                 // update (rank 2w for a pair's second panel; critical path)
-                solve_l21(k0, w, rest);
-                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s);
+                solve_l21(k0, w, rest, 0, rest, pln, ctx.scale_l16.ptr, rb_pad, second);
+                if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, second);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
-                pack(0, rb_pad, r0, w2, cbn, pun, ctx.scale_u16n.ptr);
                 gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
                         ctx.status.ptr, ctx, s);
                 start_next_diag();
             }
             if (rest2 > 0) {
-                ctx.pack_u16.reserve((size_t)kc * cb16 * 128 * sizeof(uint4));
-                ctx.scale_u16.reserve((size_t)cb16 * 16);
-                uint4 *pu = reinterpret_cast<uint4 *>(ctx.pack_u16.ptr);
                 if (!second) {
                     // the rest of U12, packed by the solve itself (blocks past
                     // the last column are never read into a stored result);
@@ -2637,10 +2615,9 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                     gemm_h3(dA, lda, r0, r0 + w2, w2, rest2, w, rb_pad, cb16, pu, ctx.scale_u16.ptr,
                             kKernelLuUpdateThin, ctx.status.ptr, ctx, s);
                 } else {
-                    // the rest of this panel's U12, then both panels' U12 rows
-                    // packed together, and the rank-2w trailing update
-                    solve_u12(k0, w, r0 + w2, n, s);
-                    pack(0, 0, r0 + w2, rest2, cb16, pu, ctx.scale_u16.ptr);
+                    // the rest of this panel's U12, packed with the first
+                    // panel's U12 rows, and the rank-2w trailing update
+                    solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16, true);
                     gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, ctx.scale_u16.ptr,
                             kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s);
                 }
