@@ -14,53 +14,13 @@
 #include "assembly.h"
 #include "harmonic_kernels.h"
 #include "lu.h"
+#include "mna.h"
 #include "rcs_kernels.h"
 
 namespace peec {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kC0 = 299792458.0;  // main.c:47-54
-
-// One entry of Z = [[R + jwL, A], [-P A^T, jw I]].
-__device__ inline double2 mna_entry(int row, int col, int ne, int nv, const double *L, const double2 *Lw,
-                                    const double *R, const double *P, const double2 *Pw, const double *PAT,
-                                    const int *edges, double omega) {
-    double2 z;
-    if (row < ne) {
-        if (col < ne) {
-            const size_t k = (size_t)row * ne + col;
-            const double r = (row == col && R) ? R[row] : 0.0;
-            if (Lw) {
-                const double2 l = Lw[k];
-                z = make_double2(r - omega * l.y, omega * l.x);
-            } else {
-                z = make_double2(r, omega * L[k]);
-            }
-        } else {
-            const int node = col - ne;
-            double a = 0.0;
-            if (node == edges[2 * row]) a = -1.0;
-            if (node == edges[2 * row + 1]) a = 1.0;
-            z = make_double2(a, 0.0);
-        }
-    } else {
-        const int i = row - ne;
-        if (col < ne) {
-            const int a = edges[2 * col], b = edges[2 * col + 1];
-            if (Pw) {
-                const double2 pb = Pw[(size_t)i * nv + b], pa = Pw[(size_t)i * nv + a];
-                z = make_double2(-(pb.x - pa.x), -(pb.y - pa.y));
-            } else if (PAT) {
-                z = make_double2(-PAT[(size_t)i * ne + col], 0.0);
-            } else {
-                z = make_double2(-(P[(size_t)i * nv + b] - P[(size_t)i * nv + a]), 0.0);
-            }
-        } else {
-            z = make_double2(0.0, (col - ne) == i ? omega : 0.0);
-        }
-    }
-    return z;
-}
 
 // Z = [[R + jwL, A], [-P A^T, jw I]] written row-major with leading
 // dimension ldz; A is implicit in the edge end points (incidence.c:34-115).
@@ -74,13 +34,24 @@ __global__ __launch_bounds__(256) void mna_fill_kernel(double2 *Z, double2 *Z0, 
                                                        const int *edges, double omega, int chunks, double *rowmax) {
     const int n = ne + nv;
     const int row = blockIdx.x / chunks;
+    MnaOperands mo;
+    mo.ne = ne;
+    mo.nv = nv;
+    mo.L = L;
+    mo.Lw = Lw;
+    mo.R = R;
+    mo.P = P;
+    mo.Pw = Pw;
+    mo.PAT = PAT;
+    mo.edges = edges;
+    mo.omega = omega;
     // four entries per thread, 1024 columns per workgroup
     const int col0 = (blockIdx.x % chunks) * 1024 + threadIdx.x;
     double2 z[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int col = col0 + 256 * u;
-        z[u] = col < n ? mna_entry(row, col, ne, nv, L, Lw, R, P, Pw, PAT, edges, omega) : double2{};
+        z[u] = col < n ? mna_entry(row, col, mo) : double2{};
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -251,6 +222,11 @@ struct PeecEngine {
     // the factor held for (cached_f, cached_mode) may serve further
     // right-hand sides at that point without a new assembly or factor
     bool z_equilibrated = false;  // d_Z holds this point's equilibrated FP64 matrix
+    // the MNA matrix of the current point as a function of its blocks; its
+    // dense copy d_Z0 is written only when a pass needs it (the FP64
+    // fallback, residuals of many right-hand sides)
+    MnaOperands mna;
+    bool z0_ready = false;
     bool shift_ok = true;         // shifted-center moments may still be built
     bool cache_ok = false;
     double cached_f = 0.0;
@@ -304,6 +280,16 @@ __global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *
 
 static int assemble_retarded(PeecEngine *e, double k, double *center = nullptr);
 
+// the dense copy of the current point's MNA matrix in d_Z0
+static void ensure_z0(PeecEngine *e) {
+    if (e->z0_ready) return;
+    const MnaOperands &m = e->mna;
+    const int n = m.ne + m.nv;
+    e->d_Z0.resize((size_t)n * n);
+    launch_mna_fill(nullptr, e->d_Z0.ptr, n, m.ne, m.nv, m.L, m.Lw, m.R, m.P, m.Pw, m.PAT, m.edges, m.omega, e->stream);
+    e->z0_ready = true;
+}
+
 static int factor_error(const LUStatus &st) {
     if (st.code == -2) PEEC_ERR("complex LU singular at column %lld.", st.column);
     else PEEC_ERR("LU failed (code %d).", st.code);
@@ -323,6 +309,7 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
     if (fp64) {
         if (!e->z_equilibrated) {
             // mixed point falling back to FP64: form the equilibrated matrix
+            ensure_z0(e);
             e->d_Z.resize((size_t)n * n);
             copy_equilibrated(e->d_Z0.ptr, e->d_Z.ptr, e->d_row.ptr, e->d_col.ptr, n, s);
             e->z_equilibrated = true;
@@ -330,8 +317,8 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
         st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
     } else {
         double max_scaled = 0.0;
-        if (convert_equilibrated_to_f32(e->d_Z0.ptr, e->d_Z32.ptr, nullptr, e->d_row.ptr, e->d_col.ptr, n,
-                                        &max_scaled, *e->lu) != 0)
+        if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr, nullptr, e->d_row.ptr, e->d_col.ptr, &max_scaled,
+                                        *e->lu) != 0)
             return -2;
         e->limit = 1.0e-14 * max_scaled;
         st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
@@ -366,22 +353,29 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
     if (ret) report.moment_terms = assemble_retarded(e, omega / kC0, &report.moment_center);
     HIP_CHECK(hipEventRecord(e->ev[1], s));
     const bool mixed = mode == PEEC_SOLVE_MIXED;
-    e->d_Z0.resize(nn);
+    MnaOperands &m = e->mna;
+    m = MnaOperands{};
+    m.ne = ne;
+    m.nv = nv;
+    m.L = e->d_L.ptr;
+    m.Lw = ret ? e->d_Lw.ptr : nullptr;
+    m.P = e->d_P.ptr;
+    m.Pw = ret ? e->d_Pw.ptr : nullptr;
+    m.edges = e->d_edges.ptr;
+    m.omega = omega;
+    e->z0_ready = false;
     e->d_row.resize(n);
     e->d_col.resize(n);
     e->z_equilibrated = false;
     if (mixed) {
-        // the FP32 factor is formed straight from Z0 with the equilibration's
-        // scales (read-only passes); the equilibrated FP64 matrix is only
-        // built if the point falls back to FP64
-        zero_doubles(e->d_row.ptr, n, s);
-        launch_mna_fill(nullptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
-                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s, e->d_row.ptr);
-        if (equilibration_scales(e->d_Z0.ptr, n, n, e->d_row.ptr, e->d_col.ptr, *e->lu, true) != 0) return -2;
+        // the FP32 factor is formed from the MNA blocks with the
+        // equilibration's scales (read-only passes that evaluate the
+        // entries); the equilibrated FP64 matrix is only built if the point
+        // falls back to FP64
+        if (equilibration_scales(m, e->d_row.ptr, e->d_col.ptr, *e->lu) != 0) return -2;
     } else {
         e->d_Z.resize(nn);
-        launch_mna_fill(e->d_Z.ptr, e->d_Z0.ptr, n, ne, nv, e->d_L.ptr, ret ? e->d_Lw.ptr : nullptr, nullptr,
-                        e->d_P.ptr, ret ? e->d_Pw.ptr : nullptr, nullptr, e->d_edges.ptr, omega, s);
+        launch_mna_fill(e->d_Z.ptr, nullptr, n, ne, nv, m.L, m.Lw, nullptr, m.P, m.Pw, nullptr, m.edges, omega, s);
         double max_scaled = 0.0;
         if (equilibrate_complex(e->d_Z.ptr, n, n, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) != 0) return -2;
         e->z_equilibrated = true;
@@ -397,8 +391,8 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
         e->d_Z32.resize(nn);
         if (e->opt.reuse_pivots && e->have_reuse) {
             double max_scaled = 0.0;
-            if (convert_equilibrated_to_f32(e->d_Z0.ptr, e->d_Z32.ptr, e->d_reuse_perm.ptr, e->d_row.ptr,
-                                            e->d_col.ptr, n, &max_scaled, *e->lu) != 0)
+            if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr, e->d_reuse_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+                                            &max_scaled, *e->lu) != 0)
                 return -2;
             e->limit = 1.0e-14 * max_scaled;
             static constexpr int kTrialWidths[2] = {128, 64};
@@ -456,7 +450,12 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
     std::vector<double> eta(nrhs);
     std::vector<unsigned long long> bits(nrhs);
     auto measure = [&]() {
-        residual_multi(e->d_Z0.ptr, n, n, e->d_X.ptr, e->d_B.ptr, e->d_R.ptr, nrhs, e->d_ax.ptr, e->d_eta.ptr, s);
+        if (nrhs < kResidualDenseMinRhs) {
+            residual_multi(e->mna, e->d_X.ptr, e->d_B.ptr, e->d_R.ptr, nrhs, e->d_ax.ptr, e->d_eta.ptr, s);
+        } else {
+            ensure_z0(e);
+            residual_multi(e->d_Z0.ptr, n, n, e->d_X.ptr, e->d_B.ptr, e->d_R.ptr, nrhs, e->d_ax.ptr, e->d_eta.ptr, s);
+        }
         e->d_eta.download(bits.data(), nrhs, s);
         e->stream.sync();
         for (int q = 0; q < nrhs; ++q) {

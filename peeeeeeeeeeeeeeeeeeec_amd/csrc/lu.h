@@ -8,6 +8,7 @@
 #pragma once
 
 #include "common.h"
+#include "mna.h"
 
 namespace peec {
 
@@ -68,6 +69,9 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
 // already holds the row maxima) ...
 int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx,
                          bool rows_ready = false);
+// ... or of the MNA matrix of m, evaluated from its blocks (row maxima
+// included) ...
+int equilibration_scales(const MnaOperands &m, double *d_row, double *d_col, LUContext &ctx);
 // p[0..n) = 0 on stream s
 void zero_doubles(double *p, size_t n, hipStream_t s);
 // ... the equilibrated entries (a_ij / r_i) / c_j in FP32, rows perm[i] (or
@@ -75,6 +79,8 @@ void zero_doubles(double *p, size_t n, hipStream_t s);
 // are those equilibrate_complex would produce, rounded once to FP32 ...
 int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_perm, const double *d_row,
                                 const double *d_col, int n, double *max_scaled, LUContext &ctx);
+int convert_equilibrated_to_f32(const MnaOperands &m, float2 *dst, const int *d_perm, const double *d_row,
+                                const double *d_col, double *max_scaled, LUContext &ctx);
 // ... and the equilibrated FP64 matrix when it is needed after all.
 void copy_equilibrated(const double2 *src, double2 *dst, const double *d_row, const double *d_col, int n,
                        hipStream_t s);
@@ -120,6 +126,12 @@ void complex_residual(const double2 *dA, int rows, int cols, size_t lda, const d
 // a non-negative double in d_eta_bits[q]. d_ax: n * nrhs doubles of scratch.
 void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, const double2 *dB, double2 *dR,
                     int nrhs, double *d_ax, unsigned long long *d_eta_bits, hipStream_t s);
+// The same with the MNA matrix of m evaluated from its blocks (the same
+// entries, sums and result as with its dense copy), for fewer than
+// kResidualDenseMinRhs right-hand sides; more take the dense matrix.
+constexpr int kResidualDenseMinRhs = 16;
+void residual_multi(const MnaOperands &m, const double2 *dX, const double2 *dB, double2 *dR, int nrhs, double *d_ax,
+                    unsigned long long *d_eta_bits, hipStream_t s);
 
 // In-bounds self-test of the MFMA lane layout (1024 doubles out).
 void mfma_layout_probe(double *host_out);

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "lu.h"
+#include "mna.h"
 
 namespace peec {
 
@@ -173,13 +174,27 @@ __device__ inline double eshfl(double v, int src) { return __shfl(v, src); }
 // ---------------------------------------------------------------------------
 // equilibration (complex_lu.c:275-320)
 
-__global__ void row_max_kernel(const double2 *A, int n, size_t lda, double *row, int *bad) {
+// Entry sources of the read-only passes over a complex matrix: a dense
+// row-major array, or the MNA matrix evaluated from its blocks (the same
+// value per entry as the dense copy the fill kernel would write).
+struct DenseSource {
+    const double2 *A;
+    size_t lda;
+    __device__ double2 operator()(int i, int j) const { return A[(size_t)i * lda + j]; }
+};
+struct MnaSource {
+    MnaOperands m;
+    __device__ double2 operator()(int i, int j) const { return mna_entry(i, j, m); }
+};
+
+template <class Src>
+__global__ void row_max_kernel(Src src, int n, double *row, int *bad) {
     __shared__ double red[256];
     const int i = blockIdx.x;
     double m = 0.0;
     bool finite = true;
     for (int j = threadIdx.x; j < n; j += 256) {
-        const double v = mag(A[(size_t)i * lda + j]);
+        const double v = mag(src(i, j));
         if (!isfinite(v)) finite = false;
         m = fmax(m, v);
     }
@@ -237,13 +252,14 @@ __global__ void scale_cols_kernel(double2 *A, int n, size_t lda, const double *c
 
 // column maxima of the row-scaled matrix, |a_ij / r_i| (the arithmetic of
 // scale_rows_colmax_kernel), without writing the matrix
-__global__ void colmax_scaled_kernel(const double2 *A, int n, size_t lda, const double *row, double *col) {
+template <class Src>
+__global__ void colmax_scaled_kernel(Src src, int n, const double *row, double *col) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const int i0 = blockIdx.y * 64, i1 = min(n, i0 + 64);
     double m = 0.0;
     for (int i = i0; i < i1; ++i) {
-        double2 v = A[(size_t)i * lda + j];
+        double2 v = src(i, j);
         const double r = row[i];
         v.x = v.x / r;
         v.y = v.y / r;
@@ -266,7 +282,8 @@ __device__ inline double2 equilibrated(double2 v, double r, double c) {
 // |equilibrated entry| into maxv[0 .. kMaxSlots). One workgroup per (row,
 // 256 columns).
 constexpr int kMaxSlots = 1024;
-__global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(const double2 *src, float2 *dst, const int *perm,
+template <class Src>
+__global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(Src src, float2 *dst, const int *perm,
                                                                   const double *row, const double *col, int n,
                                                                   double *maxv) {
     __shared__ double red[4];
@@ -274,10 +291,9 @@ __global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(const double2 
     const int i = blockIdx.y, c0 = blockIdx.x * 1024 + threadIdx.x;
     const int si = perm ? perm[i] : i;
     const double ri = row[si];
-    const double2 *srow = src + (size_t)si * n;
     double2 a[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = c0 + 256 * u < n ? srow[c0 + 256 * u] : double2{};
+    for (int u = 0; u < 4; ++u) a[u] = c0 + 256 * u < n ? src(si, c0 + 256 * u) : double2{};
     double m = 0.0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -362,7 +378,7 @@ int equilibrate_complex(double2 *dA, int n, size_t lda, double *d_row, double *d
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, s, mx.ptr, (size_t)1);
     hipLaunchKernelGGL(fill_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, (size_t)n);
-    hipLaunchKernelGGL(row_max_kernel, dim3(n), dim3(256), 0, s, dA, n, lda, d_row, bad.ptr);
+    hipLaunchKernelGGL(row_max_kernel<DenseSource>, dim3(n), dim3(256), 0, s, DenseSource{dA, lda}, n, d_row, bad.ptr);
     int h_bad = 0;
     bad.download(&h_bad, 1, s);
     HIP_CHECK(hipStreamSynchronize(s));
@@ -387,8 +403,8 @@ void zero_doubles(double *p, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
 }
 
-int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx,
-                         bool rows_ready) {
+template <class Src>
+static int equilibration_scales_of(Src src, int n, double *d_row, double *d_col, LUContext &ctx, bool rows_ready) {
     hipStream_t s = ctx.stream;
     DeviceBuffer<int> bad(1);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
@@ -396,18 +412,26 @@ int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, do
     if (rows_ready)
         hipLaunchKernelGGL(row_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_row, n, bad.ptr);
     else
-        hipLaunchKernelGGL(row_max_kernel, dim3(n), dim3(256), 0, s, dA, n, lda, d_row, bad.ptr);
+        hipLaunchKernelGGL(row_max_kernel<Src>, dim3(n), dim3(256), 0, s, src, n, d_row, bad.ptr);
     dim3 grid((n + 255) / 256, (n + 63) / 64);
-    hipLaunchKernelGGL(colmax_scaled_kernel, grid, dim3(256), 0, s, dA, n, lda, d_row, d_col);
+    hipLaunchKernelGGL(colmax_scaled_kernel<Src>, grid, dim3(256), 0, s, src, n, d_row, d_col);
     HIP_CHECK(hipGetLastError());
     int h_bad = 0;
     bad.download(&h_bad, 1, s);
     HIP_CHECK(hipStreamSynchronize(s));
     return h_bad ? -2 : 0;
 }
+int equilibration_scales(const double2 *dA, int n, size_t lda, double *d_row, double *d_col, LUContext &ctx,
+                         bool rows_ready) {
+    return equilibration_scales_of(DenseSource{dA, lda}, n, d_row, d_col, ctx, rows_ready);
+}
+int equilibration_scales(const MnaOperands &m, double *d_row, double *d_col, LUContext &ctx) {
+    return equilibration_scales_of(MnaSource{m}, m.ne + m.nv, d_row, d_col, ctx, false);
+}
 
-int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_perm, const double *d_row,
-                                const double *d_col, int n, double *max_scaled, LUContext &ctx) {
+template <class Src>
+static int convert_equilibrated_to_f32_of(Src src, float2 *dst, const int *d_perm, const double *d_row,
+                                          const double *d_col, int n, double *max_scaled, LUContext &ctx) {
     hipStream_t s = ctx.stream;
     DeviceBuffer<double> mx(kMaxSlots);
     hipLaunchKernelGGL(fill_zero_kernel, dim3(kMaxSlots / 256), dim3(256), 0, s, mx.ptr, (size_t)kMaxSlots);
@@ -415,8 +439,8 @@ int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_per
     DeviceBuffer<int> bad(1);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
     hipLaunchKernelGGL(zero_scale_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, n, bad.ptr);
-    hipLaunchKernelGGL(to_f32_equilibrated_kernel, dim3((unsigned)((n + 1023) / 1024), (unsigned)n), dim3(256), 0, s,
-                       dA, dst, d_perm, d_row, d_col, n, mx.ptr);
+    hipLaunchKernelGGL(to_f32_equilibrated_kernel<Src>, dim3((unsigned)((n + 1023) / 1024), (unsigned)n), dim3(256), 0,
+                       s, src, dst, d_perm, d_row, d_col, n, mx.ptr);
     HIP_CHECK(hipGetLastError());
     int h_bad = 0;
     std::vector<double> slots(kMaxSlots);
@@ -425,6 +449,14 @@ int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_per
     HIP_CHECK(hipStreamSynchronize(s));
     *max_scaled = *std::max_element(slots.begin(), slots.end());
     return h_bad ? -2 : 0;
+}
+int convert_equilibrated_to_f32(const double2 *dA, float2 *dst, const int *d_perm, const double *d_row,
+                                const double *d_col, int n, double *max_scaled, LUContext &ctx) {
+    return convert_equilibrated_to_f32_of(DenseSource{dA, (size_t)n}, dst, d_perm, d_row, d_col, n, max_scaled, ctx);
+}
+int convert_equilibrated_to_f32(const MnaOperands &m, float2 *dst, const int *d_perm, const double *d_row,
+                                const double *d_col, double *max_scaled, LUContext &ctx) {
+    return convert_equilibrated_to_f32_of(MnaSource{m}, dst, d_perm, d_row, d_col, m.ne + m.nv, max_scaled, ctx);
 }
 
 void copy_equilibrated(const double2 *src, double2 *dst, const double *d_row, const double *d_col, int n,
@@ -3327,10 +3359,10 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n) {
 // One workgroup per row (a wavefront per row left most of the device idle
 // in the last of the ~1.5 rounds of rows); NQ = 1, or up to 8 right-hand
 // sides, so the one-right-hand-side case keeps one set of sums.
-template <int NQ>
-__global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, size_t lda, int n, const double2 *X,
-                                                             const double *AX, const double2 *B, double2 *R,
-                                                             int nq, unsigned long long *eta_bits) {
+template <int NQ, class Src>
+__global__ __launch_bounds__(256) void residual_multi_kernel(Src src, int n, const double2 *X, const double *AX,
+                                                             const double2 *B, double2 *R, int nq,
+                                                             unsigned long long *eta_bits) {
     __shared__ double red[4][3 * NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = (int)blockIdx.x;
@@ -3338,9 +3370,8 @@ __global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, s
     double yr[NQ], yi[NQ], den[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) yr[q] = yi[q] = den[q] = 0.0;
-    const double2 *row = A + (size_t)i * lda;
     for (int j = threadIdx.x; j < n; j += 256) {
-        const double2 z = row[j];
+        const double2 z = src(i, j);
         const double za = hypot(z.x, z.y);
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
@@ -3391,7 +3422,7 @@ __global__ __launch_bounds__(256) void residual_multi_kernel(const double2 *A, s
 // waves of 64 x 32); the epilogue forms r = Z x - b, stores it, and takes
 // the maximum of |r_i| / (den_i + |b_i|) over the wave's rows per right-hand
 // side with one atomic each.
-constexpr int kResidualMfmaMinRhs = 16;
+constexpr int kResidualMfmaMinRhs = kResidualDenseMinRhs;
 
 __global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, size_t lda, int n, const double2 *X,
                                                             const double2 *B, double2 *R, int nrhs,
@@ -3474,6 +3505,25 @@ __global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, si
     }
 }
 
+// the row-wise residual passes (up to kResidualRhs right-hand sides each)
+template <class Src>
+static void residual_rows(Src src, int n, const double2 *dX, const double2 *dB, double2 *dR, int nrhs, double *d_ax,
+                          unsigned long long *d_eta_bits, hipStream_t s) {
+    const size_t total = (size_t)n * nrhs;
+    hipLaunchKernelGGL(abs_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dX, d_ax, total);
+    for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
+        const int nq = std::min(kResidualRhs, nrhs - q0);
+        const size_t off = (size_t)q0 * n;
+        if (nq == 1)
+            hipLaunchKernelGGL((residual_multi_kernel<1, Src>), dim3((unsigned)n), dim3(256), 0, s, src, n, dX + off,
+                               d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
+        else
+            hipLaunchKernelGGL((residual_multi_kernel<kResidualRhs, Src>), dim3((unsigned)n), dim3(256), 0, s, src, n,
+                               dX + off, d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, const double2 *dB, double2 *dR,
                     int nrhs, double *d_ax, unsigned long long *d_eta_bits, hipStream_t s) {
     const size_t total = (size_t)n * nrhs;
@@ -3485,18 +3535,15 @@ void residual_multi(const double2 *dA, int n, size_t lda, const double2 *dX, con
         HIP_CHECK(hipGetLastError());
         return;
     }
-    hipLaunchKernelGGL(abs_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dX, d_ax, total);
-    for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
-        const int nq = std::min(kResidualRhs, nrhs - q0);
-        const size_t off = (size_t)q0 * n;
-        if (nq == 1)
-            hipLaunchKernelGGL(residual_multi_kernel<1>, dim3((unsigned)n), dim3(256), 0, s, dA, lda, n, dX + off,
-                               d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
-        else
-            hipLaunchKernelGGL(residual_multi_kernel<kResidualRhs>, dim3((unsigned)n), dim3(256), 0, s, dA, lda, n,
-                               dX + off, d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
-    }
-    HIP_CHECK(hipGetLastError());
+    residual_rows(DenseSource{dA, lda}, n, dX, dB, dR, nrhs, d_ax, d_eta_bits, s);
+}
+
+void residual_multi(const MnaOperands &m, const double2 *dX, const double2 *dB, double2 *dR, int nrhs, double *d_ax,
+                    unsigned long long *d_eta_bits, hipStream_t s) {
+    if (nrhs >= kResidualMfmaMinRhs)
+        throw HipError{"residual_multi: many right-hand sides need the dense matrix (MFMA residual)"};
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, s, d_eta_bits, nrhs);
+    residual_rows(MnaSource{m}, m.ne + m.nv, dX, dB, dR, nrhs, d_ax, d_eta_bits, s);
 }
 
 void mfma_layout_probe(double *host_out) {
