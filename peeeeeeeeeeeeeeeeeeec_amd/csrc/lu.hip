@@ -1986,6 +1986,241 @@ __device__ __forceinline__ void cmfma4(float ar, float ai, float br, float bi, v
     ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, ci, 0, 0, 0);
 }
 
+// The ordered factor's diagonal block for w = 16 NBK (64 or 128), blocked by
+// 16 columns, in one workgroup of eight wavefronts with the block in LDS
+// (rows of W + 4: the MFMA operand and tile accesses below are free of bank
+// conflicts). Per 16-column step p:
+//  1. wavefront 0 factors the 16 x 16 diagonal tile in registers (lane
+//     (li, lk) holds row li, columns 4 lk .. + 3; pivots and row entries
+//     are passed between lanes, no barriers), with the pivot check of the
+//     column-by-column elimination;
+//  2. wavefronts 0 and 1 form U_pp^-1 and L_pp^-1 (L unit lower) the same
+//     way; they also go to dinv (U^-1 row-major at 256 p, (L^-1)^T row-major
+//     at 2048 + 256 p) for the blocked triangular solves;
+//  3. the tiles below and right of it: L_ip = A_ip U_pp^-1 and U_pc =
+//     L_pp^-1 A_pc, one FP32 MFMA product each (L_ip also kept transposed);
+//  4. the trailing tiles: A_ic -= L_ip U_pc, one product each, spread over
+//     the wavefronts.
+// The same operations as the column-by-column elimination within a tile;
+// across tiles the updates are grouped 16 columns at a time.
+// lane N of each 16-lane row to the whole row (DPP row_newbcast)
+template <int N> __device__ inline float row_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + N, 0xF, 0xF, false));
+}
+template <int N> __device__ inline float2 row_bcast(float2 v) { return make_float2(row_bcast<N>(v.x), row_bcast<N>(v.y)); }
+
+template <int NBK> __host__ __device__ constexpr size_t diag_blocked_lds() {
+    return ((size_t)16 * NBK * (16 * NBK + 4) + (size_t)(NBK - 1) * 256 + 512) * sizeof(float2);
+}
+template <int NBK>
+__global__ __launch_bounds__(512) void diag_lu_blocked_kernel(float2 *A, size_t lda, int k0, double limit,
+                                                              int *status, float2 *dinv) {
+    constexpr int W = 16 * NBK, S = W + 4;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    float2 *sa = reinterpret_cast<float2 *>(smem_raw);  // the block, W x S
+    float2 *slt = sa + (size_t)W * S;                   // L_ip transposed, tile i - p - 1
+    float2 *suinv = slt + (NBK - 1) * 256;              // U_pp^-1, row-major
+    float2 *slinvt = suinv + 256;                       // (L_pp^-1)^T, row-major
+    __shared__ int s_bad;
+    if (status[0] != 0) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    if (tid == 0) s_bad = -1;
+    {
+        // all loads in flight before the LDS stores
+        constexpr int kPer = W * W / 512;
+        float2 g[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = tid + 512 * u, r = idx / W, c = idx % W;
+            g[u] = A[(size_t)(k0 + r) * lda + k0 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = tid + 512 * u, r = idx / W, c = idx % W;
+            sa[r * S + c] = g[u];
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < NBK; ++p) {
+        const int d = 16 * p, nb = NBK - 1 - p;
+        float2 *tile = sa + (size_t)(d + li) * S + d + 4 * lk;  // row layout of the diagonal tile
+        float2 rcps[16];  // wavefront 0: reciprocals of the pivots, for U^-1
+        if (wave == 0) {
+            float2 a[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = tile[v];
+            int bad = -1;
+            static_for<16>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const int kq = k >> 2, kv = k & 3;
+                const float2 u = make_float2(rdlane(a[kv].x, k + 16 * kq), rdlane(a[kv].y, k + 16 * kq));
+                int okk;
+                float2 rcp;
+                pivot_check(u, limit, okk, rcp);
+                rcps[k] = rcp;
+                if (!okk && bad < 0) bad = k;
+                const float2 colk = make_float2(__shfl(a[kv].x, li + 16 * kq), __shfl(a[kv].y, li + 16 * kq));
+                const float2 mult = wmul(colk, rcp);
+                float2 rowk[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) rowk[v] = row_bcast<k>(a[v]);
+                if (li > k) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int col = 4 * lk + v;
+                        if (col > k)
+                            a[v] = wfms(a[v], mult, rowk[v]);
+                        else if (col == k)
+                            a[v] = mult;
+                    }
+                }
+            });
+#pragma unroll
+            for (int v = 0; v < 4; ++v) tile[v] = a[v];
+            if (bad >= 0 && lane == 0) s_bad = d + bad;
+        }
+        __syncthreads();
+        if (s_bad >= 0) {
+            if (tid == 0) {
+                status[0] = -2;
+                status[1] = k0 + s_bad;
+            }
+            return;
+        }
+        if (wave < 2) {
+            float2 a[4], b[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                a[v] = tile[v];
+                b[v] = make_float2(li == 4 * lk + v ? 1.f : 0.f, 0.f);
+            }
+            // column k of the factor in row li, for every k (not on the
+            // dependency chain of the substitutions)
+            float2 colv[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                colv[k] = make_float2(__shfl(a[k & 3].x, li + 16 * (k >> 2)), __shfl(a[k & 3].y, li + 16 * (k >> 2)));
+            if (wave == 0) {
+                // U^-1 by rows from the last: row k final, then taken out of
+                // the rows above
+                static_for<16>([&](auto kc) {
+                    constexpr int k = 15 - decltype(kc)::value;
+                    float2 xk[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) xk[v] = wmul(row_bcast<k>(b[v]), rcps[k]);
+                    const float2 uik = colv[k];
+                    if (li == k) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) b[v] = xk[v];
+                    } else if (li < k) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) b[v] = wfms(b[v], uik, xk[v]);
+                    }
+                });
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    suinv[li * 16 + 4 * lk + v] = b[v];
+                    if (dinv) dinv[256 * p + 16 * li + 4 * lk + v] = b[v];
+                }
+            } else {
+                // L^-1 (unit diagonal) by rows from the first
+                static_for<16>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    float2 xk[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) xk[v] = row_bcast<k>(b[v]);
+                    const float2 lik = colv[k];
+                    if (li > k) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) b[v] = wfms(b[v], lik, xk[v]);
+                    }
+                });
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    slinvt[(4 * lk + v) * 16 + li] = b[v];
+                    if (dinv) dinv[2048 + 256 * p + 16 * (4 * lk + v) + li] = b[v];
+                }
+            }
+        }
+        __syncthreads();
+        // L_ip = A_ip U_pp^-1 (row layout) and U_pc = L_pp^-1 A_pc (column
+        // layout)
+        for (int it = wave; it < 2 * nb; it += 8) {
+            v4f xr, xi, nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
+            if (it < nb) {
+                const int i = p + 1 + it;
+                float2 *x = sa + (size_t)(16 * i + li) * S + d + 4 * lk;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    xr[v] = x[v].x;
+                    xi[v] = x[v].y;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float2 m = suinv[(4 * lk + t) * 16 + li];
+                    cmfma4(m.x, m.y, xr[t], xi[t], nr, ni);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float2 y = make_float2(nr[v], ni[v]);
+                    x[v] = y;
+                    slt[(size_t)(i - p - 1) * 256 + (4 * lk + v) * 16 + li] = y;
+                }
+            } else {
+                const int c = p + 1 + (it - nb);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float2 e = sa[(size_t)(d + 4 * lk + v) * S + 16 * c + li];
+                    xr[v] = e.x;
+                    xi[v] = e.y;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float2 m = slinvt[(4 * lk + t) * 16 + li];
+                    cmfma4(m.x, m.y, xr[t], xi[t], nr, ni);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sa[(size_t)(d + 4 * lk + v) * S + 16 * c + li] = make_float2(nr[v], ni[v]);
+            }
+        }
+        __syncthreads();
+        // A_ic -= L_ip U_pc (column layout)
+        for (int it = wave; it < nb * nb; it += 8) {
+            const int i = p + 1 + it / nb, c = p + 1 + it % nb;
+            v4f cr, ci;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float2 e = sa[(size_t)(16 * i + 4 * lk + v) * S + 16 * c + li];
+                cr[v] = e.x;
+                ci[v] = e.y;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 l = slt[(size_t)(i - p - 1) * 256 + (4 * lk + t) * 16 + li];
+                const float2 u = sa[(size_t)(d + 4 * lk + t) * S + 16 * c + li];
+                cmfma4(-l.x, -l.y, u.x, u.y, cr, ci);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sa[(size_t)(16 * i + 4 * lk + v) * S + 16 * c + li] = make_float2(cr[v], ci[v]);
+        }
+        __syncthreads();
+    }
+    {
+        constexpr int kPer = W * W / 512;
+        float2 g[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = tid + 512 * u, r = idx / W, c = idx % W;
+            g[u] = sa[r * S + c];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = tid + 512 * u, r = idx / W, c = idx % W;
+            A[(size_t)(k0 + r) * lda + k0 + c] = g[u];
+        }
+    }
+}
+
 // 16 x 16 operand blocks in LDS: element (k, m) at ((k % 4) 4 + k / 4) 16 + m,
 // so that the t-th MFMA's operand read (k = 4 lk + t at lane (m, lk)) is one
 // contiguous 512-byte run of the wavefront.
@@ -2528,10 +2763,21 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
                            dA, lda, c0, c1, k0, w, ctx.status.ptr, pk, pinv, pcb);
     };
+    if (blocked_solves && nb == 128)
+        set_dynamic_lds(reinterpret_cast<const void *>(diag_lu_blocked_kernel<8>), diag_blocked_lds<8>());
     auto diag = [&](int k0, int w, hipStream_t st) {
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
-        hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
-                           ctx.status.ptr, blocked_solves && w == nb ? dinv_of(k0) : nullptr);
+        if (blocked_solves && w == nb && (w == 128 || w == 64)) {
+            if (w == 128)
+                hipLaunchKernelGGL(diag_lu_blocked_kernel<8>, dim3(1), dim3(512), diag_blocked_lds<8>(), st, dA, lda,
+                                   k0, pivot_limit, ctx.status.ptr, dinv_of(k0));
+            else
+                hipLaunchKernelGGL(diag_lu_blocked_kernel<4>, dim3(1), dim3(512), diag_blocked_lds<4>(), st, dA, lda,
+                                   k0, pivot_limit, ctx.status.ptr, dinv_of(k0));
+        } else {
+            hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
+                               ctx.status.ptr, blocked_solves && w == nb ? dinv_of(k0) : nullptr);
+        }
         if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
     };
     diag(0, std::min(nb, n), s);
