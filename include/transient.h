@@ -100,6 +100,21 @@ int transient_build_shunt_operators(TransientSystem *system);
 int transient_build_system_matrix(TransientSystem *system);
 /* transient.h:337: (size_t)-1 outside the slot-cell stage */
 size_t transient_slot_current_offset(const TransientSystem *system);
+/* transient.h:324,342,355: the slot-cell stage is not built by this library;
+ * these print an ERROR line and return -1 */
+int transient_system_create_slot_cells(TransientSystem *system, const InductanceMatrix *L,
+                                       const ResistanceMatrix *R, const IncidenceMatrix *A,
+                                       const PotentialMatrix *P, const SlotCircuit *slot_circuit,
+                                       const InternalCircuitShunt *shunt, TransientScheme scheme,
+                                       double dt);
+int transient_build_rhs_slot_cells(TransientSystem *system, const double *edge_current_n,
+                                   const double *node_self_voltage_n,
+                                   const double *slot_inductor_current_n, double shunt_current_n,
+                                   const double *node_source_n, const double *node_source_np1,
+                                   const double *edge_voltage_n, const double *edge_voltage_np1);
+int transient_extract_slot_cells_solution(const TransientSystem *system, double *edge_current,
+                                          double *node_self_voltage, double *slot_inductor_current,
+                                          double *shunt_current);
 /* transient.h:364 (transient.c:2014-2066): real LU of system_matrix (GPU) */
 int transient_factorize(TransientSystem *system, int parallel_threads);
 /* transient.h:391 (transient.c:1421-1659): rhs of the step n -> n+1 from I_n,

@@ -839,6 +839,28 @@ size_t transient_slot_current_offset(const TransientSystem *sys) {
     return sys->n_edges + sys->n_nodes;
 }
 
+// The slot-cell stage (slot inductors as extra unknowns) is not built by this
+// library. Its entry points exist so that a program linked against it
+// without the reference's transient.c still links, and a slot-cell run fails
+// with -1 and an error line instead of running something else.
+int transient_system_create_slot_cells(TransientSystem *, const InductanceMatrix *, const ResistanceMatrix *,
+                                       const IncidenceMatrix *, const PotentialMatrix *, const SlotCircuit *,
+                                       const InternalCircuitShunt *, TransientScheme, double) {
+    PEEC_ERR("transient: the slot-cell stage is not provided by this library.");
+    return -1;
+}
+
+int transient_build_rhs_slot_cells(TransientSystem *, const double *, const double *, const double *, double,
+                                   const double *, const double *, const double *, const double *) {
+    PEEC_ERR("transient: the slot-cell stage is not provided by this library.");
+    return -1;
+}
+
+int transient_extract_slot_cells_solution(const TransientSystem *, double *, double *, double *, double *) {
+    PEEC_ERR("transient: the slot-cell stage is not provided by this library.");
+    return -1;
+}
+
 size_t transient_shunt_index(const TransientSystem *sys) {
     if (!sys) return (size_t)-1;
     if (sys->stage == TRANSIENT_STAGE_2) return sys->n_edges + sys->n_nodes;

@@ -106,7 +106,8 @@ EXPORTED = [
     "internal_circuit_shunt_print_info",
     "transient_system_init", "transient_system_free", "transient_system_create_stage1",
     "transient_system_create_stage2", "transient_build_shunt_operators", "transient_build_system_matrix",
-    "transient_slot_current_offset", "transient_factorize", "transient_build_rhs_stage1",
+    "transient_slot_current_offset", "transient_system_create_slot_cells", "transient_build_rhs_slot_cells",
+    "transient_extract_slot_cells_solution", "transient_factorize", "transient_build_rhs_stage1",
     "transient_build_rhs_stage2", "transient_solve_step", "transient_current_offset", "transient_voltage_offset",
     "transient_shunt_index", "transient_extract_stage1_solution", "transient_extract_stage2_solution",
     "transient_recover_node_state", "transient_print_info",
