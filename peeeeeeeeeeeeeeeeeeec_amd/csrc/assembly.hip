@@ -11,12 +11,16 @@
 //                      entry is computed as if all its piece pairs were FAR.
 //  * near_entry_kernel one wavefront per candidate near entry (host-built
 //                      superset of entries with a SELF/TOUCHING/NEAR piece
-//                      pair); each lane owns one (piece pair, triangle pair)
-//                      and runs the reference rule for its class: the 4-D GL
-//                      product rule, the closed-form identical-triangle term or
-//                      the analytic-inner / adaptive-outer rule with an exact
-//                      post-order bisection stack. Lane 0 sums the values in the
-//                      reference's order and overwrites the entry.
+//                      pair); the wavefront takes the entry's (piece pair,
+//                      triangle pair) tasks one after another and runs the
+//                      reference rule of each task's class with all 64 lanes:
+//                      the 4-D GL product rule, the closed-form identical-
+//                      triangle term, or the analytic-inner / adaptive-outer
+//                      rule, whose recursion is walked one node at a time
+//                      (the node's quadrature points spread over the lanes,
+//                      pending parents on a stack in LDS, sums formed in
+//                      post-order like the recursion). The task values are
+//                      summed in the reference's order and overwrite the entry.
 //  * smooth_tile_kernel same tiling; integrates (exp(-jkR)-1)/R with the
 //                      smooth-order rule and adds it to the static matrix.
 // This file is compiled with -ffp-contract=off so the classification and the
@@ -83,29 +87,45 @@ __device__ inline void d_tri_map(const double (&t)[3][3], double u, double v, do
     J = d_norm3(c) * (1.0 - u);
 }
 
+// Wave-wide sum as a butterfly over the 64 lanes: at every step a lane and
+// its partner add the same two values in swapped order, and IEEE addition is
+// commutative, so all lanes end with the same bits. wave_uniform() then
+// takes the first lane's copy as a scalar, so the branches on it are uniform.
+__device__ inline double wave_sum(double x) {
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+    return x;
+}
+__device__ inline double wave_uniform(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // 4-D Gauss-Legendre product rule over a triangle pair (quadrature.c:679-783)
-__device__ double d_tri_pair_gl(const double (&a)[3][3], const double (&b)[3][3], int order) {
+// with the order^4 points spread over the wavefront: for each point i of the
+// first coordinate, lane m takes the (j, k, l) combinations m, m + 64, ...
+// Each term is formed as in the reference; only the order of the sum differs.
+__device__ double wave_tri_pair_gl(const double (&a)[3][3], const double (&b)[3][3], int order, int lane) {
+    const int o2 = order * order, o3 = o2 * order;
     double sum = 0.0;
+#pragma unroll 1
     for (int i = 0; i < order; ++i) {
         const double ua = 0.5 * (c_gl_x[order][i] + 1.0), wu = 0.5 * c_gl_w[order][i];
-        for (int j = 0; j < order; ++j) {
+#pragma unroll 1
+        for (int m = lane; m < o3; m += 64) {
+            const int j = m / o2, kl = m - j * o2, k = kl / order, l = kl - k * order;
             const double va = 0.5 * (c_gl_x[order][j] + 1.0), wv = 0.5 * c_gl_w[order][j];
-            double ra[3], ja;
+            const double ub = 0.5 * (c_gl_x[order][k] + 1.0), wub = 0.5 * c_gl_w[order][k];
+            const double vb = 0.5 * (c_gl_x[order][l] + 1.0), wvb = 0.5 * c_gl_w[order][l];
+            double ra[3], ja, rb[3], jb;
             d_tri_map(a, ua, va, ra, ja);
-            for (int k = 0; k < order; ++k) {
-                const double ub = 0.5 * (c_gl_x[order][k] + 1.0), wub = 0.5 * c_gl_w[order][k];
-                for (int l = 0; l < order; ++l) {
-                    const double vb = 0.5 * (c_gl_x[order][l] + 1.0), wvb = 0.5 * c_gl_w[order][l];
-                    double rb[3], jb;
-                    d_tri_map(b, ub, vb, rb, jb);
-                    const double R = d_dist3(ra, rb);
-                    if (R <= 0.0) continue;
-                    sum += wu * wv * wub * wvb * ja * jb / R;
-                }
-            }
+            d_tri_map(b, ub, vb, rb, jb);
+            const double R = d_dist3(ra, rb);
+            if (R > 0.0) sum += wu * wv * wub * wvb * ja * jb / R;
         }
     }
-    return sum;
+    return wave_uniform(wave_sum(sum));
 }
 
 // exact potential of a uniform triangle at a point (quadrature.c:842-898)
@@ -129,6 +149,13 @@ __device__ void d_tripot_setup(const double (&p)[3][3], TriPot &T) {
         for (int k = 0; k < 3; ++k) T.t[i][k] /= T.l[i];
         d_cross3(T.t[i], T.n, T.m[i]);
     }
+}
+
+// the set-up values as wave-uniform scalars (every lane computed the same
+// bits from the same triangle), so they can live in scalar registers
+__device__ inline void d_tripot_uniform(TriPot &T) {
+    double *f = &T.p[0][0];
+    for (int k = 0; k < (int)(sizeof(TriPot) / sizeof(double)); ++k) f[k] = wave_uniform(f[k]);
 }
 
 __device__ double d_tripot_eval(const TriPot &T, const double *r) {
@@ -156,16 +183,29 @@ __device__ double d_tripot_eval(const TriPot &T, const double *r) {
     return acc;
 }
 
-// outer GL rule of the analytic inner potential (quadrature.c:900-913)
-__device__ double d_outer_rule(const double (&a)[3][3], const TriPot &T, int order) {
-    double s = 0.0;
-    for (int i = 0; i < order; ++i)
-        for (int j = 0; j < order; ++j) {
-            double r[3], J;
-            d_tri_map(a, 0.5 * (c_gl_x[order][i] + 1), 0.5 * (c_gl_x[order][j] + 1), r, J);
-            s += 0.25 * c_gl_w[order][i] * c_gl_w[order][j] * J * d_tripot_eval(T, r);
-        }
-    return s;
+// Low and high outer rules of the analytic inner potential T over triangle t
+// (quadrature.c:900-913) in one pass: the low^2 + order^2 points are spread
+// over the wavefront, each lane summing its points of either rule. Each term
+// is formed as in the reference; only the order of the sums differs.
+__device__ void wave_outer_rules(const double (&t)[3][3], const TriPot &T, int low, int order, int lane,
+                                 double &lo, double &hi) {
+    const int nl = low * low, np = nl + order * order;
+    double sl = 0.0, sh = 0.0;
+#pragma unroll 1
+    for (int p = lane; p < np; p += 64) {
+        const bool is_low = p < nl;
+        const int q = is_low ? low : order, idx = is_low ? p : p - nl;
+        const int i = idx / q, j = idx - i * q;
+        double r[3], J;
+        d_tri_map(t, 0.5 * (c_gl_x[q][i] + 1), 0.5 * (c_gl_x[q][j] + 1), r, J);
+        const double v = 0.25 * c_gl_w[q][i] * c_gl_w[q][j] * J * d_tripot_eval(T, r);
+        if (is_low)
+            sl += v;
+        else
+            sh += v;
+    }
+    lo = wave_uniform(wave_sum(sl));
+    hi = wave_uniform(wave_sum(sh));
 }
 
 // longest-edge bisection (quadrature.c:923-937); child 0 = left, 1 = right
@@ -193,76 +233,86 @@ __device__ inline void d_split(const double (&a)[3][3], int child, double (&out)
     }
 }
 
-// Adaptive outer integration (quadrature.c:915-938) with an explicit stack
-// that evaluates the recursion tree in post-order, so every partial sum is
-// formed exactly as left(value) + right(value) like the recursive original.
-struct Frame {
+// A parent triangle of the adaptive recursion whose children are pending.
+struct WaveFrame {
     double t[3][3];
-    double tol;
-    double left;
+    double tol;   // the parent's tolerance (each child gets half)
+    double left;  // value of the left child once it is done
     int depth;
-    int state;  // 0 fresh, 1 left pending, 2 right pending
+    int state;    // 1: left child in progress, 2: right child in progress
 };
 
-// n counts the analytic potential evaluations (quadrature points).
-__device__ double d_outer_adaptive(const double (&a)[3][3], const TriPot &T, int order, double tol, unsigned &n) {
-    Frame st[kMaxDepth + 1];
-    int sp = 0;
-    for (int r = 0; r < 3; ++r)
-        for (int d = 0; d < 3; ++d) st[0].t[r][d] = a[r][d];
-    st[0].tol = tol;
-    st[0].depth = 0;
-    st[0].state = 0;
-    double ret = 0.0;
+// Adaptive outer integration (quadrature.c:915-938) by the whole wavefront:
+// one node of the recursion at a time, its two rules evaluated by all lanes,
+// the accept/bisect decision uniform. The parents of the current node wait
+// on a stack in LDS (every lane writes the same values); a finished node is
+// combined with them in post-order, so each sum is left + right as in the
+// recursive original. n counts the analytic potential evaluations.
+__device__ double wave_outer_adaptive(const double (&a)[3][3], const TriPot &T, int order, double tol, int lane,
+                                      WaveFrame *st, unsigned &n) {
     const int low = order <= 6 ? 3 : 4;
+    double t[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int d = 0; d < 3; ++d) t[r][d] = a[r][d];
+    double node_tol = tol;
+    int depth = 0, sp = 0;
     for (;;) {
-        Frame &f = st[sp];
-        bool done = false;
-        if (f.state == 0) {
-            n += low * low + order * order;
-            const double lo = d_outer_rule(f.t, T, low);
-            const double hi = d_outer_rule(f.t, T, order);
-            if (!isfinite(lo) || !isfinite(hi)) return NAN;
-            if (fabs(hi - lo) <= f.tol + 2e-5 * fabs(hi)) {
-                ret = hi;
-                done = true;
-            } else {
-                if (f.depth >= kMaxDepth) return NAN;
-                f.state = 1;
-                Frame &c = st[sp + 1];
-                d_split(f.t, 0, c.t);
-                c.tol = f.tol * 0.5;
-                c.depth = f.depth + 1;
-                c.state = 0;
-                ++sp;
+        n += low * low + order * order;
+        double lo, hi;
+        wave_outer_rules(t, T, low, order, lane, lo, hi);
+        if (!isfinite(lo) || !isfinite(hi)) return NAN;
+        if (fabs(hi - lo) <= node_tol + 2e-5 * fabs(hi)) {
+            double ret = hi;
+            for (;;) {
+                if (sp == 0) return ret;
+                WaveFrame &f = st[sp - 1];
+                if (__builtin_amdgcn_readfirstlane(f.state) == 1) {
+                    // left child done: keep it, evaluate the right child
+                    f.left = ret;
+                    f.state = 2;
+                    d_split(f.t, 1, t);
+                    node_tol = f.tol * 0.5;
+                    depth = __builtin_amdgcn_readfirstlane(f.depth) + 1;
+                    break;
+                }
+                ret = f.left + ret;  // right child done
+                --sp;
             }
-        } else if (f.state == 1) {
-            f.left = ret;
-            f.state = 2;
-            Frame &c = st[sp + 1];
-            d_split(f.t, 1, c.t);
-            c.tol = f.tol * 0.5;
-            c.depth = f.depth + 1;
-            c.state = 0;
-            ++sp;
         } else {
-            ret = f.left + ret;
-            done = true;
-        }
-        if (done) {
-            if (sp == 0) return ret;
-            --sp;
+            if (depth >= kMaxDepth) return NAN;
+            WaveFrame &f = st[sp++];
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) f.t[r][d] = t[r][d];
+            f.tol = node_tol;
+            f.depth = depth;
+            f.state = 1;
+            double c[3][3];
+            d_split(t, 0, c);
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) t[r][d] = c[r][d];
+            node_tol *= 0.5;
+            ++depth;
         }
     }
 }
 
-// near-field triangle pair (quadrature.c:940-973); cls 0 self, 1 touching, 2 near
-__device__ double d_tri_pair_near(const double (&a)[3][3], const double (&b)[3][3], int cls,
-                                  const QuadratureOrders &q, unsigned &n) {
-    TriPot Ta, Tb;
-    d_tripot_setup(a, Ta);
-    d_tripot_setup(b, Tb);
-    if (Ta.area2 == 0.0 || Tb.area2 == 0.0) return 0.0;
+// twice the area of a triangle, formed as in the potential set-up
+__device__ inline double d_twice_area(const double (&p)[3][3]) {
+    double t0[3], t1[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+        t0[k] = p[1][k] - p[0][k];
+        t1[k] = p[2][k] - p[1][k];
+    }
+    d_cross3(t0, t1, c);
+    return d_norm3(c);
+}
+
+// near-field triangle pair (quadrature.c:940-973) by the whole wavefront;
+// cls 0 self, 1 touching, 2 near
+__device__ double wave_tri_pair_near(const double (&a)[3][3], const double (&b)[3][3], int cls,
+                                     const QuadratureOrders &q, int lane, WaveFrame *st, unsigned &n) {
+    const double area_a = d_twice_area(a), area_b = d_twice_area(b);
+    if (area_a == 0.0 || area_b == 0.0) return 0.0;
     bool same = true;
     for (int i = 0; i < 3; ++i) {
         bool hit = false;
@@ -270,18 +320,42 @@ __device__ double d_tri_pair_near(const double (&a)[3][3], const double (&b)[3][
             if (d_coincide(a[i], b[j])) hit = true;
         if (!hit) same = false;
     }
+    TriPot T;
     if (same) {
+        // identical triangles: (2A/3) sum over the vertices, one per lane,
+        // added in vertex order
+        d_tripot_setup(b, T);
+        double r[3];
+        for (int d = 0; d < 3; ++d) r[d] = lane == 0 ? a[0][d] : lane == 1 ? a[1][d] : a[2][d];
+        const double v = lane < 3 ? d_tripot_eval(T, r) : 0.0;
         double s = 0.0;
-        for (int i = 0; i < 3; ++i) s += d_tripot_eval(Tb, a[i]);
+        for (int i = 0; i < 3; ++i) s += wave_uniform(__shfl(v, i));
         n += 3;
-        return Ta.area2 * s / 3.0;
+        return area_a * s / 3.0;
     }
     int order = cls == 0 ? q.self_order : cls == 1 ? q.touching_order : q.near_order;
     if (order < 6) order = 6;
-    const double scale = Ta.area2 * Tb.area2 / sqrt(fmax(Ta.area2, Tb.area2));
-    const double ab = d_outer_adaptive(a, Tb, order, 2e-5 * scale, n);
-    const double ba = d_outer_adaptive(b, Ta, order, 2e-5 * scale, n);
-    return 0.5 * (ab + ba);
+    const double scale = area_a * area_b / sqrt(fmax(area_a, area_b));
+    // a over the potential of b, then b over the potential of a (one call
+    // site, so the recursion's code is emitted once)
+    double half[2];
+#pragma unroll 1
+    for (int dn = 0; dn < 2; ++dn) {
+        double outer[3][3], inner[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int d = 0; d < 3; ++d) {
+                outer[r][d] = dn ? b[r][d] : a[r][d];
+                inner[r][d] = dn ? a[r][d] : b[r][d];
+            }
+        d_tripot_setup(inner, T);
+        d_tripot_uniform(T);
+        const double v = wave_outer_adaptive(outer, T, order, 2e-5 * scale, lane, st, n);
+        if (dn)
+            half[1] = v;
+        else
+            half[0] = v;
+    }
+    return 0.5 * (half[0] + half[1]);
 }
 
 // piece-pair classification (quadrature.c:550-607); `same_object` is the
@@ -331,73 +405,52 @@ __device__ inline void d_piece_triangle(const double *pv, int which, double (&t)
     }
 }
 
-// PER_WAVE entries (i, j) per wavefront, 64 / PER_WAVE lanes each; see the
-// file header. Edge regions have at most 2 x 2 pieces (16 tasks), so four
-// Lp entries share a wavefront; a wavefront runs as long as its slowest
-// adaptive lane, and one Lp entry alone would leave 48 lanes idle.
-template <int PER_WAVE>
+// One wavefront per candidate near entry (i, j); see the file header. The
+// wavefront walks the entry's (piece pair, triangle pair) tasks in the
+// reference's order and evaluates each one with all lanes, so no lane waits
+// on another's adaptive recursion; the task values are summed in that order.
 __global__ __launch_bounds__(64) void near_entry_kernel(
     const int *__restrict__ pairs, int n_pairs, const int *__restrict__ ptr,
     const double *__restrict__ pv, QuadratureOrders q, int kind, const double *__restrict__ aux,
     const double *__restrict__ dir, double *__restrict__ out, size_t ld, int mirror,
     int *__restrict__ err, unsigned long long *__restrict__ counts = nullptr) {
-    constexpr int G = 64 / PER_WAVE, CAP = kMaxTasks / PER_WAVE;
-    __shared__ double vals[kMaxTasks];
-    const int slot = threadIdx.x / G, sub = threadIdx.x % G;
-    const int e = blockIdx.x * PER_WAVE + slot;
+    __shared__ WaveFrame st[kMaxDepth + 1];
+    const int lane = threadIdx.x;
+    const int e = blockIdx.x;
     if (e >= n_pairs) return;
-    double *v = vals + slot * CAP;
     const int i = pairs[2 * e], j = pairs[2 * e + 1];
     const int a0 = ptr[i], na = ptr[i + 1] - a0, b0 = ptr[j], nb = ptr[j + 1] - b0;
-    const int tasks = na * nb * 4;
-    if (tasks > CAP) {
-        if (sub == 0) atomicOr(err, 2);
-        return;
-    }
+    const unsigned far_points = (unsigned)(q.far_order * q.far_order * q.far_order * q.far_order);
     unsigned n_pot = 0, n_gl = 0;  // analytic potential evaluations, 1/R evaluations
-    for (int t = sub; t < tasks; t += G) {
-        const int pp = t >> 2, tp = t & 3;
+    double integral = 0.0;
+    bool finite = true;
+    for (int pp = 0; pp < na * nb; ++pp) {
         const int ia = pp / nb, ib = pp - ia * nb;
         const double *pa = pv + 12 * (size_t)(a0 + ia);
         const double *pb = pv + 12 * (size_t)(b0 + ib);
         const int cls = d_classify(pa, pb, i == j && ia == ib, q.near_factor);
-        double ta[3][3], tb[3][3];
-        d_piece_triangle(pa, tp >> 1, ta);
-        d_piece_triangle(pb, tp & 1, tb);
-        if (cls == 3) {
-            v[t] = d_tri_pair_gl(ta, tb, q.far_order);
-            n_gl += q.far_order * q.far_order * q.far_order * q.far_order;
-        } else {
-            v[t] = d_tri_pair_near(ta, tb, cls, q, n_pot);
-        }
-    }
-    if (counts) {
-        // the lanes of one entry are active together: sum them, one atomic
-        // per entry
-        unsigned long long cp = n_pot, cg = n_gl;
-        for (int m = 1; m < G; m <<= 1) {
-            cp += __shfl_xor(cp, m);
-            cg += __shfl_xor(cg, m);
-        }
-        if (sub == 0) {
-            atomicAdd(&counts[0], cp);
-            atomicAdd(&counts[1], cg);
-        }
-    }
-    // the workgroup is one wavefront: lanes that returned above are simply
-    // inactive at the barrier
-    __syncthreads();
-    if (sub != 0) return;
-    double integral = 0.0;
-    bool finite = true;
-    for (int pp = 0; pp < na * nb; ++pp) {
         double s = 0.0;
+#pragma unroll 1
         for (int tp = 0; tp < 4; ++tp) {
-            const double x = v[4 * pp + tp];
+            double ta[3][3], tb[3][3];
+            d_piece_triangle(pa, tp >> 1, ta);
+            d_piece_triangle(pb, tp & 1, tb);
+            double x;
+            if (cls == 3) {
+                x = wave_tri_pair_gl(ta, tb, q.far_order, lane);
+                n_gl += far_points;
+            } else {
+                x = wave_tri_pair_near(ta, tb, cls, q, lane, st, n_pot);
+            }
             if (!isfinite(x)) finite = false;
             s += x;
         }
         integral += s;
+    }
+    if (lane != 0) return;
+    if (counts) {
+        atomicAdd(&counts[0], (unsigned long long)n_pot);
+        atomicAdd(&counts[1], (unsigned long long)n_gl);
     }
     double value;
     if (kind == 2) {
@@ -1441,15 +1494,8 @@ void RegionSet::assemble_static(double *d_out, hipStream_t s, int *d_error, Asse
     const int nn = (int)h_near_.size() / 2;
     if (nn) {
         const int tk = timer ? timer->start(s) : -1;
-        // edge regions (at most 2 x 2 pieces): four entries per wavefront
-        if (kind_ == RegionKind::Edge && maxp_ <= 2)
-            hipLaunchKernelGGL(near_entry_kernel<4>, dim3((nn + 3) / 4), dim3(64), 0, s, d_near_.ptr, nn,
-                               d_ptr_.ptr, d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
-                               d_error, d_counts);
-        else
-            hipLaunchKernelGGL(near_entry_kernel<1>, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr,
-                               d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1,
-                               d_error, d_counts);
+        hipLaunchKernelGGL(near_entry_kernel, dim3(nn), dim3(64), 0, s, d_near_.ptr, nn, d_ptr_.ptr, d_pv_.ptr,
+                           q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_out, (size_t)R_, 1, d_error, d_counts);
         HIP_CHECK(hipGetLastError());
         // flops follow from the evaluation counts once the kernel is done
         if (timer) timer->stop(tk, s, kKernelNearEntry, 0.0, 16.0 * nn);
@@ -1670,7 +1716,7 @@ void RegionSet::assemble_pairs(const std::vector<int> &pairs, double *d_values, 
     d_pairs.upload(pairs, s);
     const int n = (int)pairs.size() / 2;
     if (n) {
-        hipLaunchKernelGGL(near_entry_kernel<1>, dim3(n), dim3(64), 0, s, d_pairs.ptr, n, d_ptr_.ptr,
+        hipLaunchKernelGGL(near_entry_kernel, dim3(n), dim3(64), 0, s, d_pairs.ptr, n, d_ptr_.ptr,
                            d_pv_.ptr, q_, (int)kind_, d_aux_.ptr, d_dir_.ptr, d_values, (size_t)0,
                            0, d_error);
         HIP_CHECK(hipGetLastError());
@@ -1695,7 +1741,7 @@ double region_pair_integral(const std::vector<double> &pa, const std::vector<dou
     d_ptr.upload(ptr, s);
     d_pairs.upload(pairs, s);
     HIP_CHECK(hipMemsetAsync(d_err.ptr, 0, sizeof(int), s));
-    hipLaunchKernelGGL(near_entry_kernel<1>, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr, d_pv.ptr, q, 2,
+    hipLaunchKernelGGL(near_entry_kernel, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr, d_pv.ptr, q, 2,
                        d_aux.ptr, d_dir.ptr, d_val.ptr, (size_t)0, 0, d_err.ptr);
     HIP_CHECK(hipGetLastError());
     double v = NAN;
@@ -1723,7 +1769,7 @@ double patch_pair_integral(const QuadPatch *a, const QuadPatch *b, const Quadrat
     d_ptr.upload(ptr, s);
     d_pairs.upload(pairs, s);
     HIP_CHECK(hipMemsetAsync(d_err.ptr, 0, sizeof(int), s));
-    hipLaunchKernelGGL(near_entry_kernel<1>, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr,
+    hipLaunchKernelGGL(near_entry_kernel, dim3(1), dim3(64), 0, s, d_pairs.ptr, 1, d_ptr.ptr,
                        d_pv.ptr, q, 2, d_aux.ptr, d_dir.ptr, d_val.ptr, (size_t)0, 0, d_err.ptr);
     HIP_CHECK(hipGetLastError());
     double v = NAN;
