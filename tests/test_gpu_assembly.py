@@ -257,3 +257,20 @@ def test_engine_points_from_shifted_moments(require_gpu):
     for a, b in zip(res[1], res[0]):
         assert a[0] == 0 and b[0] == 0
         assert abs(a[1][0] - b[1][0]) <= 1e-10 * abs(b[1][0])
+
+
+def test_gpu_matrices_export_to_exact_capacitance_network(require_gpu, tmp_path):
+    """The reference's SPICE export check (tests/run_tests.py:166-239) on the
+    GPU-assembled Lp/P of meshes/plate.msh: written by matrix_export_peec_set
+    and read back by scipy.io.mmread they are the assembled numbers bit for
+    bit, and the capacitance network built from P reconstructs P^-1 to
+    1e-10."""
+    from tests.test_mtx import _capacitance_network_error, _export_set
+
+    with plib.Model(msh=ck.PLATE_MSH) as m:
+        L, P = m.inductance(), m.potential()
+        read, A = _export_set(tmp_path, m, L, P)
+    assert np.array_equal(read["L"], L) and np.array_equal(read["P"], P) and np.array_equal(read["A"], A)
+    err = _capacitance_network_error(read["P"])
+    print(f"capacitance network reconstruction error {err:.3e}", flush=True)
+    assert np.isfinite(err) and err <= 1e-10

@@ -65,3 +65,65 @@ def test_import_rejects_other_formats(tmp_path):
     p.write_text("%%MatrixMarket matrix array real general\n2 2\n1.0\n2.0\n3.0\n")
     with pytest.raises(RuntimeError):
         plib.import_dense_mtx(str(p))
+
+
+def _export_set(tmp_path, model, L, P):
+    """matrix_export_peec_set of (L, A, P, PAT, R = 0) for `model`, read back
+    with scipy's Matrix Market reader."""
+    from scipy.io import mmread
+
+    lib = plib.load()
+    ne, nv = L.shape[0], P.shape[0]
+    A = np.zeros((ne, nv))
+    e = model.edges()
+    A[np.arange(ne), e[:, 0]], A[np.arange(ne), e[:, 1]] = -1.0, 1.0
+    PAT = P @ A.T
+    Lm = api.InductanceMatrix(n=ne, data=api.c_malloc_copy(np.ascontiguousarray(L), C.c_double))
+    Pm = api.PotentialMatrix(n=nv, data=api.c_malloc_copy(np.ascontiguousarray(P), C.c_double))
+    PATm = api.Matrix(rows=nv, cols=ne, data=api.c_malloc_copy(np.ascontiguousarray(PAT), C.c_double))
+    Rm = api.ResistanceMatrix(n=ne, diagonal=api.c_malloc_copy(np.zeros(ne), C.c_double))
+    d = str(tmp_path / "set")
+    assert lib.matrix_export_peec_set(d.encode(), C.byref(model.mesh), C.byref(Lm), C.byref(Pm), C.byref(PATm),
+                                      C.byref(Rm)) == 0
+    read = {}
+    for name in ("L", "P", "A", "R"):
+        v = mmread(os.path.join(d, name + ".mtx"))
+        read[name] = np.asarray(v.toarray() if hasattr(v, "toarray") else v, dtype=float)
+    return read, A
+
+
+def _capacitance_network_error(P):
+    """The reference's SPICE check (tests/run_tests.py:166-239): the nodal
+    capacitance matrix C = P^-1 written as a network of two-terminal
+    capacitors (c_ij = -C_ij between nodes i and j, sum_j C_ij from node i to
+    ground) is stamped back into a nodal matrix; returns the relative
+    infinity-norm error of that reconstruction against P^-1."""
+    Cm = np.linalg.solve(P, np.eye(P.shape[0]))
+    n = P.shape[0]
+    rec = np.zeros_like(Cm)
+    for i in range(n):
+        g = Cm[i].sum()  # capacitor from node i to ground
+        rec[i, i] += g
+        for j in range(i + 1, n):
+            c = -Cm[i, j]
+            rec[i, i] += c
+            rec[j, j] += c
+            rec[i, j] -= c
+            rec[j, i] -= c
+    return np.linalg.norm(rec - Cm, np.inf) / np.linalg.norm(Cm, np.inf)
+
+
+def test_exported_reference_matrices_give_exact_capacitance_network(tmp_path):
+    """The reference's own Lp/P of meshes/plate.msh (golden), exported by this
+    library and read back by scipy.io.mmread, are the same numbers bit for
+    bit, A holds one -1/+1 pair per edge, and the capacitance network built
+    from the exported P reconstructs P^-1 to 1e-10 (run_tests.py:209-226)."""
+    g = np.load(os.path.join(ck.ROOT, "tests", "golden", "plate_msh.npz"))
+    with plib.Model(msh=ck.PLATE_MSH) as m:
+        read, A = _export_set(tmp_path, m, g["L"], g["P"])
+    assert np.array_equal(read["L"], g["L"]) and np.array_equal(read["P"], g["P"])
+    assert np.array_equal(read["A"], A) and not read["R"].any()
+    assert np.all(np.abs(read["A"]).sum(axis=1) == 2) and np.all(read["A"].sum(axis=1) == 0)
+    err = _capacitance_network_error(read["P"])
+    print(f"capacitance network reconstruction error {err:.3e}")
+    assert np.isfinite(err) and err <= 1e-10
