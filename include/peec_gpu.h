@@ -115,6 +115,8 @@ typedef struct
                                    two such factors try 128 and 64 and keep the faster */
     double moment_center;       /* wave number k0 of the shifted-center moments that
                                    gave this point's retarded matrices (0 otherwise) */
+    int batch_size;             /* points whose factors shared this point's launches
+                                   (peec_engine_solve_batch; 1 or 0 otherwise) */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;
@@ -142,6 +144,21 @@ int peec_engine_solve(PeecEngine *engine, double frequency, int mode,
                       const int *port_nodes, size_t n_ports, Complex *z_in,
                       Complex *edge_current, Complex *node_voltage,
                       PeecPointReport *report);
+
+/* Mixed-precision points of this engine's mesh solved together: point p at
+ * frequency[p] with n_ports[p] port pairs ports[p] (as peec_engine_solve)
+ * and impedances z_in[p]. Every point's retarded blocks, equilibration and
+ * FP32 matrix are formed in the engine's reused row order, then one ordered
+ * factor runs over all of them in shared launches (the members of a launch
+ * are its grid's z index), then each point is solved and refined on its own
+ * factor with the single point's fallbacks. Points met before the engine has
+ * a row order (its first mixed point) are solved alone first. The factors
+ * are those of single points in the same row order and block width, so the
+ * results equal single calls'. status[p] gets each point's code (as
+ * peec_engine_solve); returns 0 or the first non-zero one. */
+int peec_engine_solve_batch(PeecEngine *engine, size_t n_points, const double *frequency,
+                            const int *const *ports, const size_t *n_ports, Complex *const *z_in,
+                            PeecPointReport *reports, int *status);
 
 /* Monostatic radar cross section on one factor (the workload of
  * main_rcs.c:1380-1500 for many incidences at once). For incidence i the

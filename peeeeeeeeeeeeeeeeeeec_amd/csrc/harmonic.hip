@@ -236,7 +236,20 @@ struct PeecEngine {
     int tuned_nb = 0;
     double trial_s[2] = {0.0, 0.0};
     int trials = 0;
-    DeviceBuffer<float2> d_Z32;
+    DeviceBuffer<float2> d_Z32;  // FP32 factors: one, or a batch side by side
+    float2 *z32 = nullptr;       // the current point's FP32 factor (in d_Z32)
+    // mixed points factored together (peec_engine_solve_batch): each one's
+    // retarded blocks and equilibration scales; the engine's own buffers are
+    // swapped with a point's while that point is prepared or solved
+    struct PointSlot {
+        DeviceBuffer<double2> Lw, Pw;
+        DeviceBuffer<double> row, col;
+        MnaOperands mna;
+        double limit = 0.0;
+    };
+    std::vector<PointSlot> slots;
+    DeviceBuffer<double> d_limits;
+    DeviceBuffer<int> d_batch_perm;  // the row order a batch was factored in
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
     PeecStaticStats static_stats{};
@@ -317,11 +330,11 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
         st = lu_factor<double2>(e->d_Z.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
     } else {
         double max_scaled = 0.0;
-        if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr, nullptr, e->d_row.ptr, e->d_col.ptr, &max_scaled,
-                                        *e->lu) != 0)
+        if (convert_equilibrated_to_f32(e->mna, e->z32, nullptr, e->d_row.ptr, e->d_col.ptr, &max_scaled, *e->lu) !=
+            0)
             return -2;
         e->limit = 1.0e-14 * max_scaled;
-        st = lu_factor<float2>(e->d_Z32.ptr, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
+        st = lu_factor<float2>(e->z32, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
     }
     if (st.code != 0) return factor_error(st);
     permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
@@ -333,26 +346,19 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
 
 // Retarded assembly, MNA fill, equilibration and the factor (FP64, or FP32
 // in the previous point's row order / with partial pivoting). 0, -1 or -2.
-static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &report) {
+// Retarded blocks, MNA operands and equilibration of point f in the
+// engine's current buffers: the scales only (mixed mode: the FP32 matrix is
+// formed from the blocks later), or the equilibrated FP64 matrix in d_Z.
+// Events ev[1] / ev[2] mark the end of the assembly / of the equilibration.
+static int prepare_point(PeecEngine *e, double f, bool mixed, PeecPointReport &report) {
     hipStream_t s = e->stream;
     const int n = e->n, ne = e->ne, nv = e->nv;
     const size_t nn = (size_t)n * n;
     const double omega = kTwoPi * f;
-    HIP_CHECK(hipEventRecord(e->ev[0], s));
-    if (e->cache_ok && f == e->cached_f && mode == e->cached_mode) {
-        // the factor of this point is still held: nothing to assemble,
-        // fill or factor
-        for (int i = 1; i <= 3; ++i) HIP_CHECK(hipEventRecord(e->ev[i], s));
-        report.pivot_reuse = e->ordered ? 1 : 0;
-        report.factor_cached = 1;
-        return 0;
-    }
-    e->cache_ok = false;
     const bool ret = e->opt.retarded != 0;
     e->retarded_now = ret;
     if (ret) report.moment_terms = assemble_retarded(e, omega / kC0, &report.moment_center);
     HIP_CHECK(hipEventRecord(e->ev[1], s));
-    const bool mixed = mode == PEEC_SOLVE_MIXED;
     MnaOperands &m = e->mna;
     m = MnaOperands{};
     m.ne = ne;
@@ -382,16 +388,37 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
         e->limit = 1.0e-14 * max_scaled;
     }
     HIP_CHECK(hipEventRecord(e->ev[2], s));
+    return 0;
+}
+
+static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &report) {
+    hipStream_t s = e->stream;
+    const int n = e->n;
+    const size_t nn = (size_t)n * n;
+    HIP_CHECK(hipEventRecord(e->ev[0], s));
+    if (e->cache_ok && f == e->cached_f && mode == e->cached_mode) {
+        // the factor of this point is still held: nothing to assemble,
+        // fill or factor
+        for (int i = 1; i <= 3; ++i) HIP_CHECK(hipEventRecord(e->ev[i], s));
+        report.pivot_reuse = e->ordered ? 1 : 0;
+        report.factor_cached = 1;
+        return 0;
+    }
+    e->cache_ok = false;
+    const bool mixed = mode == PEEC_SOLVE_MIXED;
+    const int prep = prepare_point(e, f, mixed, report);
+    if (prep) return prep;
     e->d_ipiv.resize(n);
     e->d_perm.resize(n);
     e->d_reuse_perm.resize(n);
     LUStatus st;
     e->ordered = false;
     if (mixed) {
-        e->d_Z32.resize(nn);
+        e->d_Z32.reserve(nn);
+        e->z32 = e->d_Z32.ptr;
         if (e->opt.reuse_pivots && e->have_reuse) {
             double max_scaled = 0.0;
-            if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr, e->d_reuse_perm.ptr, e->d_row.ptr, e->d_col.ptr,
+            if (convert_equilibrated_to_f32(e->mna, e->z32, e->d_reuse_perm.ptr, e->d_row.ptr, e->d_col.ptr,
                                             &max_scaled, *e->lu) != 0)
                 return -2;
             e->limit = 1.0e-14 * max_scaled;
@@ -401,7 +428,7 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
             if (nb == 0) nb = trial ? kTrialWidths[e->trials] : e->tuned_nb;
             if (trial) e->stream.sync();
             const auto t0 = std::chrono::steady_clock::now();
-            st = lu_factor_ordered(e->d_Z32.ptr, n, n, e->limit, *e->lu, nb);
+            st = lu_factor_ordered(e->z32, n, n, e->limit, *e->lu, nb);
             report.block_size = nb;
             if (trial && st.code == 0) {
                 // lu_factor_ordered returns after its stream has drained
@@ -483,8 +510,8 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
     };
     auto refine = [&]() {
         HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
-        lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr,
-                                           nrhs, *e->lu);
+        lu_solve_permuted<float2, double2>(e->z32, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_X.ptr, nrhs,
+                                           *e->lu);
         measure();
         std::vector<double> best = eta;
         std::vector<int> since(nrhs, 0), active(nrhs), flags(nrhs);
@@ -494,8 +521,8 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             bool any = false;
             for (int q = 0; q < nrhs; ++q) any = any || active[q];
             if (!any) break;
-            lu_solve_permuted<float2, double2>(e->d_Z32.ptr, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr,
-                                               e->d_R.ptr, nrhs, *e->lu, true);
+            lu_solve_permuted<float2, double2>(e->z32, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, e->d_R.ptr,
+                                               nrhs, *e->lu, true);
             const int *dfl;
             unsigned long long msk;
             flag_args(active, dfl, msk);
@@ -932,6 +959,177 @@ int peec_engine_solve(PeecEngine *e, double f, int mode, const int *ports, size_
         if (rep) *rep = report;
         return -1;
     }
+}
+
+// The engine's per-point buffers and a batch slot's trade places (the
+// engine then works on that point's blocks and scales).
+static void swap_point(PeecEngine *e, PeecEngine::PointSlot &p) {
+    std::swap(e->d_Lw, p.Lw);
+    std::swap(e->d_Pw, p.Pw);
+    std::swap(e->d_row, p.row);
+    std::swap(e->d_col, p.col);
+}
+
+int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *frequency, const int *const *ports,
+                            const size_t *n_ports, Complex *const *z_in, PeecPointReport *reports, int *status) {
+    if (!e || n_points == 0 || !frequency || !ports || !n_ports || !z_in || !status) return -1;
+    for (size_t p = 0; p < n_points; ++p) {
+        if (!(frequency[p] > 0.0) || !std::isfinite(frequency[p]) || !ports[p] || n_ports[p] == 0 || !z_in[p])
+            return -1;
+        for (size_t k = 0; k < 2 * n_ports[p]; ++k)
+            if (ports[p][k] < 0 || ports[p][k] >= e->nv) return -1;
+    }
+    auto worst = [&]() {
+        for (size_t p = 0; p < n_points; ++p)
+            if (status[p] != 0) return status[p];
+        return 0;
+    };
+    auto single = [&](size_t p) {
+        PeecPointReport r{};
+        status[p] = peec_engine_solve(e, frequency[p], PEEC_SOLVE_MIXED, ports[p], n_ports[p], z_in[p], nullptr,
+                                      nullptr, &r);
+        r.batch_size = 1;
+        if (reports) reports[p] = r;
+    };
+    // points before the engine has a row order to reuse go alone (the first
+    // one's partial-pivoting factor sets it)
+    size_t first = 0;
+    while (first < n_points && !(e->opt.reuse_pivots && e->have_reuse)) single(first++);
+    const size_t m = n_points - first;
+    if (m <= 1) {
+        if (m == 1) single(first);
+        return worst();
+    }
+    std::vector<PeecPointReport> rep(m);
+    for (size_t b = 0; b < m; ++b) status[first + b] = -1;  // until the point is done
+    try {
+        hipStream_t s = e->stream;
+        const int n = e->n, ne = e->ne, nv = e->nv;
+        const size_t nn = (size_t)n * n;
+        const int nb = e->opt.block_size > 0 ? e->opt.block_size : e->tuned_nb > 0 ? e->tuned_nb : 128;
+        e->cache_ok = false;
+        e->slots.resize(m);
+        e->d_Z32.reserve(m * nn);
+        e->d_batch_perm.resize(n);
+        HIP_CHECK(hipMemcpyAsync(e->d_batch_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                                 s));
+        std::vector<int> prep(m, 0);
+        std::vector<double> limits(m, 0.0);
+        std::vector<float> ms_prep(m, 0.f), ms_asm(m, 0.f);
+        // each point's retarded blocks, scales and FP32 matrix in the batch's
+        // row order, the matrices side by side in d_Z32
+        for (size_t b = 0; b < m; ++b) {
+            PeecEngine::PointSlot &sl = e->slots[b];
+            rep[b].frequency = frequency[first + b];
+            rep[b].batch_size = (int)m;
+            swap_point(e, sl);
+            HIP_CHECK(hipEventRecord(e->ev[0], s));
+            prep[b] = prepare_point(e, frequency[first + b], true, rep[b]);
+            if (prep[b] == 0) {
+                double max_scaled = 0.0;
+                if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr + b * nn, e->d_batch_perm.ptr, e->d_row.ptr,
+                                                e->d_col.ptr, &max_scaled, *e->lu) != 0)
+                    prep[b] = -2;
+                limits[b] = 1.0e-14 * max_scaled;
+            }
+            HIP_CHECK(hipEventRecord(e->ev[3], s));
+            sl.mna = e->mna;
+            sl.limit = limits[b];
+            swap_point(e, sl);
+            e->stream.sync();
+            HIP_CHECK(hipEventElapsedTime(&ms_asm[b], e->ev[0], e->ev[1]));
+            HIP_CHECK(hipEventElapsedTime(&ms_prep[b], e->ev[1], e->ev[3]));
+        }
+        // one ordered factor over the batch (a member that failed above is
+        // factored too, from whatever its slot holds, and its result ignored)
+        e->d_limits.upload(limits.data(), m, s);
+        std::vector<LUStatus> fst(m);
+        HIP_CHECK(hipEventRecord(e->ev[0], s));
+        lu_factor_ordered(e->d_Z32.ptr, n, n, 0.0, *e->lu, nb, (int)m, nn, e->d_limits.ptr, fst.data());
+        HIP_CHECK(hipEventRecord(e->ev[1], s));
+        e->stream.sync();
+        float ms_factor = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms_factor, e->ev[0], e->ev[1]));
+        // each point's solves and refinement on its own factor (with the
+        // single point's fallbacks: a rejected order is refactored with
+        // pivoting, a point above the gate in FP64)
+        for (size_t b = 0; b < m; ++b) {
+            const size_t p = first + b;
+            PeecPointReport &r = rep[b];
+            r.ms_assembly = ms_asm[b];
+            r.ms_fill = ms_prep[b];
+            r.ms_factor = ms_factor;
+            r.block_size = nb;
+            if (prep[b] != 0) {
+                status[p] = prep[b];
+                continue;
+            }
+            PeecEngine::PointSlot &sl = e->slots[b];
+            swap_point(e, sl);
+            e->mna = sl.mna;
+            e->limit = sl.limit;
+            e->z32 = e->d_Z32.ptr + b * nn;
+            e->retarded_now = e->opt.retarded != 0;
+            e->z0_ready = false;
+            e->z_equilibrated = false;
+            int code = 0;
+            if (fst[b].code == 0) {
+                e->ordered = true;
+                r.pivot_reuse = 1;
+                HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_batch_perm.ptr, sizeof(int) * n,
+                                         hipMemcpyDeviceToDevice, s));
+            } else if (fst[b].code == -2) {
+                e->ordered = false;
+                code = factor_pivoted(e, false);
+            } else {
+                code = factor_error(fst[b]);
+            }
+            if (code == 0) {
+                const int nr = (int)n_ports[p];
+                const size_t tot = (size_t)n * nr;
+                e->d_ports.upload(ports[p], 2 * n_ports[p], s);
+                e->d_B.resize(tot);
+                HIP_CHECK(hipEventRecord(e->ev[3], s));
+                hipLaunchKernelGGL(port_rhs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, e->d_B.ptr,
+                                   ne, nv, e->d_P.ptr, e->retarded_now ? e->d_Pw.ptr : nullptr, e->d_ports.ptr, nr);
+                code = engine_solve(e, PEEC_SOLVE_MIXED, nr, r);
+                if (code == 0 || code == -3) {
+                    std::vector<double2> x(tot);
+                    e->d_X.download(x.data(), tot, s);
+                    e->stream.sync();
+                    for (int k = 0; k < nr; ++k) {
+                        const double2 *xk = x.data() + (size_t)k * n;
+                        const double2 va = xk[ne + ports[p][2 * k]], vb = xk[ne + ports[p][2 * k + 1]];
+                        z_in[p][k] = Complex{va.x - vb.x, va.y - vb.y};
+                    }
+                    float a = 0.f, c = 0.f;
+                    HIP_CHECK(hipEventElapsedTime(&a, e->ev[3], e->ev[4]));
+                    HIP_CHECK(hipEventElapsedTime(&c, e->ev[4], e->ev[5]));
+                    r.ms_solve = a;
+                    r.ms_check = c;
+                }
+                if (code == -3)
+                    PEEC_ERR("backward error %.3e above the gate %.1e at f = %.6g Hz.", r.backward_error,
+                             e->opt.backward_error_gate, frequency[p]);
+            }
+            status[p] = code;
+            swap_point(e, sl);
+        }
+        // the engine's own buffers are back; nothing it holds is a factor of
+        // a single point
+        e->cache_ok = false;
+        e->z32 = e->d_Z32.ptr;
+    } catch (const HipError &err) {
+        PEEC_ERR("%s", err.what.c_str());
+        e->cache_ok = false;
+        e->z32 = e->d_Z32.ptr;
+        if (reports)
+            for (size_t b = 0; b < m; ++b) reports[first + b] = rep[b];
+        return -1;
+    }
+    if (reports)
+        for (size_t b = 0; b < m; ++b) reports[first + b] = rep[b];
+    return worst();
 }
 
 void peec_rcs_options_default(PeecRcsOptions *o) {

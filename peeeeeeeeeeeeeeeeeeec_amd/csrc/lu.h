@@ -17,6 +17,39 @@ struct LUStatus {
     long long column = -1;
 };
 
+// p += blockIdx.z * stride (null pointers stay null)
+template <class P> __device__ inline void batch_shift(P &p, size_t stride) {
+    if (p) p += (size_t)blockIdx.z * stride;
+}
+
+// Strides between the members of a batch of equally sized factors that run
+// in the same launches, one member per grid z index; all zero for a single
+// factor. A kernel moves its pointers to member blockIdx.z on entry.
+struct BatchStride {
+    size_t a = 0;                    // float2 elements between the matrices
+    size_t dinv = 0;                 // float2 elements between the 16 x 16 inverse sets
+    size_t pl = 0, pu = 0;           // uint4 units between the L21 / U12 operand packs
+    size_t sl = 0, su = 0;           // floats between their scales
+    int status = 0;                  // ints between the status pairs
+    const double *limits = nullptr;  // pivot limit of each member (null: the kernel argument)
+    template <class TA, class TS> __device__ void apply(TA &A, TS &st) const {
+        batch_shift(A, a);
+        batch_shift(st, (size_t)status);
+    }
+    template <class TA, class TS, class TD> __device__ void apply(TA &A, TS &st, TD &d) const {
+        apply(A, st);
+        batch_shift(d, dinv);
+    }
+    template <class TA, class TS, class TD> __device__ void apply(TA &A, TS &st, TD &d, double &limit) const {
+        apply(A, st, d);
+        if (limits) limit = limits[blockIdx.z];
+    }
+    template <class TP, class TF> __device__ void apply_pack(TP &p, TF &f, size_t ps, size_t fs) const {
+        batch_shift(p, ps);
+        batch_shift(f, fs);
+    }
+};
+
 // Scratch owned by one factorization context (one stream).
 struct LUContext {
     DeviceBuffer<unsigned> barrier;   // grid barrier counter
@@ -94,8 +127,14 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
 // (no pivot search, no row exchange between workgroups); status -2 when a
 // pivot has |u_kk| <= pivot_limit. Used by the mixed-precision path with the
 // row order of an earlier partial-pivoting factorization.
+// A batch of `batch` equally sized matrices batch_stride elements apart
+// (block widths 64 or 128) is factored in the same launches, one member per
+// grid z index, with pivot limits d_limits[b] (device); the result is the
+// first failing member's status, and member_status[b] (host, optional)
+// receives each one's. A single factor (batch 1) ignores the batch arguments.
 LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx,
-                           int block_size = 0);
+                           int block_size = 0, int batch = 1, size_t batch_stride = 0,
+                           const double *d_limits = nullptr, LUStatus *member_status = nullptr);
 
 // Solve with the factors for `nrhs` right-hand sides stored rhs-major
 // (X[r*n + i]); applies the row scale, pivots, L, U and column scale like

@@ -1696,7 +1696,10 @@ __global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t 
                                                               const uint4 *__restrict__ Uh,
                                                               const float *__restrict__ linv,
                                                               const float *__restrict__ uinv, int rb_pad, int cb_count,
-                                                              const int *status) {
+                                                              const int *status, BatchStride bs) {
+    bs.apply(A, status);
+    bs.apply_pack(Lh, linv, bs.pl, bs.sl);
+    bs.apply_pack(Uh, uinv, bs.pu, bs.su);
     gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
 }
 template <int KC, int MB, int NB, int ND>
@@ -1705,7 +1708,10 @@ __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t 
                                                               const uint4 *__restrict__ Uh,
                                                               const float *__restrict__ linv,
                                                               const float *__restrict__ uinv, int rb_pad, int cb_count,
-                                                              const int *status) {
+                                                              const int *status, BatchStride bs) {
+    bs.apply(A, status);
+    bs.apply_pack(Lh, linv, bs.pl, bs.sl);
+    bs.apply_pack(Uh, uinv, bs.pu, bs.su);
     gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
 }
 
@@ -1714,7 +1720,7 @@ __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t 
 // the next-column kernel's name.
 static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
                     const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
-                    hipStream_t s, int rb_first = 0) {
+                    hipStream_t s, int rb_first = 0, const BatchStride &bs = BatchStride{}, int batch = 1) {
     if (m <= 0 || ncols <= 0) return;
     // rows from row0 are L21 row blocks rb_first.. of the pack
     const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr) + (size_t)rb_first * 128;
@@ -1729,8 +1735,8 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     const int grid = (tiles + 7) / 8 * 8;
 #define PEEC_H3(KC, MB, NB, ND)                                                                                     \
     hipLaunchKernelGGL((next ? gemm_next_h3_kernel<KC, MB, NB, ND> : gemm_tile_h3_kernel<KC, MB, NB, ND>),          \
-                       dim3(grid), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad, cb_count, \
-                       status)
+                       dim3(grid, 1, batch), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad,  \
+                       cb_count, status, bs)
     if (kw == 256)
         PEEC_H3(16, 2, 4, 4);
     else if (kw == 128)
@@ -1742,8 +1748,8 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
 #undef PEEC_H3
     HIP_CHECK(hipGetLastError());
     if (ctx.timer)
-        ctx.timer->stop(tg, s, kind, 8.0 * m * ncols * kw,
-                        (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
+        ctx.timer->stop(tg, s, kind, 8.0 * batch * m * ncols * kw,
+                        batch * (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
 }
 
 // ---------------------------------------------------------------------------
@@ -1795,13 +1801,14 @@ __device__ inline void pivot_check(double u, double limit, int &ok, double &rcp)
 // at dinv[256 b], and (L_b^-1)^T (L_b unit lower) row-major at
 // dinv[2048 + 256 b], each column by substitution in FP32.
 __global__ __launch_bounds__(1024) void diag_lu_tiled_kernel(float2 *A, size_t lda, int k0, int w, double limit,
-                                                             int *status, float2 *dinv) {
+                                                             int *status, float2 *dinv, BatchStride bs) {
     // double-buffered by panel parity: L11 (unit lower) and U11 of the
     // diagonal tile, reciprocals of its pivots, L21 tiles by tile row and
     // U12 tiles by tile column
     __shared__ float2 s_l11[2][16], s_u11[2][16], s_rd[2][4];
     __shared__ float2 s_l21[2][32][16], s_u12[2][32][16];
     __shared__ int s_ok[2];
+    bs.apply(A, status, dinv, limit);
     if (status[0] != 0) return;
     const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
     const int ti = 8 * (wv >> 2) + (ln >> 3), tc = 8 * (wv & 3) + (ln & 7);
@@ -2014,7 +2021,7 @@ template <int NBK> __host__ __device__ constexpr size_t diag_blocked_lds() {
 }
 template <int NBK>
 __global__ __launch_bounds__(512) void diag_lu_blocked_kernel(float2 *A, size_t lda, int k0, double limit,
-                                                              int *status, float2 *dinv) {
+                                                              int *status, float2 *dinv, BatchStride bs) {
     constexpr int W = 16 * NBK, S = W + 4;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *sa = reinterpret_cast<float2 *>(smem_raw);  // the block, W x S
@@ -2022,6 +2029,7 @@ __global__ __launch_bounds__(512) void diag_lu_blocked_kernel(float2 *A, size_t 
     float2 *suinv = slt + (NBK - 1) * 256;              // U_pp^-1, row-major
     float2 *slinvt = suinv + 256;                       // (L_pp^-1)^T, row-major
     __shared__ int s_bad;
+    bs.apply(A, status, dinv, limit);
     if (status[0] != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     if (tid == 0) s_bad = -1;
@@ -2296,10 +2304,12 @@ __device__ __forceinline__ void pack_h3_regs(const v4f (&xr)[NBK], const v4f (&x
 template <int NBK, bool PRE>
 __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t lda, int k0, int r_begin, int r_end,
                                                               const float2 *__restrict__ dinv, const int *status,
-                                                              uint4 *pk, float *pinv, int rb_pad) {
+                                                              uint4 *pk, float *pinv, int rb_pad, BatchStride bs) {
     constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2, NP = PRE ? NBK : 0;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_u = s_inv + NBK * 256;
+    bs.apply(A, status, dinv);
+    bs.apply_pack(pk, pinv, bs.pl, bs.sl);
     if (status[0] != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int rb = r_begin + ((int)blockIdx.x * 4 + wave) * 16;
@@ -2381,10 +2391,12 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
 template <int NBK, bool PRE>
 __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t lda, int k0, int c0, int c1,
                                                              const float2 *__restrict__ dinv, const int *status,
-                                                             uint4 *pk, float *pinv, int pcb) {
+                                                             uint4 *pk, float *pinv, int pcb, BatchStride bs) {
     constexpr int W = 16 * NBK, NU = NBK * (NBK - 1) / 2, NP = PRE ? NBK : 0;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *s_inv = reinterpret_cast<float2 *>(smem_raw), *s_l = s_inv + NBK * 256;
+    bs.apply(A, status, dinv);
+    bs.apply_pack(pk, pinv, bs.pu, bs.su);
     if (status[0] != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int tile = (int)blockIdx.x * 4 + wave, cb = c0 + 16 * tile;
@@ -2670,11 +2682,14 @@ static void set_dynamic_lds(const void *fn, size_t bytes) {
 // updated first and its diagonal block is factored on the context's second
 // stream while the rest of the trailing update runs on the first; the next
 // panel waits for that factor through an event.
-LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx, int block_size) {
+LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LUContext &ctx, int block_size,
+                           int batch, size_t batch_stride, const double *d_limits, LUStatus *member_status) {
     using S = float2;
     hipStream_t s = ctx.stream, aux = ctx.aux;
     LUStatus out;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2, 0);
+    if (batch < 1) throw HipError{"ordered factor: empty batch"};
+    ctx.status.reserve((size_t)2 * batch);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.status.ptr, 2 * batch, 0);
     int nb = block_size > 0 ? std::min(block_size, kMaxNB) : kMaxNB;
     while (nb > 16 && (size_t)nb * nb * sizeof(S) > (size_t)ctx.lds_per_block) nb /= 2;
     const size_t split_lds = (size_t)4 * 2 * 4096 * sizeof(float);
@@ -2694,7 +2709,19 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     // factor then also leaves its 16 x 16 block inverses, in the set of the
     // panel's parity); substitution kernels otherwise
     const bool blocked_solves = nb % 16 == 0 && (nb / 16 == 2 || nb / 16 == 4 || nb / 16 == 8);
-    ctx.dblk_inv.reserve(2 * 4096);
+    ctx.dblk_inv.reserve((size_t)2 * 4096 * batch);
+    // members of a batch: the matrices batch_stride elements apart, each
+    // with its own status pair, inverse sets, operand packs and pivot limit
+    BatchStride bs;
+    if (batch > 1) {
+        bs.a = batch_stride;
+        bs.status = 2;
+        bs.dinv = 2 * 4096;
+        bs.limits = d_limits;
+    }
+    // the same with the strides of the L21 pack and of the next block
+    // column's / the rest's U12 pack (set with the packs below)
+    BatchStride bs_n = bs, bs_r = bs;
     if (blocked_solves && nb == 128) {
         set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8, false>), trsm_mfma_lds<8>());
         set_dynamic_lds(reinterpret_cast<const void *>(trsm_right_mfma_kernel<8, true>), trsm_mfma_lds<8>());
@@ -2710,9 +2737,9 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         if (r_end < 0) r_end = rest;
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((r_end - r_begin + 63) / 64);
-#define PEEC_TRSM_R(K, P)                                                                                          \
-    hipLaunchKernelGGL((trsm_right_mfma_kernel<K, P>), dim3(grid), dim3(256), trsm_mfma_lds<K>(), s, dA, lda, k0, \
-                       r_begin, r_end, dinv_of(k0), ctx.status.ptr, pk, pinv, stride)
+#define PEEC_TRSM_R(K, P)                                                                                  \
+    hipLaunchKernelGGL((trsm_right_mfma_kernel<K, P>), dim3(grid, 1, batch), dim3(256), trsm_mfma_lds<K>(), s, dA, \
+                       lda, k0, r_begin, r_end, dinv_of(k0), ctx.status.ptr, pk, pinv, stride, bs_n)
             if (w == 128 && pre)
                 PEEC_TRSM_R(8, true);
             else if (w == 128)
@@ -2729,6 +2756,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             return;
         }
         if (pk) throw HipError{"ordered factor: packed L21 solves need the blocked solve kernels"};
+        if (batch > 1) throw HipError{"ordered factor: a batch needs the blocked solve kernels"};
         const int grid = std::max(1, std::min((r_end - r_begin + 15) / 16, ctx.cus));
         hipLaunchKernelGGL(trsm_right_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), (size_t)w * w * sizeof(S), s,
                            dA, lda, k0, w, r_begin, r_end, ctx.status.ptr);
@@ -2739,9 +2767,10 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                          int pcb = 0, bool pre = false) {
         if (blocked_solves && w == nb) {
             const unsigned grid = (unsigned)((c1 - c0 + 63) / 64);
-#define PEEC_TRSM_L(K, P)                                                                                          \
-    hipLaunchKernelGGL((trsm_left_mfma_kernel<K, P>), dim3(grid), dim3(256), trsm_mfma_lds<K>(), st, dA, lda, k0, \
-                       c0, c1, dinv_of(k0), ctx.status.ptr, pk, pinv, pcb)
+            const BatchStride &bu = pk == reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr) ? bs_n : bs_r;
+#define PEEC_TRSM_L(K, P)                                                                                  \
+    hipLaunchKernelGGL((trsm_left_mfma_kernel<K, P>), dim3(grid, 1, batch), dim3(256), trsm_mfma_lds<K>(), st, dA, \
+                       lda, k0, c0, c1, dinv_of(k0), ctx.status.ptr, pk, pinv, pcb, bu)
             if (w == 128 && pre)
                 PEEC_TRSM_L(8, true);
             else if (w == 128)
@@ -2758,6 +2787,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
             return;
         }
         if (pre) throw HipError{"ordered factor: two-panel U12 packs need the blocked solve kernels"};
+        if (batch > 1) throw HipError{"ordered factor: a batch needs the blocked solve kernels"};
         const int tiles = (c1 - c0 + kTrsmWaveCols - 1) / kTrsmWaveCols;
         const int grid = std::max(1, std::min(tiles, ctx.cus));
         hipLaunchKernelGGL(trsm_wave_kernel<S>, dim3((unsigned)grid), dim3(1024), trsm_wave_lds(w) * sizeof(S), st,
@@ -2769,14 +2799,14 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         const int tp = ctx.timer ? ctx.timer->start(st) : -1;
         if (blocked_solves && w == nb && (w == 128 || w == 64)) {
             if (w == 128)
-                hipLaunchKernelGGL(diag_lu_blocked_kernel<8>, dim3(1), dim3(512), diag_blocked_lds<8>(), st, dA, lda,
-                                   k0, pivot_limit, ctx.status.ptr, dinv_of(k0));
+                hipLaunchKernelGGL(diag_lu_blocked_kernel<8>, dim3(1, 1, batch), dim3(512), diag_blocked_lds<8>(), st,
+                                   dA, lda, k0, pivot_limit, ctx.status.ptr, dinv_of(k0), bs);
             else
-                hipLaunchKernelGGL(diag_lu_blocked_kernel<4>, dim3(1), dim3(512), diag_blocked_lds<4>(), st, dA, lda,
-                                   k0, pivot_limit, ctx.status.ptr, dinv_of(k0));
+                hipLaunchKernelGGL(diag_lu_blocked_kernel<4>, dim3(1, 1, batch), dim3(512), diag_blocked_lds<4>(), st,
+                                   dA, lda, k0, pivot_limit, ctx.status.ptr, dinv_of(k0), bs);
         } else {
-            hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
-                               ctx.status.ptr, blocked_solves && w == nb ? dinv_of(k0) : nullptr);
+            hipLaunchKernelGGL(diag_lu_tiled_kernel, dim3(1, 1, batch), dim3(1024), 0, st, dA, lda, k0, w, pivot_limit,
+                               ctx.status.ptr, blocked_solves && w == nb ? dinv_of(k0) : nullptr, bs);
         }
         if (ctx.timer) ctx.timer->stop(tp, st, kKernelPanel, 0.0, 2.0 * (double)w * w * sizeof(S));
     };
@@ -2787,16 +2817,28 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     // split-FP16 operand buffers at their largest (the first panels), so
     // that no buffer moves while the other stream may use it
     const bool split_panels = split16 && blocked_solves && (nb == 128 || nb == 64) && n > nb;
+    if (batch > 1 && n > nb && !split_panels)
+        throw HipError{"ordered factor: a batch needs the split-FP16 update with a block width of 64 or 128"};
     if (split_panels) {
         int rbp, cbp;
         gemm_pads<S>(n, n, rbp, cbp);
         const int kcm = 2 * nb / 16, cbm = (n + 127) / 128 * 8;
-        ctx.pack_l16.reserve((size_t)kcm * rbp * 128 * sizeof(uint4));
-        ctx.scale_l16.reserve((size_t)rbp * 16);
-        ctx.pack_u16n.reserve((size_t)kcm * 8 * 128 * sizeof(uint4));
-        ctx.scale_u16n.reserve((size_t)8 * 16);
-        ctx.pack_u16.reserve((size_t)kcm * cbm * 128 * sizeof(uint4));
-        ctx.scale_u16.reserve((size_t)cbm * 16);
+        // per member (in uint4 units / floats)
+        const size_t pl = (size_t)kcm * rbp * 128, pun = (size_t)kcm * 8 * 128, pu = (size_t)kcm * cbm * 128;
+        ctx.pack_l16.reserve(pl * batch * sizeof(uint4));
+        ctx.scale_l16.reserve((size_t)rbp * 16 * batch);
+        ctx.pack_u16n.reserve(pun * batch * sizeof(uint4));
+        ctx.scale_u16n.reserve((size_t)8 * 16 * batch);
+        ctx.pack_u16.reserve(pu * batch * sizeof(uint4));
+        ctx.scale_u16.reserve((size_t)cbm * 16 * batch);
+        if (batch > 1) {
+            bs_n.pl = bs_r.pl = pl;
+            bs_n.sl = bs_r.sl = (size_t)rbp * 16;
+            bs_n.pu = pun;
+            bs_n.su = (size_t)8 * 16;
+            bs_r.pu = pu;
+            bs_r.su = (size_t)cbm * 16;
+        }
     }
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int w = std::min(nb, n - k0), rest = n - k0 - w;
@@ -2867,12 +2909,12 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                 if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, false);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
                 gemm_h3(dA, lda, r0, r0, w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
-                        ctx.status.ptr, ctx, s);
+                        ctx.status.ptr, ctx, s, 0, bs_n, batch);
                 start_next_diag();
                 if (rest > w2) {
                     solve_l21(k0, w, rest, w2, rest, pln, ctx.scale_l16.ptr, rb_pad, false);
                     gemm_h3(dA, lda, r0 + w2, r0, rest - w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr,
-                            kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd);
+                            kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd, bs_n, batch);
                 }
             } else {
                 // L21 (rows r0..) and U12 of the next block column, then its
@@ -2881,7 +2923,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                 if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, second);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
                 gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
-                        ctx.status.ptr, ctx, s);
+                        ctx.status.ptr, ctx, s, 0, bs_n, batch);
                 start_next_diag();
             }
             if (rest2 > 0) {
@@ -2891,13 +2933,13 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                     // then only the next panel's rows
                     solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16);
                     gemm_h3(dA, lda, r0, r0 + w2, w2, rest2, w, rb_pad, cb16, pu, ctx.scale_u16.ptr,
-                            kKernelLuUpdateThin, ctx.status.ptr, ctx, s);
+                            kKernelLuUpdateThin, ctx.status.ptr, ctx, s, 0, bs_r, batch);
                 } else {
                     // the rest of this panel's U12, packed with the first
                     // panel's U12 rows, and the rank-2w trailing update
                     solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16, true);
                     gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, ctx.scale_u16.ptr,
-                            kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s);
+                            kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s, 0, bs_r, batch);
                 }
             }
             pair_open = !second;
@@ -2951,11 +2993,16 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         HIP_CHECK(hipGetLastError());
     }
     if (pending) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
-    int st[2] = {0, 0};
-    ctx.status.download(st, 2, s);
+    std::vector<int> st((size_t)2 * batch, 0);
+    ctx.status.download(st.data(), st.size(), s);
     HIP_CHECK(hipStreamSynchronize(s));
-    out.code = st[0];
-    out.column = st[1];
+    for (int b = 0; b < batch; ++b) {
+        LUStatus m;
+        m.code = st[2 * b];
+        m.column = st[2 * b + 1];
+        if (member_status) member_status[b] = m;
+        if (out.code == 0 && m.code != 0) out = m;
+    }
     return out;
 }
 

@@ -38,7 +38,8 @@ class PeecPointReport(C.Structure):
                 ("pivot_reuse", C.c_int), ("pivot_refactor", C.c_int), ("reuse_backward_error", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int),
-                ("moment_terms", C.c_int), ("block_size", C.c_int), ("moment_center", C.c_double)]
+                ("moment_terms", C.c_int), ("block_size", C.c_int), ("moment_center", C.c_double),
+                ("batch_size", C.c_int)]
 
 
 class PeecServerOptions(C.Structure):
@@ -127,6 +128,7 @@ EXPORTED = [
     "harmonic_solve", "harmonic_compute_residual", "harmonic_compute_residual_noalloc", "harmonic_print_info",
     # peec_gpu.h
     "peec_engine_options_default", "peec_engine_create", "peec_engine_create_with_matrices", "peec_engine_solve",
+    "peec_engine_solve_batch",
     "peec_engine_static_matrices", "peec_engine_retarded_matrices", "peec_engine_edges",
     "peec_engine_nodes", "peec_engine_far_evaluations", "peec_engine_smooth_evaluations",
     "peec_engine_destroy", "peec_engine_set_profiling", "peec_engine_kernel_stats", "peec_engine_static_stats",
@@ -283,6 +285,8 @@ def load():
          C.POINTER(C.c_void_p))
     _sig(lib, "peec_engine_solve", C.c_int, C.c_void_p, C.c_double, C.c_int, _ip, C.c_size_t, _cp, _cp,
          _cp, C.POINTER(PeecPointReport))
+    _sig(lib, "peec_engine_solve_batch", C.c_int, C.c_void_p, C.c_size_t, _dp, C.POINTER(_ip),
+         C.POINTER(C.c_size_t), C.POINTER(_cp), C.POINTER(PeecPointReport), _ip)
     _sig(lib, "peec_engine_static_matrices", C.c_int, C.c_void_p, _dp, _dp)
     _sig(lib, "peec_engine_retarded_matrices", C.c_int, C.c_void_p, C.c_double, _cp, _cp)
     _sig(lib, "peec_engine_edges", C.c_size_t, C.c_void_p)
@@ -664,6 +668,27 @@ class Engine:
             I = I.reshape(k, self.ne)
             V = V.reshape(k, self.nv)
         return st, z, I, V, rep
+
+    def solve_batch(self, frequencies, ports):
+        """Mixed-precision points solved together (peec_engine_solve_batch):
+        ports[p] is the list of (a, b) node pairs of point p (one list for
+        all points if a single list of pairs is given). Returns (status,
+        per-point statuses, per-point z_in arrays, reports)."""
+        f = np.ascontiguousarray(frequencies, np.float64)
+        npt = f.size
+        if len(ports) and np.ndim(ports[0]) == 1:
+            ports = [ports] * npt
+        parr = [np.ascontiguousarray(np.asarray(p, dtype=np.int32).reshape(-1, 2)) for p in ports]
+        assert len(parr) == npt
+        zs = [np.zeros(p.shape[0], np.complex128) for p in parr]
+        pp = (_ip * npt)(*[p.ctypes.data_as(_ip) for p in parr])
+        npo = (C.c_size_t * npt)(*[p.shape[0] for p in parr])
+        zp = (_cp * npt)(*[z.ctypes.data_as(_cp) for z in zs])
+        reps = (PeecPointReport * npt)()
+        st = np.zeros(npt, np.int32)
+        code = load().peec_engine_solve_batch(self.handle, npt, f.ctypes.data_as(_dp), pp, npo, zp, reps,
+                                              st.ctypes.data_as(_ip))
+        return code, st, zs, list(reps)
 
     def monostatic_rcs(self, frequency, theta_deg, phi_deg, mode=0, polarization=1, E_amp=1.0, phase=0.0,
                        excitation=1, integration=1, order=6, want_currents=False):
