@@ -16,6 +16,14 @@
  * request's point is preferred, and the request then only solves its
  * right-hand sides.
  *
+ * Continuous batching: when a worker admits a mixed-precision request, the
+ * mixed requests of the same mesh waiting in the queue at that moment (up to
+ * max_batch in all, and as many as the budget allows) join it, and the whole
+ * group runs as one peec_engine_solve_batch on one engine: their FP32
+ * factors share every launch of the ordered factor. Requests that arrive
+ * meanwhile form the next batch. A member's answer is that of the request
+ * solved alone in the same row order.
+ *
  * The static Lp and P of a mesh are assembled once when the mesh is added;
  * further engines of that mesh start from copies of them. Mesh and DualMesh
  * must stay valid while the server exists. Calls on one server may come from
@@ -39,6 +47,8 @@ typedef struct
     int slots;                 /* worker threads = requests running at once, default 4 */
     size_t hbm_budget_bytes;   /* device memory the engines may hold; 0 = 90% of free memory
                                   when the server is created */
+    int max_batch;             /* mixed requests of one mesh solved together, default 8;
+                                  1 = one request at a time */
 } PeecServerOptions;
 
 typedef struct PeecServer PeecServer;

@@ -60,6 +60,13 @@ size_t engine_bytes(size_t ne, size_t nv, bool retarded) {
     return 8 * (ne * ne + nv * nv) + peec::engine_point_bytes(ne, nv, retarded);
 }
 
+// Further device bytes of each batch member beyond the first: its FP32
+// factor, retarded Lp(w)/P(w), scales and operand packs.
+size_t member_bytes(size_t ne, size_t nv, bool retarded) {
+    const size_t n = ne + nv;
+    return 8 * n * n + (retarded ? 16 * (ne * ne + nv * nv) : 0) + 64 * n;
+}
+
 }  // namespace
 
 struct PeecServer {
@@ -78,7 +85,35 @@ struct PeecServer {
 
     void worker();
     Job *pick(EngineEntry *&e, std::vector<PeecEngine *> &evicted);
+    void gather(Job *j, EngineEntry *e, std::vector<Job *> &batch);
 };
+
+// Under the lock: the mixed requests of j's mesh waiting in the queue join
+// j's batch on engine e (up to max_batch in all), each as long as the
+// engine's grown working set fits under the budget. A request that finds
+// its factor held by e runs alone.
+void PeecServer::gather(Job *j, EngineEntry *e, std::vector<Job *> &batch) {
+    batch.assign(1, j);
+    if (j->mode != PEEC_SOLVE_MIXED || opt.max_batch <= 1) return;
+    if (e->has_factor && e->f == j->f && e->mode == j->mode) return;
+    const MeshEntry &me = *meshes[j->mesh];
+    const size_t extra = member_bytes(me.ne, me.nv, opt.engine.retarded != 0);
+    for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < opt.max_batch;) {
+        Job *o = *it;
+        if (o->mesh != j->mesh || o->mode != PEEC_SOLVE_MIXED) {
+            ++it;
+            continue;
+        }
+        const size_t need = me.bytes + batch.size() * extra;
+        if (need > e->bytes) {
+            if (used - e->bytes + need > budget) break;
+            used += need - e->bytes;
+            e->bytes = need;
+        }
+        batch.push_back(o);
+        it = queue.erase(it);
+    }
+}
 
 // Under the lock: the first queued request that can run now, and its engine
 // (an idle one holding the request's factor, an idle one of its mesh, or a
@@ -144,9 +179,12 @@ void PeecServer::worker() {
             return j != nullptr;
         });
         if (!j) return;
+        std::vector<Job *> batch;
+        gather(j, e, batch);
         lk.unlock();
         for (PeecEngine *x : evicted) peec_engine_destroy(x);
-        j->t_start = Clock::now();
+        const Clock::time_point t_start = Clock::now();
+        for (Job *b : batch) b->t_start = t_start;
         int status = 0;
         if (!e->ready) {
             const MeshEntry &m = *meshes[j->mesh];
@@ -154,12 +192,29 @@ void PeecServer::worker() {
             status = peec_engine_create_with_matrices(m.mesh, m.dual, &opt.engine, m.L.data(), m.P.data(), &eng);
             e->eng = eng;
         }
-        PeecPointReport rep{};
-        if (status == 0)
-            status = peec_engine_solve(e->eng, j->f, j->mode, j->ports.data(), j->ports.size() / 2, j->z.data(),
-                                       nullptr, nullptr, &rep);
+        const size_t nb = batch.size();
+        std::vector<PeecPointReport> rep(nb);
+        std::vector<int> st(nb, status);
+        if (status == 0 && nb == 1) {
+            st[0] = peec_engine_solve(e->eng, j->f, j->mode, j->ports.data(), j->ports.size() / 2, j->z.data(),
+                                      nullptr, nullptr, &rep[0]);
+        } else if (status == 0) {
+            std::vector<double> f(nb);
+            std::vector<const int *> ports(nb);
+            std::vector<size_t> nports(nb);
+            std::vector<Complex *> z(nb);
+            for (size_t i = 0; i < nb; ++i) {
+                f[i] = batch[i]->f;
+                ports[i] = batch[i]->ports.data();
+                nports[i] = batch[i]->ports.size() / 2;
+                z[i] = batch[i]->z.data();
+            }
+            peec_engine_solve_batch(e->eng, nb, f.data(), ports.data(), nports.data(), z.data(), rep.data(),
+                                    st.data());
+        }
         const Clock::time_point t_end = Clock::now();
         lk.lock();
+        int engine_id = 0;
         if (!e->ready && !e->eng) {
             // creation failed: release the reservation
             used -= e->bytes;
@@ -168,16 +223,21 @@ void PeecServer::worker() {
         } else {
             e->ready = true;
             e->busy = false;
-            e->has_factor = status == 0 && !rep.used_fp64_fallback;
+            // a batch leaves no single point's factor behind
+            e->has_factor = nb == 1 && st[0] == 0 && !rep[0].used_fp64_fallback;
             e->f = j->f;
             e->mode = j->mode;
-            j->result.engine = e->id;
+            engine_id = e->id;
         }
-        j->result.status = status;
-        j->result.report = rep;
-        j->result.queue_s = std::chrono::duration<double>(j->t_start - j->t_submit).count();
-        j->result.latency_s = std::chrono::duration<double>(t_end - j->t_submit).count();
-        j->done = true;
+        for (size_t i = 0; i < nb; ++i) {
+            Job *b = batch[i];
+            b->result.engine = engine_id;
+            b->result.status = st[i];
+            b->result.report = rep[i];
+            b->result.queue_s = std::chrono::duration<double>(b->t_start - b->t_submit).count();
+            b->result.latency_s = std::chrono::duration<double>(t_end - b->t_submit).count();
+            b->done = true;
+        }
         cv_done.notify_all();
         cv_work.notify_all();  // an engine is free again
     }
@@ -191,10 +251,11 @@ void peec_server_options_default(PeecServerOptions *o) {
     peec_engine_options_default(&o->engine);
     o->slots = 4;
     o->hbm_budget_bytes = 0;
+    o->max_batch = 8;
 }
 
 int peec_server_create(const PeecServerOptions *o, PeecServer **out) {
-    if (!o || !out || o->slots < 1 || o->slots > 64) return -1;
+    if (!o || !out || o->slots < 1 || o->slots > 64 || o->max_batch < 1 || o->max_batch > 64) return -1;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         PEEC_ERR("no usable GPU for the request server.");

@@ -43,7 +43,8 @@ class PeecPointReport(C.Structure):
 
 
 class PeecServerOptions(C.Structure):
-    _fields_ = [("engine", PeecEngineOptions), ("slots", C.c_int), ("hbm_budget_bytes", C.c_size_t)]
+    _fields_ = [("engine", PeecEngineOptions), ("slots", C.c_int), ("hbm_budget_bytes", C.c_size_t),
+                ("max_batch", C.c_int)]
 
 
 class PeecRequestResult(C.Structure):
@@ -781,17 +782,21 @@ def import_dense_mtx(path):
 class Server:
     """Request loop over device-resident engines (include/peec_serve.h):
     admission by HBM budget, factor reuse keyed by (mesh, frequency, mode),
-    up to `slots` requests running at once on their own streams."""
+    up to `slots` requests running at once on their own streams, queued
+    mixed requests of one mesh batched into shared factor launches (up to
+    `max_batch`)."""
 
     FP64 = 0
     MIXED = 1
 
-    def __init__(self, slots=4, hbm_budget_bytes=0, retarded=False, **engine_opts):
+    def __init__(self, slots=4, hbm_budget_bytes=0, retarded=False, max_batch=None, **engine_opts):
         lib = load()
         o = PeecServerOptions()
         lib.peec_server_options_default(C.byref(o))
         o.slots = slots
         o.hbm_budget_bytes = int(hbm_budget_bytes)
+        if max_batch is not None:
+            o.max_batch = int(max_batch)
         o.engine.retarded = int(bool(retarded))
         for k, v in engine_opts.items():
             setattr(o.engine, k, v)

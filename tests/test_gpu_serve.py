@@ -97,3 +97,38 @@ def test_server_rejects_bad_requests(require_gpu):
     with pytest.raises(RuntimeError):
         sv.submit(mid, -1.0, [(0, 1)])
     sv.close()
+
+
+def test_server_batches_queued_mixed_requests(require_gpu):
+    """Continuous batching: mixed requests of one mesh that wait in the queue
+    while the worker is busy run as one batched factor; each answer equals a
+    standalone engine solving the same requests one by one in the same row
+    order (set by an FP64 request first) and block width."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 10, 6, 6)
+    nv = len(xyz)
+    reqs = [(2e6, [(0, nv - 1)]), (7e6, [(1, nv - 2), (4, 50)]), (2e7, [(0, nv - 1)]),
+            (6e7, [(7, nv - 9), (0, 30), (2, 60)]), (1.2e8, [(0, nv - 1)]), (1.9e8, [(20, 300)])]
+    opts = dict(retarded=True, block_size=128)
+    with plib.Model(xyz, quads) as m:
+        sv = plib.Server(slots=1, max_batch=8, **opts)
+        mid = sv.add_mesh(m)
+        t0 = sv.submit(mid, 5e5, [(0, nv - 1)], plib.Server.FP64)
+        tickets = [sv.submit(mid, f, p, plib.Server.MIXED) for f, p in reqs]
+        st0, z0, _ = sv.wait(t0)
+        out = [sv.wait(t) for t in tickets]
+        sv.close()
+        eng = plib.Engine(m, smooth_moments=0, **opts)
+        s64, z64, _, _, _ = eng.solve(5e5, [(0, nv - 1)], plib.Engine.FP64, want_solution=False)
+        single = [eng.solve(f, p, plib.Engine.MIXED, want_solution=False) for f, p in reqs]
+        eng.close()
+    assert st0 == 0 and s64 == 0
+    np.testing.assert_allclose(z0, z64, rtol=1e-13, atol=0)
+    sizes = [r.report.batch_size for _, _, r in out]
+    print(f"batch sizes {sizes}", flush=True)
+    assert max(sizes) >= 2
+    for (st, z, r), (s1, z1, _, _, r1), (f, _) in zip(out, single, reqs):
+        assert st == 0 and s1 == 0, f
+        assert r.report.backward_error <= 1e-10
+        d = np.max(np.abs(z - z1) / np.abs(z1))
+        print(f"f={f:g} batch {r.report.batch_size} vs single {d:.2e}", flush=True)
+        assert d <= 1e-13
