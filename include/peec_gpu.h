@@ -89,6 +89,11 @@ typedef struct
                                    two FP16 parts (hi + lo, power-of-two scaled per row
                                    / column; three products, FP32 accumulation, ~2^-22
                                    per product). 0 = FP32 MFMA. Default 1 */
+    int panel_lds_kib;          /* cap (KiB) on the LDS a workgroup of the partial-pivoting
+                                   panel may hold: a block column whose panel slab does
+                                   not fit is factored as narrower sub-panels (each with
+                                   its U12 and update inside the block column), as tall
+                                   matrices are; 0 (default) = the device's limit */
 } PeecEngineOptions;
 
 typedef struct

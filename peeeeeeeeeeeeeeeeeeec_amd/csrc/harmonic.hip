@@ -695,6 +695,7 @@ static int engine_create(const Mesh *mesh, const DualMesh *dual, const PeecEngin
         e->lu = std::make_unique<LUContext>(s);
         e->lu->timer = &e->timer;
         e->lu->split_fp16_update = o->split_fp16_update != 0;
+        if (o->panel_lds_kib > 0) e->lu->panel_lds_limit = (size_t)o->panel_lds_kib * 1024;
         if (mesh->edge_centers && mesh->edge_vectors) e->rcs = std::make_unique<RcsGeometry>(mesh, dual, s);
         std::vector<int> edges(mesh->edges, mesh->edges + 2 * e->ne);
         e->d_edges.upload(edges, s);

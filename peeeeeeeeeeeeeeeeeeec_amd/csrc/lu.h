@@ -82,6 +82,7 @@ struct LUContext {
     const void *dinv_factor = nullptr;            // factor whose inverses diag_inv holds
     int dinv_n = 0;
     int lds_per_block = 64 * 1024;
+    size_t panel_lds_limit = 0;                   // cap on the pivoted panel's LDS per workgroup (0: none)
     hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
     hipEvent_t ev_next = nullptr, ev_diag = nullptr, ev_prev = nullptr, ev_u12 = nullptr;
     explicit LUContext(hipStream_t s);

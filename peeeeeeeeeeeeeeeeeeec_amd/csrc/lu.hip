@@ -1417,34 +1417,31 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
     hipLaunchKernelGGL(fill_uint_kernel, dim3((unsigned)((kBarWords + 255) / 256)), dim3(256), 0, s, ctx.barrier.ptr,
                        (int)kBarWords, 0u);
     HIP_CHECK(hipGetLastError());
-    // dynamic LDS of the panel kernel: the device's per-workgroup limit less
-    // the kernel's static arrays
-    const size_t lds_budget = (size_t)ctx.lds_per_block - 8 * 1024;
-    int k0 = 0;
-    while (k0 < n) {
-        const int m = n - k0;
-        // workgroups: one per CU at most, at least 8 rows each
-        int G = std::max(1, std::min(ctx.cus, (m + 7) / 8));
-        int nb = block_size > 0 ? block_size : 128;
-        nb = std::min(nb, kMaxNB);
-        int rpw = (m + G - 1) / G;
-        while (nb > 16 && (size_t)rpw * nb * sizeof(T) > lds_budget) nb /= 2;
-        while ((size_t)rpw * nb * sizeof(T) > lds_budget) {
-            // very tall panels: more rows than the LDS budget allows per CU
-            out.code = -1;
-            PEEC_ERR("LU panel of %d rows exceeds the LDS budget.", m);
-            return out;
-        }
-        const int w = std::min(nb, m);
-        rpw = (m + G - 1) / G;
+    // dynamic LDS of the panel kernel: the device's per-workgroup limit (or
+    // the context's cap) less the kernel's static arrays
+    size_t lds_cap = (size_t)ctx.lds_per_block;
+    if (ctx.panel_lds_limit > 0 && ctx.panel_lds_limit < lds_cap) lds_cap = ctx.panel_lds_limit;
+    const size_t lds_budget = lds_cap > 16 * 1024 ? lds_cap - 8 * 1024 : lds_cap / 2;
+    // the cooperative panel over rows j0.. and columns j0 .. j0 + wj - 1
+    // (its slab in the LDS of one workgroup per CU), then its row swaps on
+    // every other column
+    auto panel = [&](int j0, int wj) -> bool {
+        const int mj = n - j0;
+        const int G = std::max(1, std::min(ctx.cus, (mj + 7) / 8));
+        const int rpw = (mj + G - 1) / G;
         double *slots = ctx.slots.ptr;
         double *mags = ctx.cand.ptr;
         unsigned long long *tags = ctx.tags.ptr;
-        const size_t shm = (size_t)rpw * w * sizeof(T);
-        int nn = n, kk0 = k0, ww = w, rr = rpw;
+        const size_t shm = (size_t)rpw * wj * sizeof(T);
+        if (shm > lds_budget) {
+            out.code = -1;
+            PEEC_ERR("LU panel of %d rows exceeds the LDS budget.", mj);
+            return false;
+        }
+        int nn = n, kk0 = j0, ww = wj, rr = rpw;
         size_t ld = lda;
         unsigned round0 = ctx.round;
-        ctx.round += (unsigned)w + 1u;
+        ctx.round += (unsigned)wj + 1u;
         double lim = pivot_limit;
         void *args[] = {&dA, &ld, &nn, &kk0, &ww, &rr, &slots, &mags, &tags, &d_ipiv, &round0, &lim, &ctx.status.ptr};
         if (shm > 64 * 1024)
@@ -1453,31 +1450,64 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         const int tp = ctx.timer ? ctx.timer->start(s) : -1;
         HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<void *>(panel_kernel<S>), dim3(G), dim3(256),
                                              args, (unsigned)shm, s));
-        if (ctx.timer) ctx.timer->stop(tp, s, kKernelPanel, 0.0, 2.0 * m * w * sizeof(S));
-        if (n - w > 0) {
-            ctx.swap_stage.reserve((size_t)2 * w * n * sizeof(S));
+        if (ctx.timer) ctx.timer->stop(tp, s, kKernelPanel, 0.0, 2.0 * mj * wj * sizeof(S));
+        if (n - wj > 0) {
+            ctx.swap_stage.reserve((size_t)2 * wj * n * sizeof(S));
             S *stage = reinterpret_cast<S *>(ctx.swap_stage.ptr);
-            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(256), 0, s, d_ipiv, k0, w, ctx.swap_plan.ptr,
+            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(256), 0, s, d_ipiv, j0, wj, ctx.swap_plan.ptr,
                                ctx.status.ptr);
-            const dim3 sg((unsigned)((n - w + 255) / 256), (unsigned)(2 * w));
-            hipLaunchKernelGGL(swap_gather_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, k0, w, ctx.swap_plan.ptr,
+            const dim3 sg((unsigned)((n - wj + 255) / 256), (unsigned)(2 * wj));
+            hipLaunchKernelGGL(swap_gather_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, j0, wj, ctx.swap_plan.ptr,
                                stage, ctx.status.ptr);
-            hipLaunchKernelGGL(swap_scatter_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, k0, w, ctx.swap_plan.ptr,
+            hipLaunchKernelGGL(swap_scatter_kernel<S>, sg, dim3(256), 0, s, dA, lda, n, j0, wj, ctx.swap_plan.ptr,
                                stage, ctx.status.ptr);
+        }
+        return true;
+    };
+    // U12 = L11^-1 A12 of the w columns from k0, over columns c0 .. c1 - 1
+    auto trsm = [&](int k0, int w, int c0, int c1) -> bool {
+        const size_t tshm = trsm_lds_elems(w) * sizeof(S);
+        if (tshm > (size_t)ctx.lds_per_block) {
+            out.code = -1;
+            PEEC_ERR("LU block of %d columns needs %zu bytes of LDS per workgroup.", w, tshm);
+            return false;
+        }
+        if (tshm > 64 * 1024)
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(trsm_kernel<S>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)tshm));
+        const int cols = c1 - c0;
+        hipLaunchKernelGGL(trsm_kernel<S>, dim3((cols + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC)), dim3(256), tshm, s,
+                           dA, lda, c0, c1, k0, w, ctx.status.ptr);
+        return true;
+    };
+    int k0 = 0;
+    while (k0 < n) {
+        const int m = n - k0;
+        // workgroups: one per CU at most, at least 8 rows each
+        const int G = std::max(1, std::min(ctx.cus, (m + 7) / 8));
+        int nb = block_size > 0 ? block_size : 128;
+        nb = std::min(nb, kMaxNB);
+        const int w = std::min(nb, m);
+        // Block columns of w = 128: when the slab of a w-wide panel does not
+        // fit the LDS of the CUs (tall matrices), the block column is
+        // factored as narrower sub-panels that fit, each followed by its
+        // U12 and rank-ws update within the block column only; the trailing
+        // matrix then still takes one rank-w update per block column.
+        const int rpw = (m + G - 1) / G;
+        int ws = w;
+        while (ws > 16 && (size_t)rpw * ws * sizeof(T) > lds_budget) ws /= 2;
+        for (int j0 = k0; j0 < k0 + w; j0 += ws) {
+            const int wj = std::min(ws, k0 + w - j0);
+            if (!panel(j0, wj)) return out;
+            const int right = k0 + w - (j0 + wj);
+            if (right > 0) {
+                if (!trsm(j0, wj, j0 + wj, k0 + w)) return out;
+                launch_gemm<S>(dA, lda, j0 + wj, j0 + wj, n - j0 - wj, right, j0, wj, ctx.status.ptr, ctx, s);
+            }
         }
         const int rest = n - k0 - w;
         if (rest > 0) {
-            const size_t tshm = trsm_lds_elems(w) * sizeof(S);
-            if (tshm > (size_t)ctx.lds_per_block) {
-                out.code = -1;
-                PEEC_ERR("LU block of %d columns needs %zu bytes of LDS per workgroup.", w, tshm);
-                return out;
-            }
-            if (tshm > 64 * 1024)
-                HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(trsm_kernel<S>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)tshm));
-            hipLaunchKernelGGL(trsm_kernel<S>, dim3((rest + 256 / kTrsmTPC - 1) / (256 / kTrsmTPC)), dim3(256),
-                               tshm, s, dA, lda, k0 + w, n, k0, w, ctx.status.ptr);
+            if (!trsm(k0, w, k0 + w, n)) return out;
             launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s);
         }
         HIP_CHECK(hipGetLastError());

@@ -29,7 +29,7 @@ class PeecEngineOptions(C.Structure):
                 ("block_size", C.c_int),
                 ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
                 ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int), ("smooth_moments", C.c_int),
-                ("shifted_moments", C.c_int), ("split_fp16_update", C.c_int)]
+                ("shifted_moments", C.c_int), ("split_fp16_update", C.c_int), ("panel_lds_kib", C.c_int)]
 
 
 class PeecPointReport(C.Structure):

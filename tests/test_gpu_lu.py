@@ -557,3 +557,35 @@ def test_engine_rcs_fallback_then_solve_at_same_point(require_gpu):
             assert abs(val - z64) <= 1e-12 * abs(z64)
         else:
             np.testing.assert_allclose(val, s64, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("kib", [20, 12])
+def test_engine_subpanels_of_tall_block_columns(require_gpu, kib):
+    """Block columns whose partial-pivoting panel slab does not fit the LDS
+    are factored as narrower sub-panels (tall matrices, e.g. N ~ 57000);
+    forced here at N = 938 by capping the panel's LDS (20 KiB: 64-column
+    sub-panels of the FP64 factor; 12 KiB: 32-column FP64 and 64-column FP32
+    sub-panels). The FP64 impedances match the unrestricted factor to 1e-12
+    and the mixed point (whose first factor pivots in FP32 sub-panels)
+    matches FP64 to 1e-10."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 10, 6, 6)
+    port = (0, len(xyz) - 1)
+    freqs = [1e6, 5e7]
+    with plib.Model(xyz, quads) as m:
+        ref = plib.Engine(m, retarded=True)
+        z_ref = [ref.solve(f, [port], plib.Engine.FP64, want_solution=False)[1][0] for f in freqs]
+        ref.close()
+        eng = plib.Engine(m, retarded=True, panel_lds_kib=kib)
+        out64 = [eng.solve(f, [port], plib.Engine.FP64, want_solution=False) for f in freqs]
+        eng.close()
+        eng = plib.Engine(m, retarded=True, panel_lds_kib=kib)
+        outm = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+        eng.close()
+    for f, zr, (s64, z64, _, _, r64), (sm, zm, _, _, rm) in zip(freqs, z_ref, out64, outm):
+        print(f"f={f:g} cap {kib} KiB: fp64 {abs(z64[0] - zr) / abs(zr):.2e} vs unrestricted, mixed "
+              f"{abs(zm[0] - z64[0]) / abs(z64[0]):.2e} vs fp64, eta {r64.backward_error:.1e} / "
+              f"{rm.backward_error:.1e}, reuse {rm.pivot_reuse}", flush=True)
+        assert s64 == 0 and sm == 0
+        assert abs(z64[0] - zr) <= 1e-12 * abs(zr)
+        assert abs(zm[0] - z64[0]) <= 1e-10 * abs(z64[0])
+        assert r64.backward_error <= 1e-10 and rm.backward_error <= 1e-10
