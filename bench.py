@@ -363,8 +363,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=200, help="sweep: points timed (200 = the whole sweep)")
-    ap.add_argument("--warmup", type=int, default=3,
-                    help="sweep: untimed points (the first factor with pivoting and the two block-width trials)")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="sweep: untimed points (the first factor with pivoting and the autotuner's four "
+                         "factor-shape trials)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -47,8 +47,9 @@ typedef struct
     int retarded;               /* 0 quasi-static (reference model), 1 exp(-jkR)/R */
     int smooth_order;           /* GL order of the retarded remainder rule, default 2 */
     int near_smooth_order;      /* same, for entries of nearby regions, default 4 */
-    int block_size;             /* LU panel width, 0 = automatic (autotuned for the
-                                   reused-row-order factor, see PeecPointReport) */
+    int block_size;             /* LU panel width, 0 = automatic (the reused-row-order
+                                   factor's width and update tile shape autotuned, see
+                                   PeecPointReport); a given width uses 64-row tiles */
     int max_refinement_steps;   /* mixed mode cap, default 30 */
     double refinement_target;   /* mixed mode stops at this backward error, default 1e-12
                                    (a hundredth of the gate; the FP64 LU itself reaches
@@ -116,12 +117,17 @@ typedef struct
     int moment_terms;           /* retarded Lp(w) assembled from that many stored moment
                                    terms (0: by quadrature, or not retarded) */
     int block_size;             /* block width of a reused-row-order factor (0 otherwise);
-                                   with block_size = 0 in the options the engine's first
-                                   two such factors try 128 and 64 and keep the faster */
+                                   with block_size = 0 in the options the engine's
+                                   autotuner tries the shapes (block width 128 / 64 x
+                                   update tiles of 64 / 32 rows) on its first such
+                                   factors, single ones and batches apart, and keeps the
+                                   fastest per factor */
     double moment_center;       /* wave number k0 of the shifted-center moments that
                                    gave this point's retarded matrices (0 otherwise) */
     int batch_size;             /* points whose factors shared this point's launches
                                    (peec_engine_solve_batch; 1 or 0 otherwise) */
+    int update_tile_rows;       /* rows of the split-FP16 update tiles of that factor
+                                   (64 or 32; 0 when no reused-row-order factor ran) */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;

@@ -201,6 +201,32 @@ using namespace peec;
 // ---------------------------------------------------------------------------
 // engine
 
+// Autotuner of the ordered factor's shape: block width (128 or 64) and the
+// rows of the split-FP16 update's tiles (64 or 32; with the FP32 update only
+// the width varies). Each candidate is tried on one real factor, timed per
+// member of its launches, and the fastest is kept; single factors and
+// batches are tuned apart (they fill the device differently).
+struct FactorShape {
+    int nb, rows;
+};
+static constexpr FactorShape kShapes[4] = {{128, 64}, {128, 32}, {64, 64}, {64, 32}};
+struct ShapeTuner {
+    int tried = 0, best = -1;
+    double seconds[4] = {0.0, 0.0, 0.0, 0.0};
+    int candidates(bool split16) const { return split16 ? 4 : 2; }
+    // the candidate to try next (split16 off: 128 and 64 with 64-row tiles)
+    FactorShape trial(bool split16) const { return split16 ? kShapes[tried] : kShapes[2 * tried]; }
+    void record(double s, bool split16) {
+        seconds[tried++] = s;
+        if (tried == candidates(split16)) {
+            best = 0;
+            for (int i = 1; i < tried; ++i)
+                if (seconds[i] < seconds[best]) best = i;
+        }
+    }
+    FactorShape chosen(bool split16) const { return split16 ? kShapes[best] : kShapes[2 * best]; }
+};
+
 struct PeecEngine {
     PeecEngineOptions opt{};
     int ne = 0, nv = 0, n = 0;
@@ -231,11 +257,9 @@ struct PeecEngine {
     bool cache_ok = false;
     double cached_f = 0.0;
     int cached_mode = -1;
-    // block-width autotuner of the ordered factor (block_size = 0): the
-    // first two ordered factors run at 128 and 64, the faster width is kept
-    int tuned_nb = 0;
-    double trial_s[2] = {0.0, 0.0};
-    int trials = 0;
+    // shape autotuners of the ordered factor (block_size = 0): single
+    // factors and batched ones
+    ShapeTuner tune_single, tune_batch;
     DeviceBuffer<float2> d_Z32;  // FP32 factors: one, or a batch side by side
     float2 *z32 = nullptr;       // the current point's FP32 factor (in d_Z32)
     // mixed points factored together (peec_engine_solve_batch): each one's
@@ -422,19 +446,20 @@ static int engine_factor(PeecEngine *e, double f, int mode, PeecPointReport &rep
                                             &max_scaled, *e->lu) != 0)
                 return -2;
             e->limit = 1.0e-14 * max_scaled;
-            static constexpr int kTrialWidths[2] = {128, 64};
-            int nb = e->opt.block_size;
-            const bool trial = nb == 0 && e->tuned_nb == 0;
-            if (nb == 0) nb = trial ? kTrialWidths[e->trials] : e->tuned_nb;
+            const bool split16 = e->lu->split_fp16_update;
+            ShapeTuner &tu = e->tune_single;
+            const bool trial = e->opt.block_size == 0 && tu.best < 0;
+            FactorShape shape = e->opt.block_size > 0 ? FactorShape{e->opt.block_size, 64}
+                                : trial                ? tu.trial(split16)
+                                                       : tu.chosen(split16);
+            e->lu->h3_tile_rows = shape.rows;
             if (trial) e->stream.sync();
             const auto t0 = std::chrono::steady_clock::now();
-            st = lu_factor_ordered(e->z32, n, n, e->limit, *e->lu, nb);
-            report.block_size = nb;
-            if (trial && st.code == 0) {
-                // lu_factor_ordered returns after its stream has drained
-                e->trial_s[e->trials++] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                if (e->trials == 2) e->tuned_nb = e->trial_s[1] < e->trial_s[0] ? kTrialWidths[1] : kTrialWidths[0];
-            }
+            st = lu_factor_ordered(e->z32, n, n, e->limit, *e->lu, shape.nb);
+            report.block_size = shape.nb;
+            report.update_tile_rows = shape.rows;
+            if (trial && st.code == 0)  // lu_factor_ordered returns after its stream has drained
+                tu.record(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), split16);
             if (st.code == 0) {
                 e->ordered = true;
                 HIP_CHECK(hipMemcpyAsync(e->d_perm.ptr, e->d_reuse_perm.ptr, sizeof(int) * n, hipMemcpyDeviceToDevice,
@@ -1007,7 +1032,13 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
         hipStream_t s = e->stream;
         const int n = e->n, ne = e->ne, nv = e->nv;
         const size_t nn = (size_t)n * n;
-        const int nb = e->opt.block_size > 0 ? e->opt.block_size : e->tuned_nb > 0 ? e->tuned_nb : 128;
+        const bool split16 = e->lu->split_fp16_update;
+        ShapeTuner &tu = e->tune_batch;
+        const bool trial = e->opt.block_size == 0 && tu.best < 0;
+        const FactorShape shape = e->opt.block_size > 0 ? FactorShape{e->opt.block_size, 64}
+                                  : trial                ? tu.trial(split16)
+                                                         : tu.chosen(split16);
+        const int nb = shape.nb;
         e->cache_ok = false;
         e->slots.resize(m);
         e->d_Z32.reserve(m * nn);
@@ -1046,11 +1077,14 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
         e->d_limits.upload(limits.data(), m, s);
         std::vector<LUStatus> fst(m);
         HIP_CHECK(hipEventRecord(e->ev[0], s));
-        lu_factor_ordered(e->d_Z32.ptr, n, n, 0.0, *e->lu, nb, (int)m, nn, e->d_limits.ptr, fst.data());
+        e->lu->h3_tile_rows = shape.rows;
+        const LUStatus bst =
+            lu_factor_ordered(e->d_Z32.ptr, n, n, 0.0, *e->lu, nb, (int)m, nn, e->d_limits.ptr, fst.data());
         HIP_CHECK(hipEventRecord(e->ev[1], s));
         e->stream.sync();
         float ms_factor = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_factor, e->ev[0], e->ev[1]));
+        if (trial && bst.code == 0) tu.record(1e-3 * ms_factor / (double)m, split16);
         // each point's solves and refinement on its own factor (with the
         // single point's fallbacks: a rejected order is refactored with
         // pivoting, a point above the gate in FP64)
@@ -1061,6 +1095,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             r.ms_fill = ms_prep[b];
             r.ms_factor = ms_factor;
             r.block_size = nb;
+            r.update_tile_rows = shape.rows;
             if (prep[b] != 0) {
                 status[p] = prep[b];
                 continue;

@@ -1757,20 +1757,28 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
     const float *ls = ctx.scale_l16.ptr + (size_t)rb_first * 16;
     const bool next = kind != kKernelLuUpdateSplit16;
-    // 64 x 128 tiles (128 x 128 tiles, with a third less operand traffic per
-    // output, measured no faster: the update is bound by latency, not by
-    // operand bandwidth)
-    const int BM = 64, BN = 128;
+    // 64 x 128 tiles by default (128 x 128 tiles, with a third less operand
+    // traffic per output, measured no faster: the update is bound by
+    // latency, not by operand bandwidth); 32 x 128 tiles (twice the
+    // workgroups for a small update) when the context's tuner picks them
+    const bool short_tiles = ctx.h3_tile_rows == 32;
+    const int BM = short_tiles ? 32 : 64, BN = 128;
     const int tiles = ((m + BM - 1) / BM) * ((ncols + BN - 1) / BN);
     const int grid = (tiles + 7) / 8 * 8;
 #define PEEC_H3(KC, MB, NB, ND)                                                                                     \
     hipLaunchKernelGGL((next ? gemm_next_h3_kernel<KC, MB, NB, ND> : gemm_tile_h3_kernel<KC, MB, NB, ND>),          \
                        dim3(grid, 1, batch), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad,  \
                        cb_count, status, bs)
-    if (kw == 256)
+    if (kw == 256 && short_tiles)
+        PEEC_H3(16, 1, 4, 4);
+    else if (kw == 256)
         PEEC_H3(16, 2, 4, 4);
+    else if (kw == 128 && short_tiles)
+        PEEC_H3(8, 1, 4, 4);
     else if (kw == 128)
         PEEC_H3(8, 2, 4, 4);
+    else if (kw == 64 && short_tiles)
+        PEEC_H3(4, 1, 4, 4);
     else if (kw == 64)
         PEEC_H3(4, 2, 4, 4);
     else

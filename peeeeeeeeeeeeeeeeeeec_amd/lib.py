@@ -39,7 +39,7 @@ class PeecPointReport(C.Structure):
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int),
                 ("moment_terms", C.c_int), ("block_size", C.c_int), ("moment_center", C.c_double),
-                ("batch_size", C.c_int)]
+                ("batch_size", C.c_int), ("update_tile_rows", C.c_int)]
 
 
 class PeecServerOptions(C.Structure):

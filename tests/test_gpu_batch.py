@@ -52,3 +52,33 @@ def test_batch_equals_single_points(require_gpu, block):
         assert r.refinement_steps == r1.refinement_steps
         assert d <= 1e-14
         assert e64 <= 1e-10
+
+
+def test_batch_shape_autotuner(require_gpu):
+    """With block_size = 0 the batched factors are autotuned apart from the
+    single ones: successive batches try block widths 128 / 64 with update
+    tiles of 64 / 32 rows, then keep the fastest per member; every answer
+    stays within 1e-10 of FP64."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 10, 6, 6)
+    port = (0, len(xyz) - 1)
+    batches = [[2e6 * (1.3 ** (3 * b + i)) for i in range(3)] for b in range(5)]
+    with plib.Model(xyz, quads) as m:
+        eng = plib.Engine(m, retarded=True)
+        st0, _, _, _, r0 = eng.solve(1e6, [port], plib.Engine.MIXED, want_solution=False)
+        assert st0 == 0 and r0.pivot_reuse == 0
+        shapes, answers = [], []
+        for fs in batches:
+            code, st, zs, reps = eng.solve_batch(fs, [port])
+            assert code == 0 and not st.any()
+            assert all(r.batch_size == 3 for r in reps)
+            shapes.append({(r.block_size, r.update_tile_rows) for r in reps})
+            answers += [(f, z[0], r) for f, z, r in zip(fs, zs, reps)]
+        for f, z, r in answers:
+            _, z64, _, _, _ = eng.solve(f, [port], plib.Engine.FP64, want_solution=False)
+            assert abs(z - z64[0]) <= 1e-10 * abs(z64[0]) and r.backward_error <= 1e-10
+        eng.close()
+    print(f"batch shapes {shapes}", flush=True)
+    assert all(len(s) == 1 for s in shapes)
+    tried = [s.pop() for s in shapes]
+    assert tried[:4] == [(128, 64), (128, 32), (64, 64), (64, 32)]
+    assert tried[4] in tried[:4]

@@ -315,7 +315,7 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
         z64 = [eng.solve(f, [port], plib.Engine.FP64)[1][0] for f in freqs]
         eng.close()
         eng = plib.Engine(m, retarded=True)
-        reports, zm, widths = [], [], []
+        reports, zm, widths, rows = [], [], [], []
         for f in freqs:
             st, z, _, _, rep = eng.solve(f, [port], plib.Engine.MIXED)
             assert st == 0
@@ -323,6 +323,7 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
             reports.append((rep.pivot_reuse, rep.pivot_refactor, rep.refinement_steps, rep.backward_error,
                             rep.used_fp64_fallback, rep.reuse_backward_error))
             widths.append(rep.block_size)
+            rows.append(rep.update_tile_rows)
         eng.close()
     for f, a, b, r in zip(freqs, zm, z64, reports):
         print(f"f={f:g} mixed {a:.12e} fp64 {b:.12e} rel={abs(a - b) / abs(b):.2e} reuse/refactor/ir/eta/fb/eta_reused={r}",
@@ -333,10 +334,13 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
     assert all(r[0] == 1 for r in reports[1:])
     # the reused order is kept at most points of the sweep
     assert sum(1 for r in reports[1:] if not r[1]) >= len(freqs) // 2
-    # block-width autotuner: the first two ordered factors try 128 and 64,
-    # the later ones keep one of them
-    assert widths[0] == 0 and widths[1:3] == [128, 64]
-    assert len(set(widths[3:])) == 1 and widths[3] in (64, 128)
+    # shape autotuner: the first four ordered factors try block widths 128
+    # and 64 with split-FP16 update tiles of 64 and 32 rows, the later ones
+    # keep one of the four
+    print(f"block widths {widths}, update tile rows {rows}", flush=True)
+    assert widths[0] == 0 and rows[0] == 0
+    assert widths[1:5] == [128, 128, 64, 64] and rows[1:5] == [64, 32, 64, 32]
+    assert len(set(zip(widths[5:], rows[5:]))) == 1 and (widths[5], rows[5]) in set(zip(widths[1:5], rows[1:5]))
 
 
 def test_engine_fixed_block_width(require_gpu):
@@ -354,7 +358,8 @@ def test_engine_fixed_block_width(require_gpu):
             assert all(r[0] == 0 for r in res)
             out[nb] = ([r[1][0] for r in res], [r[4].block_size for r in res])
     assert out[64][1] == [0, 64, 64, 64]
-    assert out[0][1][:3] == [0, 128, 64]
+    # the autotuner's first trials: widths 128, 128, 64 (update tiles 64, 32, 64 rows)
+    assert out[0][1] == [0, 128, 128, 64]
     np.testing.assert_allclose(out[64][0], out[0][0], rtol=1e-10, atol=0)
 
 
