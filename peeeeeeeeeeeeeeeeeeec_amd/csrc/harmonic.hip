@@ -244,6 +244,7 @@ struct PeecEngine {
     std::unique_ptr<RcsGeometry> rcs;           // edge geometry for excitation and far field
     double limit = 0.0;                         // pivot limit of the current point
     bool ordered = false;                       // current FP32 factor uses the reused row order
+    bool split16_pivoted = false;               // ... or is a partial-pivoting one with split-FP16 updates
     bool retarded_now = false;
     // the factor held for (cached_f, cached_mode) may serve further
     // right-hand sides at that point without a new assembly or factor
@@ -335,8 +336,12 @@ static int factor_error(const LUStatus &st) {
 
 // Partial-pivoting factor (FP64 in d_Z, or FP32 in d_Z32 from the
 // equilibrated d_Z); its row order becomes the order reused by the next
-// mixed-precision point.
-static int factor_pivoted(PeecEngine *e, bool fp64) {
+// mixed-precision point. The FP32 factor's trailing updates run on
+// split-FP16 MFMA (with split_fp16_update) unless `accurate` asks for the
+// FP32 MFMA update: the split keeps 22 bits of each operand relative to its
+// row's largest part, FP32 24 bits of every entry, which is what a refactor
+// after a stalled refinement needs at badly scaled (low-frequency) points.
+static int factor_pivoted(PeecEngine *e, bool fp64, bool accurate = false) {
     hipStream_t s = e->stream;
     const int n = e->n;
     LUStatus st;
@@ -358,7 +363,9 @@ static int factor_pivoted(PeecEngine *e, bool fp64) {
             0)
             return -2;
         e->limit = 1.0e-14 * max_scaled;
+        e->lu->pivoted_split16 = e->lu->split_fp16_update && !accurate;
         st = lu_factor<float2>(e->z32, n, n, e->d_ipiv.ptr, e->limit, *e->lu, e->opt.block_size);
+        e->split16_pivoted = e->lu->pivoted_split16;
     }
     if (st.code != 0) return factor_error(st);
     permutation_from_pivots(e->d_ipiv.ptr, n, e->d_perm.ptr, s);
@@ -602,7 +609,15 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             // refactor with partial pivoting and refine again
             report.pivot_refactor = 1;
             report.reuse_backward_error = worst;
-            const int code = factor_pivoted(e, false);
+            const int code = factor_pivoted(e, false, true);
+            if (code) return code;
+            worst = refine();
+        } else if (!e->ordered && e->split16_pivoted && !(worst <= e->opt.backward_error_gate)) {
+            // a partial-pivoting factor with split-FP16 updates that stalled
+            // above the gate: the same factor with the FP32 update first
+            report.pivot_refactor = 1;
+            report.reuse_backward_error = worst;
+            const int code = factor_pivoted(e, false, true);
             if (code) return code;
             worst = refine();
         }

@@ -77,6 +77,7 @@ struct LUContext {
     DeviceBuffer<float2> dblk_inv;
     bool split_fp16_update = true;                // ordered factor: FP16 hi/lo update (else FP32 MFMA)
     int h3_tile_rows = 64;                        // rows of the split-FP16 update's tiles (64 or 32)
+    bool pivoted_split16 = false;                 // partial-pivoting FP32 factor: split-FP16 trailing updates
     DeviceBuffer<int> swap_plan;                  // row moves of one panel
     DeviceBuffer<unsigned char> swap_stage;       // staged source rows of those moves
     DeviceBuffer<unsigned char> diag_inv;         // inverses of the 64x64 diagonal blocks (solves)

@@ -1408,6 +1408,10 @@ __global__ void mfma_layout_probe_kernel(double *out) {
 // ---------------------------------------------------------------------------
 // host driver
 
+// The trailing update of a partial-pivoting FP32 factor on split-FP16 MFMA
+// (defined with the split-FP16 kernels below).
+static void pivoted_update_h3(float2 *A, size_t lda, int k0, int w, int n, LUContext &ctx, hipStream_t s);
+
 template <class S>
 LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LUContext &ctx, int block_size) {
     using T = typename Elem<S>::A;
@@ -1508,7 +1512,12 @@ LUStatus lu_factor(S *dA, int n, size_t lda, int *d_ipiv, double pivot_limit, LU
         const int rest = n - k0 - w;
         if (rest > 0) {
             if (!trsm(k0, w, k0 + w, n)) return out;
-            launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s);
+            bool split = false;
+            if constexpr (std::is_same<S, float2>::value) split = ctx.pivoted_split16 && (w == 128 || w == 64);
+            if (split)
+                pivoted_update_h3(reinterpret_cast<float2 *>(dA), lda, k0, w, n, ctx, s);
+            else
+                launch_gemm<S>(dA, lda, k0 + w, k0 + w, rest, rest, k0, w, ctx.status.ptr, ctx, s);
         }
         HIP_CHECK(hipGetLastError());
         k0 += w;
@@ -1788,6 +1797,85 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     if (ctx.timer)
         ctx.timer->stop(tg, s, kind, 8.0 * batch * m * ncols * kw,
                         batch * (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
+}
+
+// Split-FP16 operand pack of `nlines` lines of the k range k0 .. k0 + 16 CH - 1
+// of A, in the layout pack_h3_regs writes: rows of L21 (ROWS, as (re, -im))
+// or columns of U12 from line0, one wavefront per 16-line block, each line
+// scaled by split_scale of its largest part over the k range; the blocks
+// past nlines up to `blocks` (the pack's padding) are written as zeros.
+template <bool ROWS, int CH>
+__global__ __launch_bounds__(256) void pack_split16_kernel(const float2 *__restrict__ A, size_t lda, int line0,
+                                                           int nlines, int k0, int blocks, uint4 *pk, float *pinv,
+                                                           const int *status) {
+    if (status[0] != 0) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int blk = (int)blockIdx.x * 4 + wave;
+    if (blk >= blocks) return;
+    const int line = 16 * blk + li;
+    const bool in = line < nlines;
+    float vr[CH][4], vi[CH][4];
+    float mx = 0.f;
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = 16 * c + 4 * lk + v;
+            float2 x = make_float2(0.f, 0.f);
+            if (in) x = ROWS ? A[(size_t)(line0 + line) * lda + k0 + k] : A[(size_t)(k0 + k) * lda + line0 + line];
+            vr[c][v] = x.x;
+            vi[c][v] = ROWS ? -x.y : x.y;
+            mx = fmaxf(mx, fmaxf(fabsf(x.x), fabsf(x.y)));
+        }
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            _Float16 xh, xl, yh, yl;
+            split_half(vr[c][v] * sc, xh, xl);
+            split_half(vi[c][v] * sc, yh, yl);
+            hw[v] = pack_half2(xh, yh);
+            lw[v] = pack_half2(xl, yl);
+        }
+        uint4 *o = pk + ((size_t)(c * blocks + blk) * 2) * 64 + lane;
+        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+    if (lk == 0) pinv[16 * blk + li] = 1.f / sc;
+}
+
+// A[k0 + w.., k0 + w..] -= L21 U12 of the partial-pivoting FP32 factor's
+// block column at k0 (w = 64 or 128), on the split-FP16 operands (the same
+// update kernel as the ordered factor's, k width w).
+static void pivoted_update_h3(float2 *A, size_t lda, int k0, int w, int n, LUContext &ctx, hipStream_t s) {
+    const int r0 = k0 + w, rest = n - r0;
+    if (rest <= 0) return;
+    int rb_pad, cb_pad;
+    gemm_pads<float2>(rest, rest, rb_pad, cb_pad);
+    const int cb = (rest + 127) / 128 * 8, kc = w / 16;
+    ctx.pack_l16.reserve((size_t)kc * rb_pad * 128 * sizeof(uint4));
+    ctx.scale_l16.reserve((size_t)rb_pad * 16);
+    ctx.pack_u16.reserve((size_t)kc * cb * 128 * sizeof(uint4));
+    ctx.scale_u16.reserve((size_t)cb * 16);
+    uint4 *pl = reinterpret_cast<uint4 *>(ctx.pack_l16.ptr), *pu = reinterpret_cast<uint4 *>(ctx.pack_u16.ptr);
+    if (w == 128) {
+        hipLaunchKernelGGL((pack_split16_kernel<true, 8>), dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, A, lda,
+                           r0, rest, k0, rb_pad, pl, ctx.scale_l16.ptr, ctx.status.ptr);
+        hipLaunchKernelGGL((pack_split16_kernel<false, 8>), dim3((unsigned)((cb + 3) / 4)), dim3(256), 0, s, A, lda,
+                           r0, rest, k0, cb, pu, ctx.scale_u16.ptr, ctx.status.ptr);
+    } else {
+        hipLaunchKernelGGL((pack_split16_kernel<true, 4>), dim3((unsigned)((rb_pad + 3) / 4)), dim3(256), 0, s, A, lda,
+                           r0, rest, k0, rb_pad, pl, ctx.scale_l16.ptr, ctx.status.ptr);
+        hipLaunchKernelGGL((pack_split16_kernel<false, 4>), dim3((unsigned)((cb + 3) / 4)), dim3(256), 0, s, A, lda,
+                           r0, rest, k0, cb, pu, ctx.scale_u16.ptr, ctx.status.ptr);
+    }
+    HIP_CHECK(hipGetLastError());
+    gemm_h3(A, lda, r0, r0, rest, rest, w, rb_pad, cb, pu, ctx.scale_u16.ptr, kKernelLuUpdateSplit16, ctx.status.ptr,
+            ctx, s);
 }
 
 // ---------------------------------------------------------------------------
