@@ -95,6 +95,12 @@ typedef struct
                                    not fit is factored as narrower sub-panels (each with
                                    its U12 and update inside the block column), as tall
                                    matrices are; 0 (default) = the device's limit */
+    int gmres_krylov;           /* mixed mode: when plain refinement stalls above the gate,
+                                   GMRES-IR on the same factor first (each correction
+                                   A d = r solved by GMRES on the system preconditioned
+                                   with the FP32 factor, this Krylov dimension, up to 10
+                                   outer steps), then the FP64 refactor. 0 = off,
+                                   default 20 */
 } PeecEngineOptions;
 
 typedef struct
@@ -128,6 +134,8 @@ typedef struct
                                    (peec_engine_solve_batch; 1 or 0 otherwise) */
     int update_tile_rows;       /* rows of the split-FP16 update tiles of that factor
                                    (64 or 32; 0 when no reused-row-order factor ran) */
+    int gmres_steps;            /* outer GMRES-IR steps (0: plain refinement sufficed) */
+    int gmres_iterations;       /* GMRES iterations over all of them */
 } PeecPointReport;
 
 typedef struct PeecEngine PeecEngine;

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -275,6 +276,11 @@ struct PeecEngine {
     std::vector<PointSlot> slots;
     DeviceBuffer<double> d_limits;
     DeviceBuffer<int> d_batch_perm;  // the row order a batch was factored in
+    // GMRES-IR workspace: Krylov basis, work vector, zero right-hand side,
+    // partial dot products, projections / solution coefficients, |x|, eta
+    DeviceBuffer<double2> d_gV, d_gw, d_gzero, d_gpart, d_gh;
+    DeviceBuffer<double> d_gax;
+    DeviceBuffer<unsigned long long> d_geta;
     hipEvent_t ev[6] = {};
     long long far_evals = 0, smooth_evals = 0;
     PeecStaticStats static_stats{};
@@ -317,6 +323,57 @@ __global__ void masked_copy_kernel(double2 *dst, const double2 *src, const int *
 }
 
 static int assemble_retarded(PeecEngine *e, double k, double *center = nullptr);
+
+// ---------------------------------------------------------------------------
+// GMRES-IR (refinement whose correction solves A d = r by GMRES on the
+// system preconditioned with the FP32 factor, Carson & Higham): complex
+// FP64 vector kernels over n entries
+
+constexpr int kGmresDotBlocks = 128;
+constexpr int kGmresOuter = 10;  // outer GMRES-IR steps at most
+// partial[i * gridDim.x + b] = sum over the entries of block b of conj(V_i) w
+__global__ __launch_bounds__(256) void gmres_dots_kernel(const double2 *V, size_t ldv, const double2 *w, int n,
+                                                         double2 *partial) {
+    __shared__ double sr[256], si[256];
+    const int i = blockIdx.y, tid = threadIdx.x;
+    const double2 *v = V + (size_t)i * ldv;
+    double ar = 0.0, ai = 0.0;
+    for (int t = blockIdx.x * 256 + tid; t < n; t += gridDim.x * 256) {
+        const double2 a = v[t], b = w[t];
+        ar += a.x * b.x + a.y * b.y;
+        ai += a.x * b.y - a.y * b.x;
+    }
+    sr[tid] = ar;
+    si[tid] = ai;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            sr[tid] += sr[tid + h];
+            si[tid] += si[tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)i * gridDim.x + blockIdx.x] = make_double2(sr[0], si[0]);
+}
+// w -= sum_{i < k} h_i V_i
+__global__ void gmres_axpy_kernel(const double2 *V, size_t ldv, const double2 *h, int k, double2 *w, int n) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    double2 acc = w[t];
+    for (int i = 0; i < k; ++i) {
+        const double2 a = h[i], v = V[(size_t)i * ldv + t];
+        acc.x -= a.x * v.x - a.y * v.y;
+        acc.y -= a.x * v.y + a.y * v.x;
+    }
+    w[t] = acc;
+}
+// y = a x
+__global__ void gmres_scale_kernel(const double2 *x, double2 a, double2 *y, int n) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const double2 v = x[t];
+    y[t] = make_double2(a.x * v.x - a.y * v.y, a.x * v.y + a.y * v.x);
+}
 
 // the dense copy of the current point's MNA matrix in d_Z0
 static void ensure_z0(PeecEngine *e) {
@@ -598,6 +655,146 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
         }
         return w;
     };
+    // GMRES-IR on the current FP32 factor: per right-hand side, outer steps
+    // x <- x - d with A d = r (r = A x - b in FP64) solved by restarted-free
+    // GMRES of Krylov dimension m on M^-1 A d = M^-1 r (M the FP32 factor,
+    // applied in FP64 arithmetic), classical Gram-Schmidt applied twice,
+    // Givens rotations on the host; each right-hand side keeps its best
+    // iterate and stops at the refinement target or after two outer steps
+    // without a new best. Returns the worst best backward error.
+    auto gmres_refine = [&]() {
+        const int m = e->opt.gmres_krylov;
+        const int nblk = std::min(kGmresDotBlocks, (n + 255) / 256);
+        const unsigned vg = (unsigned)((n + 255) / 256);
+        e->d_gV.resize((size_t)(m + 1) * n);
+        e->d_gw.resize(n);
+        e->d_gzero.resize(n);
+        e->d_gpart.resize((size_t)nblk * (m + 1));
+        e->d_gh.resize(m + 1);
+        e->d_gax.resize(n);
+        e->d_geta.resize(1);
+        fill_zero(e->d_gzero.ptr, n, s);
+        std::vector<double2> part((size_t)nblk * (m + 1));
+        // conj(V_i)^T w for i < k, summed on the host in a fixed order
+        auto dots = [&](const double2 *V, int k, const double2 *w, std::vector<std::complex<double>> &out) {
+            hipLaunchKernelGGL(gmres_dots_kernel, dim3((unsigned)nblk, (unsigned)k), dim3(256), 0, s, V, (size_t)n, w,
+                               n, e->d_gpart.ptr);
+            e->d_gpart.download(part.data(), (size_t)nblk * k, s);
+            e->stream.sync();
+            out.assign(k, std::complex<double>(0.0, 0.0));
+            for (int i = 0; i < k; ++i)
+                for (int b = 0; b < nblk; ++b) out[i] += std::complex<double>(part[(size_t)i * nblk + b].x,
+                                                                              part[(size_t)i * nblk + b].y);
+        };
+        // w <- M^-1 w
+        auto precond = [&](double2 *w) {
+            lu_solve_permuted<float2, double2>(e->z32, n, n, e->d_perm.ptr, e->d_row.ptr, e->d_col.ptr, w, 1, *e->lu,
+                                               true);
+        };
+        measure();
+        std::vector<double> best = eta;
+        std::vector<int> since(nrhs, 0);
+        std::vector<double2> hbuf(m + 1);
+        HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, e->d_X.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+        for (int outer = 0; outer < kGmresOuter; ++outer) {
+            bool any = false;
+            for (int q = 0; q < nrhs; ++q) {
+                if (best[q] <= e->opt.refinement_target || since[q] >= 2) continue;
+                any = true;
+                double2 *r = e->d_R.ptr + (size_t)q * n, *x = e->d_X.ptr + (size_t)q * n;
+                double2 *V = e->d_gV.ptr, *w = e->d_gw.ptr;
+                // v_0 = M^-1 r / beta
+                HIP_CHECK(hipMemcpyAsync(w, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+                precond(w);
+                std::vector<std::complex<double>> hv;
+                dots(w, 1, w, hv);
+                const double beta = std::sqrt(std::fabs(hv[0].real()));
+                if (!(beta > 0.0) || !std::isfinite(beta)) continue;
+                hipLaunchKernelGGL(gmres_scale_kernel, dim3(vg), dim3(256), 0, s, w, make_double2(1.0 / beta, 0.0), V,
+                                   n);
+                std::vector<std::complex<double>> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m);
+                g[0] = beta;
+                int kdim = 0;
+                for (int j = 0; j < m; ++j) {
+                    // w = M^-1 A v_j (A from its blocks: the residual kernel with b = 0)
+                    residual_multi(e->mna, V + (size_t)j * n, e->d_gzero.ptr, w, 1, e->d_gax.ptr, e->d_geta.ptr, s);
+                    precond(w);
+                    for (int pass = 0; pass < 2; ++pass) {
+                        dots(V, j + 1, w, hv);
+                        for (int i = 0; i <= j; ++i) {
+                            H[(size_t)i * m + j] += hv[i];
+                            hbuf[i] = make_double2(hv[i].real(), hv[i].imag());
+                        }
+                        e->d_gh.upload(hbuf.data(), j + 1, s);
+                        hipLaunchKernelGGL(gmres_axpy_kernel, dim3(vg), dim3(256), 0, s, V, (size_t)n, e->d_gh.ptr,
+                                           j + 1, w, n);
+                    }
+                    dots(w, 1, w, hv);
+                    const double hn = std::sqrt(std::fabs(hv[0].real()));
+                    H[(size_t)(j + 1) * m + j] = hn;
+                    // previous rotations, then this column's
+                    for (int i = 0; i < j; ++i) {
+                        const std::complex<double> a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+                        H[(size_t)i * m + j] = std::conj(cs[i]) * a + std::conj(sn[i]) * b;
+                        H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+                    }
+                    const std::complex<double> a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+                    const double den = std::sqrt(std::norm(a) + std::norm(b));
+                    if (den == 0.0) break;
+                    cs[j] = a / den;
+                    sn[j] = b / den;
+                    H[(size_t)j * m + j] = den;
+                    H[(size_t)(j + 1) * m + j] = 0.0;
+                    g[j + 1] = -sn[j] * g[j];
+                    g[j] = std::conj(cs[j]) * g[j];
+                    kdim = j + 1;
+                    // the preconditioned residual is |g_{j+1}|
+                    if (std::abs(g[j + 1]) <= 1e-15 * beta || !(hn > 0.0)) break;
+                    hipLaunchKernelGGL(gmres_scale_kernel, dim3(vg), dim3(256), 0, s, w, make_double2(1.0 / hn, 0.0),
+                                       V + (size_t)(j + 1) * n, n);
+                }
+                report.gmres_iterations += kdim;
+                // y = H^-1 g (upper triangular), x <- x - V y
+                std::vector<std::complex<double>> y(kdim);
+                for (int i = kdim - 1; i >= 0; --i) {
+                    std::complex<double> t = g[i];
+                    for (int c = i + 1; c < kdim; ++c) t -= H[(size_t)i * m + c] * y[c];
+                    y[i] = t / H[(size_t)i * m + i];
+                }
+                for (int i = 0; i < kdim; ++i) hbuf[i] = make_double2(y[i].real(), y[i].imag());
+                e->d_gh.upload(hbuf.data(), kdim, s);
+                hipLaunchKernelGGL(gmres_axpy_kernel, dim3(vg), dim3(256), 0, s, V, (size_t)n, e->d_gh.ptr, kdim, x,
+                                   n);
+            }
+            if (!any) break;
+            ++report.gmres_steps;
+            measure();
+            std::vector<int> flags(nrhs, 0);
+            bool improved = false;
+            for (int q = 0; q < nrhs; ++q) {
+                if (eta[q] < best[q]) {
+                    best[q] = eta[q];
+                    since[q] = 0;
+                    flags[q] = 1;
+                    improved = true;
+                } else {
+                    ++since[q];
+                }
+            }
+            if (improved) {
+                const int *dfl;
+                unsigned long long msk;
+                flag_args(flags, dfl, msk);
+                hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_Xbest.ptr, e->d_X.ptr, dfl,
+                                   msk, n, nrhs);
+            }
+        }
+        HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_Xbest.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+        measure();
+        double w = 0.0;
+        for (int q = 0; q < nrhs; ++q) w = std::fmax(w, eta[q]);
+        return w;
+    };
     double worst = 0.0;
     if (mode == PEEC_SOLVE_MIXED) {
         worst = refine();
@@ -620,6 +817,11 @@ static int engine_solve(PeecEngine *e, int mode, int nrhs, PeecPointReport &repo
             const int code = factor_pivoted(e, false, true);
             if (code) return code;
             worst = refine();
+        }
+        if (!(worst <= e->opt.backward_error_gate) && e->opt.gmres_krylov > 0 && std::isfinite(worst)) {
+            // plain refinement stalled above the gate: GMRES-IR on the same
+            // factor before refactoring in FP64
+            worst = gmres_refine();
         }
         if (!(worst <= e->opt.backward_error_gate)) {
             // refinement stalled above the gate: refactor in FP64
@@ -687,6 +889,7 @@ void peec_engine_options_default(PeecEngineOptions *o) {
     o->smooth_moments = -1;
     o->split_fp16_update = 1;
     o->shifted_moments = 0;
+    o->gmres_krylov = 20;
 }
 
 // Moment terms stored by default: the series then covers k * diameter up to

@@ -29,7 +29,8 @@ class PeecEngineOptions(C.Structure):
                 ("block_size", C.c_int),
                 ("max_refinement_steps", C.c_int), ("refinement_target", C.c_double),
                 ("backward_error_gate", C.c_double), ("reuse_pivots", C.c_int), ("smooth_moments", C.c_int),
-                ("shifted_moments", C.c_int), ("split_fp16_update", C.c_int), ("panel_lds_kib", C.c_int)]
+                ("shifted_moments", C.c_int), ("split_fp16_update", C.c_int), ("panel_lds_kib", C.c_int),
+                ("gmres_krylov", C.c_int)]
 
 
 class PeecPointReport(C.Structure):
@@ -39,7 +40,8 @@ class PeecPointReport(C.Structure):
                 ("ms_assembly", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("ms_check", C.c_double), ("factor_cached", C.c_int),
                 ("moment_terms", C.c_int), ("block_size", C.c_int), ("moment_center", C.c_double),
-                ("batch_size", C.c_int), ("update_tile_rows", C.c_int)]
+                ("batch_size", C.c_int), ("update_tile_rows", C.c_int), ("gmres_steps", C.c_int),
+                ("gmres_iterations", C.c_int)]
 
 
 class PeecServerOptions(C.Structure):

@@ -405,7 +405,7 @@ def test_engine_mixed_fp64_fallback_matches_fp64_mode(require_gpu):
     xyz, quads = meshgen.box(0.8, 0.5, 0.5, 8, 5, 5)
     port = (0, len(xyz) - 1)
     with plib.Model(xyz, quads) as m:
-        mixed = plib.Engine(m, retarded=True, max_refinement_steps=0, backward_error_gate=1e-13)
+        mixed = plib.Engine(m, retarded=True, max_refinement_steps=0, backward_error_gate=1e-13, gmres_krylov=0)
         fp64 = plib.Engine(m, retarded=True)
         for f in (2e5, 3e7):
             st, z, _, _, rep = mixed.solve(f, [port], plib.Engine.MIXED, want_solution=False)
@@ -545,7 +545,7 @@ def test_engine_rcs_fallback_then_solve_at_same_point(require_gpu):
         z64 = fp64.solve(f, [port], plib.Engine.FP64, want_solution=False)[1][0]
         s64 = [r.sigma for r in fp64.monostatic_rcs(f, th, ph, mode=plib.Engine.FP64)[1]]
         fp64.close()
-        mixed = plib.Engine(m, max_refinement_steps=0)
+        mixed = plib.Engine(m, max_refinement_steps=0, gmres_krylov=0)
         out = []
         for call in ("solve", "rcs", "solve", "rcs"):
             if call == "solve":
@@ -594,3 +594,35 @@ def test_engine_subpanels_of_tall_block_columns(require_gpu, kib):
         assert abs(z64[0] - zr) <= 1e-12 * abs(zr)
         assert abs(zm[0] - z64[0]) <= 1e-10 * abs(z64[0])
         assert r64.backward_error <= 1e-10 and rm.backward_error <= 1e-10
+
+
+def test_engine_gmres_ir_fallback(require_gpu):
+    """When plain refinement stalls above the gate, GMRES-IR on the same FP32
+    factor (FP64 residuals; each correction solved by GMRES on the system
+    preconditioned with the factor) reaches the gate without the FP64
+    refactor. Forced at N = 938 with one plain refinement step and a gate of
+    1e-14; with GMRES-IR off the same points refactor in FP64 (whose own
+    backward error, ~1e-14 here, need not reach that gate)."""
+    xyz, quads = meshgen.box(1.0, 0.6, 0.6, 10, 6, 6)
+    port = (0, len(xyz) - 1)
+    freqs = [1e6, 3e6, 1e7]
+    strict = dict(retarded=True, max_refinement_steps=1, backward_error_gate=1e-14, refinement_target=1e-15)
+    with plib.Model(xyz, quads) as m:
+        ref = plib.Engine(m, retarded=True)
+        z64 = [ref.solve(f, [port], plib.Engine.FP64, want_solution=False)[1][0] for f in freqs]
+        ref.close()
+        eng = plib.Engine(m, **strict)
+        out = [eng.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+        eng.close()
+        off = plib.Engine(m, gmres_krylov=0, **strict)
+        out_off = [off.solve(f, [port], plib.Engine.MIXED, want_solution=False) for f in freqs]
+        off.close()
+    for f, zr, (st, z, _, _, r), (st2, _, _, _, r2) in zip(freqs, z64, out, out_off):
+        print(f"f={f:g}: steps {r.refinement_steps} gmres {r.gmres_steps}/{r.gmres_iterations} eta "
+              f"{r.backward_error:.2e} fallback {r.used_fp64_fallback} | off: fallback {r2.used_fp64_fallback} "
+              f"eta {r2.backward_error:.2e} | vs fp64 {abs(z[0] - zr) / abs(zr):.2e}", flush=True)
+        assert st == 0 and st2 in (0, -3)
+        assert r.used_fp64_fallback == 0 and r.backward_error <= 1e-14
+        assert abs(z[0] - zr) <= 1e-10 * abs(zr)
+        assert (r.gmres_steps > 0) == (r2.used_fp64_fallback == 1)
+    assert any(r.gmres_steps > 0 for _, _, _, _, r in out)
