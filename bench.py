@@ -363,10 +363,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=200, help="sweep: points timed (200 = the whole sweep)")
-    ap.add_argument("--warmup", type=int, default=5,
-                    help="sweep: untimed points (the first factor with pivoting and the autotuner's four "
-                         "factor-shape trials)")
+    ap.add_argument("--warmup", type=int, default=17,
+                    help="sweep: untimed points (the first factor with pivoting, then batches covering the "
+                         "autotuner's four factor-shape trials: 1 + 4 x batch)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=4,
+                    help="sweep: mixed points factored together in shared launches (1 = one at a time)")
     ap.add_argument("--cells", type=float, default=0.05, help="quad size in metres (default 5 cm)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
@@ -426,23 +428,43 @@ def main():
         np.arange(K) % len(freqs)
     timed = [freqs[i] for i in idx]
     warm = [freqs[(i * 37 + 11) % len(freqs)] for i in range(W)]
-    for f in warm:
+    B = max(1, args.batch)
+    # the first warm-up point alone (its partial-pivoting factor sets the row
+    # order), the rest like the timed points
+    for i, f in enumerate(warm[:1]):
         st, *_ = eng.solve(f, [port], lib.Engine.MIXED, want_solution=False)
         assert st == 0, f"warmup point failed ({st})"
-    print(f"[bench] mesh N={n} static assembly {t_static:.2f} s; timing {len(timed)} mixed points", flush=True)
+    rest = warm[1:]
+    for i in range(0, len(rest), B):
+        chunk = rest[i:i + B]
+        if B == 1:
+            st = eng.solve(chunk[0], [port], lib.Engine.MIXED, want_solution=False)[0]
+        else:
+            st = eng.solve_batch(chunk, [port])[0]
+        assert st == 0, f"warmup point failed ({st})"
+    print(f"[bench] mesh N={n} static assembly {t_static:.2f} s; timing {len(timed)} mixed points"
+          f"{'' if B == 1 else f' in batches of {B}'}", flush=True)
 
     eng.set_profiling(True)
     sync()
     lat, zm, reports = [], [], []
     t_start = time.perf_counter()
-    for f in timed:
+    for i in range(0, len(timed), B):
+        chunk = timed[i:i + B]
         t1 = time.perf_counter()
-        st, z, _, _, rep = eng.solve(f, [port], lib.Engine.MIXED, want_solution=False)
-        lat.append(time.perf_counter() - t1)
-        if st != 0:
-            raise SystemExit(f"mixed point f={f} failed with status {st}")
-        zm.append(z[0])
-        reports.append(rep)
+        if B == 1:
+            st, z, _, _, rep = eng.solve(chunk[0], [port], lib.Engine.MIXED, want_solution=False)
+            zs, reps, sts = [z], [rep], [st]
+        else:
+            _, sts, zs, reps = eng.solve_batch(chunk, [port])
+        el = time.perf_counter() - t1
+        for f, st, z, rep in zip(chunk, sts, zs, reps):
+            if st != 0:
+                raise SystemExit(f"mixed point f={f} failed with status {st}")
+            # a batch member's answer is ready when its batch is done
+            lat.append(el)
+            zm.append(z[0])
+            reports.append(rep)
     sync()
     t_mixed = time.perf_counter() - t_start
     gemm = eng.kernel_stats("lu_update")
@@ -497,7 +519,10 @@ def main():
                          f"Nv={n_v}, N={n}), retarded exp(-jkR)/R, {len(timed)} of the 200 log-spaced points "
                          f"10 kHz-200 MHz, 1 port node {port[0]} -> {port[1]}"),
             "points": [float(f) for f in timed],
+            "mixed_batch": B,
         },
+        "latency_note": ("per point, submission to answer; the points of a mixed batch are answered together "
+                         "when the batch is done" if B > 1 else "per point"),
         "p50_latency_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_latency_ms": 1e3 * float(np.percentile(lat, 99)),
         "fp64": {
@@ -515,8 +540,10 @@ def main():
         "pivot_order_reused": int(sum(r.pivot_reuse and not r.pivot_refactor for r in reports)),
         "pivot_order_refactors": int(sum(r.pivot_refactor for r in reports)),
         "ordered_factor_block_widths": sorted({int(r.block_size) for r in reports if r.block_size}),
-        "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) for r in reports]))
+        "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) / (max(1, r.batch_size) if k == "factor" else 1)
+                                            for r in reports]))
                           for k in ("assembly", "fill", "factor", "solve", "check")},
+        "stage_note": "factor: a batch's shared factor time divided by its members",
         "static_assembly_s": t_static,
         "sweep_amortized": {
             "points": len(freqs),

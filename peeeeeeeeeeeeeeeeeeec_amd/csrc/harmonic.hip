@@ -1300,6 +1300,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             lu_factor_ordered(e->d_Z32.ptr, n, n, 0.0, *e->lu, nb, (int)m, nn, e->d_limits.ptr, fst.data());
         HIP_CHECK(hipEventRecord(e->ev[1], s));
         e->stream.sync();
+        e->timer.resolve();
         float ms_factor = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_factor, e->ev[0], e->ev[1]));
         if (trial && bst.code == 0) tu.record(1e-3 * ms_factor / (double)m, split16);
@@ -1373,6 +1374,8 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
         // a single point
         e->cache_ok = false;
         e->z32 = e->d_Z32.ptr;
+        e->stream.sync();
+        e->timer.resolve();
     } catch (const HipError &err) {
         PEEC_ERR("%s", err.what.c_str());
         e->cache_ok = false;
