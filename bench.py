@@ -555,7 +555,8 @@ def main():
         "roofline": {
             "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, rank 256 (two "
                        "128-wide panels per pass), complex FP32 storage, split-FP16 operands on "
-                       "v_mfma_f32_16x16x32_f16 (gemm_tile_h3_kernel<16, 2>)"),
+                       "v_mfma_f32_16x16x32_f16 (gemm_tile_h3_kernel<16, 2>)"
+                       + (f"; each launch updates the {B} points of a batch (grid z)" if B > 1 else "")),
             "bound": "mfma",
             "achieved": g16_tf / 1e12,
             "peak": PEAKS["fp16_mfma"] / 1e12,
