@@ -540,10 +540,9 @@ def main():
         "pivot_order_reused": int(sum(r.pivot_reuse and not r.pivot_refactor for r in reports)),
         "pivot_order_refactors": int(sum(r.pivot_refactor for r in reports)),
         "ordered_factor_block_widths": sorted({int(r.block_size) for r in reports if r.block_size}),
-        "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) / (max(1, r.batch_size) if k == "factor" else 1)
-                                            for r in reports]))
+        "stage_ms_mean": {k: float(np.mean([getattr(r, "ms_" + k) for r in reports]))
                           for k in ("assembly", "fill", "factor", "solve", "check")},
-        "stage_note": "factor: a batch's shared factor time divided by its members",
+        "stage_note": "a batch member's factor and solve times are its share of the batch's shared launches",
         "static_assembly_s": t_static,
         "sweep_amortized": {
             "points": len(freqs),

@@ -170,10 +170,12 @@ int peec_engine_solve(PeecEngine *engine, double frequency, int mode,
  * FP32 matrix are formed in the engine's reused row order, then one ordered
  * factor runs over all of them in shared launches (the members of a launch
  * are its grid's z index), then each point is solved and refined on its own
- * factor with the single point's fallbacks. Points met before the engine has
- * a row order (its first mixed point) are solved alone first. The factors
- * are those of single points in the same row order and block width, so the
- * results equal single calls'. status[p] gets each point's code (as
+ * factor with the single point's fallbacks (points of one port each are
+ * refined in lock-step, their triangular solves in shared launches; a
+ * member's ms_factor / ms_solve are its share of the batch's). Points met
+ * before the engine has a row order (its first mixed point) are solved alone
+ * first. The factors and solves are those of single points in the same row
+ * order and block width, so the results equal single calls'. status[p] gets each point's code (as
  * peec_engine_solve); returns 0 or the first non-zero one. */
 int peec_engine_solve_batch(PeecEngine *engine, size_t n_points, const double *frequency,
                             const int *const *ports, const size_t *n_ports, Complex *const *z_in,

@@ -276,6 +276,7 @@ struct PeecEngine {
     std::vector<PointSlot> slots;
     DeviceBuffer<double> d_limits;
     DeviceBuffer<int> d_batch_perm;  // the row order a batch was factored in
+    DeviceBuffer<double> d_mrow, d_mcol;  // a batch's equilibration scales side by side
     // GMRES-IR workspace: Krylov basis, work vector, zero right-hand side,
     // partial dot products, projections / solution coefficients, |x|, eta
     DeviceBuffer<double2> d_gV, d_gw, d_gzero, d_gpart, d_gh;
@@ -1304,15 +1305,137 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
         float ms_factor = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_factor, e->ev[0], e->ev[1]));
         if (trial && bst.code == 0) tu.record(1e-3 * ms_factor / (double)m, split16);
+        // one port per point and every factor good: the points are refined
+        // in lock-step, their triangular solves in shared launches
+        // (lu_solve_members); a point that stays above the gate goes on
+        // alone below
+        std::vector<char> done(m, 0);
+        bool lockstep = m <= 64;  // the members' step masks are one 64-bit word
+        for (size_t b = 0; b < m; ++b) lockstep = lockstep && prep[b] == 0 && fst[b].code == 0 && n_ports[first + b] == 1;
+        if (lockstep) {
+            const size_t tot = (size_t)m * n;
+            e->d_B.resize(tot);
+            e->d_X.resize(tot);
+            e->d_R.resize(tot);
+            e->d_Xbest.resize(tot);
+            e->d_ax.resize(tot);
+            e->d_eta.resize(m);
+            e->d_mrow.resize(tot);
+            e->d_mcol.resize(tot);
+            std::vector<int> hp(2 * m);
+            for (size_t b = 0; b < m; ++b) {
+                hp[2 * b] = ports[first + b][0];
+                hp[2 * b + 1] = ports[first + b][1];
+            }
+            e->d_ports.upload(hp.data(), 2 * m, s);
+            HIP_CHECK(hipEventRecord(e->ev[0], s));
+            const bool ret = e->opt.retarded != 0;
+            for (size_t b = 0; b < m; ++b) {
+                PeecEngine::PointSlot &sl = e->slots[b];
+                hipLaunchKernelGGL(port_rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                                   e->d_B.ptr + b * n, ne, nv, e->d_P.ptr, ret ? sl.Pw.ptr : nullptr,
+                                   e->d_ports.ptr + 2 * b, 1);
+                HIP_CHECK(hipMemcpyAsync(e->d_mrow.ptr + b * n, sl.row.ptr, sizeof(double) * n,
+                                         hipMemcpyDeviceToDevice, s));
+                HIP_CHECK(hipMemcpyAsync(e->d_mcol.ptr + b * n, sl.col.ptr, sizeof(double) * n,
+                                         hipMemcpyDeviceToDevice, s));
+            }
+            std::vector<double> eta(m), best(m);
+            std::vector<unsigned long long> bits(m);
+            std::vector<int> since(m, 0), active(m), flags(m), steps(m, 0);
+            auto measure = [&]() {
+                for (size_t b = 0; b < m; ++b)
+                    residual_multi(e->slots[b].mna, e->d_X.ptr + b * n, e->d_B.ptr + b * n, e->d_R.ptr + b * n, 1,
+                                   e->d_ax.ptr + b * n, e->d_eta.ptr + b, s);
+                e->d_eta.download(bits.data(), m, s);
+                e->stream.sync();
+                for (size_t b = 0; b < m; ++b) {
+                    double v;
+                    std::memcpy(&v, &bits[b], sizeof v);
+                    eta[b] = std::isfinite(v) ? v : INFINITY;
+                }
+            };
+            auto mask_of = [&](const std::vector<int> &fl) {
+                unsigned long long mk = 0ull;
+                for (size_t b = 0; b < m; ++b)
+                    if (fl[b]) mk |= 1ull << b;
+                return mk;
+            };
+            const unsigned grid = (unsigned)((tot + 255) / 256);
+            HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+            lu_solve_members(e->d_Z32.ptr, nn, n, e->d_batch_perm.ptr, e->d_mrow.ptr, e->d_mcol.ptr, e->d_X.ptr,
+                             (int)m, *e->lu, false);
+            measure();
+            best = eta;
+            for (size_t b = 0; b < m; ++b) active[b] = eta[b] > e->opt.refinement_target;
+            HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, e->d_X.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
+            for (int step = 0; step < e->opt.max_refinement_steps; ++step) {
+                bool any = false;
+                for (size_t b = 0; b < m; ++b) any = any || active[b];
+                if (!any) break;
+                lu_solve_members(e->d_Z32.ptr, nn, n, e->d_batch_perm.ptr, e->d_mrow.ptr, e->d_mcol.ptr, e->d_R.ptr,
+                                 (int)m, *e->lu, true);
+                hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr,
+                                   (const int *)nullptr, mask_of(active), n, (int)m);
+                for (size_t b = 0; b < m; ++b) steps[b] += active[b];
+                measure();
+                bool improved = false;
+                for (size_t b = 0; b < m; ++b) {
+                    flags[b] = 0;
+                    if (!active[b]) continue;
+                    if (eta[b] < best[b]) {
+                        best[b] = eta[b];
+                        since[b] = 0;
+                        flags[b] = 1;
+                        improved = true;
+                    } else if (++since[b] >= 2) {
+                        active[b] = 0;
+                    }
+                    if (best[b] <= e->opt.refinement_target) active[b] = 0;
+                }
+                if (improved)
+                    hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_Xbest.ptr, e->d_X.ptr,
+                                       (const int *)nullptr, mask_of(flags), n, (int)m);
+            }
+            for (size_t b = 0; b < m; ++b) flags[b] = eta[b] > best[b];
+            hipLaunchKernelGGL(masked_copy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_Xbest.ptr,
+                               (const int *)nullptr, mask_of(flags), n, (int)m);
+            HIP_CHECK(hipEventRecord(e->ev[1], s));
+            std::vector<double2> x(tot);
+            e->d_X.download(x.data(), tot, s);
+            e->stream.sync();
+            float ms_solve = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms_solve, e->ev[0], e->ev[1]));
+            for (size_t b = 0; b < m; ++b) {
+                if (!(best[b] <= e->opt.backward_error_gate)) continue;  // alone below, with the fallbacks
+                const size_t p = first + b;
+                const double2 *xb = x.data() + b * n;
+                const double2 va = xb[ne + ports[p][0]], vb = xb[ne + ports[p][1]];
+                z_in[p][0] = Complex{va.x - vb.x, va.y - vb.y};
+                PeecPointReport &r = rep[b];
+                r.ms_assembly = ms_asm[b];
+                r.ms_fill = ms_prep[b];
+                r.ms_factor = ms_factor / (double)m;
+                r.ms_solve = ms_solve / (double)m;
+                r.block_size = nb;
+                r.update_tile_rows = shape.rows;
+                r.pivot_reuse = 1;
+                r.refinement_steps = steps[b];
+                r.backward_error = best[b];
+                status[p] = 0;
+                done[b] = 1;
+            }
+        }
         // each point's solves and refinement on its own factor (with the
         // single point's fallbacks: a rejected order is refactored with
         // pivoting, a point above the gate in FP64)
         for (size_t b = 0; b < m; ++b) {
+            if (done[b]) continue;
             const size_t p = first + b;
             PeecPointReport &r = rep[b];
             r.ms_assembly = ms_asm[b];
             r.ms_fill = ms_prep[b];
-            r.ms_factor = ms_factor;
+            r.ms_factor = ms_factor / (double)m;
             r.block_size = nb;
             r.update_tile_rows = shape.rows;
             if (prep[b] != 0) {

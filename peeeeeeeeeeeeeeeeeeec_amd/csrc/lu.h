@@ -83,6 +83,7 @@ struct LUContext {
     DeviceBuffer<unsigned char> diag_inv;         // inverses of the 64x64 diagonal blocks (solves)
     const void *dinv_factor = nullptr;            // factor whose inverses diag_inv holds
     int dinv_n = 0;
+    int dinv_members = 1;                         // ... and of how many batch members
     int lds_per_block = 64 * 1024;
     size_t panel_lds_limit = 0;                   // cap on the pivoted panel's LDS per workgroup (0: none)
     hipStream_t aux = nullptr;                    // look-ahead stream (default priority)
@@ -155,6 +156,16 @@ template <class S, class V>
 void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const double *d_row,
                        const double *d_col, V *dX, int nrhs, LUContext &ctx, bool same_factor = false);
 void permutation_from_pivots(const int *d_ipiv, int n, int *d_perm, hipStream_t s);
+
+// One right-hand side per member of a batch of FP32 factors (a batched
+// ordered factor: the members lu_stride elements apart with leading dimension
+// n, all in the row order d_perm; member b's scales at d_row + b n and
+// d_col + b n, its vector at dX + b n): X_b <- A_b^-1 X_b as
+// lu_solve_permuted does, with every member's chained solves in the same
+// cooperative launches (grid y = member). same_factors: the factors are
+// those of the previous call on this context (block inverses reused).
+void lu_solve_members(const float2 *dLU, size_t lu_stride, int n, const int *d_perm, const double *d_row,
+                      const double *d_col, double2 *dX, int members, LUContext &ctx, bool same_factors);
 
 // y = A x for a complex FP64 rows x cols matrix (leading dimension lda),
 // and the reference's component-wise backward error terms per row:

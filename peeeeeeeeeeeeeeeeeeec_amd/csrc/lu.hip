@@ -3257,8 +3257,13 @@ __device__ inline double2 quad_sum(double2 v) { return make_double2(quad_sum(v.x
 template <class S, class V>
 __global__ __launch_bounds__(256) void diag_inverse_kernel(const S *M, size_t lda, int n,
                                                            typename VecT<V>::A *linv, typename VecT<V>::A *uinv,
-                                                           int *bad) {
+                                                           int *bad, size_t m_stride = 0, size_t d_stride = 0) {
     using T = typename VecT<V>::A;
+    // member blockIdx.y of a batch of factors (m_stride apart), its inverses
+    // d_stride apart
+    M += (size_t)blockIdx.y * m_stride;
+    linv += (size_t)blockIdx.y * d_stride;
+    uinv += (size_t)blockIdx.y * d_stride;
     constexpr int TB = kChainTB;
     __shared__ T blk[TB][TB];
     __shared__ T xrow[2][TB];  // the finished row of X at each step (double-buffered)
@@ -3364,8 +3369,16 @@ __device__ inline bool chain_empty(double v) { return (unsigned long long)__doub
 // factors are widened on load).
 template <class S, class V, bool UPPER, int NQ>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda, const V *B, V *Y, int n, int nrhs,
-                                                         const typename VecT<V>::A *dinv, int *bad) {
+                                                         const typename VecT<V>::A *dinv, int *bad,
+                                                         size_t m_stride, size_t v_stride, size_t d_stride) {
     using T = typename VecT<V>::A;
+    // member blockIdx.y of a batch: its factor, vectors and block inverses
+    // (strides zero for a single solve); each member's chain has the grid's
+    // x dimension to itself
+    M += (size_t)blockIdx.y * m_stride;
+    B += (size_t)blockIdx.y * v_stride;
+    Y += (size_t)blockIdx.y * v_stride;
+    dinv += (size_t)blockIdx.y * d_stride;
     constexpr int TB = kChainTB;
     __shared__ T dblk[TB * kChainRow];
     __shared__ T xs[NQ][kChainRow];
@@ -3496,8 +3509,8 @@ void chain_solve(const S *dLU, int n, size_t lda, const V *B, V *Y, int nrhs, co
         const V *Bq = B + (size_t)q0 * n;
         V *Yq = Y + (size_t)q0 * n;
         int nq = std::min(8, nrhs - q0), nn = n;
-        size_t ld = lda;
-        void *args[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, &nq, (void *)&dinv, &bad};
+        size_t ld = lda, zero = 0;
+        void *args[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, &nq, (void *)&dinv, &bad, &zero, &zero, &zero};
         void *fn = nq == 1 ? reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 1>)
                            : reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 8>);
         HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, ctx.stream));
@@ -3625,11 +3638,12 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
     T *uinv = linv + (size_t)nblk * kChainTB * kChainTB;
     // the block inverses depend on the factor only: computed again unless the
     // caller solves with the factor of the previous call
-    if (!(same_factor && ctx.dinv_factor == dLU && ctx.dinv_n == n)) {
+    if (!(same_factor && ctx.dinv_factor == dLU && ctx.dinv_n == n && ctx.dinv_members == 1)) {
         hipLaunchKernelGGL((diag_inverse_kernel<S, V>), dim3(nblk), dim3(256), 0, s, dLU, lda, n, linv, uinv,
                            ctx.solve_bad.ptr);
         ctx.dinv_factor = dLU;
         ctx.dinv_n = n;
+        ctx.dinv_members = 1;
     }
     const int grid = chain_grid<S, V>(n, ctx);
     if (nrhs >= kBlockedSolveMinRhs) {
@@ -3668,6 +3682,87 @@ void lu_solve_permuted(const S *dLU, int n, size_t lda, const int *d_perm, const
         }
     }
     if (d_col) hipLaunchKernelGGL(col_scale_kernel<V>, dim3((total + 255) / 256), dim3(256), 0, s, tmp, d_col, n, nrhs);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(dX, tmp, sizeof(V) * total, hipMemcpyDeviceToDevice, s));
+    int h_bad = 0;
+    HIP_CHECK(hipMemcpyAsync(&h_bad, ctx.solve_bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h_bad & 2) throw HipError{"triangular solve chain timed out"};
+    if (h_bad) throw HipError{"singular U diagonal in solve"};
+}
+
+// X[b][i] = B[b][perm[i]] / row[b][perm[i]] for the members b of a batch
+__global__ void permute_scale_members_kernel(double2 *X, const double2 *B, const int *perm, const double *row, int n,
+                                             int members) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)n * members) return;
+    const size_t b = idx / n;
+    const int p = perm[idx % n];
+    const double2 v = B[b * n + p];
+    const double r = row[b * n + p];
+    X[idx] = make_double2(v.x / r, v.y / r);
+}
+// X[b][i] /= col[b][i]
+__global__ void col_scale_members_kernel(double2 *X, const double *col, size_t total) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    X[idx].x /= col[idx];
+    X[idx].y /= col[idx];
+}
+
+void lu_solve_members(const float2 *dLU, size_t lu_stride, int n, const int *d_perm, const double *d_row,
+                      const double *d_col, double2 *dX, int members, LUContext &ctx, bool same_factors) {
+    using S = float2;
+    using V = double2;
+    using T = double2;
+    hipStream_t s = ctx.stream;
+    if (members < 1) return;
+    const size_t total = (size_t)n * members;
+    if (ctx.solve_tmp.count < 4 * total) ctx.solve_tmp.resize(4 * total);
+    V *tmp = reinterpret_cast<V *>(ctx.solve_tmp.ptr);
+    V *ybuf = tmp + total;
+    const unsigned g = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(permute_scale_members_kernel, dim3(g), dim3(256), 0, s, tmp, dX, d_perm, d_row, n, members);
+    const int nblk = (n + kChainTB - 1) / kChainTB;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, ctx.solve_bad.ptr, 1, 0);
+    // the block inverses of every member: L's of all members, then U's
+    const size_t dstride = (size_t)nblk * kChainTB * kChainTB;
+    const size_t need = 2 * (size_t)members * dstride * sizeof(T);
+    if (ctx.diag_inv.count < need) ctx.dinv_factor = nullptr;
+    ctx.diag_inv.reserve(need);
+    T *linv = reinterpret_cast<T *>(ctx.diag_inv.ptr);
+    T *uinv = linv + (size_t)members * dstride;
+    if (!(same_factors && ctx.dinv_factor == dLU && ctx.dinv_n == n && ctx.dinv_members == members)) {
+        hipLaunchKernelGGL((diag_inverse_kernel<S, V>), dim3(nblk, members), dim3(256), 0, s, dLU, (size_t)n, n, linv,
+                           uinv, ctx.solve_bad.ptr, lu_stride, dstride);
+        ctx.dinv_factor = dLU;
+        ctx.dinv_n = n;
+        ctx.dinv_members = members;
+    }
+    // every member's chain gets an equal share of the co-resident workgroups
+    int f = 0, bw = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &f, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, false, 1>), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &bw, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, true, 1>), 256, 0));
+    const long long cap = (long long)std::min(f, bw) * ctx.cus / members;
+    if (cap < 1) throw HipError{"batched triangular solve: too many members for co-resident chains"};
+    const int G = (int)std::min<long long>(nblk, cap);
+    auto chain = [&](bool upper, const V *B, V *Y, const T *dinv) {
+        const size_t words = total * 2;
+        hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<unsigned long long *>(Y), words, kChainEmpty);
+        int nn = n, nq = 1;
+        size_t ld = (size_t)n, ms = lu_stride, vs = (size_t)n, ds = dstride;
+        int *bad = ctx.solve_bad.ptr;
+        void *args[] = {(void *)&dLU, &ld, (void *)&B, &Y, &nn, &nq, (void *)&dinv, &bad, &ms, &vs, &ds};
+        void *fn = upper ? reinterpret_cast<void *>(trsv_chain_kernel<S, V, true, 1>)
+                         : reinterpret_cast<void *>(trsv_chain_kernel<S, V, false, 1>);
+        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(G, members), dim3(256), args, 0, s));
+    };
+    chain(false, tmp, ybuf, linv);
+    chain(true, ybuf, tmp, uinv);
+    if (d_col) hipLaunchKernelGGL(col_scale_members_kernel, dim3(g), dim3(256), 0, s, tmp, d_col, total);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(dX, tmp, sizeof(V) * total, hipMemcpyDeviceToDevice, s));
     int h_bad = 0;
