@@ -332,8 +332,11 @@ def test_engine_mixed_sweep_reuses_row_order(require_gpu):
         assert r[3] <= 1e-10
     assert reports[0][0] == 0  # the first point has no order to reuse
     assert all(r[0] == 1 for r in reports[1:])
-    # the reused order is kept at most points of the sweep
-    assert sum(1 for r in reports[1:] if not r[1]) >= len(freqs) // 2
+    # the reused order is kept: measured with no refactor over this sweep
+    # (from 10 kHz up to 200 MHz in the 10 kHz point's order); one is allowed
+    # for a point whose reused-order refinement lands just above its keep
+    # threshold
+    assert sum(1 for r in reports[1:] if r[1]) <= 1
     # shape autotuner: the first four ordered factors try block widths 128
     # and 64 with split-FP16 update tiles of 64 and 32 rows, the later ones
     # keep one of the four
