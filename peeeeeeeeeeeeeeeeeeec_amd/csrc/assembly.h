@@ -65,6 +65,14 @@ class RegionSet {
     // d_out = d_static + remainder from the stored moments (terms <= moment_terms())
     void assemble_retarded_moments(const double *d_static, double wave_number, int terms, double2 *d_out,
                                    hipStream_t s, KernelTimer *timer = nullptr);
+    // The same for `count` wave numbers at once (count <= kMaxMomentBatch),
+    // d_out[b] receiving the matrix of wave_number[b] with terms[b] terms:
+    // one pass over the moments and the static matrix for all of them, each
+    // result bitwise equal to assemble_retarded_moments of its wave number.
+    static constexpr int kMaxMomentBatch = 4;
+    void assemble_retarded_moments_batch(const double *d_static, const double *wave_number, const int *terms,
+                                         double2 *const *d_out, int count, hipStream_t s,
+                                         KernelTimer *timer = nullptr);
     // Complex moments around a shifted wave number k0 (points above the
     // range of the moments at 0): V_0 = scale * sum_w (exp(-jk0R) - 1)/R and
     // V_m = scale * sum_w exp(-jk0R) R^(m-1), m = 1..terms-1, so that the

@@ -973,6 +973,103 @@ __global__ __launch_bounds__(256) void moment_eval_kernel(
     }
 }
 
+// The same series for up to four wave numbers per pass (a batch of sweep
+// points): 1024 threads per 64 x 64 upper tile, each with four rows of the
+// tile for every member, so each moment and static value is read once for
+// all members. Per value and member the sum runs over the same terms in the
+// same order as moment_eval_kernel (a member's coefficients past its own
+// term count are zero and leave the sum at +0), so the results are
+// identical.
+struct MomentBatch {
+    MomentCoefficients cf[RegionSet::kMaxMomentBatch];
+    double2 *out[RegionSet::kMaxMomentBatch];
+};
+
+template <int B>
+__global__ __launch_bounds__(1024) void moment_eval_batch_kernel(const double *__restrict__ mom, size_t mstride,
+                                                                 int terms, MomentBatch mb,
+                                                                 const int *__restrict__ tile_ids, int R,
+                                                                 const double *__restrict__ stat, size_t ld) {
+    __shared__ double tr[64][65];
+    const int t = blockIdx.x;
+    const int ta = tile_ids[2 * t], tb = tile_ids[2 * t + 1];
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;  // rows r0 + 16 q
+    const double *base = mom + (size_t)t * 4096 + (size_t)r0 * 64 + c;
+    double re[B][4], im[B][4];
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) re[b][q] = im[b][q] = 0.0;
+    // two terms per step (their loads issued together), still smallest first
+    auto term = [&](int m, const double (&u)[4]) {
+        if (m & 1) {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) re[b][q] = __builtin_fma(mb.cf[b].c[m], u[q], re[b][q]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) im[b][q] = __builtin_fma(mb.cf[b].c[m], u[q], im[b][q]);
+        }
+    };
+    int m = terms - 1;
+    for (; m >= 1; m -= 2) {
+        const double *src = base + (size_t)m * mstride, *nxt = src - mstride;
+        double u[4], v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            u[q] = src[q * 1024];
+            v[q] = nxt[q * 1024];
+        }
+        term(m, u);
+        term(m - 1, v);
+    }
+    if (m == 0) {
+        double u[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[q] = base[q * 1024];
+        term(0, u);
+    }
+    const int j = tb * 64 + c;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = ta * 64 + r0 + 16 * q;
+        if (i < R && j < R) {
+            const size_t o = (size_t)i * ld + j;
+            const double sv = stat[o];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                re[b][q] += sv;
+                mb.out[b][o] = make_double2(re[b][q], im[b][q]);
+            }
+        }
+    }
+    if (ta == tb) return;  // the diagonal tile holds both halves
+    // lower triangle of each member: out[j][i] = value(i, j), through LDS
+    const int i = ta * 64 + c;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        double tre[4];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tr[r0 + 16 * q][c] = re[b][q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tre[q] = tr[c][r0 + 16 * q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tr[r0 + 16 * q][c] = im[b][q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int jj = tb * 64 + r0 + 16 * q;
+            if (i < R && jj < R) mb.out[b][(size_t)jj * ld + i] = make_double2(tre[q], tr[c][r0 + 16 * q]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // moments around a shifted wave number k0 (the points a sweep has above the
 // range of the moments at 0): with d = k - k0,
@@ -1622,6 +1719,44 @@ void RegionSet::assemble_retarded_moments(const double *d_static, double k, int 
     if (timer)
         timer->stop(tk, s, kKernelSmoothMoments, 2.0 * terms * mstride,
                     8.0 * (terms + 1) * mstride + 16.0 * R_ * R_);
+}
+
+void RegionSet::assemble_retarded_moments_batch(const double *d_static, const double *k, const int *terms,
+                                                double2 *const *d_out, int count, hipStream_t s,
+                                                KernelTimer *timer) {
+    if (count < 1 || count > kMaxMomentBatch) throw HipError{"moment batch size out of range"};
+    MomentBatch mb{};
+    int tmax = 0;
+    for (int b = 0; b < count; ++b) {
+        if (terms[b] < 1 || terms[b] > moment_terms_) throw HipError{"moment terms out of range"};
+        // the coefficients of assemble_retarded_moments
+        double km = 1.0;
+        for (int m = 1; m <= terms[b]; ++m) {
+            km *= k[b] / m;
+            const int r = m & 3;
+            mb.cf[b].c[m - 1] = (r == 1 || r == 2) ? -km : km;
+        }
+        mb.out[b] = d_out[b];
+        tmax = std::max(tmax, terms[b]);
+    }
+    const int tk = timer ? timer->start(s) : -1;
+    const size_t mstride = (size_t)moment_tiles_ * 4096;
+#define PEEC_MB(B)                                                                                                \
+    hipLaunchKernelGGL(moment_eval_batch_kernel<B>, dim3(moment_tiles_), dim3(1024), 0, s, d_moments_.ptr, mstride, \
+                       tmax, mb, d_moment_tiles_.ptr, R_, d_static, (size_t)R_)
+    if (count == 1)
+        PEEC_MB(1);
+    else if (count == 2)
+        PEEC_MB(2);
+    else if (count == 3)
+        PEEC_MB(3);
+    else
+        PEEC_MB(4);
+#undef PEEC_MB
+    HIP_CHECK(hipGetLastError());
+    if (timer)
+        timer->stop(tk, s, kKernelSmoothMoments, 2.0 * count * tmax * mstride,
+                    8.0 * (tmax + 1) * mstride + 16.0 * count * R_ * R_);
 }
 
 int RegionSet::shifted_terms_for(double k) const {

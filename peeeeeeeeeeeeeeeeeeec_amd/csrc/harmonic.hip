@@ -439,14 +439,22 @@ static int factor_pivoted(PeecEngine *e, bool fp64, bool accurate = false) {
 // engine's current buffers: the scales only (mixed mode: the FP32 matrix is
 // formed from the blocks later), or the equilibrated FP64 matrix in d_Z.
 // Events ev[1] / ev[2] mark the end of the assembly / of the equilibration.
-static int prepare_point(PeecEngine *e, double f, bool mixed, PeecPointReport &report) {
+// assembled_terms > 0: the retarded blocks of this point are already in
+// d_Lw / d_Pw, from that many stored moments (a batch evaluates its members'
+// series together).
+static int prepare_point(PeecEngine *e, double f, bool mixed, PeecPointReport &report, int assembled_terms = 0) {
     hipStream_t s = e->stream;
     const int n = e->n, ne = e->ne, nv = e->nv;
     const size_t nn = (size_t)n * n;
     const double omega = kTwoPi * f;
     const bool ret = e->opt.retarded != 0;
     e->retarded_now = ret;
-    if (ret) report.moment_terms = assemble_retarded(e, omega / kC0, &report.moment_center);
+    if (ret && assembled_terms > 0) {
+        report.moment_terms = assembled_terms;
+        report.moment_center = 0.0;
+    } else if (ret) {
+        report.moment_terms = assemble_retarded(e, omega / kC0, &report.moment_center);
+    }
     HIP_CHECK(hipEventRecord(e->ev[1], s));
     MnaOperands &m = e->mna;
     m = MnaOperands{};
@@ -1267,6 +1275,45 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
         std::vector<int> prep(m, 0);
         std::vector<double> limits(m, 0.0);
         std::vector<float> ms_prep(m, 0.f), ms_asm(m, 0.f);
+        // retarded members whose series the stored moments cover: their
+        // blocks in shared passes over the moments, up to four at a time
+        // (each member's time is its share of the pass)
+        std::vector<int> pre_terms(m, 0);
+        if (e->opt.retarded != 0) {
+            constexpr int G = RegionSet::kMaxMomentBatch;
+            for (size_t g = 0; g < m; g += G) {
+                const int cnt = (int)std::min<size_t>(G, m - g);
+                double ks[G];
+                int tl[G], tp[G];
+                double2 *ol[G], *op[G];
+                bool ok = cnt > 1;
+                for (int b = 0; b < cnt; ++b) {
+                    ks[b] = kTwoPi * frequency[first + g + b] / kC0;
+                    tl[b] = e->edges->series_terms(ks[b]);
+                    tp[b] = e->nodes->series_terms(ks[b]);
+                    ok = ok && tl[b] <= e->edges->moment_terms() && tp[b] <= e->nodes->moment_terms();
+                }
+                if (!ok) continue;
+                for (int b = 0; b < cnt; ++b) {
+                    PeecEngine::PointSlot &sl = e->slots[g + b];
+                    sl.Lw.resize((size_t)ne * ne);
+                    sl.Pw.resize((size_t)nv * nv);
+                    ol[b] = sl.Lw.ptr;
+                    op[b] = sl.Pw.ptr;
+                }
+                HIP_CHECK(hipEventRecord(e->ev[0], s));
+                e->edges->assemble_retarded_moments_batch(e->d_L.ptr, ks, tl, ol, cnt, s, &e->timer);
+                e->nodes->assemble_retarded_moments_batch(e->d_P.ptr, ks, tp, op, cnt, s, &e->timer);
+                HIP_CHECK(hipEventRecord(e->ev[1], s));
+                HIP_CHECK(hipEventSynchronize(e->ev[1]));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+                for (int b = 0; b < cnt; ++b) {
+                    pre_terms[g + b] = tl[b];
+                    ms_asm[g + b] = ms / (float)cnt;
+                }
+            }
+        }
         // each point's retarded blocks, scales and FP32 matrix in the batch's
         // row order, the matrices side by side in d_Z32
         for (size_t b = 0; b < m; ++b) {
@@ -1275,7 +1322,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             rep[b].batch_size = (int)m;
             swap_point(e, sl);
             HIP_CHECK(hipEventRecord(e->ev[0], s));
-            prep[b] = prepare_point(e, frequency[first + b], true, rep[b]);
+            prep[b] = prepare_point(e, frequency[first + b], true, rep[b], pre_terms[b]);
             if (prep[b] == 0) {
                 double max_scaled = 0.0;
                 if (convert_equilibrated_to_f32(e->mna, e->d_Z32.ptr + b * nn, e->d_batch_perm.ptr, e->d_row.ptr,
@@ -1288,7 +1335,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             sl.limit = limits[b];
             swap_point(e, sl);
             e->stream.sync();
-            HIP_CHECK(hipEventElapsedTime(&ms_asm[b], e->ev[0], e->ev[1]));
+            if (pre_terms[b] == 0) HIP_CHECK(hipEventElapsedTime(&ms_asm[b], e->ev[0], e->ev[1]));
             HIP_CHECK(hipEventElapsedTime(&ms_prep[b], e->ev[1], e->ev[3]));
         }
         // one ordered factor over the batch (a member that failed above is
