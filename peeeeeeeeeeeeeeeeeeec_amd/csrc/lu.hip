@@ -4019,6 +4019,105 @@ __global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, si
     }
 }
 
+// The one-right-hand-side residual of the MNA matrix evaluated from its
+// blocks, visiting only the entries that can be nonzero: row i of an edge
+// has the Lp (R + jwL) part and two incidence entries (-1, +1 at its end
+// nodes), row i of a node has the -P A^T part and jw on the diagonal. Each
+// thread sums the same columns in the same order as residual_multi_kernel
+// (whose other terms are exact zeros), with four columns' loads issued
+// together, so the sums and the result are the same.
+__global__ __launch_bounds__(256) void residual_mna1_kernel(MnaOperands m, const double2 *__restrict__ X,
+                                                            const double *__restrict__ AX,
+                                                            const double2 *__restrict__ B, double2 *__restrict__ R,
+                                                            unsigned long long *eta_bits) {
+    __shared__ double red[4][3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
+    const int i = (int)blockIdx.x, ne = m.ne, n = m.ne + m.nv;
+    double yr = 0.0, yi = 0.0, den = 0.0;
+    auto add = [&](double2 z, int j) {
+        const double za = hypot(z.x, z.y);
+        const double2 v = X[j];
+        yr += z.x * v.x - z.y * v.y;
+        yi += z.x * v.y + z.y * v.x;
+        den += za * AX[j];
+    };
+    if (i < ne) {
+        const double2 *lrow = m.Lw + (size_t)i * ne;
+        const double rd = m.R ? m.R[i] : 0.0;
+        int j = t;
+        for (; j + 768 < ne; j += 1024) {
+            double2 l[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) l[u] = lrow[j + 256 * u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double r = (j + 256 * u == i) ? rd : 0.0;
+                add(make_double2(r - m.omega * l[u].y, m.omega * l[u].x), j + 256 * u);
+            }
+        }
+        for (; j < ne; j += 256) {
+            const double2 l = lrow[j];
+            const double r = j == i ? rd : 0.0;
+            add(make_double2(r - m.omega * l.y, m.omega * l.x), j);
+        }
+        // incidence entries, in column order within the thread that owns them
+        const int ja = ne + m.edges[2 * i], jb = ne + m.edges[2 * i + 1];
+        const int j1 = min(ja, jb), j2 = max(ja, jb);
+        if ((j1 & 255) == t) add(make_double2(j1 == jb ? 1.0 : -1.0, 0.0), j1);
+        if ((j2 & 255) == t) add(make_double2(j2 == jb ? 1.0 : -1.0, 0.0), j2);
+    } else {
+        const int p = i - ne;
+        const double2 *prow = m.Pw + (size_t)p * m.nv;
+        int j = t;
+        for (; j + 768 < ne; j += 1024) {
+            int a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = m.edges[2 * (j + 256 * u)];
+                b[u] = m.edges[2 * (j + 256 * u) + 1];
+            }
+            double2 pa[4], pb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pb[u] = prow[b[u]];
+                pa[u] = prow[a[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) add(make_double2(-(pb[u].x - pa[u].x), -(pb[u].y - pa[u].y)), j + 256 * u);
+        }
+        for (; j < ne; j += 256) {
+            const int a = m.edges[2 * j], b = m.edges[2 * j + 1];
+            const double2 pb = prow[b], pa = prow[a];
+            add(make_double2(-(pb.x - pa.x), -(pb.y - pa.y)), j);
+        }
+        const int jd = ne + p;
+        if ((jd & 255) == t) add(make_double2(0.0, m.omega), jd);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        yr += __shfl_xor(yr, off);
+        yi += __shfl_xor(yi, off);
+        den += __shfl_xor(den, off);
+    }
+    if (lane == 0) {
+        red[wave][0] = yr;
+        red[wave][1] = yi;
+        red[wave][2] = den;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double sr = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+        const double si = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+        const double sd = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+        const double2 bi = B[i];
+        const double rr = sr - bi.x, ri = si - bi.y;
+        R[i] = make_double2(rr, ri);
+        const double ra = hypot(rr, ri), d = sd + hypot(bi.x, bi.y);
+        const double eta = d > 0.0 ? ra / d : ra;
+        atomicMax(eta_bits, (unsigned long long)__double_as_longlong(eta));
+    }
+    (void)n;
+}
+
 // the row-wise residual passes (up to kResidualRhs right-hand sides each)
 template <class Src>
 static void residual_rows(Src src, int n, const double2 *dX, const double2 *dB, double2 *dR, int nrhs, double *d_ax,
@@ -4028,7 +4127,14 @@ static void residual_rows(Src src, int n, const double2 *dX, const double2 *dB, 
     for (int q0 = 0; q0 < nrhs; q0 += kResidualRhs) {
         const int nq = std::min(kResidualRhs, nrhs - q0);
         const size_t off = (size_t)q0 * n;
-        if (nq == 1)
+        bool mna1 = false;
+        if constexpr (std::is_same<Src, MnaSource>::value)
+            mna1 = nq == 1 && src.m.Lw && src.m.Pw && !src.m.PAT;
+        if (mna1) {
+            if constexpr (std::is_same<Src, MnaSource>::value)
+                hipLaunchKernelGGL(residual_mna1_kernel, dim3((unsigned)n), dim3(256), 0, s, src.m, dX + off,
+                                   d_ax + off, dB + off, dR + off, d_eta_bits + q0);
+        } else if (nq == 1)
             hipLaunchKernelGGL((residual_multi_kernel<1, Src>), dim3((unsigned)n), dim3(256), 0, s, src, n, dX + off,
                                d_ax + off, dB + off, dR + off, nq, d_eta_bits + q0);
         else
