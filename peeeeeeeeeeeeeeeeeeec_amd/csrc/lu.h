@@ -71,6 +71,11 @@ struct LUContext {
     DeviceBuffer<float> scale_l16, scale_u16;
     DeviceBuffer<unsigned char> pack_u16n;  // the next block column's U12 (look-ahead update)
     DeviceBuffer<float> scale_u16n;
+    // a second set of L21 / U12 packs: a panel pair whose trailing update is
+    // deferred keeps its packs while the next pair writes its own
+    DeviceBuffer<unsigned char> pack_l16b, pack_u16b;
+    DeviceBuffer<float> scale_l16b, scale_u16b;
+    bool defer_updates = true;                    // ordered factor: every other pair's trailing update merged
     // inverses of the ordered factor's 16 x 16 diagonal blocks, two panels'
     // worth (by panel parity): the look-ahead factor of the next panel writes
     // its set while the current panel's solves read theirs

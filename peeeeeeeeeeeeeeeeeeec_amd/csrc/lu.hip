@@ -1587,6 +1587,15 @@ __device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
 // (split_scale of its largest magnitude over all k of the update), whose
 // inverse is stored beside the pack.
 
+// The operands of one segment of an update: split-FP16 packs of L21 (from
+// the update's first row) and U12 (from its first column), their per-row /
+// per-column inverse scales, and the row / column blocks per chunk.
+struct H3Seg {
+    const uint4 *L = nullptr, *U = nullptr;
+    const float *linv = nullptr, *uinv = nullptr;
+    int rb_pad = 0, cb_count = 0;
+};
+
 // C -= L21 U12 on the split-FP16 operands, one tile of (32 MB) x (32 NB),
 // NB = 8 / MB, per workgroup of four waves in a 2 x 2 grid (each wave MB x NB
 // blocks of 16 x 16), two workgroups per CU so that one's waits fall under
@@ -1599,38 +1608,53 @@ __device__ inline void split_half(float x, _Float16 &hi, _Float16 &lo) {
 // Per chunk and 16 x 16 block: three MFMAs for the real and three for the
 // imaginary part. The update uses MB = 2 (64 x 128 tiles): rows of C are
 // then read and written in 1 KB runs.
+// With SEG = 2 the tile takes two updates of KC chunks each (two panel
+// pairs, each packed with its own scales) in one pass over C: after the
+// first segment's chunks the sums are brought to the second segment's
+// scales, (linv_a / linv_b)(uinv_a / uinv_b) per entry, a power of two and
+// so exact, and the second segment's products accumulate on top.
 // MB x NB blocks per wave (2 x 2 waves: tiles of 32 MB x 32 NB); ND staging
 // register sets (chunks loaded ahead).
-template <int KC, int MB, int NB, int ND>
+template <int KC, int MB, int NB, int ND, int SEG>
 __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
-                                                  const uint4 *__restrict__ Lh, const uint4 *__restrict__ Uh,
-                                                  const float *__restrict__ linv, const float *__restrict__ uinv,
-                                                  int rb_pad, int cb_count, const int *status) {
+                                                  const H3Seg &sa, const H3Seg &sb, const int *status) {
     constexpr int BM = 32 * MB, BN = 32 * NB;
     constexpr int UA = 4 * MB, UNITS = UA + 4 * NB;  // L21 units, all units per chunk
+    constexpr int KT = SEG * KC;                     // chunks of the tile
     // C is read and written EG block rows of the wave at a time (at most 32
     // values in registers beside the accumulators)
     constexpr int EG = MB * NB <= 8 ? MB : 8 / NB, CL = EG * NB * 4;
     static_assert(KC % 2 == 0 && KC >= 2 && MB % EG == 0, "chunks alternate between two LDS slots");
+    static_assert(SEG == 1 || SEG == 2, "one or two segments");
     __shared__ u32x4_t slots_lds[2 * UNITS * 64];
     if (status[0] != 0) return;
     int tm, tn;
     if (!gemm_tile((m + BM - 1) / BM, (ncols + BN - 1) / BN, tm, tn)) return;
-    const u32x4_t *Lv = reinterpret_cast<const u32x4_t *>(Lh), *Uv = reinterpret_cast<const u32x4_t *>(Uh);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const size_t astride = (size_t)rb_pad * 128, bstride = (size_t)cb_count * 128;
-    // wave w loads L21 units MB w .. MB w + MB - 1 and U12 units NB w ..
-    const u32x4_t *as = Lv + (size_t)(tm * (BM / 8) + wave * MB) * 64 + lane;
-    const u32x4_t *bs = Uv + (size_t)(tn * (BN / 8) + wave * NB) * 64 + lane;
-    static_assert(KC % ND == 0, "a tile's chunks must cycle the staging sets exactly");
+    // wave w loads L21 units MB w .. MB w + MB - 1 and U12 units NB w .. of
+    // each segment
+    const u32x4_t *as[SEG], *bs[SEG];
+    size_t astride[SEG], bstride[SEG];
+    {
+        const H3Seg *sg[2] = {&sa, &sb};
+#pragma unroll
+        for (int g = 0; g < SEG; ++g) {
+            as[g] = reinterpret_cast<const u32x4_t *>(sg[g]->L) + (size_t)(tm * (BM / 8) + wave * MB) * 64 + lane;
+            bs[g] = reinterpret_cast<const u32x4_t *>(sg[g]->U) + (size_t)(tn * (BN / 8) + wave * NB) * 64 + lane;
+            astride[g] = (size_t)sg[g]->rb_pad * 128;
+            bstride[g] = (size_t)sg[g]->cb_count * 128;
+        }
+    }
+    static_assert(KT % ND == 0, "a tile's chunks must cycle the staging sets exactly");
     u32x4_t ga[ND][MB], gb[ND][NB];
-    auto load_g = [&](auto set, int c) {
-        constexpr int t = decltype(set)::value;
+    auto load_g = [&](auto set, auto chunk) {
+        constexpr int t = decltype(set)::value, c = decltype(chunk)::value;
+        constexpr int g = c / KC, cc = c % KC;
 #pragma unroll
-        for (int q = 0; q < MB; ++q) ga[t][q] = as[(size_t)c * astride + q * 64];
+        for (int q = 0; q < MB; ++q) ga[t][q] = as[g][(size_t)cc * astride[g] + q * 64];
 #pragma unroll
-        for (int q = 0; q < NB; ++q) gb[t][q] = bs[(size_t)c * bstride + q * 64];
+        for (int q = 0; q < NB; ++q) gb[t][q] = bs[g][(size_t)cc * bstride[g] + q * 64];
     };
     auto store_g = [&](auto set, int slot) {
         constexpr int t = decltype(set)::value;
@@ -1640,7 +1664,9 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
 #pragma unroll
         for (int q = 0; q < NB; ++q) base[(UA + wave * NB + q) * 64] = gb[t][q];
     };
-    static_for<ND>([&](auto q) { load_g(std::integral_constant<int, decltype(q)::value>{}, decltype(q)::value); });
+    static_for<ND>([&](auto q) {
+        load_g(std::integral_constant<int, decltype(q)::value>{}, std::integral_constant<int, decltype(q)::value>{});
+    });
     store_g(std::integral_constant<int, 0>{}, 0);
     __syncthreads();
     const int ra = (wave >> 1) * MB, cbl = (wave & 1) * NB;
@@ -1659,11 +1685,32 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
         const int col = min((cb0 + j) * 16 + li, ncols - 1);
         return A + (size_t)(row0 + row) * lda + col0 + col;
     };
-    static_for<KC>([&](auto step) {
+    static_for<KT>([&](auto step) {
         constexpr int s = decltype(step)::value;
         using cur = std::integral_constant<int, s % ND>;
         using nxt = std::integral_constant<int, (s + 1) % ND>;
-        if constexpr (s + ND < KC) load_g(cur{}, s + ND);
+        if constexpr (s + ND < KT) load_g(cur{}, std::integral_constant<int, s + ND>{});
+        if constexpr (SEG == 2 && s == KC) {
+            // the first segment's sums in the second segment's scales
+            float fc[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int col = (cb0 + j) * 16 + li;
+                fc[j] = sa.uinv[col] / sb.uinv[col];
+            }
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = (rb0 + i) * 16 + 4 * lk + v;
+                    const float fr = sa.linv[row] / sb.linv[row];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        cre[i][j][v] *= fr * fc[j];
+                        cim[i][j][v] *= fr * fc[j];
+                    }
+                }
+        }
         const u32x4_t *sl = slots_lds + (size_t)(s % 2) * UNITS * 64 + lane;
         half8_t uh[NB], ul[NB];
 #pragma unroll
@@ -1692,14 +1739,15 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
                 cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, uh[j], cim[i][j], 0, 0, 0);
             }
         }
-        if constexpr (s + 1 < KC) {
+        if constexpr (s + 1 < KT) {
             store_g(nxt{}, (s + 1) % 2);
             __syncthreads();
         }
     });
+    const H3Seg &last = SEG == 2 ? sb : sa;
     float fc[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) fc[j] = uinv[(cb0 + j) * 16 + li];
+    for (int j = 0; j < NB; ++j) fc[j] = last.uinv[(cb0 + j) * 16 + li];
 #pragma unroll
     for (int g = 0; g < MB; g += EG) {
 #pragma unroll
@@ -1713,7 +1761,7 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = (rb0 + g + i) * 16 + 4 * lk + v;
-                const float fr = linv[row];
+                const float fr = last.linv[row];
 #pragma unroll
                 for (int j = 0; j < NB; ++j) {
                     const int col = (cb0 + j) * 16 + li;
@@ -1727,45 +1775,44 @@ __device__ __forceinline__ void gemm_tile_h3_body(float2 *A, size_t lda, int row
     }
 }
 
-// The trailing update, and the same code under its own name for the next
-// block column's update (a separate entry in kernel traces).
-template <int KC, int MB, int NB, int ND>
+// the segments' pointers moved to batch member blockIdx.z
+__device__ inline void h3_seg_member(H3Seg &g, const BatchStride &bs) {
+    bs.apply_pack(g.L, g.linv, bs.pl, bs.sl);
+    bs.apply_pack(g.U, g.uinv, bs.pu, bs.su);
+}
+
+// The trailing update, and the same code under its own name for the other
+// updates (a separate entry in kernel traces).
+template <int KC, int MB, int NB, int ND, int SEG>
 __global__ __launch_bounds__(256, 2) void gemm_tile_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
-                                                              int ncols, const uint4 *__restrict__ Lh,
-                                                              const uint4 *__restrict__ Uh,
-                                                              const float *__restrict__ linv,
-                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
-                                                              const int *status, BatchStride bs) {
+                                                              int ncols, H3Seg sa, H3Seg sb, const int *status,
+                                                              BatchStride bs) {
     bs.apply(A, status);
-    bs.apply_pack(Lh, linv, bs.pl, bs.sl);
-    bs.apply_pack(Uh, uinv, bs.pu, bs.su);
-    gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+    h3_seg_member(sa, bs);
+    if constexpr (SEG == 2) h3_seg_member(sb, bs);
+    gemm_tile_h3_body<KC, MB, NB, ND, SEG>(A, lda, row0, col0, m, ncols, sa, sb, status);
 }
 template <int KC, int MB, int NB, int ND>
 __global__ __launch_bounds__(256, 2) void gemm_next_h3_kernel(float2 *A, size_t lda, int row0, int col0, int m,
-                                                              int ncols, const uint4 *__restrict__ Lh,
-                                                              const uint4 *__restrict__ Uh,
-                                                              const float *__restrict__ linv,
-                                                              const float *__restrict__ uinv, int rb_pad, int cb_count,
-                                                              const int *status, BatchStride bs) {
+                                                              int ncols, H3Seg sa, H3Seg sb, const int *status,
+                                                              BatchStride bs) {
     bs.apply(A, status);
-    bs.apply_pack(Lh, linv, bs.pl, bs.sl);
-    bs.apply_pack(Uh, uinv, bs.pu, bs.su);
-    gemm_tile_h3_body<KC, MB, NB, ND>(A, lda, row0, col0, m, ncols, Lh, Uh, linv, uinv, rb_pad, cb_count, status);
+    h3_seg_member(sa, bs);
+    gemm_tile_h3_body<KC, MB, NB, ND, 1>(A, lda, row0, col0, m, ncols, sa, sb, status);
 }
 
-// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs (kw = 64,
-// 128 or 256 columns of L21); kinds other than the trailing update run under
-// the next-column kernel's name.
-static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
-                    const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
-                    hipStream_t s, int rb_first = 0, const BatchStride &bs = BatchStride{}, int batch = 1) {
+// A[row0.., col0..] (m x ncols) -= L21 U12 from the split-FP16 packs of
+// segment a (kw = 64, 128 or 256 columns of L21), then, with b, those of a
+// second segment of the same k width in the same pass; kinds other than the
+// trailing update run under the next-column kernel's name (one segment).
+static void gemm_h3_segs(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, const H3Seg &a,
+                         const H3Seg *b, KernelKind kind, const int *status, LUContext &ctx, hipStream_t s,
+                         const BatchStride &bs, int batch) {
     if (m <= 0 || ncols <= 0) return;
-    // rows from row0 are L21 row blocks rb_first.. of the pack
-    const uint4 *Lh = reinterpret_cast<const uint4 *>(ctx.pack_l16.ptr) + (size_t)rb_first * 128;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-    const float *ls = ctx.scale_l16.ptr + (size_t)rb_first * 16;
     const bool next = kind != kKernelLuUpdateSplit16;
+    if (b && next) throw HipError{"split-FP16 update: two segments only in the trailing update"};
+    const H3Seg sb = b ? *b : H3Seg{};
     // 64 x 128 tiles by default (128 x 128 tiles, with a third less operand
     // traffic per output, measured no faster: the update is bound by
     // latency, not by operand bandwidth); 32 x 128 tiles (twice the
@@ -1775,9 +1822,10 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
     const int tiles = ((m + BM - 1) / BM) * ((ncols + BN - 1) / BN);
     const int grid = (tiles + 7) / 8 * 8;
 #define PEEC_H3(KC, MB, NB, ND)                                                                                     \
-    hipLaunchKernelGGL((next ? gemm_next_h3_kernel<KC, MB, NB, ND> : gemm_tile_h3_kernel<KC, MB, NB, ND>),          \
-                       dim3(grid, 1, batch), dim3(256), 0, s, A, lda, row0, col0, m, ncols, Lh, Uh, ls, us, rb_pad,  \
-                       cb_count, status, bs)
+    hipLaunchKernelGGL((next ? gemm_next_h3_kernel<KC, MB, NB, ND>                                                 \
+                        : b  ? gemm_tile_h3_kernel<KC, MB, NB, ND, 2>                                               \
+                             : gemm_tile_h3_kernel<KC, MB, NB, ND, 1>),                                             \
+                       dim3(grid, 1, batch), dim3(256), 0, s, A, lda, row0, col0, m, ncols, a, sb, status, bs)
     if (kw == 256 && short_tiles)
         PEEC_H3(16, 1, 4, 4);
     else if (kw == 256)
@@ -1794,9 +1842,32 @@ static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols,
         throw HipError{"split-FP16 update needs a k width of 64, 128 or 256"};
 #undef PEEC_H3
     HIP_CHECK(hipGetLastError());
+    const int kt = b ? 2 * kw : kw;
     if (ctx.timer)
-        ctx.timer->stop(tg, s, kind, 8.0 * batch * m * ncols * kw,
-                        batch * (2.0 * m * ncols + (double)(m + ncols) * kw) * sizeof(float2));
+        ctx.timer->stop(tg, s, kind, 8.0 * batch * m * ncols * kt,
+                        batch * (2.0 * m * ncols + (double)(m + ncols) * kt) * sizeof(float2));
+}
+
+// One segment from the context's L21 pack set `set` (rows from row0 are its
+// row blocks rb_first..) and the given U12 pack.
+static H3Seg h3_seg(LUContext &ctx, int set, int rb_first, int rb_pad, int cb_count, const uint4 *Uh,
+                    const float *us) {
+    H3Seg g;
+    g.L = reinterpret_cast<const uint4 *>(set ? ctx.pack_l16b.ptr : ctx.pack_l16.ptr) + (size_t)rb_first * 128;
+    g.linv = (set ? ctx.scale_l16b.ptr : ctx.scale_l16.ptr) + (size_t)rb_first * 16;
+    g.U = Uh;
+    g.uinv = us;
+    g.rb_pad = rb_pad;
+    g.cb_count = cb_count;
+    return g;
+}
+
+static void gemm_h3(float2 *A, size_t lda, int row0, int col0, int m, int ncols, int kw, int rb_pad, int cb_count,
+                    const uint4 *Uh, const float *us, KernelKind kind, const int *status, LUContext &ctx,
+                    hipStream_t s, int rb_first = 0, const BatchStride &bs = BatchStride{}, int batch = 1,
+                    int set = 0) {
+    const H3Seg a = h3_seg(ctx, set, rb_first, rb_pad, cb_count, Uh, us);
+    gemm_h3_segs(A, lda, row0, col0, m, ncols, kw, a, nullptr, kind, status, ctx, s, bs, batch);
 }
 
 // Split-FP16 operand pack of `nlines` lines of the k range k0 .. k0 + 16 CH - 1
@@ -2940,6 +3011,15 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
     bool pending = false;    // a diagonal factor is in flight on aux
     bool pair_open = false;  // the previous panel began a pair of split-FP16 panels
     int wprev = 0;
+    // Deferred trailing updates: the second panel of pair A (panels P0 P1)
+    // updates, of its trailing matrix, only the rows of the next pair (P2 P3;
+    // A1) and the columns of P3 and P4 below them (A2); the rest (rows below
+    // P3, columns beyond P4) is applied with pair B's own trailing update in
+    // one pass over C (two segments), from pair A's packs, kept in the other
+    // pack set. Pairs alternate between the two sets.
+    int pair_index = 0;
+    bool deferred = false;
+    int def_set = 0, def_rb_pad = 0, def_cb = 0;
     // split-FP16 operand buffers at their largest (the first panels), so
     // that no buffer moves while the other stream may use it
     const bool split_panels = split16 && blocked_solves && (nb == 128 || nb == 64) && n > nb;
@@ -2957,6 +3037,12 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         ctx.scale_u16n.reserve((size_t)8 * 16 * batch);
         ctx.pack_u16.reserve(pu * batch * sizeof(uint4));
         ctx.scale_u16.reserve((size_t)cbm * 16 * batch);
+        if (ctx.defer_updates) {
+            ctx.pack_l16b.reserve(pl * batch * sizeof(uint4));
+            ctx.scale_l16b.reserve((size_t)rbp * 16 * batch);
+            ctx.pack_u16b.reserve(pu * batch * sizeof(uint4));
+            ctx.scale_u16b.reserve((size_t)cbm * 16 * batch);
+        }
         if (batch > 1) {
             bs_n.pl = bs_r.pl = pl;
             bs_n.sl = bs_r.sl = (size_t)rbp * 16;
@@ -2982,9 +3068,12 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         const bool h3 = split_panels && w == nb;
         const bool second = h3 && pair_open && wprev == w;
         const int cbn = (w2 + 127) / 128 * 8, cb16 = (rest2 + 127) / 128 * 8;
-        uint4 *pln = reinterpret_cast<uint4 *>(ctx.pack_l16.ptr);
+        const int set = ctx.defer_updates ? (pair_index & 1) : 0;
+        uint4 *pln = reinterpret_cast<uint4 *>(set ? ctx.pack_l16b.ptr : ctx.pack_l16.ptr);
+        float *sln = set ? ctx.scale_l16b.ptr : ctx.scale_l16.ptr;
         uint4 *pun = reinterpret_cast<uint4 *>(ctx.pack_u16n.ptr);
-        uint4 *pu = reinterpret_cast<uint4 *>(ctx.pack_u16.ptr);
+        uint4 *pu = reinterpret_cast<uint4 *>(set ? ctx.pack_u16b.ptr : ctx.pack_u16.ptr);
+        float *su = set ? ctx.scale_u16b.ptr : ctx.scale_u16.ptr;
         // U12 of the next block column needs this block's diagonal factor
         // and the previous trailing update; with a look-ahead factor in
         // flight it runs on the second stream behind both, beside L21
@@ -3031,25 +3120,25 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                 // block of the L21 pack (a last, narrower next panel takes
                 // the path below).
                 const int rbd = w2 / 16;
-                solve_l21(k0, w, rest, 0, w2, pln, ctx.scale_l16.ptr, rb_pad, false);
+                solve_l21(k0, w, rest, 0, w2, pln, sln, rb_pad, false);
                 if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, false);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
                 gemm_h3(dA, lda, r0, r0, w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
-                        ctx.status.ptr, ctx, s, 0, bs_n, batch);
+                        ctx.status.ptr, ctx, s, 0, bs_n, batch, set);
                 start_next_diag();
                 if (rest > w2) {
-                    solve_l21(k0, w, rest, w2, rest, pln, ctx.scale_l16.ptr, rb_pad, false);
+                    solve_l21(k0, w, rest, w2, rest, pln, sln, rb_pad, false);
                     gemm_h3(dA, lda, r0 + w2, r0, rest - w2, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr,
-                            kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd, bs_n, batch);
+                            kKernelLuUpdate, ctx.status.ptr, ctx, s, rbd, bs_n, batch, set);
                 }
             } else {
                 // L21 (rows r0..) and U12 of the next block column, then its
                 // update (rank 2w for a pair's second panel; critical path)
-                solve_l21(k0, w, rest, 0, rest, pln, ctx.scale_l16.ptr, rb_pad, second);
+                solve_l21(k0, w, rest, 0, rest, pln, sln, rb_pad, second);
                 if (!u12_aux) solve_u12(k0, w, r0, r0 + w2, s, pun, ctx.scale_u16n.ptr, cbn, second);
                 if (u12_aux) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_u12, 0));
                 gemm_h3(dA, lda, r0, r0, rest, w2, kw, rb_pad, cbn, pun, ctx.scale_u16n.ptr, kKernelLuUpdate,
-                        ctx.status.ptr, ctx, s, 0, bs_n, batch);
+                        ctx.status.ptr, ctx, s, 0, bs_n, batch, set);
                 start_next_diag();
             }
             if (rest2 > 0) {
@@ -3057,17 +3146,46 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                     // the rest of U12, packed by the solve itself (blocks past
                     // the last column are never read into a stored result);
                     // then only the next panel's rows
-                    solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16);
-                    gemm_h3(dA, lda, r0, r0 + w2, w2, rest2, w, rb_pad, cb16, pu, ctx.scale_u16.ptr,
-                            kKernelLuUpdateThin, ctx.status.ptr, ctx, s, 0, bs_r, batch);
+                    solve_u12(k0, w, r0 + w2, n, s, pu, su, cb16);
+                    gemm_h3(dA, lda, r0, r0 + w2, w2, rest2, w, rb_pad, cb16, pu, su, kKernelLuUpdateThin,
+                            ctx.status.ptr, ctx, s, 0, bs_r, batch, set);
                 } else {
                     // the rest of this panel's U12, packed with the first
-                    // panel's U12 rows, and the rank-2w trailing update
-                    solve_u12(k0, w, r0 + w2, n, s, pu, ctx.scale_u16.ptr, cb16, true);
-                    gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, ctx.scale_u16.ptr,
-                            kKernelLuUpdateSplit16, ctx.status.ptr, ctx, s, 0, bs_r, batch);
+                    // panel's U12 rows, and the rank-2w trailing update:
+                    // deferred in part (pair A), merged with the deferred
+                    // pair's (pair B), or whole
+                    solve_u12(k0, w, r0 + w2, n, s, pu, su, cb16, true);
+                    const int span = 2 * nb;  // the next pair's rows / columns
+                    if (!deferred && ctx.defer_updates && w2 == nb && rest2 >= span + 16) {
+                        // A1: the next pair's rows, every column beyond the next panel
+                        gemm_h3(dA, lda, r0, r0 + w2, span, rest2, kw, rb_pad, cb16, pu, su, kKernelLuUpdateThin,
+                                ctx.status.ptr, ctx, s, 0, bs_r, batch, set);
+                        // A2: the rows below it, columns of its second panel and the one after
+                        gemm_h3(dA, lda, r0 + span, r0 + w2, rest - span, span, kw, rb_pad, cb16, pu, su,
+                                kKernelLuUpdateThin, ctx.status.ptr, ctx, s, span / 16, bs_r, batch, set);
+                        deferred = true;
+                        def_set = set;
+                        def_rb_pad = rb_pad;
+                        def_cb = cb16;
+                    } else if (deferred) {
+                        // pair A's packs start a span above / to the left of this update
+                        const int off = span / 16;
+                        const uint4 *pua = reinterpret_cast<const uint4 *>(def_set ? ctx.pack_u16b.ptr
+                                                                                   : ctx.pack_u16.ptr);
+                        const float *sua = def_set ? ctx.scale_u16b.ptr : ctx.scale_u16.ptr;
+                        const H3Seg ga = h3_seg(ctx, def_set, off, def_rb_pad, def_cb, pua + (size_t)off * 128,
+                                                sua + (size_t)off * 16);
+                        const H3Seg gb = h3_seg(ctx, set, 0, rb_pad, cb16, pu, su);
+                        gemm_h3_segs(dA, lda, r0, r0 + w2, rest, rest2, kw, ga, &gb, kKernelLuUpdateSplit16,
+                                     ctx.status.ptr, ctx, s, bs_r, batch);
+                        deferred = false;
+                    } else {
+                        gemm_h3(dA, lda, r0, r0 + w2, rest, rest2, kw, rb_pad, cb16, pu, su, kKernelLuUpdateSplit16,
+                                ctx.status.ptr, ctx, s, 0, bs_r, batch, set);
+                    }
                 }
             }
+            if (second) ++pair_index;
             pair_open = !second;
             wprev = w;
             HIP_CHECK(hipGetLastError());
@@ -3077,7 +3195,8 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         // panel but the last has the full width, and the last has no rows
         // below it): the first panel's thin update left the rows below the
         // pair without its contribution
-        if (pair_open) throw HipError{"ordered factor: a split-FP16 panel pair was left unfinished"};
+        if (pair_open || deferred)
+            throw HipError{"ordered factor: a split-FP16 panel pair was left unfinished"};
         ctx.pack_l.reserve((size_t)(ks_count + 2) * rb_pad * 64 * sizeof(S));
         ctx.pack_u.reserve((size_t)(ks_count + 2) * cb_pad * 64 * sizeof(S));
         S *Lp = reinterpret_cast<S *>(ctx.pack_l.ptr);
@@ -3119,6 +3238,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
         HIP_CHECK(hipGetLastError());
     }
     if (pending) HIP_CHECK(hipStreamWaitEvent(s, ctx.ev_diag, 0));
+    if (deferred) throw HipError{"ordered factor: a deferred trailing update was not applied"};
     std::vector<int> st((size_t)2 * batch, 0);
     ctx.status.download(st.data(), st.size(), s);
     HIP_CHECK(hipStreamSynchronize(s));
