@@ -278,8 +278,10 @@ typedef enum
     PEEC_KERNEL_FAR_TILE = 3,          /* static 1/R tile kernel: 12 flops/eval */
     PEEC_KERNEL_SMOOTH_MOMENTS = 4,    /* retarded remainder from stored moments */
     PEEC_KERNEL_LU_UPDATE_SPLIT16 = 5, /* split-FP16 trailing update of the ordered factor */
-    PEEC_KERNEL_LU_UPDATE_THIN = 6,    /* split-FP16 update of the next panel's rows */
-    PEEC_KERNEL_NEAR_ENTRY = 7         /* static near-field entries (timed at construction) */
+    PEEC_KERNEL_LU_UPDATE_THIN = 6,    /* split-FP16 updates of the next panel's / pair's rows and of
+                                          the block columns after a deferred pair */
+    PEEC_KERNEL_NEAR_ENTRY = 7,        /* static near-field entries (timed at construction) */
+    PEEC_KERNEL_LU_UPDATE_MERGED = 8   /* split-FP16 trailing updates of two panel pairs in one pass */
 } PeecKernel;
 
 typedef struct

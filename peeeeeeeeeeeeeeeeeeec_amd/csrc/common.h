@@ -134,7 +134,8 @@ enum KernelKind {
     kKernelLuUpdateSplit16 = 5,
     kKernelLuUpdateThin = 6,  // first panel of a pair: update of the next panel's rows only
     kKernelNearEntry = 7,     // static near-field entries
-    kKernelKinds = 8
+    kKernelLuUpdateMerged = 8,  // trailing updates of two panel pairs in one pass
+    kKernelKinds = 9
 };
 
 struct KernelTotals {

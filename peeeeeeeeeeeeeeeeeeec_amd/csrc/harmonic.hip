@@ -1390,10 +1390,13 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             std::vector<double> eta(m), best(m);
             std::vector<unsigned long long> bits(m);
             std::vector<int> since(m, 0), active(m), flags(m), steps(m, 0);
-            auto measure = [&]() {
+            // residuals of every member (all), or only of those still refined
+            // (a finished member's solution, and so its eta, no longer change)
+            auto measure = [&](bool all) {
                 for (size_t b = 0; b < m; ++b)
-                    residual_multi(e->slots[b].mna, e->d_X.ptr + b * n, e->d_B.ptr + b * n, e->d_R.ptr + b * n, 1,
-                                   e->d_ax.ptr + b * n, e->d_eta.ptr + b, s);
+                    if (all || active[b])
+                        residual_multi(e->slots[b].mna, e->d_X.ptr + b * n, e->d_B.ptr + b * n, e->d_R.ptr + b * n,
+                                       1, e->d_ax.ptr + b * n, e->d_eta.ptr + b, s);
                 e->d_eta.download(bits.data(), m, s);
                 e->stream.sync();
                 for (size_t b = 0; b < m; ++b) {
@@ -1412,7 +1415,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
             HIP_CHECK(hipMemcpyAsync(e->d_X.ptr, e->d_B.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
             lu_solve_members(e->d_Z32.ptr, nn, n, e->d_batch_perm.ptr, e->d_mrow.ptr, e->d_mcol.ptr, e->d_X.ptr,
                              (int)m, *e->lu, false);
-            measure();
+            measure(true);
             best = eta;
             for (size_t b = 0; b < m; ++b) active[b] = eta[b] > e->opt.refinement_target;
             HIP_CHECK(hipMemcpyAsync(e->d_Xbest.ptr, e->d_X.ptr, sizeof(double2) * tot, hipMemcpyDeviceToDevice, s));
@@ -1425,7 +1428,7 @@ int peec_engine_solve_batch(PeecEngine *e, size_t n_points, const double *freque
                 hipLaunchKernelGGL(masked_axpy_kernel, dim3(grid), dim3(256), 0, s, e->d_X.ptr, e->d_R.ptr,
                                    (const int *)nullptr, mask_of(active), n, (int)m);
                 for (size_t b = 0; b < m; ++b) steps[b] += active[b];
-                measure();
+                measure(false);
                 bool improved = false;
                 for (size_t b = 0; b < m; ++b) {
                     flags[b] = 0;

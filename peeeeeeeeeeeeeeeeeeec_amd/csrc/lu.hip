@@ -1810,7 +1810,7 @@ static void gemm_h3_segs(float2 *A, size_t lda, int row0, int col0, int m, int n
                          const BatchStride &bs, int batch) {
     if (m <= 0 || ncols <= 0) return;
     const int tg = ctx.timer ? ctx.timer->start(s) : -1;
-    const bool next = kind != kKernelLuUpdateSplit16;
+    const bool next = kind != kKernelLuUpdateSplit16 && kind != kKernelLuUpdateMerged;
     if (b && next) throw HipError{"split-FP16 update: two segments only in the trailing update"};
     const H3Seg sb = b ? *b : H3Seg{};
     // 64 x 128 tiles by default (128 x 128 tiles, with a third less operand
@@ -3176,7 +3176,7 @@ LUStatus lu_factor_ordered(float2 *dA, int n, size_t lda, double pivot_limit, LU
                         const H3Seg ga = h3_seg(ctx, def_set, off, def_rb_pad, def_cb, pua + (size_t)off * 128,
                                                 sua + (size_t)off * 16);
                         const H3Seg gb = h3_seg(ctx, set, 0, rb_pad, cb16, pu, su);
-                        gemm_h3_segs(dA, lda, r0, r0 + w2, rest, rest2, kw, ga, &gb, kKernelLuUpdateSplit16,
+                        gemm_h3_segs(dA, lda, r0, r0 + w2, rest, rest2, kw, ga, &gb, kKernelLuUpdateMerged,
                                      ctx.status.ptr, ctx, s, bs_r, batch);
                         deferred = false;
                     } else {
@@ -3596,6 +3596,125 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const S *M, size_t lda,
     }
 }
 
+// The one-right-hand-side chain with the same order of work per block as
+// trsv_chain_kernel<S, V, UPPER, 1>, restructured for the workgroups that
+// run behind the front of the chain (all but the last few dependencies of
+// their block are already solved when they reach them): up to K
+// dependencies are taken per round trip (their factor entries and polled
+// values loaded together, then summed in dependency order), and the block
+// inverse is staged as its packed triangle (L_bb^-1 is lower, U_bb^-1 upper
+// triangular; the zero triangle only added zeros), which also lets two
+// workgroups share a CU. Products and sums are those of the one-at-a-time
+// kernel, in the same order, so the solutions are the same. Member
+// blockIdx.y of a batch at the given strides.
+template <class S, bool UPPER, int K>
+__global__ __launch_bounds__(256) void trsv_chain1_kernel(const S *M, size_t lda, const double2 *B, double2 *Y, int n,
+                                                          const double2 *dinv, int *bad, size_t m_stride,
+                                                          size_t v_stride, size_t d_stride) {
+    using T = double2;
+    M += (size_t)blockIdx.y * m_stride;
+    B += (size_t)blockIdx.y * v_stride;
+    Y += (size_t)blockIdx.y * v_stride;
+    dinv += (size_t)blockIdx.y * d_stride;
+    constexpr int TB = kChainTB, NT = TB * (TB + 1) / 2;
+    __shared__ T dtri[NT];
+    __shared__ T xs[K][kChainRow];
+    __shared__ T bs[TB];
+    __shared__ int s_abort;
+    const int nblk = (n + TB - 1) / TB;
+    const int tid = threadIdx.x;
+    const int i = tid >> 2, qt = tid & 3, c0 = qt * 16;
+    const int cs = chain_col(c0);
+    // packed position of inverse entry (r, c) of the stored triangle
+    auto tri = [](int r, int c) { return UPPER ? c * (c + 1) / 2 + r : r * (r + 1) / 2 + c; };
+    if (tid == 0) s_abort = 0;
+    for (int w = blockIdx.x; w < nblk; w += gridDim.x) {
+        const int b = UPPER ? nblk - 1 - w : w;
+        const int r0 = b * TB, rows = min(TB, n - r0);
+        const int ri = min(i, rows - 1);
+        T acc = ezero<T>();
+        __syncthreads();  // the previous block is done with dtri, xs, bs
+        for (int idx = tid; idx < TB * TB; idx += 256) {
+            const int r = idx / TB, c = idx % TB;
+            if (UPPER ? c >= r : c <= r) dtri[tri(r, c)] = dinv[(size_t)b * TB * TB + idx];
+        }
+        if (tid < TB) bs[tid] = tid < rows ? B[r0 + tid] : ezero<T>();
+        const S *mrow = M + (size_t)(r0 + ri) * lda;
+        for (int d0 = 0; d0 < w; d0 += K) {
+            const int kk = min(K, w - d0);
+            // this thread's 16 factor entries of each dependency of the round
+            // (clamped columns past n meet zero values below)
+            S mf[K][16];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < kk) {
+                    const int jb = UPPER ? nblk - 1 - (d0 + k) : d0 + k;
+                    const int cb = jb * TB, cols = min(TB, n - cb);
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) mf[k][c] = mrow[cb + min(c0 + c, cols - 1)];
+                }
+            if (tid < TB) {
+                T xv[K];
+                // every dependency's value first, then wait on those not yet there
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    xv[k] = ezero<T>();
+                    const int jb = UPPER ? nblk - 1 - (d0 + k) : d0 + k;
+                    if (k < kk && tid < min(TB, n - jb * TB))
+                        sub_elem(reinterpret_cast<const double *>(Y + (size_t)jb * TB + tid), xv[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int jb = UPPER ? nblk - 1 - (d0 + k) : d0 + k;
+                    if (k < kk && tid < min(TB, n - jb * TB)) {
+                        const double *p = reinterpret_cast<const double *>(Y + (size_t)jb * TB + tid);
+                        unsigned spins = 0;
+                        while (chain_empty(xv[k])) {
+                            if (++spins > kSpinLimit) {
+                                atomicOr(bad, 2);
+                                s_abort = 1;
+                                xv[k] = ezero<T>();
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                            sub_elem(p, xv[k]);
+                        }
+                    }
+                    xs[k][chain_col(tid)] = xv[k];
+                }
+            }
+            __syncthreads();
+            if (s_abort) return;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < kk) {
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) acc = eadd(acc, emul(ld(&mf[k][c]), xs[k][cs + c]));
+                }
+            __syncthreads();
+        }
+        __syncthreads();  // bs and dtri of a block without dependencies
+        {
+            const T v = quad_sum(acc);
+            if (qt == 0) xs[0][chain_col(i)] = esub(bs[i], v);
+        }
+        __syncthreads();
+        T v = ezero<T>();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int c = c0 + k;
+            if (UPPER ? c >= i : c <= i) v = eadd(v, emul(dtri[tri(i, c)], xs[0][cs + k]));
+        }
+        v = quad_sum(v);
+        if (qt == 0 && i < rows) pub_elem(reinterpret_cast<double *>(Y + r0 + i), v);
+    }
+}
+
+// dependencies per round trip of trsv_chain1_kernel: two for FP32 factors
+// (fastest measured of 2 / 3 / 4: three workgroups fit a CU), one for FP64
+template <class S> constexpr int chain_deps() { return sizeof(S) == sizeof(float2) ? 2 : 1; }
+constexpr int kChainDeps = chain_deps<float2>();  // the FP32 factors of a batch
+
 // number of chain workgroups that can be resident at once (the chain
 // kernel's workgroups wait on each other)
 template <class S, class V>
@@ -3631,6 +3750,16 @@ void chain_solve(const S *dLU, int n, size_t lda, const V *B, V *Y, int nrhs, co
         int nq = std::min(8, nrhs - q0), nn = n;
         size_t ld = lda, zero = 0;
         void *args[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, &nq, (void *)&dinv, &bad, &zero, &zero, &zero};
+        if constexpr (std::is_same<V, double2>::value) {
+            if (nq == 1) {
+                // one right-hand side: the dependency-batched chain
+                void *args1[] = {(void *)&dLU, &ld, (void *)&Bq, &Yq, &nn, (void *)&dinv, &bad, &zero, &zero, &zero};
+                HIP_CHECK(hipLaunchCooperativeKernel(
+                    reinterpret_cast<const void *>(trsv_chain1_kernel<S, UPPER, chain_deps<S>()>), dim3(grid),
+                    dim3(256), args1, 0, ctx.stream));
+                continue;
+            }
+        }
         void *fn = nq == 1 ? reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 1>)
                            : reinterpret_cast<void *>(trsv_chain_kernel<S, V, UPPER, 8>);
         HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, ctx.stream));
@@ -3860,11 +3989,11 @@ void lu_solve_members(const float2 *dLU, size_t lu_stride, int n, const int *d_p
         ctx.dinv_members = members;
     }
     // every member's chain gets an equal share of the co-resident workgroups
+    const void *fwd = reinterpret_cast<const void *>(trsv_chain1_kernel<S, false, kChainDeps>);
+    const void *bwd = reinterpret_cast<const void *>(trsv_chain1_kernel<S, true, kChainDeps>);
     int f = 0, bw = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &f, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, false, 1>), 256, 0));
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &bw, reinterpret_cast<const void *>(trsv_chain_kernel<S, V, true, 1>), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, fwd, 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, bwd, 256, 0));
     const long long cap = (long long)std::min(f, bw) * ctx.cus / members;
     if (cap < 1) throw HipError{"batched triangular solve: too many members for co-resident chains"};
     const int G = (int)std::min<long long>(nblk, cap);
@@ -3872,13 +4001,11 @@ void lu_solve_members(const float2 *dLU, size_t lu_stride, int n, const int *d_p
         const size_t words = total * 2;
         hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
                            reinterpret_cast<unsigned long long *>(Y), words, kChainEmpty);
-        int nn = n, nq = 1;
+        int nn = n;
         size_t ld = (size_t)n, ms = lu_stride, vs = (size_t)n, ds = dstride;
         int *bad = ctx.solve_bad.ptr;
-        void *args[] = {(void *)&dLU, &ld, (void *)&B, &Y, &nn, &nq, (void *)&dinv, &bad, &ms, &vs, &ds};
-        void *fn = upper ? reinterpret_cast<void *>(trsv_chain_kernel<S, V, true, 1>)
-                         : reinterpret_cast<void *>(trsv_chain_kernel<S, V, false, 1>);
-        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(G, members), dim3(256), args, 0, s));
+        void *args[] = {(void *)&dLU, &ld, (void *)&B, &Y, &nn, (void *)&dinv, &bad, &ms, &vs, &ds};
+        HIP_CHECK(hipLaunchCooperativeKernel(upper ? bwd : fwd, dim3(G, members), dim3(256), args, 0, s));
     };
     chain(false, tmp, ybuf, linv);
     chain(true, ybuf, tmp, uinv);

@@ -65,7 +65,7 @@ class PeecKernelStats(C.Structure):
 
 
 KERNELS = {"lu_update": 0, "panel": 1, "smooth_tile": 2, "far_tile": 3, "smooth_moments": 4,
-           "lu_update_split16": 5, "lu_update_thin": 6, "near_entry": 7}
+           "lu_update_split16": 5, "lu_update_thin": 6, "near_entry": 7, "lu_update_merged": 8}
 
 # flop model of the static assembly: 1/R evaluation, analytic triangle potential
 FLOPS_PER_INV_R = 12.0
