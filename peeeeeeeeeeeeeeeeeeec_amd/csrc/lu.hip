@@ -2431,7 +2431,121 @@ __global__ __launch_bounds__(512) void diag_lu_blocked_kernel(float2 *A, size_t 
 // contiguous 512-byte run of the wavefront.
 __device__ __forceinline__ int blk_slot(int k, int m) { return (((k & 3) << 2) + (k >> 2)) * 16 + m; }
 template <int NBK> __host__ __device__ constexpr size_t trsm_mfma_lds() {
-    return (size_t)(NBK + NBK * (NBK - 1) / 2) * 256 * sizeof(float2);
+    // the 16 x 16 operand blocks, then (split-FP16 solves) 16 row scales per block
+    return (size_t)(NBK + NBK * (NBK - 1) / 2) * (256 * sizeof(float2) + 16 * sizeof(float));
+}
+
+// Split-FP16 products in the MFMA triangular solves. A 16 x 16 operand block
+// staged in LDS in the FP32 layout above (operand entry A[m][k] at
+// blk_slot(k, m)) is rewritten in place by one wavefront as the A operand of
+// v_mfma_f32_16x16x32_f16: lane (m, lk) holds k = 4 lk .. + 3 as (re, -im)
+// half pairs, the hi vectors at [lane] and the lo vectors at [64 + lane]
+// (uint4 units, the same 2 KB), each row m scaled by split_scale of its
+// largest part; sinv[m] = 1 / scale.
+__device__ __forceinline__ void split_block_lds(float2 *blk, float *sinv, int lane) {
+    const int li = lane & 15, lk = lane >> 4;
+    float2 a[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] = blk[blk_slot(4 * lk + v, li)];
+    float mx = 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(a[v].x), fabsf(a[v].y)));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+    unsigned hw[4], lw[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        _Float16 xh, xl, yh, yl;
+        split_half(a[v].x * sc, xh, xl);
+        split_half(-a[v].y * sc, yh, yl);
+        hw[v] = pack_half2(xh, yh);
+        lw[v] = pack_half2(xl, yl);
+    }
+    u32x4_t *o = reinterpret_cast<u32x4_t *>(blk);
+    o[lane] = u32x4_t{hw[0], hw[1], hw[2], hw[3]};
+    o[64 + lane] = u32x4_t{lw[0], lw[1], lw[2], lw[3]};
+    if (lk == 0) sinv[li] = 1.f / sc;
+}
+
+// The B operand of a wavefront's 16 lines x 16 k (lane (li, lk): line li, k =
+// 4 lk + v in re[v], im[v]) as split-FP16 halves, the line scaled by
+// split_scale of its largest part; returns 1 / scale.
+__device__ __forceinline__ float split_lines(const v4f &re, const v4f &im, u32x4_t &hi, u32x4_t &lo) {
+    float mx = 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mx = fmaxf(mx, fmaxf(fabsf(re[v]), fabsf(im[v])));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float sc = split_scale(mx);
+    unsigned hw[4], lw[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        _Float16 xh, xl, yh, yl;
+        split_half(re[v] * sc, xh, xl);
+        split_half(im[v] * sc, yh, yl);
+        hw[v] = pack_half2(xh, yh);
+        lw[v] = pack_half2(xl, yl);
+    }
+    hi = u32x4_t{hw[0], hw[1], hw[2], hw[3]};
+    lo = u32x4_t{lw[0], lw[1], lw[2], lw[3]};
+    return 1.f / sc;
+}
+
+// (cr, ci) = A B of one 16 x 16 block pair on split operands (A from a block
+// rewritten by split_block_lds, B from split_lines): hi hi + hi lo + lo hi
+// for the real part and for the imaginary part (A's pairs turned).
+__device__ __forceinline__ void cmul_h3(const u32x4_t *ablk, int lane, const u32x4_t &bh, const u32x4_t &bl, v4f &cr,
+                                        v4f &ci) {
+    const u32x4_t ah = ablk[lane], al = ablk[64 + lane];
+    const half8_t rh = __builtin_bit_cast(half8_t, ah), rl = __builtin_bit_cast(half8_t, al);
+    const half8_t ih = __builtin_bit_cast(half8_t, turn_pairs(ah)), il = __builtin_bit_cast(half8_t, turn_pairs(al));
+    const half8_t hb = __builtin_bit_cast(half8_t, bh), lb = __builtin_bit_cast(half8_t, bl);
+    const v4f z = {0.f, 0.f, 0.f, 0.f};
+    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, hb, z, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, hb, z, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, lb, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, lb, ci, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(rl, hb, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, hb, ci, 0, 0, 0);
+}
+
+// The block solve of a wavefront's 16 lines on split-FP16 products: for j in
+// order, X_j = A_j B_j^-1 (inverse block j of s_inv, as the operand of the
+// transposed product), then A_c -= X_j B_jc (block p of s_off) for c > j; the
+// products are unscaled, (row scale of the operand block)(line scale of X),
+// exact powers of two.
+template <int NBK>
+__device__ __forceinline__ void solve_lines_h3(v4f (&xr)[NBK], v4f (&xi)[NBK], const float2 *s_inv, const float2 *s_off,
+                                               const float *sc_inv, const float *sc_off, int lane) {
+    const int lk = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < NBK; ++j) {
+        u32x4_t bh, bl;
+        float fb = split_lines(xr[j], xi[j], bh, bl);
+        v4f cr, ci;
+        cmul_h3(reinterpret_cast<const u32x4_t *>(s_inv + 256 * j), lane, bh, bl, cr, ci);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float f = sc_inv[16 * j + 4 * lk + v] * fb;
+            xr[j][v] = cr[v] * f;
+            xi[j][v] = ci[v] * f;
+        }
+        if (j + 1 < NBK) {
+            fb = split_lines(xr[j], xi[j], bh, bl);
+#pragma unroll
+            for (int c = j + 1; c < NBK; ++c) {
+                const int p = j * (NBK - 1) - j * (j - 1) / 2 + (c - j - 1);
+                cmul_h3(reinterpret_cast<const u32x4_t *>(s_off + 256 * p), lane, bh, bl, cr, ci);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float f = sc_off[16 * p + 4 * lk + v] * fb;
+                    xr[c][v] -= cr[v] * f;
+                    xi[c][v] -= ci[v] * f;
+                }
+            }
+        }
+    }
 }
 
 // The split-FP16 operand block of a solved 16-row (16-column) block straight
@@ -2498,7 +2612,7 @@ __device__ __forceinline__ void pack_h3_regs(const v4f (&xr)[NBK], const v4f (&x
 // the t-th wavefront, rb_pad blocks per chunk); with PRE, the k range of the
 // operand starts at k0 - 16 NBK (the first panel of a pair, whose L21 is in
 // place) and covers both panels.
-template <int NBK, bool PRE>
+template <int NBK, bool PRE, bool H3 = true>
 __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t lda, int k0, int r_begin, int r_end,
                                                               const float2 *__restrict__ dinv, const int *status,
                                                               uint4 *pk, float *pinv, int rb_pad, BatchStride bs) {
@@ -2546,10 +2660,17 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
 #pragma unroll
     for (int q = 0; q < NBK + NU; ++q) s_inv[q * 256 + blk_slot(er, ec)] = g[q];
     __syncthreads();
+    float *s_sc = reinterpret_cast<float *>(s_inv + (NBK + NU) * 256);
+    if constexpr (H3) {
+        // the blocks as split-FP16 operands (one wavefront per block)
+        for (int q = wave; q < NBK + NU; q += 4) split_block_lds(s_inv + 256 * q, s_sc + 16 * q, lane);
+        __syncthreads();
+    }
     if (!live) return;
     const int lo = lk * 16 + li;  // + 64 t: operand of the t-th MFMA
+    if constexpr (H3) solve_lines_h3<NBK>(xr, xi, s_inv, s_u, s_sc, s_sc + 16 * NBK, lane);
 #pragma unroll
-    for (int j = 0; j < NBK; ++j) {
+    for (int j = 0; j < (H3 ? 0 : NBK); ++j) {
         // X_j^T = (U_jj^-1)^T (A_j - ...)^T
         v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -2585,7 +2706,7 @@ __global__ __launch_bounds__(256) void trsm_right_mfma_kernel(float2 *A, size_t 
 // columns are also written as the split-FP16 U12 operand of the update
 // (block `tile` = the wavefront's 16-column block from c0, pcb blocks per
 // chunk); with PRE its k range starts at k0 - 16 NBK (both panels of a pair).
-template <int NBK, bool PRE>
+template <int NBK, bool PRE, bool H3 = true>
 __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t lda, int k0, int c0, int c1,
                                                              const float2 *__restrict__ dinv, const int *status,
                                                              uint4 *pk, float *pinv, int pcb, BatchStride bs) {
@@ -2637,10 +2758,17 @@ __global__ __launch_bounds__(256) void trsm_left_mfma_kernel(float2 *A, size_t l
 #pragma unroll
     for (int q = 0; q < NU; ++q) s_l[q * 256 + blk_slot(ec, er)] = g[NBK + q];
     __syncthreads();
+    float *s_sc = reinterpret_cast<float *>(s_inv + (NBK + NU) * 256);
+    if constexpr (H3) {
+        // the blocks as split-FP16 operands (one wavefront per block)
+        for (int q = wave; q < NBK + NU; q += 4) split_block_lds(s_inv + 256 * q, s_sc + 16 * q, lane);
+        __syncthreads();
+    }
     if (!live) return;
     const int lo = lk * 16 + li;
+    if constexpr (H3) solve_lines_h3<NBK>(xr, xi, s_inv, s_l, s_sc, s_sc + 16 * NBK, lane);
 #pragma unroll
-    for (int j = 0; j < NBK; ++j) {
+    for (int j = 0; j < (H3 ? 0 : NBK); ++j) {
         v4f nr = {0.f, 0.f, 0.f, 0.f}, ni = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
