@@ -182,3 +182,11 @@ def test_gpu_entry_points_fail_loudly_without_gpu():
         assert st != 0
     finally:
         m.close()
+
+
+def test_kernel_kinds_match_header():
+    """The Python names of the per-kernel timers map to the PeecKernel values
+    include/peec_gpu.h declares, one for one."""
+    text = open(os.path.join(ROOT, "include", "peec_gpu.h")).read()
+    declared = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"PEEC_KERNEL_([A-Z0-9_]+)\s*=\s*(\d+)", text)}
+    assert declared == plib.KERNELS
