@@ -3781,8 +3781,12 @@ __global__ __launch_bounds__(256) void trsv_chain1_kernel(const S *M, size_t lda
         }
         if (tid < TB) bs[tid] = tid < rows ? B[r0 + tid] : ezero<T>();
         const S *mrow = M + (size_t)(r0 + ri) * lda;
+        __builtin_amdgcn_s_setprio(0);
         for (int d0 = 0; d0 < w; d0 += K) {
             const int kk = min(K, w - d0);
+            // the last round is the front of the chain: its waves go first on
+            // a CU shared with workgroups still summing earlier dependencies
+            if (d0 + K >= w) __builtin_amdgcn_s_setprio(3);
             // this thread's 16 factor entries of each dependency of the round
             // (clamped columns past n meet zero values below)
             S mf[K][16];
