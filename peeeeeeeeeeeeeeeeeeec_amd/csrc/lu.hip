@@ -2233,6 +2233,9 @@ template <int NBK>
 __global__ __launch_bounds__(512) void diag_lu_blocked_kernel(float2 *A, size_t lda, int k0, double limit,
                                                               int *status, float2 *dinv, BatchStride bs) {
     constexpr int W = 16 * NBK, S = W + 4;
+    // the look-ahead stream's critical path: ahead of the trailing update's
+    // waves on the CUs it shares with them
+    __builtin_amdgcn_s_setprio(2);
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2 *sa = reinterpret_cast<float2 *>(smem_raw);  // the block, W x S
     float2 *slt = sa + (size_t)W * S;                   // L_ip transposed, tile i - p - 1
