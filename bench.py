@@ -564,7 +564,8 @@ def main():
             "kernel": ("LU trailing update A22 -= L21 U12 of the reused-row-order factor, two panel pairs "
                        "(2 x 256 columns of L21, each pair with its own split-FP16 scales) merged in one pass "
                        "over C, complex FP32 storage, split-FP16 operands on v_mfma_f32_16x16x32_f16 "
-                       "(gemm_tile_h3_kernel<16, 2, 4, 4, 2>)"
+                       "(gemm_tile_h3_kernel<16, 4, 4, 2, 2>: 128 x 128 tiles; the small updates at the end of "
+                       "the factor <16, 2, 4, 4, 2>, 64 x 128)"
                        + (f"; each launch updates the {B} points of a batch (grid z)" if B > 1 else "")),
             "bound": "mfma",
             "achieved": g16_tf / 1e12,
