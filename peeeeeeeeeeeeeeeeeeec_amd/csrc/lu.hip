@@ -4354,28 +4354,34 @@ __global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, si
     for (int i = 0; i < MB; ++i) arow[i] = A + (size_t)min(r0 + 16 * i + li, n - 1) * lda;
 #pragma unroll
     for (int j = 0; j < NB; ++j) xcol[j] = X + (size_t)min(q0 + 16 * j + li, nrhs - 1) * n;
-    double ar[MB], ai[MB], am[MB], br[NB], bi[NB], bm[NB];
-    auto load = [&](int k0) {
-        const int k = k0 + lk;
-        const bool in = k < n;
-        const int kc = min(k, n - 1);
+    // raw operands of two k steps in flight (loaded two steps ahead of their
+    // products); each step's MFMAs take the same operands in the same order
+    // as a load-then-multiply loop, so the sums are the same
+    double2 zr[2][MB], xr[2][NB];
+    auto load = [&](int st, int k0) {
+        const int kc = min(k0 + lk, n - 1);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) zr[st][i] = arow[i][kc];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) xr[st][j] = xcol[j][kc];
+    };
+    auto step = [&](int st, int k0) {
+        const bool in = k0 + lk < n;
+        double ar[MB], ai[MB], am[MB], br[NB], bi[NB], bm[NB];
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
-            const double2 z = arow[i][kc];
-            ar[i] = in ? z.x : 0.0;
-            ai[i] = in ? z.y : 0.0;
+            ar[i] = in ? zr[st][i].x : 0.0;
+            ai[i] = in ? zr[st][i].y : 0.0;
             am[i] = sqrt(fma(ar[i], ar[i], ai[i] * ai[i]));
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const double2 v = xcol[j][kc];
-            br[j] = in ? v.x : 0.0;
-            bi[j] = in ? v.y : 0.0;
+            br[j] = in ? xr[st][j].x : 0.0;
+            bi[j] = in ? xr[st][j].y : 0.0;
             bm[j] = sqrt(fma(br[j], br[j], bi[j] * bi[j]));
         }
-    };
-    for (int k0 = 0; k0 < n; k0 += 4) {
-        load(k0);
+        // this stage's registers are free again: the step after next
+        if (k0 + 8 < n) load(st, k0 + 8);
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
             const double nai = -ai[i];
@@ -4388,7 +4394,15 @@ __global__ __launch_bounds__(256) void residual_mfma_kernel(const double2 *A, si
                 cden[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[i], bm[j], cden[i][j], 0, 0, 0);
             }
         }
+    };
+    load(0, 0);
+    if (4 < n) load(1, 4);
+    int k0 = 0;
+    for (; k0 + 4 < n; k0 += 8) {
+        step(0, k0);
+        step(1, k0 + 4);
     }
+    if (k0 < n) step(0, k0);
     // accumulator element v of lane (li, lk): row 4 v + lk of the 16-row
     // block, right-hand side li of the 16-column block
 #pragma unroll
