@@ -278,19 +278,23 @@ __device__ inline double2 equilibrated(double2 v, double r, double c) {
     return v;
 }
 
-// dst row i (FP32) = equilibrated row perm[i] (or i) of src; the largest
-// |equilibrated entry| into maxv[0 .. kMaxSlots). One workgroup per (row,
-// 256 columns).
+// dst row i (FP32) = equilibrated row perm[i] (or i) of src, formed as
+// (a_ij * (1 / r_i)) * (1 / c_j) with the scales' reciprocals: within a few
+// FP64 ulps of the divided value (a_ij / r_i) / c_j, which the rounding to
+// FP32 absorbs, at a fraction of four FP64 divisions per entry. The largest
+// |equilibrated entry| (the root of the largest squared magnitude; entries
+// are at most 1 in magnitude) into maxv[0 .. kMaxSlots). One workgroup per
+// (row, 1024 columns).
 constexpr int kMaxSlots = 1024;
 template <class Src>
 __global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(Src src, float2 *dst, const int *perm,
-                                                                  const double *row, const double *col, int n,
+                                                                  const double *row, const double *inv_col, int n,
                                                                   double *maxv) {
     __shared__ double red[4];
     // four entries per thread (1024 per workgroup), loads issued first
     const int i = blockIdx.y, c0 = blockIdx.x * 1024 + threadIdx.x;
     const int si = perm ? perm[i] : i;
-    const double ri = row[si];
+    const double ri = 1.0 / row[si];
     double2 a[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) a[u] = c0 + 256 * u < n ? src(si, c0 + 256 * u) : double2{};
@@ -299,9 +303,10 @@ __global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(Src src, float
     for (int u = 0; u < 4; ++u) {
         const int c = c0 + 256 * u;
         if (c < n) {
-            const double2 v = equilibrated(a[u], ri, col[c]);
-            dst[(size_t)i * n + c] = make_float2((float)v.x, (float)v.y);
-            m = fmax(m, mag(v));
+            const double ic = inv_col[c];
+            const double vx = (a[u].x * ri) * ic, vy = (a[u].y * ri) * ic;
+            dst[(size_t)i * n + c] = make_float2((float)vx, (float)vy);
+            m = fmax(m, vx * vx + vy * vy);
         }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
@@ -310,7 +315,15 @@ __global__ __launch_bounds__(256) void to_f32_equilibrated_kernel(Src src, float
     // one of kMaxSlots slots per workgroup: no single contended address
     if (threadIdx.x == 0)
         atomic_max_pos(&maxv[(blockIdx.x + (size_t)blockIdx.y * gridDim.x) % kMaxSlots],
-                       fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+                       sqrt(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+}
+
+// inv[j] = 1 / col[j]; a zero scale is reported in *bad
+__global__ void inverse_scale_kernel(const double *col, int n, double *inv, int *bad) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    if (col[j] == 0.0) atomicOr(bad, 1);
+    inv[j] = 1.0 / col[j];
 }
 
 // dst (FP64) = the equilibrated src
@@ -320,11 +333,6 @@ __global__ void equilibrated_copy_kernel(const double2 *src, double2 *dst, const
     if (idx >= (size_t)n * n) return;
     const int i = (int)(idx / n), c = (int)(idx % n);
     dst[idx] = equilibrated(src[idx], row[i], col[c]);
-}
-
-__global__ void zero_scale_check_kernel(const double *col, int n, int *bad) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j < n && col[j] == 0.0) atomicOr(bad, 1);
 }
 
 __global__ void fill_zero_kernel(double *p, size_t n) {
@@ -433,14 +441,16 @@ template <class Src>
 static int convert_equilibrated_to_f32_of(Src src, float2 *dst, const int *d_perm, const double *d_row,
                                           const double *d_col, int n, double *max_scaled, LUContext &ctx) {
     hipStream_t s = ctx.stream;
-    DeviceBuffer<double> mx(kMaxSlots);
+    // slot maxima, then the column scales' reciprocals
+    DeviceBuffer<double> mx(kMaxSlots + (size_t)n);
     hipLaunchKernelGGL(fill_zero_kernel, dim3(kMaxSlots / 256), dim3(256), 0, s, mx.ptr, (size_t)kMaxSlots);
     // zero column maxima would divide by zero: reported like the in-place path
     DeviceBuffer<int> bad(1);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, bad.ptr, 1, 0);
-    hipLaunchKernelGGL(zero_scale_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, n, bad.ptr);
+    hipLaunchKernelGGL(inverse_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_col, n, mx.ptr + kMaxSlots,
+                       bad.ptr);
     hipLaunchKernelGGL(to_f32_equilibrated_kernel<Src>, dim3((unsigned)((n + 1023) / 1024), (unsigned)n), dim3(256), 0,
-                       s, src, dst, d_perm, d_row, d_col, n, mx.ptr);
+                       s, src, dst, d_perm, d_row, mx.ptr + kMaxSlots, n, mx.ptr);
     HIP_CHECK(hipGetLastError());
     int h_bad = 0;
     std::vector<double> slots(kMaxSlots);
